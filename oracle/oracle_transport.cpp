@@ -1,0 +1,335 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+// CPU ORACLE — test infrastructure only (see oracle.h).  Serial restatement
+// of the reference's inviscid transport operators for backend 0:
+// Cells / Faces (HLL part) / Rhs / Advance / FaceSpeeds / Step and the
+// Minimum and BoundaryPartial/BoundaryFinish reductions
+// (src/reactiveTransport/reactiveTransportKernels.h, reactiveTransport.cpp).
+// Scope: viscosity = conductivity = diffusivity = 0, no mechanical closure,
+// no WALE or capillary; those configurations are rejected at create.
+#include "oracle.h"
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace oracle {
+namespace {
+
+inline double mn(double a, double b) { return a < b ? a : b; }
+inline double mx(double a, double b) { return a > b ? a : b; }
+void require(bool ok, const char* message) { if (!ok) throw std::runtime_error(message); }
+
+struct Prim { double rho, u[3]; };
+struct Work { double advectL, advectR, pressureMomentum, pressureEnergy; };
+
+struct Transport {
+    Config cfg;
+    std::vector<Face> faces;
+    std::vector<double> inverseVolume;
+    std::vector<size_t> row, boundary;
+    std::vector<int64_t> incidence;
+    std::vector<double> q, initial, rhs;  // SoA: qi(c,k) = k*(cells+fixed)+c
+    std::vector<TState> state;
+    std::vector<Prim> prim;
+    std::vector<Work> work;
+    std::vector<double> faceSpeed, step;
+    size_t partitions = 1;
+    bool haveInitial = false;
+    double stageDt = 0;
+    std::string error;
+
+    size_t all() const { return cfg.cells + cfg.fixed; }
+    size_t qi(size_t c, size_t k) const { return k * all() + c; }
+
+    // Constructor checks and CSR incidence (reactiveTransport.cpp:410-488).
+    Transport(const Config& c, const double* volume, const Face* f, const double* fixedQ, const TState* fixedStates)
+        : cfg(c) {
+        require(cfg.cells && cfg.species && cfg.faces, "Empty transport mesh/model");
+        require(cfg.variables >= cfg.species + 4 && cfg.variables <= cfg.species + 6, "Wrong transport conserved-variable count");
+        require(std::isfinite(cfg.waveFactor) && cfg.waveFactor >= 1, "Invalid transport wave factor");
+        require(cfg.viscosity == 0 && cfg.conductivity == 0 && cfg.diffusivity == 0 && !cfg.mechanical,
+                "Oracle covers the inviscid, non-diffusive, non-mechanical transport only");
+        require(volume && f, "Null transport geometry");
+        row.assign(cfg.cells + 1, 0);
+        for (size_t i = 0; i < cfg.cells; ++i) require(std::isfinite(volume[i]) && volume[i] > 0, "Invalid transport volume");
+        for (size_t i = 0; i < cfg.faces; ++i) {
+            const Face& x = f[i];
+            require(x.owner >= 0 && size_t(x.owner) < cfg.cells && x.neighbour >= -1 &&
+                        (x.neighbour < 0 || size_t(x.neighbour) < cfg.cells), "Invalid transport face cell");
+            require(x.kind >= 0 && x.kind <= 3 && (x.kind == 0) == (x.neighbour >= 0), "Invalid transport face kind");
+            require(x.kind != 3 || (x.fixed >= 0 && size_t(x.fixed) < cfg.fixed), "Invalid fixed boundary state index");
+            double norm = 0;
+            for (double component : x.normal) norm += component * component;
+            require(std::isfinite(norm) && std::abs(norm - 1) < 1e-8, "Invalid transport unit normal");
+            require(std::isfinite(x.area) && std::isfinite(x.distance) && std::isfinite(x.ownerWeight) && x.area > 0 &&
+                        x.distance > 0 && x.ownerWeight >= 0 && x.ownerWeight <= 1, "Invalid transport face geometry");
+            ++row[x.owner + 1];
+            if (x.neighbour >= 0) ++row[x.neighbour + 1];
+            else boundary.push_back(i);
+        }
+        for (size_t i = 0; i < cfg.cells; ++i) row[i + 1] += row[i];
+        incidence.resize(row.back());
+        std::vector<size_t> next(row);
+        for (size_t i = 0; i < cfg.faces; ++i) {
+            incidence[next[f[i].owner]++] = -int64_t(i) - 1;
+            if (f[i].neighbour >= 0) incidence[next[f[i].neighbour]++] = int64_t(i) + 1;
+        }
+        faces.assign(f, f + cfg.faces);
+        inverseVolume.resize(cfg.cells);
+        for (size_t i = 0; i < cfg.cells; ++i) inverseVolume[i] = 1 / volume[i];
+        q.assign(all() * cfg.variables, 0);
+        initial.assign(all() * cfg.variables, 0);
+        state.assign(all(), TState{});
+        prim.assign(all(), Prim{});
+        work.assign(cfg.faces, Work{});
+        faceSpeed.assign(cfg.faces, 0);
+        step.assign(cfg.cells, 0);
+        partitions = std::min(size_t(256), std::max(size_t(1), (boundary.size() + 255) / 256));
+        for (size_t i = 0; i < cfg.fixed; ++i)
+            for (size_t k = 0; k < cfg.variables; ++k)
+                q[qi(cfg.cells + i, k)] = initial[qi(cfg.cells + i, k)] = fixedQ[i * cfg.variables + k];
+        require(!cfg.fixed || fixedStates, "Null transport state");
+        validateStates(fixedStates, cfg.fixed);
+        for (size_t i = 0; i < cfg.fixed; ++i) state[cfg.cells + i] = fixedStates[i];
+    }
+    static void validateStates(const TState* s, size_t n) {  // reactiveTransport.cpp:537-545
+        for (size_t c = 0; c < n; ++c)
+            require(std::isfinite(s[c].p) && s[c].p > 0 && std::isfinite(s[c].T) && s[c].T > 0 && std::isfinite(s[c].rho) &&
+                        s[c].rho > 0 && std::isfinite(s[c].sound) && s[c].sound > 0 && std::isfinite(s[c].cv) && s[c].cv > 0 &&
+                        std::isfinite(s[c].gasMass) && s[c].gasMass >= 0 && std::isfinite(s[c].dilatation),
+                    "Invalid recovered transport state");
+    }
+    size_t rightCell(const Face& f) const {  // reactiveTransportKernels.h:134-136
+        return f.neighbour >= 0 ? size_t(f.neighbour) : (f.kind == 3 ? cfg.cells + size_t(f.fixed) : size_t(f.owner));
+    }
+    double rightValue(const Face& f, size_t k) const {  // :137-144
+        const size_t r = rightCell(f);
+        double value = q[qi(r, k)];
+        if (f.kind == 1 && k >= cfg.species && k < cfg.species + 3) {
+            double normal = 0;
+            for (int d = 0; d < 3; ++d) normal += q[qi(r, cfg.species + d)] * f.normal[d];
+            value -= 2 * normal * f.normal[k - cfg.species];
+        }
+        return value;
+    }
+    void velocities(const Face& f, double* ul, double* ur) const {  // :145-149
+        const Prim& l = prim[f.owner];
+        const Prim& r = prim[rightCell(f)];
+        double dot = 0;
+        for (int d = 0; d < 3; ++d) { ul[d] = l.u[d]; ur[d] = r.u[d]; dot += r.u[d] * f.normal[d]; }
+        if (f.kind == 1)
+            for (int d = 0; d < 3; ++d) ur[d] -= 2 * dot * f.normal[d];
+    }
+    double faceFlux(size_t fi, size_t k) const {  // :180-191 (inviscid)
+        const Face& f = faces[fi];
+        const Work& w = work[fi];
+        const double ql = q[qi(f.owner, k)], qr = rightValue(f, k);
+        double flux = w.advectL * ql + w.advectR * qr;
+        if (k >= cfg.species && k < cfg.species + 3) flux += (w.pressureMomentum - 0.0) * f.normal[k - cfg.species];
+        if (k == cfg.species + 3) flux += w.pressureEnergy;
+        if (k < cfg.species) flux += 0.0;                        // species diffusion, D = 0
+        else if (k < cfg.species + 3) flux -= 0.0;               // viscous traction, mu = 0
+        else if (k == cfg.species + 3) flux += 0.0;              // viscous/conductive energy
+        return flux * f.area;
+    }
+    void cells() {  // Cells, :193-199
+        for (size_t c = 0; c < all(); ++c) {
+            Prim p{};
+            for (size_t k = 0; k < cfg.species; ++k) p.rho += q[qi(c, k)];
+            for (int d = 0; d < 3; ++d) p.u[d] = q[qi(c, cfg.species + d)] / p.rho;
+            prim[c] = p;
+        }
+    }
+    void facesHll() {  // Faces HLL coefficients, :343-356
+        for (size_t fi = 0; fi < cfg.faces; ++fi) {
+            const Face& f = faces[fi];
+            const TState& sl = state[f.owner];
+            const TState& sr = state[rightCell(f)];
+            double ul[3], ur[3], unL = 0, unR = 0;
+            velocities(f, ul, ur);
+            for (int d = 0; d < 3; ++d) { unL += ul[d] * f.normal[d]; unR += ur[d] * f.normal[d]; }
+            const double aL = cfg.waveFactor * sl.sound, aR = cfg.waveFactor * sr.sound;
+            const double left = mn(0, mn(unL - aL, unR - aR)), right = mx(0, mx(unL + aL, unR + aR));
+            const double invWave = 1 / (right - left);
+            Work w;
+            w.advectL = right * (unL - left) * invWave;
+            w.advectR = left * (right - unR) * invWave;
+            w.pressureMomentum = (right * sl.p - left * sr.p) * invWave;
+            w.pressureEnergy = (right * sl.p * unL - left * sr.p * unR) * invWave;
+            work[fi] = w;
+        }
+    }
+    void faceSpeeds() {  // FaceSpeeds, :315-342 (D = 0)
+        for (size_t fi = 0; fi < cfg.faces; ++fi) {
+            const Face& f = faces[fi];
+            const TState& sl = state[f.owner];
+            const TState& sr = state[rightCell(f)];
+            double ul[3], ur[3], unL = 0, unR = 0;
+            velocities(f, ul, ur);
+            for (int d = 0; d < 3; ++d) { unL += ul[d] * f.normal[d]; unR += ur[d] * f.normal[d]; }
+            const double aL = cfg.waveFactor * sl.sound, aR = cfg.waveFactor * sr.sound;
+            const double left = mn(0, mn(unL - aL, unR - aR)), right = mx(0, mx(unL + aL, unR + aR));
+            const double invWave = 1 / (right - left);
+            const double muFace = cfg.viscosity + 0;
+            const double D = cfg.diffusivity + mx(mx(4 * muFace / (3 * sl.rho), 4 * muFace / (3 * sr.rho)),
+                                                  mx(cfg.conductivity / (sl.rho * sl.cv), cfg.conductivity / (sr.rho * sr.cv)));
+            double speed = mx(std::fabs(unL) + aL, std::fabs(unR) + aR) + 2 * (D + 0) / f.distance;
+            if (!std::isfinite(unL) || !std::isfinite(unR) || !std::isfinite(speed) || !std::isfinite(invWave) || right <= left)
+                speed = -1;
+            faceSpeed[fi] = speed;
+        }
+    }
+    double rate(size_t c, size_t k) const {  // Rhs::value, :502-526
+        double r = 0;
+        for (size_t j = row[c]; j < row[c + 1]; ++j) {
+            const int64_t entry = incidence[j];
+            const size_t fi = size_t(entry < 0 ? -entry - 1 : entry - 1);
+            const double sign = entry < 0 ? -1 : 1;
+            r += sign * faceFlux(fi, k) * inverseVolume[c];
+        }
+        return r;
+    }
+    static double tree256(double* v) {  // reduceGroups combine order, reactiveTransport.cpp:70-81
+        for (size_t stride = 128; stride; stride /= 2)
+            for (size_t j = 0; j < stride; ++j) v[j] = v[j] + v[j + stride];
+        return v[0];
+    }
+    void boundaryRates(double* out) const {  // BoundaryPartial/Finish, :233-246
+        std::vector<double> partial(cfg.variables * partitions);
+        double v[256];
+        for (size_t g = 0; g < cfg.variables * partitions; ++g) {
+            const size_t k = g / partitions, tile = g % partitions;
+            for (size_t lane = 0; lane < 256; ++lane) {
+                double sum = 0;
+                for (size_t j = tile * 256 + lane; j < boundary.size(); j += partitions * 256) sum += faceFlux(boundary[j], k);
+                v[lane] = sum;
+            }
+            partial[g] = tree256(v);
+        }
+        for (size_t k = 0; k < cfg.variables; ++k) {
+            for (size_t lane = 0; lane < 256; ++lane) v[lane] = lane < partitions ? partial[k * partitions + lane] : 0;
+            out[k] = tree256(v);
+        }
+    }
+    void upload(const double* hostQ, const TState* states) {
+        require(hostQ, "Null packed source");
+        for (size_t c = 0; c < cfg.cells; ++c)
+            for (size_t k = 0; k < cfg.variables; ++k) q[qi(c, k)] = hostQ[c * cfg.variables + k];
+        require(states, "Null transport state");
+        validateStates(states, cfg.cells);
+        for (size_t c = 0; c < cfg.cells; ++c) state[c] = states[c];
+        cells();
+    }
+    double minimumStep(double cfl, double maximum) {  // Step + Minimum, :529-550 and reactiveTransport.cpp:1445-1456
+        require(std::isfinite(cfl) && cfl > 0 && cfl <= .5 && std::isfinite(maximum) && maximum > 0, "Invalid transport CFL controls");
+        for (size_t c = 0; c < cfg.cells; ++c) {
+            double denominator = 0;
+            bool bad = false;
+            for (size_t j = row[c]; j < row[c + 1]; ++j) {
+                const int64_t entry = incidence[j];
+                const size_t fi = size_t(entry < 0 ? -entry - 1 : entry - 1);
+                const double speed = faceSpeed[fi];
+                if (!std::isfinite(speed) || speed < 0) { bad = true; break; }
+                denominator += faces[fi].area * speed;
+            }
+            if (bad) { step[c] = -1; continue; }
+            step[c] = denominator > 0 ? mn(maximum, cfl / (inverseVolume[c] * denominator)) : maximum;
+            if (!std::isfinite(denominator) || denominator < 0) step[c] = -1;
+        }
+        double dt;
+        if (cfg.cells == 1) dt = step[0];
+        else {
+            dt = HUGE_VAL;
+            for (size_t c = 0; c < cfg.cells; ++c) dt = mn(dt, std::isfinite(step[c]) && step[c] > 0 ? step[c] : -1);
+        }
+        require(std::isfinite(dt) && dt > 0, "Invalid wave/diffusion timestep");
+        return dt;
+    }
+    void advance(double dt, int stage) {  // Advance, :551-560, then swap (reactiveTransport.cpp:1461-1468)
+        for (size_t j = 0; j < cfg.cells * cfg.variables; ++j) {
+            const size_t c = j % cfg.cells, k = j / cfg.cells, index = qi(c, k);
+            const double r = rate(c, k);
+            if (stage == 0) initial[index] = q[index] + dt * r;
+            else initial[index] = .5 * initial[index] + .5 * (q[index] + dt * r);
+        }
+        std::swap(q, initial);
+        haveInitial = stage == 0;
+        stageDt = dt;
+    }
+};
+
+template <class F> int guard(void* h, F f) {
+    if (!h) return -1;
+    Transport& t = *static_cast<Transport*>(h);
+    try {
+        f(t);
+        t.error.clear();
+        return 0;
+    } catch (const std::exception& e) {
+        t.haveInitial = false;
+        t.error = e.what();
+        return 1;
+    }
+}
+
+}  // namespace
+}  // namespace oracle
+
+using namespace oracle;
+
+extern "C" {
+
+void* oracle_transport_create(const Config* cfg, const double* volume, const Face* faces, const double* fixedQ,
+                              const TState* fixedStates, char* error, size_t size) {
+    try {
+        require(cfg, "Null transport configuration");
+        return new Transport(*cfg, volume, faces, fixedQ, fixedStates);
+    } catch (const std::exception& e) {
+        if (error && size) std::snprintf(error, size, "%s", e.what());
+        return nullptr;
+    }
+}
+void oracle_transport_destroy(void* h) { delete static_cast<Transport*>(h); }
+const char* oracle_transport_error(void* h) { return h ? static_cast<Transport*>(h)->error.c_str() : "Null transport"; }
+
+int oracle_transport_rhs(void* h, const double* q, const TState* states, double* rhs, double* boundary) {
+    return guard(h, [&](Transport& t) {
+        t.haveInitial = false;
+        t.upload(q, states);
+        t.facesHll();
+        for (size_t c = 0; c < t.cfg.cells; ++c)
+            for (size_t k = 0; k < t.cfg.variables; ++k) rhs[c * t.cfg.variables + k] = t.rate(c, k);
+        t.boundaryRates(boundary);
+    });
+}
+int oracle_transport_stage(void* h, double* q, const TState* states, double dt, int stage, double* boundary) {
+    return guard(h, [&](Transport& t) {
+        require(std::isfinite(dt) && dt > 0 && (stage == 0 || stage == 1), "Invalid RK stage");
+        require(stage == 0 || (t.haveInitial && t.stageDt == dt), "RK stage 1 requires matching stage 0");
+        t.upload(q, states);
+        t.facesHll();
+        t.boundaryRates(boundary);
+        t.advance(dt, stage);
+        for (size_t c = 0; c < t.cfg.cells; ++c)
+            for (size_t k = 0; k < t.cfg.variables; ++k) q[c * t.cfg.variables + k] = t.q[t.qi(c, k)];
+    });
+}
+int oracle_transport_stable_step(void* h, const double* q, const TState* states, double cfl, double maximum, double* dt) {
+    return guard(h, [&](Transport& t) {
+        require(dt && std::isfinite(cfl) && cfl > 0 && cfl <= .5 && std::isfinite(maximum) && maximum > 0,
+                "Invalid transport CFL controls");
+        t.upload(q, states);
+        t.faceSpeeds();
+        *dt = t.minimumStep(cfl, maximum);
+    });
+}
+// Conservation totals sum_c q_ck V_c in serial cell order (ReactiveFoam.C:953-958).
+void oracle_totals(size_t cells, size_t variables, const double* q, const double* volume, double* out) {
+    for (size_t k = 0; k < variables; ++k) out[k] = 0;
+    for (size_t c = 0; c < cells; ++c)
+        for (size_t k = 0; k < variables; ++k) out[k] += q[c * variables + k] * volume[c];
+}
+
+}

@@ -1,0 +1,140 @@
+"""ctypes mirrors of the C ABI structures (include/reactive_transport.h,
+include/reactive_gpu_hem.h) plus numpy dtypes of the same layouts."""
+import ctypes as C
+import numpy as np
+
+
+class ThermoState(C.Structure):
+    """ReactiveThermoState (reactiveThermo.h:11-19, 176 B)."""
+    _fields_ = [("p", C.c_double), ("T", C.c_double), ("rho", C.c_double), ("e", C.c_double),
+                ("entropy", C.c_double), ("soundFrozen", C.c_double), ("soundEquilibrium", C.c_double),
+                ("volumeResidual", C.c_double), ("energyResidual", C.c_double),
+                ("chemicalResidual", C.c_double), ("alphaGas", C.c_double),
+                ("alphaLiquid", C.c_double * 2), ("rhoGas", C.c_double), ("rhoLiquid", C.c_double * 2),
+                ("liquidMass", C.c_double * 2), ("gasMass", C.c_double), ("cp", C.c_double),
+                ("cv", C.c_double), ("activeLiquids", C.c_int), ("iterations", C.c_int)]
+
+
+STATE_DTYPE = np.dtype([("p", "<f8"), ("T", "<f8"), ("rho", "<f8"), ("e", "<f8"), ("entropy", "<f8"),
+                        ("soundFrozen", "<f8"), ("soundEquilibrium", "<f8"), ("volumeResidual", "<f8"),
+                        ("energyResidual", "<f8"), ("chemicalResidual", "<f8"), ("alphaGas", "<f8"),
+                        ("alphaLiquid", "<f8", (2,)), ("rhoGas", "<f8"), ("rhoLiquid", "<f8", (2,)),
+                        ("liquidMass", "<f8", (2,)), ("gasMass", "<f8"), ("cp", "<f8"), ("cv", "<f8"),
+                        ("activeLiquids", "<i4"), ("iterations", "<i4")])
+assert C.sizeof(ThermoState) == 176 and STATE_DTYPE.itemsize == 176
+
+
+class HemProfile(C.Structure):
+    """ReactiveGpuHemProfileV1 (reactiveGpuHem.h:14-21, 144 B)."""
+    _fields_ = [("abiVersion", C.c_uint32), ("structBytes", C.c_uint32)] + [
+        (n, C.c_uint64) for n in ("batches", "submitted", "succeeded", "cpuFallbacks", "deviceFailures",
+                                  "phaseEvaluations", "residualEvaluations", "flashCandidates",
+                                  "stableCandidates", "analyticJacobians", "finiteDifferenceJacobians",
+                                  "transferBytes", "hostBytes", "deviceBytes")] + [
+        (n, C.c_double) for n in ("wallSeconds", "kernelSeconds", "copySeconds")]
+
+    @classmethod
+    def fresh(cls):
+        p = cls()
+        p.abiVersion = 1
+        p.structBytes = C.sizeof(cls)
+        return p
+
+
+assert C.sizeof(HemProfile) == 144
+
+
+class CapillaryInput(C.Structure):
+    _fields_ = [("color", C.c_double), ("pressureJump", C.c_double), ("equilibrium", C.c_int)]
+
+
+class HemOutput(C.Structure):
+    _fields_ = [("state", ThermoState), ("success", C.c_int), ("reserved", C.c_int)]
+
+
+assert C.sizeof(CapillaryInput) == 24 and C.sizeof(HemOutput) == 184
+HEM_OUTPUT_DTYPE = np.dtype([("state", STATE_DTYPE), ("success", "<i4"), ("reserved", "<i4")])
+
+
+class Face(C.Structure):
+    """ReactiveTransportFace (reactiveTransport.h:10-14, 80 B)."""
+    _fields_ = [("owner", C.c_int64), ("neighbour", C.c_int64), ("fixed", C.c_int64),
+                ("normal", C.c_double * 3), ("area", C.c_double), ("distance", C.c_double),
+                ("ownerWeight", C.c_double), ("kind", C.c_int)]
+
+
+FACE_DTYPE = np.dtype([("owner", "<i8"), ("neighbour", "<i8"), ("fixed", "<i8"), ("normal", "<f8", (3,)),
+                       ("area", "<f8"), ("distance", "<f8"), ("ownerWeight", "<f8"), ("kind", "<i4"),
+                       ("pad", "<i4")])
+assert C.sizeof(Face) == 80 and FACE_DTYPE.itemsize == 80
+
+
+class TransportState(C.Structure):
+    """ReactiveTransportState (reactiveTransport.h:15-17, 56 B)."""
+    _fields_ = [(n, C.c_double) for n in ("p", "T", "rho", "cv", "sound", "gasMass", "dilatation")]
+
+
+TSTATE_DTYPE = np.dtype([(n, "<f8") for n in ("p", "T", "rho", "cv", "sound", "gasMass", "dilatation")])
+
+
+class Config(C.Structure):
+    """ReactiveTransportConfig (reactiveTransport.h:18-23, 88 B)."""
+    _fields_ = [("cells", C.c_size_t), ("species", C.c_size_t), ("variables", C.c_size_t),
+                ("faces", C.c_size_t), ("fixed", C.c_size_t), ("viscosity", C.c_double),
+                ("conductivity", C.c_double), ("diffusivity", C.c_double), ("waveFactor", C.c_double),
+                ("maxBytes", C.c_double), ("mechanical", C.c_int)]
+
+
+class Token(C.Structure):
+    _fields_ = [("attemptId", C.c_uint64), ("stageId", C.c_uint64), ("contentVersion", C.c_uint64)]
+
+
+class OptionsV2(C.Structure):
+    _fields_ = [("abiVersion", C.c_uint32), ("structBytes", C.c_uint32), ("bridgeCells", C.c_size_t),
+                ("recomputeGas", C.c_int), ("physicalModelHash", C.c_char * 65)]
+
+
+class OptionsV21(C.Structure):
+    _fields_ = [("abiVersion", C.c_uint32), ("structBytes", C.c_uint32), ("slotBytes", C.c_size_t),
+                ("pinnedBudgetBytes", C.c_size_t), ("bridgeCellsOverride", C.c_size_t),
+                ("slots", C.c_uint32), ("blockThreads", C.c_uint32), ("recomputeGas", C.c_int),
+                ("detailedGasCounters", C.c_int), ("physicalModelHash", C.c_char * 65)]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("allocatedBytes", "uploadedBytes", "downloadedBytes",
+                                          "kernelLaunches", "stages", "stepQueries")]
+
+
+class Primitive(C.Structure):
+    _fields_ = [("rho", C.c_double), ("u", C.c_double * 3)]
+
+
+class Profile(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("conservedUploads", "conservedUploadBytes", "conservedDownloadBytes",
+                                          "stateUploadBytes", "primitiveUploadBytes", "gasUploadBytes",
+                                          "boundaryPartitions", "residentStages")]
+
+
+class DeviceProfile(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("gasPropertyBuilds", "gasPropertyCells", "partitionUploadBytes",
+                                          "thermoTableBytes", "gasStatusChecks", "cflFaceLaunches",
+                                          "transportFaceLaunches", "faceWorkspaceBytes")]
+
+
+class MemoryV2(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("liveBytes", "peakBytes", "bridgeBytes", "conservedWorkspaceBytes",
+                                          "gasWorkspaceBytes", "rhsWorkspaceBytes", "deviceToDeviceBytes",
+                                          "synchronizations")]
+
+
+class ProfileV21(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("layoutTiles", "layoutLaunches", "memcpyCalls", "payloadBytes",
+                                          "nasaValidationEvaluations", "nasaFaceEvaluations",
+                                          "temperatureBasisBuilds", "pinnedBytes", "deviceStagingBytes",
+                                          "temperatureBasisBytes", "eventWaits", "slotCells", "slots",
+                                          "blockThreads", "failedCalls")]
+
+
+assert C.sizeof(Config) == 88 and C.sizeof(OptionsV2) == 88 and C.sizeof(OptionsV21) == 120
+assert C.sizeof(Token) == 24 and C.sizeof(TransportState) == 56 and C.sizeof(Primitive) == 32
