@@ -1,0 +1,237 @@
+#!/usr/bin/env python3
+"""HLL + SSPRK2 + HEM-flash cell-steps/s on one MI355X.
+
+Workload: pure N2O Peng-Robinson HEM in a 160^3 uniform Cartesian box (80 mm
+cube, extrapolated boundaries, waveFactor 1.1, CFL 0.4), seeded state
+T ~ U(180, 300) K, rho log-uniform 1-1000 kg/m^3, u ~ N(0, 20 m/s), internal
+energy from the isochoric equilibrium at (rho, T).  One step = CFL, stage 0,
+flash, CFL recheck, stage 1, flash, conservation totals, all resident on the
+device (reactive_amd_step_run).  Inputs are resident in HBM before timing.
+
+Prints one JSON line (the bench contract) with:
+  roofline      fused flux sweep (both SSPRK2 stages): algorithmic bytes per
+                cell per stage / event-timed kernel duration vs 8 TB/s HBM
+  roofline_flash the dominant kernel (UV flash), FP64 VALU: see DESIGN.md
+  cpu_baseline  the reference's own CPU code (oracle/_ref) on a bounded sample
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "spuma-multiphysics_amd"))
+from reactive_amd import abi, cases, lib as rlib, model  # noqa: E402
+
+METRIC = "HLL+HEM-flash cell-steps/s at 160³: mixed vs FP64 GPU at ≤1e-10 rel T,p"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def prepare(n, seed):
+    """Seeded case + flash-consistent step-start states (one device flash)."""
+    faces, volume, q, seeds, info = cases.pure_n2o_cube(n, seed=seed)
+    m = model.model_bytes(model.build_model())
+    lib = rlib.load()
+    err = C.create_string_buffer(1024)
+    cells = len(q)
+    hem = lib.reactive_gpu_hem_create_v1(m, len(m), cells, err, 1024)
+    if not hem:
+        raise RuntimeError(err.value.decode())
+    e = cases.internal_energy(q, 1)
+    states = seeds.copy()
+    success = np.zeros(cells, np.int32)
+    prof = abi.HemProfile.fresh()
+    rc = lib.reactive_gpu_hem_run_v1(hem, rlib.ptr(np.ascontiguousarray(q[:, :1])), rlib.ptr(e), cells,
+                                     rlib.ptr(states), rlib.ptr(success), C.byref(prof), err, 1024)
+    if rc:
+        raise RuntimeError(err.value.decode())
+    bad = np.where(success == 0)[0]
+    if bad.size:  # a failed synthetic cell takes the state and conserved values of a converged neighbour
+        good = np.where(success == 1)[0]
+        src = good[np.searchsorted(good, bad).clip(0, good.size - 1)]
+        q[bad] = q[src]
+        states[bad] = states[src]
+    info["prep_failures"] = int(bad.size)
+    info["prep_flash_seconds"] = prof.kernelSeconds
+    return faces, volume, q, states, m, hem, info
+
+
+def cpu_baseline(m, q_end, seeds_start, threads, sample, seed):
+    """Reference CPU path on a bounded sample: the reference's own flash
+    (oracle/_ref, compiled from its sources) on `sample` cells of a real step's
+    second flash, in `threads` workers; the reference transport (serial,
+    backend 0) on a 48^3 box; per-cell costs scaled to cell-steps/s."""
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+    kind = "reference" if os.path.exists(ref_path) else "port"
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(len(q_end), size=min(sample, len(q_end)), replace=False))
+    qs = np.ascontiguousarray(q_end[idx, :1])
+    e = cases.internal_energy(q_end[idx], 1)
+    st = seeds_start[idx].copy()
+    mb = C.create_string_buffer(m, len(m))
+    chunks = np.array_split(np.arange(len(idx)), threads * 4)
+    if kind == "reference":
+        ref = C.CDLL(ref_path)
+        ref.ref_flash_batch.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                        C.c_void_p, C.c_void_p, C.c_void_p]
+        out = st.copy()
+        succ = np.zeros(len(idx), np.int32)
+
+        def work(ch):
+            if ch.size:
+                a, b = ch[0], ch[-1] + 1
+                ref.ref_flash_batch(mb, b - a, rlib.ptr(qs[a:b]), rlib.ptr(e[a:b]), rlib.ptr(st[a:b]), 0,
+                                    rlib.ptr(out[a:b]), rlib.ptr(succ[a:b]), None)
+    else:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from helpers import oracle
+        ora = oracle()
+        out = st.copy()
+        succ = np.zeros(len(idx), np.int32)
+
+        def work(ch):
+            if ch.size:
+                a, b = ch[0], ch[-1] + 1
+                ora.oracle_flash_batch(mb, b - a, rlib.ptr(qs[a:b]), rlib.ptr(e[a:b]), rlib.ptr(out[a:b]),
+                                       rlib.ptr(succ[a:b]), None, 1)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as pool:
+        list(pool.map(work, chunks))
+    flash_wall = time.perf_counter() - t0
+    # transport: reference serial operators (backend 0) on a 48^3 box, two stages
+    tn = 48
+    faces, volume, q0, seeds0, _ = cases.pure_n2o_cube(tn, seed=seed)
+    tst = np.zeros(len(q0), dtype=abi.TSTATE_DTYPE)
+    tst["p"] = 1e5; tst["T"] = 250.0; tst["rho"] = q0[:, 0]; tst["cv"] = 800.0; tst["sound"] = 200.0
+    tst["gasMass"] = q0[:, 0]
+    cfg = abi.Config(cells=len(q0), species=1, variables=5, faces=len(faces), fixed=0, viscosity=0,
+                     conductivity=0, diffusivity=0, waveFactor=1.1, maxBytes=1e12, mechanical=0)
+    err = C.create_string_buffer(1024)
+    if kind == "reference":
+        lib = C.CDLL(ref_path)
+        lib.reactive_transport_create.restype = C.c_void_p
+        lib.reactive_transport_create.argtypes = [C.c_int, C.c_void_p] + [C.c_void_p] * 6 + [C.c_char_p, C.c_size_t]
+        lib.reactive_transport_stage.argtypes = [C.c_void_p] * 5 + [C.c_double, C.c_int, C.c_void_p]
+        lib.reactive_transport_destroy.argtypes = [C.c_void_p]
+        h = lib.reactive_transport_create(0, C.byref(cfg), rlib.ptr(volume), rlib.ptr(faces), None, None, None,
+                                          None, err, 1024)
+        qq = q0.copy()
+        b = np.zeros(5)
+        t0 = time.perf_counter()
+        for stage in (0, 1):
+            lib.reactive_transport_stage(h, rlib.ptr(qq), rlib.ptr(tst), None, None, 1e-8, stage, rlib.ptr(b))
+        transport_wall = time.perf_counter() - t0
+        lib.reactive_transport_destroy(h)
+    else:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from helpers import OracleTransport
+        ot = OracleTransport(cfg, volume, faces)
+        qq = q0.copy()
+        t0 = time.perf_counter()
+        for stage in (0, 1):
+            ot.stage(qq, tst, 1e-8, stage)
+        transport_wall = time.perf_counter() - t0
+        ot.close()
+    flash_per_cell = flash_wall / len(idx)            # wall seconds per cell-flash with `threads` workers
+    transport_per_cell = transport_wall / len(q0)     # serial seconds per cell for both stages
+    per_cell_step = 2 * flash_per_cell + transport_per_cell
+    return {"value": 1.0 / per_cell_step, "unit": "cell-steps/s", "cores": threads, "kind": kind,
+            "sample": (f"{len(idx)} seeded-random cells of a real step's second flash (flash x2 per step, "
+                       f"{threads} threads, {flash_wall:.2f} s wall) + reference serial transport, both SSPRK2 "
+                       f"stages on a {tn}^3 box ({transport_wall:.2f} s); per-cell costs scaled; "
+                       f"{int(succ.sum())}/{len(idx)} sample flashes converged")}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1, help="untimed steps (at least one is always run)")
+    ap.add_argument("--n", type=int, default=160, help="cells per edge")
+    ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--cfl", type=float, default=0.4)
+    ap.add_argument("--cpu-sample", type=int, default=12288)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (the multi-GPU design is "
+              "documented, not built)", file=sys.stderr)
+        sys.exit(2)
+    lib = rlib.load()
+    log(f"bench: building {a.n}^3 pure-N2O case (seed {a.seed})")
+    faces, volume, q, states, m, hem, info = prepare(a.n, a.seed)
+    cells = len(q)
+    cfg = abi.Config(cells=cells, species=1, variables=5, faces=len(faces), fixed=0, viscosity=0, conductivity=0,
+                     diffusivity=0, waveFactor=1.1, maxBytes=200e9, mechanical=0)
+    err = C.create_string_buffer(1024)
+    tr = lib.reactive_transport_create(1, C.byref(cfg), rlib.ptr(volume), rlib.ptr(faces), None, None, None, None,
+                                       err, 1024)
+    if not tr:
+        raise RuntimeError(err.value.decode())
+    step = lib.reactive_amd_step_create(tr, hem, err, 1024)
+    if not step:
+        raise RuntimeError(err.value.decode())
+
+    def check(rc):
+        if rc:
+            raise RuntimeError(lib.reactive_amd_step_error(step).decode())
+
+    check(lib.reactive_amd_step_load(step, rlib.ptr(q), rlib.ptr(states)))
+    rep = abi.StepReport.fresh()
+    log(f"bench: warmup {a.warmup} step(s)")
+    check(lib.reactive_amd_step_run(step, a.cfl, 1.0, 1, 0, C.byref(rep)))  # first warmup step
+    q_one = np.zeros_like(q)  # q after one step from (q, states): the CPU sample flashes exactly this step's flash #2
+    check(lib.reactive_amd_step_read(step, rlib.ptr(q_one), None))
+    if a.warmup > 1:
+        check(lib.reactive_amd_step_run(step, a.cfl, 1.0, a.warmup - 1, 0, C.byref(rep)))
+    log(f"bench: timing {a.steps} step(s)")
+    rep = abi.StepReport.fresh()
+    t0 = time.perf_counter()
+    check(lib.reactive_amd_step_run(step, a.cfl, 1.0, a.steps, 0, C.byref(rep)))
+    wall = time.perf_counter() - t0
+    log(f"bench: {wall:.3f} s for {a.steps} steps (device {rep.seconds:.3f} s)")
+    value = cells * a.steps / wall
+    nv = 5
+    bytes0, bytes1 = 8 * (2 * nv + 2), 8 * (3 * nv + 2)  # algorithmic bytes per cell, stage 0 / stage 1
+    t_stage = (rep.stageSeconds[0] + rep.stageSeconds[1]) / a.steps
+    achieved = (bytes0 + bytes1) * cells / t_stage / 1e9
+    t_flash = (rep.flashSeconds[0] + rep.flashSeconds[1]) / (2 * a.steps)
+    result = {
+        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"cube{a.n}-n2o-pure", "cells": cells, "faces": len(faces), "cfl": a.cfl,
+                   "species": ["N2O"], "liquids": ["N2O"], "flash": "fp64 (plain, reference arithmetic)",
+                   "seed": a.seed, "two_phase_fraction": round(info["two_phase_fraction"], 4)},
+        "roofline": {"bound": "hbm", "kernel": "cartesianStageKernel (both SSPRK2 stages)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_cell": [bytes0, bytes1],
+                     "avg_launch_ms": [1e3 * rep.stageSeconds[0] / a.steps, 1e3 * rep.stageSeconds[1] / a.steps]},
+        "roofline_flash": {"bound": "valu-fp64", "kernel": "stepFlashKernel", "avg_launch_ms": 1e3 * t_flash,
+                           "phase_evaluations_per_cell_step": rep.phaseEvaluations / (cells * a.steps),
+                           "share_of_step": (rep.flashSeconds[0] + rep.flashSeconds[1]) / rep.seconds},
+        "flash_failures": [int(rep.flashFailures[0]), int(rep.flashFailures[1])],
+        "cfl_violations": int(rep.cflViolations), "dt": rep.dt,
+    }
+    if not a.no_cpu:
+        log(f"bench: cpu baseline ({a.cpu_sample} cells, {a.cpu_threads} threads)")
+        result["cpu_baseline"] = cpu_baseline(m, q_one, states, a.cpu_threads, a.cpu_sample, a.seed)
+    lib.reactive_amd_step_destroy(step)
+    lib.reactive_transport_destroy(tr)
+    lib.reactive_gpu_hem_destroy_v1(hem)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,64 @@
+/* SPDX-License-Identifier: GPL-3.0-or-later
+ *
+ * Additive entry points of the MI355X build (no counterpart in the reference
+ * ABI).  They keep one full SSPRK2 step resident on the device: the
+ * reference driver's Flow::step (src/reactiveFoam/ReactiveFoam.C:961-1019,
+ * without chemistry) runs there as a host loop with a conserved download
+ * after every stage and a host-packed flash batch per stage
+ * (ReactiveFoam.C:651-661, 689-724).  Here the whole step is one stream of
+ * kernels over device-resident q and states: CFL, stage 0, flash, CFL
+ * recheck, stage 1, flash, conservation totals; dt never leaves the device.
+ *
+ * The step uses an existing transport handle (structured box, see
+ * reactive_transport_create) and HEM handle (reactive_gpu_hem_create_v1) on
+ * the same device; both must describe the same species.  Return codes
+ * follow the transport ABI: 0 ok, 1 failure (reactive_amd_step_error), -1
+ * null handle.
+ */
+#ifndef REACTIVE_AMD_H
+#define REACTIVE_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+#include "reactive_gpu_hem.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define REACTIVE_AMD_MAX_VARIABLES 32
+
+typedef struct ReactiveAmdStepReport {
+    uint32_t abiVersion, structBytes;   /* 1, sizeof */
+    uint64_t steps;                     /* steps run by the call */
+    uint64_t flashFailures[2];          /* cells whose flash #1 / #2 failed, summed over the call */
+    uint64_t cflViolations;             /* steps whose post-stage-0 CFL recheck failed */
+    uint64_t phaseEvaluations;          /* PR phase evaluations, summed over the call */
+    double dt, recheckDt;               /* last step: dt and the post-stage-0 stable step */
+    double totals[REACTIVE_AMD_MAX_VARIABLES];           /* sum_c q_ck V_c after the last step */
+    double boundaryIntegral[REACTIVE_AMD_MAX_VARIABLES]; /* .5 dt (bA + bB) of the last step */
+    double seconds;                     /* device time of the call (HIP events) */
+    /* per-launch device times (HIP events on the step stream), summed over the call */
+    double stageSeconds[2];             /* fused transport stage 0 / stage 1 kernels */
+    double flashSeconds[2];             /* flash #1 / flash #2 kernels */
+    double cflSeconds;                  /* both CFL sweeps (+ their final minimum) */
+} ReactiveAmdStepReport;
+
+/* Flash arithmetic of a step (0 = plain FP64, the reference's arithmetic). */
+#define REACTIVE_AMD_FLASH_FP64 0
+
+void* reactive_amd_step_create(void* transport, void* hem, char* error, size_t errorSize);
+/* Host cell-major q (cells x variables) and the step-start thermo states. */
+int reactive_amd_step_load(void* step, const double* q, const ReactiveThermoState* states);
+/* Runs `steps` SSPRK2 steps with CFL number cfl (0 < cfl <= 0.5, the
+ * reactive_transport_stable_step contract) and dt <= maximum.  A flash
+ * failure keeps that cell's step-start state and is counted. */
+int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int flashMode,
+                          ReactiveAmdStepReport* report);
+/* Copies the current q (cell-major) and states to host memory. */
+int reactive_amd_step_read(void* step, double* q, ReactiveThermoState* states);
+const char* reactive_amd_step_error(void* step);
+void reactive_amd_step_destroy(void* step);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,19 @@
+#!/usr/bin/env bash
+# Builds lib/libreactiveTransport.so (HIP, gfx950) from csrc/.
+# Rounding policy: IEEE FP64, no contraction (-ffp-contract=off), no fast-math.
+set -euo pipefail
+here="$(cd -- "$(dirname -- "${BASH_SOURCE[0]}")" && pwd)"
+hipcc="${HIPCC:-/opt/rocm/bin/hipcc}"
+mkdir -p "$here/lib" "$here/_build"
+flags=(--offload-arch=gfx950 -O3 -std=c++17 -fPIC -ffp-contract=off -fno-fast-math
+       -Xclang -target-feature -Xclang -packed-fp32-ops
+       -I"$here/../include" -Wall -Wno-unused-result)
+objs=()
+for src in "$here"/csrc/*.hip; do
+  obj="$here/_build/$(basename "${src%.hip}").o"
+  "$hipcc" "${flags[@]}" -c "$src" -o "$obj" 2> >(grep -v "packed-fp32-ops" >&2) &
+  objs+=("$obj")
+done
+wait
+"$hipcc" --offload-arch=gfx950 -shared -fPIC "${objs[@]}" -o "$here/lib/libreactiveTransport.so.tmp"
+mv "$here/lib/libreactiveTransport.so.tmp" "$here/lib/libreactiveTransport.so"

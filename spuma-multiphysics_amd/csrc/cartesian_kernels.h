@@ -1,0 +1,251 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+// Fused structured-box stage kernel (see cartesian.hip for the parity argument).
+//
+// One thread per cell: the cell's six faces are visited in its CSR order,
+// each face's HLL coefficients rebuilt from the two adjacent states, and the
+// SSPRK2 update written once.  Workgroups are remapped so that each of the
+// eight XCDs sweeps one contiguous z-range: the +/-nx*ny neighbours a cell
+// reads were then loaded by the same XCD moments earlier and hit its L2.
+#ifndef REACTIVE_AMD_CARTESIAN_KERNELS_H
+#define REACTIVE_AMD_CARTESIAN_KERNELS_H
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "transport_internal.h"
+
+namespace ramd {
+
+struct CartesianArgs {
+    int nx, ny, nz;
+    double area, invV, waveFactor;
+    uint32_t slotOrder[64];
+    double normal[6][2][3];
+};
+
+inline CartesianArgs makeCartesianArgs(const CartesianInfo& c, double waveFactor) {
+    CartesianArgs a;
+    a.nx = c.nx; a.ny = c.ny; a.nz = c.nz;
+    a.area = c.area; a.invV = c.inverseVolume; a.waveFactor = waveFactor;
+    for (int t = 0; t < 64; ++t) a.slotOrder[t] = c.slotOrder[t];
+    for (int s = 0; s < 6; ++s)
+        for (int b = 0; b < 2; ++b)
+            for (int d = 0; d < 3; ++d) a.normal[s][b][d] = c.normal[s][b][d];
+    return a;
+}
+
+constexpr int kXcds = 8;
+
+// Logical block of a physical workgroup: physical blocks are dispatched
+// round-robin over the XCDs, so XCD x runs logical blocks [x*per, (x+1)*per).
+__device__ inline long xcdBlock(long physical, long logicalBlocks) {
+    const long per = (logicalBlocks + kXcds - 1) / kXcds;
+    return (physical % kXcds) * per + physical / kXcds;
+}
+inline unsigned xcdGrid(long logicalBlocks) {
+    const long per = (logicalBlocks + kXcds - 1) / kXcds;
+    return unsigned(per * kXcds);
+}
+
+template <int NS, int NV>
+__global__ __launch_bounds__(256) void cartesianStageKernel(CartesianArgs a, const double* __restrict__ q,
+                                                            size_t stride, const double* __restrict__ pArr,
+                                                            const double* __restrict__ cArr, double dtHost,
+                                                            const double* __restrict__ dtDev, int mode,
+                                                            double* __restrict__ out) {
+    const long cells = long(a.nx) * a.ny * a.nz;
+    const long blocks = (cells + 255) / 256;
+    const long b = xcdBlock(blockIdx.x, blocks);
+    const long c = b * 256 + threadIdx.x;
+    if (b >= blocks || c >= cells) return;
+    const double dt = dtDev ? *dtDev : dtHost;
+    if (!(dt > 0)) return;  // an invalid device-side CFL step leaves q untouched
+    const long nxy = long(a.nx) * a.ny;
+    const int i = int(c % a.nx), j = int((c / a.nx) % a.ny), k = int(c / nxy);
+    const int type = (i == 0) | (i == a.nx - 1) << 1 | (j == 0) << 2 | (j == a.ny - 1) << 3 | (k == 0) << 4 |
+                     (k == a.nz - 1) << 5;
+    const uint32_t order = a.slotOrder[type];
+    double qc[NV];
+    for (int v = 0; v < NV; ++v) qc[v] = q[size_t(v) * stride + c];
+    const double pc = pArr[c], cc = cArr[c];
+    double rate[NV];
+    for (int v = 0; v < NV; ++v) rate[v] = 0;
+    for (int pos = 0; pos < 6; ++pos) {
+        const int slot = int((order >> (3 * pos)) & 7u);
+        const int ax = slot >> 1, plus = slot & 1;
+        const int coord = ax == 0 ? i : (ax == 1 ? j : k);
+        const int ext = ax == 0 ? a.nx : (ax == 1 ? a.ny : a.nz);
+        const bool bnd = plus ? coord == ext - 1 : coord == 0;
+        const long step = ax == 0 ? 1 : (ax == 1 ? long(a.nx) : nxy);
+        const double* n = a.normal[slot][bnd ? 1 : 0];
+        double qn[NV], pn = pc, cn = cc;
+        if (bnd) {
+            for (int v = 0; v < NV; ++v) qn[v] = qc[v];
+        } else {
+            const long nb = plus ? c + step : c - step;
+            for (int v = 0; v < NV; ++v) qn[v] = q[size_t(v) * stride + nb];
+            pn = pArr[nb];
+            cn = cArr[nb];
+        }
+        // owner (L) and right (R) of this face
+        const bool selfLeft = bnd || plus;
+        double qL[NV], qR[NV];
+        for (int v = 0; v < NV; ++v) {
+            qL[v] = selfLeft ? qc[v] : qn[v];
+            qR[v] = selfLeft ? qn[v] : qc[v];
+        }
+        const double pL = selfLeft ? pc : pn, pR = selfLeft ? pn : pc;
+        const double aL = a.waveFactor * (selfLeft ? cc : cn), aR = a.waveFactor * (selfLeft ? cn : cc);
+        double rhoL = 0, rhoR = 0;
+        for (int s = 0; s < NS; ++s) { rhoL += qL[s]; rhoR += qR[s]; }
+        double unL = 0, unR = 0;
+        for (int d = 0; d < 3; ++d) {
+            unL += qL[NS + d] / rhoL * n[d];
+            unR += qR[NS + d] / rhoR * n[d];
+        }
+        const double left = unL - aL < unR - aR ? unL - aL : unR - aR;
+        const double right = unL + aL > unR + aR ? unL + aL : unR + aR;
+        const double wl = 0 < left ? 0 : left, wr = 0 > right ? 0 : right;
+        const double invWave = 1 / (wr - wl);
+        const double advectL = wr * (unL - wl) * invWave;
+        const double advectR = wl * (wr - unR) * invWave;
+        const double pM = (wr * pL - wl * pR) * invWave;
+        const double pE = (wr * pL * unL - wl * pR * unR) * invWave;
+        const double sign = selfLeft ? -1.0 : 1.0;
+        for (int v = 0; v < NV; ++v) {
+            double flux = advectL * qL[v] + advectR * qR[v];
+            if (v >= NS && v < NS + 3) flux += (pM - 0.0) * n[v - NS];
+            if (v == NS + 3) flux += pE;
+            if (v < NS) flux += 0.0;
+            else if (v < NS + 3) flux -= 0.0;
+            else if (v == NS + 3) flux += 0.0;
+            rate[v] += sign * (flux * a.area) * a.invV;
+        }
+    }
+    for (int v = 0; v < NV; ++v) {
+        const size_t idx = size_t(v) * stride + c;
+        if (mode == 0) out[idx] = qc[v] + dt * rate[v];
+        else out[idx] = .5 * out[idx] + .5 * (qc[v] + dt * rate[v]);
+    }
+}
+
+// FaceSpeeds + Step + the block level of Minimum (reactiveTransportKernels.h:
+// 315-342, 529-550; reactiveTransport.cpp:222-232) on the structured box.
+// Face speeds are rebuilt per cell and summed in CSR order; invalid cell
+// steps map to -1 (absorbing).  maximum comes from maxDev when non-null.
+template <int NS>
+__global__ __launch_bounds__(256) void cartesianCflKernel(CartesianArgs a, const double* __restrict__ q,
+                                                          size_t stride, const double* __restrict__ pArr,
+                                                          const double* __restrict__ cArr, double cfl,
+                                                          double maxHost, const double* __restrict__ maxDev,
+                                                          double* __restrict__ blockMin) {
+    (void)pArr;
+    const long cells = long(a.nx) * a.ny * a.nz;
+    const long blocks = (cells + 255) / 256;
+    const long b = xcdBlock(blockIdx.x, blocks);
+    if (b >= blocks) return;
+    const long c = b * 256 + threadIdx.x;
+    double v = __builtin_huge_val();
+    if (c < cells) {
+        const double maximum = maxDev ? *maxDev : maxHost;
+        const long nxy = long(a.nx) * a.ny;
+        const int i = int(c % a.nx), j = int((c / a.nx) % a.ny), k = int(c / nxy);
+        const int type = (i == 0) | (i == a.nx - 1) << 1 | (j == 0) << 2 | (j == a.ny - 1) << 3 | (k == 0) << 4 |
+                         (k == a.nz - 1) << 5;
+        const uint32_t order = a.slotOrder[type];
+        double qc[NS + 3];
+        for (int s = 0; s < NS + 3; ++s) qc[s] = q[size_t(s) * stride + c];
+        const double cc = cArr[c];
+        double denominator = 0, step;
+        bool bad = false;
+        for (int pos = 0; pos < 6 && !bad; ++pos) {
+            const int slot = int((order >> (3 * pos)) & 7u);
+            const int ax = slot >> 1, plus = slot & 1;
+            const int coord = ax == 0 ? i : (ax == 1 ? j : k);
+            const int ext = ax == 0 ? a.nx : (ax == 1 ? a.ny : a.nz);
+            const bool bnd = plus ? coord == ext - 1 : coord == 0;
+            const long stepIdx = ax == 0 ? 1 : (ax == 1 ? long(a.nx) : nxy);
+            const double* n = a.normal[slot][bnd ? 1 : 0];
+            double qn[NS + 3], cn = cc;
+            if (bnd) {
+                for (int s = 0; s < NS + 3; ++s) qn[s] = qc[s];
+            } else {
+                const long nb = plus ? c + stepIdx : c - stepIdx;
+                for (int s = 0; s < NS + 3; ++s) qn[s] = q[size_t(s) * stride + nb];
+                cn = cArr[nb];
+            }
+            const bool selfLeft = bnd || plus;
+            double rhoL = 0, rhoR = 0, unL = 0, unR = 0;
+            for (int s = 0; s < NS; ++s) {
+                rhoL += selfLeft ? qc[s] : qn[s];
+                rhoR += selfLeft ? qn[s] : qc[s];
+            }
+            for (int d = 0; d < 3; ++d) {
+                unL += (selfLeft ? qc[NS + d] : qn[NS + d]) / rhoL * n[d];
+                unR += (selfLeft ? qn[NS + d] : qc[NS + d]) / rhoR * n[d];
+            }
+            const double aL = a.waveFactor * (selfLeft ? cc : cn), aR = a.waveFactor * (selfLeft ? cn : cc);
+            const double left0 = unL - aL < unR - aR ? unL - aL : unR - aR;
+            const double right0 = unL + aL > unR + aR ? unL + aL : unR + aR;
+            const double left = 0 < left0 ? 0 : left0, right = 0 > right0 ? 0 : right0;
+            const double invWave = 1 / (right - left);
+            const double fl = ::fabs(unL) + aL, fr = ::fabs(unR) + aR;
+            double speed = (fl > fr ? fl : fr) + 0.0;
+            if (!isfinite(unL) || !isfinite(unR) || !isfinite(speed) || !isfinite(invWave) || right <= left) speed = -1;
+            if (!isfinite(speed) || speed < 0) { bad = true; break; }
+            denominator += a.area * speed;
+        }
+        if (bad) step = -1;
+        else {
+            const double r = cfl / (a.invV * denominator);
+            step = denominator > 0 ? (maximum < r ? maximum : r) : maximum;
+            if (!isfinite(denominator) || denominator < 0) step = -1;
+        }
+        v = isfinite(step) && step > 0 ? step : -1;
+    }
+    __shared__ double lane[4];
+    for (int offset = 32; offset > 0; offset >>= 1) {
+        const double o = __shfl_xor(v, offset, 64);
+        v = o < v ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0) lane[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = lane[0];
+        for (int w = 1; w < 4; ++w) m = lane[w] < m ? lane[w] : m;
+        blockMin[b] = m;
+    }
+}
+
+inline void launchCartesianCfl(const CartesianArgs& a, int ns, const double* q, size_t stride, const double* p,
+                               const double* c, double cfl, double maxHost, const double* maxDev, double* blockMin,
+                               hipStream_t stream) {
+    const long cells = long(a.nx) * a.ny * a.nz;
+    const unsigned grid = xcdGrid((cells + 255) / 256);
+    if (ns == 1)
+        hipLaunchKernelGGL((cartesianCflKernel<1>), dim3(grid), dim3(256), 0, stream, a, q, stride, p, c, cfl, maxHost,
+                           maxDev, blockMin);
+    else
+        hipLaunchKernelGGL((cartesianCflKernel<2>), dim3(grid), dim3(256), 0, stream, a, q, stride, p, c, cfl, maxHost,
+                           maxDev, blockMin);
+}
+
+inline bool cartesianSupported(int ns, int nv) {
+    return (ns == 1 && (nv == 5 || nv == 6)) || (ns == 2 && (nv == 6 || nv == 7));
+}
+
+inline void launchCartesianStage(const CartesianArgs& a, int ns, int nv, const double* q, size_t stride,
+                                 const double* p, const double* c, double dt, const double* dtDev, int mode,
+                                 double* out, hipStream_t stream) {
+    const long cells = long(a.nx) * a.ny * a.nz;
+    const unsigned grid = xcdGrid((cells + 255) / 256);
+#define RAMD_CART(NS_, NV_)                                                                                         \
+    hipLaunchKernelGGL((cartesianStageKernel<NS_, NV_>), dim3(grid), dim3(256), 0, stream, a, q, stride, p, c, dt, \
+                       dtDev, mode, out)
+    if (ns == 1 && nv == 5) RAMD_CART(1, 5);
+    else if (ns == 1 && nv == 6) RAMD_CART(1, 6);
+    else if (ns == 2 && nv == 6) RAMD_CART(2, 6);
+    else RAMD_CART(2, 7);
+#undef RAMD_CART
+}
+
+}  // namespace ramd
+#endif

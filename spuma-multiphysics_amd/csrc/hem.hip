@@ -1,0 +1,340 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+// Batched HEM UV flash plugin for gfx950: the reactive_gpu_hem_* ABI
+// (src/reactiveThermo/reactiveGpuHem.h:40-84) over the device flash of
+// flash_device.h.
+//
+// Execution: one work-item per cell in 64-lane workgroups (one wavefront),
+// cells visited in heavy-first bucket order (the reference's scheduling key,
+// src/reactiveTransport/reactiveHemBucket.h:13-18): liquid-active cells run
+// the long seeded searches, so they go first and cheap gas cells fill the
+// tail.  Work counters are reduced per wavefront and accumulated with one
+// 64-bit atomic per counter per wavefront.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "hip_common.h"
+#include "flash_device.h"
+#include "model_image.h"
+#include "hem_internal.h"
+#include "reactive_gpu_hem.h"
+
+namespace ramd {
+
+static_assert(sizeof(ThermoOut) == sizeof(ReactiveThermoState), "state layout");
+static_assert(sizeof(HemOutputDev) == sizeof(ReactiveGpuHemOutputV1) &&
+                  offsetof(HemOutputDev, success) == offsetof(ReactiveGpuHemOutputV1, success),
+              "output layout");
+
+// mode 0: equilibrium UV flash (recover, reactiveDeviceFlash.h:564-567)
+// mode 1: frozen inventory at the seed's liquid masses (recoverCapillary with
+//         equilibrium = 0 and no interface, reactiveDeviceFlash.h:572-590)
+template <int NS, int NL>
+__global__ __launch_bounds__(64) void hemFlashKernel(const HemModel<NS>* __restrict__ model,
+                                                     const double* __restrict__ q, const double* __restrict__ energy,
+                                                     const ThermoOut* __restrict__ guess,
+                                                     const uint32_t* __restrict__ order, uint32_t count,
+                                                     const int* __restrict__ mode, HemOutputDev* __restrict__ out,
+                                                     unsigned long long* __restrict__ totals,
+                                                     uint32_t* __restrict__ cellPhases) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    Counts n = {0, 0, 0, 0, 0, 0};
+    if (i < count) {
+        const uint32_t c = order ? order[i] : i;
+        Flash<NS, NL> f(*model, n);
+        for (int k = 0; k < NS; ++k) f.q[k] = q[size_t(c) * NS + k];
+        f.energy = energy[c];
+        const ThermoOut& g = guess[c];
+        f.guessP = g.p;
+        f.guessT = g.T;
+        f.guessLiquid = g.liquidMass[0];
+        Eval result;
+        zeroEval(result);
+        bool ok;
+        if (mode && mode[c]) ok = f.frozen(NL ? g.liquidMass[0] : 0.0, result, 0, g.liquidMass[1]);
+        else ok = f.run(result);
+        HemOutputDev& o = out[c];
+        if (ok) writeOut(result, o.state);
+        else std::memset(&o.state, 0, sizeof(o.state));
+        o.success = ok ? 1 : 0;
+        o.reserved = 0;
+        if (cellPhases) cellPhases[c] = n.phases;
+    }
+    if (totals) {
+        const uint32_t s0 = waveSum(n.phases), s1 = waveSum(n.residuals), s2 = waveSum(n.candidates);
+        const uint32_t s3 = waveSum(n.stable), s4 = waveSum(n.failures), s5 = waveSum(n.fd);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&totals[0], (unsigned long long)s0);
+            atomicAdd(&totals[1], (unsigned long long)s1);
+            atomicAdd(&totals[2], (unsigned long long)s2);
+            atomicAdd(&totals[3], (unsigned long long)s3);
+            atomicAdd(&totals[4], (unsigned long long)s4);
+            atomicAdd(&totals[5], (unsigned long long)s5);
+        }
+    }
+}
+
+__global__ void zeroTotalsKernel(unsigned long long* totals, int n) {
+    if (threadIdx.x < n) totals[threadIdx.x] = 0;
+}
+
+// Scheduling key (reactiveHemBucket.h:13-18).
+inline unsigned bucketKey(const double* q, int ns, int activeLiquids) {
+    unsigned present = 0, dominant = 0;
+    for (int k = 0; k < ns; ++k) {
+        if (q[k] > 0) ++present;
+        if (q[k] > q[dominant]) dominant = unsigned(k);
+    }
+    const unsigned liquids = activeLiquids < 0 ? 0u : (activeLiquids > 3 ? 3u : unsigned(activeLiquids));
+    return (liquids << 6) | ((present > 7 ? 7u : present) << 3) | (dominant > 7 ? 7u : dominant);
+}
+
+HemHandle::~HemHandle() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto e : events)
+        if (e) (void)hipEventDestroy(e);
+    for (void* p : allocations) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void* HemHandle::allocate(size_t bytes) {
+    void* p = nullptr;
+    RAMD_HIP(hipMalloc(&p, bytes));
+    allocations.push_back(p);
+    deviceBytes += bytes;
+    return p;
+}
+
+void HemHandle::ensureHostPath() {
+    if (hostReady) return;
+    dq = static_cast<double*>(allocate(capacity * ns * sizeof(double)));
+    de = static_cast<double*>(allocate(capacity * sizeof(double)));
+    dguess = static_cast<ThermoOut*>(allocate(capacity * sizeof(ThermoOut)));
+    dout = static_cast<HemOutputDev*>(allocate(capacity * sizeof(HemOutputDev)));
+    dorder = static_cast<uint32_t*>(allocate(capacity * sizeof(uint32_t)));
+    hostOrder.resize(capacity);
+    hostKey.resize(capacity);
+    hostOut.resize(capacity);
+    hostReady = true;
+}
+
+void HemHandle::launch(const double* q, const double* e, const ThermoOut* g, const uint32_t* order, size_t count,
+                       const int* mode, HemOutputDev* out, uint32_t* cellPhases, hipStream_t s) {
+    if (!count) return;
+    const unsigned blocks = unsigned((count + 63) / 64);
+    hipLaunchKernelGGL(zeroTotalsKernel, dim3(1), dim3(64), 0, s, totals, 6);
+#define RAMD_HEM_LAUNCH(NS_, NL_)                                                                                  \
+    hipLaunchKernelGGL((hemFlashKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, s,                                   \
+                       static_cast<const HemModel<NS_>*>(model), q, e, g, order, uint32_t(count), mode, out, totals, \
+                       cellPhases)
+    if (ns == 1 && nl == 0) RAMD_HEM_LAUNCH(1, 0);
+    else if (ns == 1 && nl == 1) RAMD_HEM_LAUNCH(1, 1);
+    else if (ns == 2 && nl == 0) RAMD_HEM_LAUNCH(2, 0);
+    else RAMD_HEM_LAUNCH(2, 1);
+#undef RAMD_HEM_LAUNCH
+    RAMD_HIP(hipGetLastError());
+}
+
+HemHandle* createHem(const void* raw, size_t bytes, size_t capacity, bool hostPath) {
+    using namespace image;
+    require(raw && bytes == sizeof(Model) && capacity && capacity <= UINT32_MAX, "Invalid HIP HEM model/capacity");
+    Model m;
+    std::memcpy(static_cast<void*>(&m), raw, sizeof(m));
+    require(validModel(m), "Invalid HIP HEM model tables or domain");
+    require(m.ns == 1 || m.ns == 2, "This HEM build supports 1 or 2 gas species");
+    require(m.nl <= 1, "This HEM build supports at most one pure liquid");
+    require(!(m.nl && m.condensedKind[0]), "Solid condensed phases are not provided by this HEM build");
+    require(!m.boundaryRecovery, "Boundary-recovery (adaptive vapour) search is not provided by this HEM build");
+    require(!m.analyticJacobian, "Analytic flash Jacobian is not provided by this HEM build; use the FD model");
+    int devices = 0;
+    RAMD_HIP(hipGetDeviceCount(&devices));
+    require(devices > 0, "No HIP device");
+    std::unique_ptr<HemHandle> h(new HemHandle);
+    h->ns = m.ns;
+    h->nl = m.nl;
+    h->capacity = capacity;
+    h->device = currentDevice();
+    RAMD_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    for (auto& e : h->events) RAMD_HIP(hipEventCreate(&e));
+    if (m.ns == 1) {
+        HemModel<1> d;
+        buildDeviceModel<1>(m, d);
+        h->model = h->allocate(sizeof(d));
+        RAMD_HIP(hipMemcpy(h->model, &d, sizeof(d), hipMemcpyHostToDevice));
+    } else {
+        HemModel<2> d;
+        buildDeviceModel<2>(m, d);
+        h->model = h->allocate(sizeof(d));
+        RAMD_HIP(hipMemcpy(h->model, &d, sizeof(d), hipMemcpyHostToDevice));
+    }
+    h->totals = static_cast<unsigned long long*>(h->allocate(8 * sizeof(unsigned long long)));
+    h->modelBytes = h->deviceBytes;
+    if (hostPath) h->ensureHostPath();
+    return h.release();
+}
+
+// Host-array batch (reactiveGpuHem.cuh:227-349 semantics).
+int runHost(HemHandle* h, const double* q, const double* energy, const ReactiveGpuHemCapillaryInputV2* capillary,
+            size_t count, ReactiveThermoState* states, int* success, ReactiveGpuHemProfileV1* profile) {
+    require(h && q && energy && states && success && profile && profile->abiVersion == 1 &&
+                profile->structBytes == sizeof(*profile),
+            "Invalid HIP HEM batch arguments");
+    require(count && count <= h->capacity, "HIP HEM batch exceeds capacity");
+    std::vector<int> mode;
+    if (capillary) {
+        mode.resize(count);
+        for (size_t c = 0; c < count; ++c) {
+            const auto& x = capillary[c];
+            require(fin(x.color) && x.color >= 0 && x.color <= 1 && fin(x.pressureJump) &&
+                        (x.equilibrium == 0 || x.equilibrium == 1),
+                    "Invalid HIP HEM capillary cell input");
+            require(x.color == 0 && x.pressureJump == 0,
+                    "Capillary interfaces (color, pressureJump != 0) are not provided by this HEM build");
+            require(x.equilibrium == 0,
+                    "Capillary equilibrium cells need the boundary-recovery search, not provided by this HEM build");
+            mode[c] = 1;
+        }
+    }
+    RAMD_HIP(hipSetDevice(h->device));
+    h->ensureHostPath();
+    if (capillary && !h->dmode) h->dmode = static_cast<int*>(h->allocate(h->capacity * sizeof(int)));
+    const auto begin = std::chrono::steady_clock::now();
+    // stable counting sort on the scheduling key, heavy buckets first
+    std::array<size_t, 256> counts{};
+    for (size_t c = 0; c < count; ++c) {
+        const unsigned key = bucketKey(q + c * h->ns, h->ns, states[c].activeLiquids);
+        h->hostKey[c] = uint8_t(key);
+        ++counts[key];
+    }
+    std::array<size_t, 256> offset{};
+    size_t next = 0;
+    for (int b = 255; b >= 0; --b) { offset[b] = next; next += counts[b]; }
+    for (size_t c = 0; c < count; ++c) h->hostOrder[offset[h->hostKey[c]]++] = uint32_t(c);
+    hipStream_t s = h->stream;
+    RAMD_HIP(hipEventRecord(h->events[0], s));
+    RAMD_HIP(hipMemcpyAsync(h->dq, q, count * h->ns * sizeof(double), hipMemcpyHostToDevice, s));
+    RAMD_HIP(hipMemcpyAsync(h->de, energy, count * sizeof(double), hipMemcpyHostToDevice, s));
+    RAMD_HIP(hipMemcpyAsync(h->dguess, states, count * sizeof(ThermoOut), hipMemcpyHostToDevice, s));
+    RAMD_HIP(hipMemcpyAsync(h->dorder, h->hostOrder.data(), count * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (capillary) RAMD_HIP(hipMemcpyAsync(h->dmode, mode.data(), count * sizeof(int), hipMemcpyHostToDevice, s));
+    RAMD_HIP(hipEventRecord(h->events[1], s));
+    h->launch(h->dq, h->de, h->dguess, h->dorder, count, capillary ? h->dmode : nullptr, h->dout, nullptr, s);
+    RAMD_HIP(hipEventRecord(h->events[2], s));
+    RAMD_HIP(hipMemcpyAsync(h->hostOut.data(), h->dout, count * sizeof(HemOutputDev), hipMemcpyDeviceToHost, s));
+    unsigned long long totals[6] = {};
+    RAMD_HIP(hipMemcpyAsync(totals, h->totals, sizeof(totals), hipMemcpyDeviceToHost, s));
+    RAMD_HIP(hipEventRecord(h->events[3], s));
+    RAMD_HIP(hipEventSynchronize(h->events[3]));
+    float upload = 0, kernel = 0, download = 0;
+    RAMD_HIP(hipEventElapsedTime(&upload, h->events[0], h->events[1]));
+    RAMD_HIP(hipEventElapsedTime(&kernel, h->events[1], h->events[2]));
+    RAMD_HIP(hipEventElapsedTime(&download, h->events[2], h->events[3]));
+    ReactiveGpuHemProfileV1 p{};
+    p.abiVersion = 1;
+    p.structBytes = sizeof(p);
+    p.batches = 1;
+    p.submitted = count;
+    for (size_t c = 0; c < count; ++c) {
+        const HemOutputDev& o = h->hostOut[c];
+        success[c] = o.success;
+        if (o.success) {
+            std::memcpy(&states[c], &o.state, sizeof(ReactiveThermoState));
+            ++p.succeeded;
+        }
+    }
+    p.deviceFailures = count - p.succeeded;
+    p.phaseEvaluations = totals[0];
+    p.residualEvaluations = totals[1];
+    p.flashCandidates = totals[2];
+    p.stableCandidates = totals[3];
+    p.finiteDifferenceJacobians = totals[5];
+    p.transferBytes = count * (h->ns * sizeof(double) + sizeof(double) + sizeof(ThermoOut) + sizeof(uint32_t) +
+                               sizeof(HemOutputDev) + (capillary ? sizeof(int) : 0)) +
+                      sizeof(totals);
+    p.deviceBytes = h->deviceBytes;
+    p.hostBytes = h->capacity * (sizeof(HemOutputDev) + sizeof(uint32_t) + sizeof(uint8_t));
+    p.wallSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - begin).count();
+    p.kernelSeconds = kernel / 1000.;
+    p.copySeconds = (upload + download) / 1000.;
+    *profile = p;
+    return 0;
+}
+
+}  // namespace ramd
+
+using namespace ramd;
+
+extern "C" {
+
+const char* reactive_gpu_hem_numerical_policy_v1(void) { return ""; }
+
+void* reactive_gpu_hem_create_v1(const void* model, size_t bytes, size_t capacity, char* error, size_t size) {
+    try {
+        return createHem(model, bytes, capacity, true);
+    } catch (const std::exception& e) {
+        writeError(error, size, e.what());
+        return nullptr;
+    }
+}
+
+void* reactive_gpu_hem_create_resident_v1(const void* model, size_t bytes, size_t capacity, char* error, size_t size) {
+    try {
+        return createHem(model, bytes, capacity, false);
+    } catch (const std::exception& e) {
+        writeError(error, size, e.what());
+        return nullptr;
+    }
+}
+
+size_t reactive_gpu_hem_device_bytes_v1(void* raw) { return raw ? static_cast<HemHandle*>(raw)->deviceBytes : 0; }
+
+void reactive_gpu_hem_destroy_v1(void* raw) { delete static_cast<HemHandle*>(raw); }
+
+int reactive_gpu_hem_run_v1(void* raw, const double* q, const double* energy, size_t count, ReactiveThermoState* states,
+                            int* success, ReactiveGpuHemProfileV1* profile, char* error, size_t size) {
+    try {
+        return runHost(static_cast<HemHandle*>(raw), q, energy, nullptr, count, states, success, profile);
+    } catch (const std::exception& e) {
+        writeError(error, size, e.what());
+        return 1;
+    }
+}
+
+int reactive_gpu_hem_run_v2(void* raw, const double* q, const double* energy,
+                            const ReactiveGpuHemCapillaryInputV2* capillary, size_t count, ReactiveThermoState* states,
+                            int* success, ReactiveGpuHemProfileV1* profile, char* error, size_t size) {
+    if (!capillary) {
+        writeError(error, size, "Null HIP HEM capillary input");
+        return 1;
+    }
+    try {
+        return runHost(static_cast<HemHandle*>(raw), q, energy, capillary, count, states, success, profile);
+    } catch (const std::exception& e) {
+        writeError(error, size, e.what());
+        return 1;
+    }
+}
+
+int reactive_gpu_hem_initialize_tp_v1(void*, double*, double*, const ReactiveGpuHemCapillaryInputV2*, size_t,
+                                      ReactiveThermoState*, int*, ReactiveGpuHemProfileV1*, char* error, size_t size) {
+    writeError(error, size, "reactive_gpu_hem_initialize_tp_v1: capillary primitive initialisation is not provided "
+                            "by the MI355X HEM build");
+    return 1;
+}
+
+int reactive_gpu_hem_run_resident_v1(void* raw, const double* q, const double* energy, const ReactiveThermoState* seed,
+                                     const ReactiveGpuHemCapillaryInputV2* capillary, const uint32_t* order,
+                                     size_t count, ReactiveGpuHemOutputV1* output, ReactiveGpuHemProfileV1* profile,
+                                     char* error, size_t size) {
+    (void)raw; (void)q; (void)energy; (void)seed; (void)capillary; (void)order; (void)count; (void)output;
+    (void)profile;
+    writeError(error, size, "reactive_gpu_hem_run_resident_v1: the capillary resident batch needs the "
+                            "boundary-recovery search, not provided by the MI355X HEM build");
+    return 1;
+}
+
+}

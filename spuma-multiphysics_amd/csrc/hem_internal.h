@@ -1,0 +1,48 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+// HEM handle shared by the flash plugin and the device-resident step.
+#ifndef REACTIVE_AMD_HEM_INTERNAL_H
+#define REACTIVE_AMD_HEM_INTERNAL_H
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <vector>
+#include "flash_device.h"
+
+namespace ramd {
+
+// Flash output record, byte-compatible with ReactiveGpuHemOutputV1 (184 B).
+struct HemOutputDev {
+    ThermoOut state;
+    int success, reserved;
+};
+
+struct HemHandle {
+    int ns = 0, nl = 0, device = 0;
+    size_t capacity = 0, deviceBytes = 0, modelBytes = 0;
+    void* model = nullptr;  // HemModel<ns> in device memory
+    unsigned long long* totals = nullptr;  // phases residuals candidates stable failures fd
+    // host-path staging (allocated on the first host batch)
+    bool hostReady = false;
+    double *dq = nullptr, *de = nullptr;
+    ThermoOut* dguess = nullptr;
+    HemOutputDev* dout = nullptr;
+    uint32_t* dorder = nullptr;
+    int* dmode = nullptr;
+    std::vector<uint32_t> hostOrder;
+    std::vector<uint8_t> hostKey;
+    std::vector<HemOutputDev> hostOut;
+    hipStream_t stream = nullptr;
+    hipEvent_t events[4] = {};
+    std::vector<void*> allocations;
+
+    ~HemHandle();
+    void* allocate(size_t bytes);
+    void ensureHostPath();
+    // Flash count cells named by order (or 0..count-1) on stream s.
+    void launch(const double* q, const double* e, const ThermoOut* guess, const uint32_t* order, size_t count,
+                const int* mode, HemOutputDev* out, uint32_t* cellPhases, hipStream_t s);
+};
+
+HemHandle* createHem(const void* raw, size_t bytes, size_t capacity, bool hostPath);
+
+}  // namespace ramd
+#endif

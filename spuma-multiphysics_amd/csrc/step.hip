@@ -1,0 +1,478 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+// Device-resident SSPRK2 step (reactive_amd_step_*): the reference driver's
+// Flow::step (src/reactiveFoam/ReactiveFoam.C:961-1019, no chemistry) as one
+// stream of kernels with q, states and dt resident in HBM:
+//
+//   dt  = CFL(qA, stateA)                        stable_step_primitives
+//   qB  = qA + dt L(qA)           [bA]           stage 0
+//   sB  = flash(qB | seeds sA)                   recover (ReactiveFoam.C:992)
+//   dt <= CFL(qB, sB, max dt) (1 + 1e-10) ?      recheck (ReactiveFoam.C:993)
+//   qA  = .5 qA + .5 (qB + dt L(qB))  [bB]       stage 1
+//   sA  = flash(qA | seeds sA)                   recover from the step-start
+//                                                seeds (ReactiveFoam.C:1011-1013)
+//   totals = sum q V, boundary = .5 dt (bA + bB) (ReactiveFoam.C:953-958, 1016)
+//
+// Flash inputs are read straight from the SoA conserved arrays: species
+// masses and e = rhoE - |rho u|^2 / (2 rho) (ReactiveFoam.C:727-728).  The
+// flash epilogue writes p and c_frozen as SoA for the next transport stage.
+// Cells are flashed in heavy-first bucket order built on the device.
+#include <hip/hip_runtime.h>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "hip_common.h"
+#include "flash_device.h"
+#include "hem_internal.h"
+#include "transport_internal.h"
+#include "cartesian_kernels.h"
+#include "reactive_amd.h"
+
+namespace ramd {
+
+// ---------------------------------------------------------------- kernels
+template <int NS, int NL>
+__global__ __launch_bounds__(64) void stepFlashKernel(const HemModel<NS>* __restrict__ model,
+                                                      const double* __restrict__ q, size_t stride,
+                                                      const ThermoOut* __restrict__ seeds,
+                                                      const uint32_t* __restrict__ order, uint32_t count,
+                                                      const double* __restrict__ dtDev, ThermoOut* __restrict__ out,
+                                                      double* __restrict__ pOut, double* __restrict__ cOut,
+                                                      unsigned long long* __restrict__ counters) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    Counts n = {0, 0, 0, 0, 0, 0};
+    uint32_t failed = 0;
+    if (i < count && *dtDev > 0) {
+        const uint32_t c = order[i];
+        Flash<NS, NL> f(*model, n);
+        double rho = 0;
+        for (int k = 0; k < NS; ++k) {
+            f.q[k] = q[size_t(k) * stride + c];
+            rho += f.q[k];
+        }
+        const double m0 = q[size_t(NS) * stride + c], m1 = q[size_t(NS + 1) * stride + c];
+        const double m2 = q[size_t(NS + 2) * stride + c], E = q[size_t(NS + 3) * stride + c];
+        f.energy = E - .5 * (m0 * m0 + m1 * m1 + m2 * m2) / rho;
+        const ThermoOut& g = seeds[c];
+        f.guessP = g.p;
+        f.guessT = g.T;
+        f.guessLiquid = g.liquidMass[0];
+        Eval result;
+        zeroEval(result);
+        if (f.run(result)) {
+            writeOut(result, out[c]);
+            pOut[c] = result.p;
+            cOut[c] = result.cf;
+        } else {  // the cell keeps its step-start state (the reference commits nothing for it)
+            out[c] = g;
+            pOut[c] = g.p;
+            cOut[c] = g.soundFrozen;
+            failed = 1;
+        }
+    }
+    const uint32_t sumFailed = waveSum(failed), sumPhases = waveSum(n.phases);
+    if ((threadIdx.x & 63) == 0) {
+        if (sumFailed) atomicAdd(&counters[0], (unsigned long long)sumFailed);
+        atomicAdd(&counters[1], (unsigned long long)sumPhases);
+    }
+}
+
+// Heavy-first scheduling key (reactiveHemBucket.h:13-18) and a counting
+// sort: histogram, exclusive scan from the heaviest bucket down, scatter.
+__global__ __launch_bounds__(256) void bucketHistogramKernel(const double* __restrict__ q, size_t stride, int ns,
+                                                             const ThermoOut* __restrict__ seeds, size_t cells,
+                                                             uint8_t* __restrict__ key,
+                                                             unsigned* __restrict__ count) {
+    __shared__ unsigned hist[256];
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c < cells) {
+        unsigned present = 0, dominant = 0;
+        double best = q[c];
+        for (int k = 0; k < ns; ++k) {
+            const double v = q[size_t(k) * stride + c];
+            if (v > 0) ++present;
+            if (v > best) { best = v; dominant = unsigned(k); }
+        }
+        const int al = seeds[c].activeLiquids;
+        const unsigned liquids = al < 0 ? 0u : (al > 3 ? 3u : unsigned(al));
+        const unsigned k = (liquids << 6) | ((present > 7 ? 7u : present) << 3) | (dominant > 7 ? 7u : dominant);
+        key[c] = uint8_t(k);
+        atomicAdd(&hist[k], 1u);
+    }
+    __syncthreads();
+    if (hist[threadIdx.x]) atomicAdd(&count[threadIdx.x], hist[threadIdx.x]);
+}
+
+__global__ void bucketScanKernel(unsigned* __restrict__ count, unsigned* __restrict__ offset) {
+    if (threadIdx.x != 0) return;
+    unsigned next = 0;
+    for (int b = 255; b >= 0; --b) {
+        offset[b] = next;
+        next += count[b];
+        count[b] = 0;  // leaves the histogram cleared for the next ordering
+    }
+}
+
+__global__ __launch_bounds__(256) void bucketScatterKernel(const uint8_t* __restrict__ key, size_t cells,
+                                                           unsigned* __restrict__ offset,
+                                                           uint32_t* __restrict__ order) {
+    const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    order[atomicAdd(&offset[key[c]], 1u)] = uint32_t(c);
+}
+
+// Neumaier-compensated sum of q_k V over the cells: per-block partials in a
+// fixed order, then one block merges them in block order (deterministic).
+__device__ inline void twoSumAdd(double& s, double& comp, double x) {
+    const double t = s + x;
+    comp += ::fabs(s) >= ::fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+}
+constexpr int kTotalBlocks = 512;
+__global__ __launch_bounds__(256) void totalsPartialKernel(const double* __restrict__ q, size_t stride, size_t cells,
+                                                           int nv, double volume, double2* __restrict__ partial) {
+    __shared__ double ls[256], lc[256];
+    for (int k = 0; k < nv; ++k) {
+        double s = 0, comp = 0;
+        for (size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x; c < cells; c += size_t(gridDim.x) * blockDim.x)
+            twoSumAdd(s, comp, q[size_t(k) * stride + c] * volume);
+        ls[threadIdx.x] = s;
+        lc[threadIdx.x] = comp;
+        __syncthreads();
+        for (unsigned st = 128; st; st >>= 1) {
+            if (threadIdx.x < st) {
+                double a = ls[threadIdx.x], ac = lc[threadIdx.x] + lc[threadIdx.x + st];
+                twoSumAdd(a, ac, ls[threadIdx.x + st]);
+                ls[threadIdx.x] = a;
+                lc[threadIdx.x] = ac;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[size_t(k) * gridDim.x + blockIdx.x] = make_double2(ls[0], lc[0]);
+        __syncthreads();
+    }
+}
+__global__ void totalsFinishKernel(const double2* __restrict__ partial, int blocks, int nv, double* __restrict__ out) {
+    const int k = threadIdx.x;
+    if (k >= nv) return;
+    double s = 0, comp = 0;
+    for (int b = 0; b < blocks; ++b) {
+        comp += partial[size_t(k) * blocks + b].y;
+        twoSumAdd(s, comp, partial[size_t(k) * blocks + b].x);
+    }
+    out[k] = s + comp;
+}
+
+// scalars: [0] dt, [1] recheck dt, [2] cfl-violation counter (as double), [3] max dt of this call
+__global__ void finishCflKernel(const double* __restrict__ blockMin, size_t n, double* __restrict__ out) {
+    double v = __builtin_huge_val();
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) v = blockMin[i] < v ? blockMin[i] : v;
+    __shared__ double lane[4];
+    v = waveMin(v);
+    if ((threadIdx.x & 63) == 0) lane[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = lane[0];
+        for (int w = 1; w < int(blockDim.x >> 6); ++w) m = lane[w] < m ? lane[w] : m;
+        *out = isfinite(m) && m > 0 ? m : -1;
+    }
+}
+__global__ void recheckKernel(double* __restrict__ scalars, unsigned long long* __restrict__ counters) {
+    if (threadIdx.x) return;
+    const double dt = scalars[0], stable = scalars[1];
+    if (!(dt > 0) || !(dt <= stable * (1 + 1e-10))) atomicAdd(&counters[4], 1ull);
+}
+__global__ void boundaryIntegralKernel(const double* __restrict__ bA, const double* __restrict__ bB,
+                                      const double* __restrict__ scalars, int nv, double* __restrict__ out) {
+    const int k = threadIdx.x;
+    if (k < nv) out[k] = .5 * scalars[0] * (bA[k] + bB[k]);
+}
+__global__ void setScalarKernel(double* __restrict__ dst, double value) {
+    if (threadIdx.x == 0) *dst = value;
+}
+__global__ void clearCountersKernel(unsigned long long* __restrict__ c, int n) {
+    if (threadIdx.x < n) c[threadIdx.x] = 0;
+}
+__global__ void toThermoKernel(const ReactiveThermoState* __restrict__ in, size_t n, ThermoOut* __restrict__ out,
+                               double* __restrict__ p, double* __restrict__ c) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    std::memcpy(&out[i], &in[i], sizeof(ThermoOut));
+    p[i] = in[i].p;
+    c[i] = in[i].soundFrozen;
+}
+
+inline unsigned blocksFor(size_t n, unsigned b) { return unsigned((n + b - 1) / b); }
+
+// ---------------------------------------------------------------- handle
+struct Step {
+    Transport* t = nullptr;
+    HemHandle* h = nullptr;
+    int ns = 0, nv = 0, device = 0;
+    size_t cells = 0, all = 0;
+    std::vector<void*> allocations;
+    double *qA = nullptr, *qB = nullptr, *pA = nullptr, *cA = nullptr, *pB = nullptr, *cB = nullptr;
+    ThermoOut *sA = nullptr, *sB = nullptr;
+    uint8_t* key = nullptr;
+    uint32_t* order = nullptr;
+    unsigned *bucketCount = nullptr, *bucketOffset = nullptr;
+    double* blockMin = nullptr;
+    double* scalars = nullptr;  // dt, recheck
+    double *bA = nullptr, *bB = nullptr, *boundary = nullptr, *totals = nullptr;
+    double2* totalPartial = nullptr;
+    unsigned long long* counters = nullptr;  // fail1, phases1, fail2, phases2, violations
+    hipStream_t stream = nullptr;
+    hipEvent_t events[2] = {};
+    std::vector<hipEvent_t> marks;  // 10 per step: cfl, stage 0, flash #1, recheck, stage 1, flash #2 brackets
+    bool loaded = false;
+    char error[1024] = {};
+
+    ~Step() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto e : events)
+            if (e) (void)hipEventDestroy(e);
+        for (auto e : marks) (void)hipEventDestroy(e);
+        for (void* p : allocations) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    template <class T> T* alloc(size_t n) {
+        void* p = nullptr;
+        RAMD_HIP(hipMalloc(&p, std::max<size_t>(1, n) * sizeof(T)));
+        allocations.push_back(p);
+        return static_cast<T*>(p);
+    }
+
+    void flash(const double* q, double* pOut, double* cOut, ThermoOut* out, unsigned long long* counter) {
+        const unsigned blocks = blocksFor(cells, 64);
+#define RAMD_STEP_FLASH(NS_, NL_)                                                                                 \
+    hipLaunchKernelGGL((stepFlashKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                             \
+                       static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), scalars, out, \
+                       pOut, cOut, counter)
+        if (h->ns == 1 && h->nl == 0) RAMD_STEP_FLASH(1, 0);
+        else if (h->ns == 1) RAMD_STEP_FLASH(1, 1);
+        else if (h->nl == 0) RAMD_STEP_FLASH(2, 0);
+        else RAMD_STEP_FLASH(2, 1);
+#undef RAMD_STEP_FLASH
+        RAMD_HIP(hipGetLastError());
+    }
+
+    // One SSPRK2 step; every launch is on `stream` and reads dt from device memory.
+    void mark(hipEvent_t* m, int i) {
+        if (m) RAMD_HIP(hipEventRecord(m[i], stream));
+    }
+    void one(double cfl, double maximum, hipEvent_t* m) {
+        const CartesianArgs a = makeCartesianArgs(t->cartesian, t->cfg.waveFactor);
+        const unsigned nb = unsigned((cells + 255) / 256);
+        // CFL on the step-start state
+        mark(m, 0);
+        launchCartesianCfl(a, ns, qA, all, pA, cA, cfl, maximum, nullptr, blockMin, stream);
+        hipLaunchKernelGGL(finishCflKernel, dim3(1), dim3(256), 0, stream, blockMin, size_t(nb), scalars);
+        mark(m, 1);
+        // heavy-first order from the step-start seeds
+        hipLaunchKernelGGL(bucketHistogramKernel, dim3(blocksFor(cells, 256)), dim3(256), 0, stream, qA, all, ns, sA,
+                           cells, key, bucketCount);
+        hipLaunchKernelGGL(bucketScanKernel, dim3(1), dim3(64), 0, stream, bucketCount, bucketOffset);
+        hipLaunchKernelGGL(bucketScatterKernel, dim3(blocksFor(cells, 256)), dim3(256), 0, stream, key, cells,
+                           bucketOffset, order);
+        // stage 0 and its boundary rate
+        t->boundaryRatesFrom(qA, pA, cA, bA, stream);
+        mark(m, 2);
+        launchCartesianStage(a, ns, nv, qA, all, pA, cA, 0.0, scalars, 0, qB, stream);
+        mark(m, 3);
+        // flash #1 seeded from the step-start states
+        flash(qB, pB, cB, sB, counters);
+        mark(m, 4);
+        // CFL recheck after stage 0 (maximum = dt)
+        launchCartesianCfl(a, ns, qB, all, pB, cB, cfl, 0.0, scalars, blockMin, stream);
+        hipLaunchKernelGGL(finishCflKernel, dim3(1), dim3(256), 0, stream, blockMin, size_t(nb), scalars + 1);
+        mark(m, 5);
+        hipLaunchKernelGGL(recheckKernel, dim3(1), dim3(64), 0, stream, scalars, counters);
+        // stage 1 (in place into qA, which holds q0) and its boundary rate
+        t->boundaryRatesFrom(qB, pB, cB, bB, stream);
+        mark(m, 6);
+        launchCartesianStage(a, ns, nv, qB, all, pB, cB, 0.0, scalars, 1, qA, stream);
+        mark(m, 7);
+        // flash #2 from the step-start seeds, into the state buffer of stage 1
+        flash(qA, pA, cA, sB, counters + 2);
+        mark(m, 8);
+        std::swap(sA, sB);
+        // conservation totals and boundary integral
+        hipLaunchKernelGGL(totalsPartialKernel, dim3(kTotalBlocks), dim3(256), 0, stream, qA, all, cells, nv,
+                           t->cartesian.volume, totalPartial);
+        hipLaunchKernelGGL(totalsFinishKernel, dim3(1), dim3(64), 0, stream, totalPartial, kTotalBlocks, nv, totals);
+        hipLaunchKernelGGL(boundaryIntegralKernel, dim3(1), dim3(64), 0, stream, bA, bB, scalars, nv, boundary);
+        RAMD_HIP(hipGetLastError());
+    }
+};
+
+template <class F>
+int guard(void* handle, F f) {
+    if (!handle) return -1;
+    Step& s = *static_cast<Step*>(handle);
+    try {
+        RAMD_HIP(hipSetDevice(s.device));
+        f(s);
+        s.error[0] = 0;
+        return 0;
+    } catch (const std::exception& e) {
+        if (s.stream) (void)hipStreamSynchronize(s.stream);
+        std::snprintf(s.error, sizeof(s.error), "%s", e.what());
+        return 1;
+    }
+}
+
+}  // namespace ramd
+
+using namespace ramd;
+
+extern "C" {
+
+void* reactive_amd_step_create(void* transport, void* hem, char* error, size_t errorSize) {
+    try {
+        require(transport && hem, "Null transport or HEM handle");
+        std::unique_ptr<Step> s(new Step);
+        s->t = static_cast<Transport*>(transport);
+        s->h = static_cast<HemHandle*>(hem);
+        require(s->t->cartesian.valid, "The resident step needs a structured-box transport mesh");
+        require(s->t->device == s->h->device, "Transport and HEM handles are on different devices");
+        require(size_t(s->h->ns) == s->t->cfg.species && s->t->cfg.variables == s->t->cfg.species + 4,
+                "Transport and HEM species differ, or the transport carries liquid inventories");
+        require(s->t->cfg.variables <= REACTIVE_AMD_MAX_VARIABLES, "Too many conserved variables");
+        s->device = s->t->device;
+        RAMD_HIP(hipSetDevice(s->device));
+        s->ns = int(s->t->cfg.species);
+        s->nv = int(s->t->cfg.variables);
+        s->cells = s->t->cfg.cells;
+        s->all = s->t->all;
+        require(s->cells <= UINT32_MAX, "Too many cells");
+        RAMD_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        for (auto& e : s->events) RAMD_HIP(hipEventCreate(&e));
+        s->qA = s->alloc<double>(s->all * s->nv);
+        s->qB = s->alloc<double>(s->all * s->nv);
+        s->pA = s->alloc<double>(s->all);
+        s->cA = s->alloc<double>(s->all);
+        s->pB = s->alloc<double>(s->all);
+        s->cB = s->alloc<double>(s->all);
+        s->sA = s->alloc<ThermoOut>(s->cells);
+        s->sB = s->alloc<ThermoOut>(s->cells);
+        s->key = s->alloc<uint8_t>(s->cells);
+        s->order = s->alloc<uint32_t>(s->cells);
+        s->bucketCount = s->alloc<unsigned>(256);
+        s->bucketOffset = s->alloc<unsigned>(256);
+        s->blockMin = s->alloc<double>((s->cells + 255) / 256 + 1);
+        s->scalars = s->alloc<double>(4);
+        s->bA = s->alloc<double>(s->nv);
+        s->bB = s->alloc<double>(s->nv);
+        s->boundary = s->alloc<double>(s->nv);
+        s->totals = s->alloc<double>(s->nv);
+        s->totalPartial = s->alloc<double2>(size_t(s->nv) * kTotalBlocks);
+        s->counters = s->alloc<unsigned long long>(8);
+        RAMD_HIP(hipMemsetAsync(s->bucketCount, 0, 256 * sizeof(unsigned), s->stream));
+        RAMD_HIP(hipStreamSynchronize(s->stream));
+        return s.release();
+    } catch (const std::exception& e) {
+        writeError(error, errorSize, e.what());
+        return nullptr;
+    }
+}
+
+int reactive_amd_step_load(void* step, const double* q, const ReactiveThermoState* states) {
+    return guard(step, [&](Step& s) {
+        require(q && states, "Null step input");
+        std::vector<double> soa(s.all * s.nv, 0.0);
+        for (size_t c = 0; c < s.cells; ++c)
+            for (int k = 0; k < s.nv; ++k) soa[size_t(k) * s.all + c] = q[c * s.nv + k];
+        RAMD_HIP(hipMemcpyAsync(s.qA, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, s.stream));
+        ReactiveThermoState* staging = nullptr;
+        RAMD_HIP(hipMalloc(&staging, s.cells * sizeof(ReactiveThermoState)));
+        RAMD_HIP(hipMemcpyAsync(staging, states, s.cells * sizeof(ReactiveThermoState), hipMemcpyHostToDevice,
+                                s.stream));
+        hipLaunchKernelGGL(toThermoKernel, dim3(blocksFor(s.cells, 256)), dim3(256), 0, s.stream, staging, s.cells,
+                           s.sA, s.pA, s.cA);
+        RAMD_HIP(hipGetLastError());
+        RAMD_HIP(hipStreamSynchronize(s.stream));
+        RAMD_HIP(hipFree(staging));
+        s.loaded = true;
+    });
+}
+
+int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int flashMode,
+                          ReactiveAmdStepReport* report) {
+    return guard(step, [&](Step& s) {
+        require(s.loaded, "reactive_amd_step_load must precede reactive_amd_step_run");
+        require(report && report->abiVersion == 1 && report->structBytes == sizeof(*report), "Invalid step report ABI");
+        require(std::isfinite(cfl) && cfl > 0 && cfl <= .5 && std::isfinite(maximum) && maximum > 0,
+                "Invalid transport CFL controls");
+        require(steps >= 1, "At least one step");
+        require(flashMode == REACTIVE_AMD_FLASH_FP64, "Unknown flash arithmetic mode");
+        hipLaunchKernelGGL(clearCountersKernel, dim3(1), dim3(64), 0, s.stream, s.counters, 8);
+        while (s.marks.size() < size_t(steps) * 9) {
+            hipEvent_t e;
+            RAMD_HIP(hipEventCreate(&e));
+            s.marks.push_back(e);
+        }
+        RAMD_HIP(hipEventRecord(s.events[0], s.stream));
+        for (int i = 0; i < steps; ++i) s.one(cfl, maximum, s.marks.data() + size_t(i) * 9);
+        RAMD_HIP(hipEventRecord(s.events[1], s.stream));
+        unsigned long long counters[8];
+        double scalars[4], totals[REACTIVE_AMD_MAX_VARIABLES], boundary[REACTIVE_AMD_MAX_VARIABLES];
+        RAMD_HIP(hipMemcpyAsync(counters, s.counters, sizeof(counters), hipMemcpyDeviceToHost, s.stream));
+        RAMD_HIP(hipMemcpyAsync(scalars, s.scalars, sizeof(scalars), hipMemcpyDeviceToHost, s.stream));
+        RAMD_HIP(hipMemcpyAsync(totals, s.totals, s.nv * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+        RAMD_HIP(hipMemcpyAsync(boundary, s.boundary, s.nv * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+        RAMD_HIP(hipStreamSynchronize(s.stream));
+        float ms = 0;
+        RAMD_HIP(hipEventElapsedTime(&ms, s.events[0], s.events[1]));
+        ReactiveAmdStepReport r{};
+        r.abiVersion = 1;
+        r.structBytes = sizeof(r);
+        r.steps = uint64_t(steps);
+        r.flashFailures[0] = counters[0];
+        r.flashFailures[1] = counters[2];
+        r.phaseEvaluations = counters[1] + counters[3];
+        r.cflViolations = counters[4];
+        r.dt = scalars[0];
+        r.recheckDt = scalars[1];
+        for (int k = 0; k < s.nv; ++k) {
+            r.totals[k] = totals[k];
+            r.boundaryIntegral[k] = boundary[k];
+        }
+        r.seconds = ms / 1000.0;
+        for (int i = 0; i < steps; ++i) {
+            const hipEvent_t* m = s.marks.data() + size_t(i) * 9;
+            float d[8];
+            for (int j = 0; j < 8; ++j) RAMD_HIP(hipEventElapsedTime(&d[j], m[j], m[j + 1]));
+            r.cflSeconds += (d[0] + d[4]) / 1000.0;
+            r.stageSeconds[0] += d[2] / 1000.0;
+            r.flashSeconds[0] += d[3] / 1000.0;
+            r.stageSeconds[1] += d[6] / 1000.0;
+            r.flashSeconds[1] += d[7] / 1000.0;
+        }
+        *report = r;
+        require(r.dt > 0, "Invalid wave/diffusion timestep");
+    });
+}
+
+int reactive_amd_step_read(void* step, double* q, ReactiveThermoState* states) {
+    return guard(step, [&](Step& s) {
+        require(s.loaded, "Nothing loaded");
+        std::vector<double> soa(s.all * s.nv);
+        RAMD_HIP(hipMemcpyAsync(soa.data(), s.qA, soa.size() * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+        if (states)
+            RAMD_HIP(hipMemcpyAsync(states, s.sA, s.cells * sizeof(ThermoOut), hipMemcpyDeviceToHost, s.stream));
+        RAMD_HIP(hipStreamSynchronize(s.stream));
+        if (q)
+            for (size_t c = 0; c < s.cells; ++c)
+                for (int k = 0; k < s.nv; ++k) q[c * s.nv + k] = soa[size_t(k) * s.all + c];
+    });
+}
+
+const char* reactive_amd_step_error(void* step) { return step ? static_cast<Step*>(step)->error : "Null step"; }
+
+void reactive_amd_step_destroy(void* step) { delete static_cast<Step*>(step); }
+
+}

@@ -1,0 +1,1036 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+// Conservative transport on gfx950: first-order HLL face flux, SSPRK2 update
+// and CFL reduction behind the reactive_transport_* ABI
+// (src/reactiveTransport/reactiveTransport.h:48-137).
+//
+// Data layout in HBM: conserved q is structure-of-arrays, q[k*(cells+fixed)+c]
+// (the reference's qi(), reactiveTransportKernels.h:85), so every kernel
+// streams each variable with unit stride.  Face geometry is compacted to
+// 48-byte records (owner, resolved right cell, kind, normal, area), the cell
+// face lists are a CSR of signed face ids in the reference's order (faces
+// sorted by id within a cell, owner side negative), which fixes the FP64
+// summation order of every cell rate.
+//
+// Kernels (one launch each, no host round trip between them):
+//   primitives   rho = sum q_k, u = (rho u)/rho               (Cells)
+//   hll          per-face HLL coefficients                    (Faces)
+//   advance      per (cell, variable) CSR gather + SSPRK2     (Rhs + Advance)
+//   boundary     per-variable boundary flux, fixed 256-lane tree (BoundaryPartial/Finish)
+//   faceSpeed / cellStep / min-reduction                       (FaceSpeeds, Step, Minimum)
+// Structured Cartesian boxes take the fused stage kernel of cartesian.hip.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hip_common.h"
+#include "transport_internal.h"
+#include "reactive_transport.h"
+
+namespace ramd {
+
+static_assert(sizeof(ReactiveTransportFace) == 80 && sizeof(ReactiveTransportState) == 56, "transport ABI");
+static_assert(sizeof(ReactiveTransportConfig) == 88 && sizeof(ReactiveTransportOptionsV2) == 88, "transport ABI");
+static_assert(sizeof(ReactiveTransportOptionsV21) == 120 && sizeof(ReactiveTransportToken) == 24, "transport ABI");
+static_assert(sizeof(ReactiveTransportPrimitive) == 32, "transport ABI");
+
+__device__ inline double dmin(double a, double b) { return a < b ? a : b; }
+__device__ inline double dmax(double a, double b) { return a > b ? a : b; }
+
+// ---------------------------------------------------------------- kernels
+__global__ void primitivesKernel(const double* __restrict__ q, size_t all, int ns, double* __restrict__ prim) {
+    const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= all) return;
+    double rho = 0;
+    for (int k = 0; k < ns; ++k) rho += q[size_t(k) * all + c];
+    prim[c] = rho;
+    for (int d = 0; d < 3; ++d) prim[size_t(d + 1) * all + c] = q[size_t(ns + d) * all + c] / rho;
+}
+
+// velocities() (reactiveTransportKernels.h:145-149): normal components of
+// the owner and (mirrored for slip walls) right velocities.
+__device__ inline void normalVelocities(const FaceDev& f, const double* __restrict__ prim, size_t all, double& unL,
+                                        double& unR) {
+    double ul[3], ur[3], dot = 0;
+    for (int d = 0; d < 3; ++d) {
+        ul[d] = prim[size_t(d + 1) * all + f.owner];
+        ur[d] = prim[size_t(d + 1) * all + f.right];
+        dot += ur[d] * f.n[d];
+    }
+    if (f.kind == 1)
+        for (int d = 0; d < 3; ++d) ur[d] -= 2 * dot * f.n[d];
+    unL = 0;
+    unR = 0;
+    for (int d = 0; d < 3; ++d) {
+        unL += ul[d] * f.n[d];
+        unR += ur[d] * f.n[d];
+    }
+}
+
+// HLL coefficients (Faces, reactiveTransportKernels.h:343-356).
+__global__ void hllKernel(const FaceDev* __restrict__ faces, size_t nf, const double* __restrict__ prim, size_t all,
+                          const ReactiveTransportState* __restrict__ state, double waveFactor,
+                          double4* __restrict__ work) {
+    const size_t fi = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (fi >= nf) return;
+    const FaceDev f = faces[fi];
+    double unL, unR;
+    normalVelocities(f, prim, all, unL, unR);
+    const double pL = state[f.owner].p, pR = state[f.right].p;
+    const double aL = waveFactor * state[f.owner].sound, aR = waveFactor * state[f.right].sound;
+    const double left = dmin(0, dmin(unL - aL, unR - aR)), right = dmax(0, dmax(unL + aL, unR + aR));
+    const double invWave = 1 / (right - left);
+    double4 w;
+    w.x = right * (unL - left) * invWave;
+    w.y = left * (right - unR) * invWave;
+    w.z = (right * pL - left * pR) * invWave;
+    w.w = (right * pL * unL - left * pR * unR) * invWave;
+    work[fi] = w;
+}
+
+// faceFlux (reactiveTransportKernels.h:180-191) for the inviscid model.
+__device__ inline double faceFlux(const FaceDev& f, const double4& w, const double* __restrict__ q, size_t all, int ns,
+                                  int k) {
+    const double ql = q[size_t(k) * all + f.owner];
+    double qr = q[size_t(k) * all + f.right];
+    if (f.kind == 1 && k >= ns && k < ns + 3) {
+        double normal = 0;
+        for (int d = 0; d < 3; ++d) normal += q[size_t(ns + d) * all + f.right] * f.n[d];
+        qr -= 2 * normal * f.n[k - ns];
+    }
+    double flux = w.x * ql + w.y * qr;
+    if (k >= ns && k < ns + 3) flux += (w.z - 0.0) * f.n[k - ns];
+    if (k == ns + 3) flux += w.w;
+    if (k < ns) flux += 0.0;
+    else if (k < ns + 3) flux -= 0.0;
+    else if (k == ns + 3) flux += 0.0;
+    return flux * f.area;
+}
+
+// Rhs::value (reactiveTransportKernels.h:502-526) and Advance (:551-560).
+// mode 0: stage 0 -> out = q + dt L; 1: stage 1 -> out = .5 out + .5 (q + dt L);
+// 2: rhs only -> out = L (cell-major index of a cells x nv array).
+__global__ void advanceKernel(const FaceDev* __restrict__ faces, const double4* __restrict__ work,
+                              const uint32_t* __restrict__ row, const int32_t* __restrict__ incidence,
+                              const double* __restrict__ invV, const double* __restrict__ q, size_t cells, size_t all,
+                              int ns, int nv, double dt, int mode, double* __restrict__ out) {
+    const size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= cells * size_t(nv)) return;
+    const size_t c = j % cells;
+    const int k = int(j / cells);
+    double rate = 0;
+    const double inv = invV[c];
+    for (uint32_t e = row[c]; e < row[c + 1]; ++e) {
+        const int32_t entry = incidence[e];
+        const uint32_t fi = uint32_t(entry < 0 ? -entry - 1 : entry - 1);
+        const double sign = entry < 0 ? -1 : 1;
+        rate += sign * faceFlux(faces[fi], work[fi], q, all, ns, k) * inv;
+    }
+    const size_t index = size_t(k) * all + c;
+    if (mode == 0) out[index] = q[index] + dt * rate;
+    else if (mode == 1) out[index] = .5 * out[index] + .5 * (q[index] + dt * rate);
+    else out[c * nv + k] = rate;
+}
+
+// BoundaryPartial: block g = (variable k, tile) sums its lanes' strided faces,
+// then the 256-lane tree of reduceGroups (reactiveTransport.cpp:70-81, 233-246).
+__global__ __launch_bounds__(256) void boundaryPartialKernel(const FaceDev* __restrict__ faces,
+                                                             const double4* __restrict__ work,
+                                                             const uint32_t* __restrict__ boundary, size_t nb,
+                                                             size_t partitions, const double* __restrict__ q,
+                                                             size_t all, int ns, double* __restrict__ partial) {
+    __shared__ double values[256];
+    const size_t g = blockIdx.x;
+    const int k = int(g / partitions);
+    const size_t tile = g % partitions;
+    double sum = 0;
+    for (size_t j = tile * 256 + threadIdx.x; j < nb; j += partitions * 256) {
+        const uint32_t fi = boundary[j];
+        sum += faceFlux(faces[fi], work[fi], q, all, ns, k);
+    }
+    values[threadIdx.x] = sum;
+    __syncthreads();
+    for (unsigned stride = 128; stride; stride /= 2) {
+        if (threadIdx.x < stride) values[threadIdx.x] = values[threadIdx.x] + values[threadIdx.x + stride];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[g] = values[0];
+}
+
+__global__ __launch_bounds__(256) void boundaryFinishKernel(const double* __restrict__ partial, size_t partitions,
+                                                            double* __restrict__ rate) {
+    __shared__ double values[256];
+    const int k = blockIdx.x;
+    values[threadIdx.x] = threadIdx.x < partitions ? partial[size_t(k) * partitions + threadIdx.x] : 0;
+    __syncthreads();
+    for (unsigned stride = 128; stride; stride /= 2) {
+        if (threadIdx.x < stride) values[threadIdx.x] = values[threadIdx.x] + values[threadIdx.x + stride];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rate[k] = values[0];
+}
+
+// FaceSpeeds (reactiveTransportKernels.h:315-342) with zero diffusivities.
+__global__ void faceSpeedKernel(const FaceDev* __restrict__ faces, size_t nf, const double* __restrict__ prim,
+                                size_t all, const ReactiveTransportState* __restrict__ state, double waveFactor,
+                                double* __restrict__ speed) {
+    const size_t fi = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (fi >= nf) return;
+    const FaceDev f = faces[fi];
+    double unL, unR;
+    normalVelocities(f, prim, all, unL, unR);
+    const double aL = waveFactor * state[f.owner].sound, aR = waveFactor * state[f.right].sound;
+    const double left = dmin(0, dmin(unL - aL, unR - aR)), right = dmax(0, dmax(unL + aL, unR + aR));
+    const double invWave = 1 / (right - left);
+    double s = dmax(::fabs(unL) + aL, ::fabs(unR) + aR) + 2 * (0.0 + 0) / f.distance;
+    if (!isfinite(unL) || !isfinite(unR) || !isfinite(s) || !isfinite(invWave) || right <= left) s = -1;
+    speed[fi] = s;
+}
+
+// Step (reactiveTransportKernels.h:529-550) + the absorbing validity map of
+// Minimum (reactiveTransport.cpp:222-232): invalid -> -1; block minimum.
+__global__ __launch_bounds__(256) void cellStepKernel(const FaceDev* __restrict__ faces,
+                                                      const double* __restrict__ speed,
+                                                      const uint32_t* __restrict__ row,
+                                                      const int32_t* __restrict__ incidence,
+                                                      const double* __restrict__ invV, size_t cells, double cfl,
+                                                      double maximum, double* __restrict__ blockMin) {
+    const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    double v = __builtin_huge_val();
+    if (c < cells) {
+        double denominator = 0, step;
+        bool bad = false;
+        for (uint32_t e = row[c]; e < row[c + 1]; ++e) {
+            const int32_t entry = incidence[e];
+            const uint32_t fi = uint32_t(entry < 0 ? -entry - 1 : entry - 1);
+            const double s = speed[fi];
+            if (!isfinite(s) || s < 0) { bad = true; break; }
+            denominator += faces[fi].area * s;
+        }
+        if (bad) step = -1;
+        else {
+            step = denominator > 0 ? dmin(maximum, cfl / (invV[c] * denominator)) : maximum;
+            if (!isfinite(denominator) || denominator < 0) step = -1;
+        }
+        v = isfinite(step) && step > 0 ? step : -1;
+    }
+    __shared__ double lane[4];
+    v = waveMin(v);
+    if ((threadIdx.x & 63) == 0) lane[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = lane[0];
+        for (int i = 1; i < int(blockDim.x >> 6); ++i) m = lane[i] < m ? lane[i] : m;
+        blockMin[blockIdx.x] = m;
+    }
+}
+
+__global__ __launch_bounds__(256) void minReduceKernel(const double* __restrict__ in, size_t n,
+                                                       double* __restrict__ out) {
+    double v = __builtin_huge_val();
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) v = in[i] < v ? in[i] : v;
+    __shared__ double lane[4];
+    v = waveMin(v);
+    if ((threadIdx.x & 63) == 0) lane[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = lane[0];
+        for (int i = 1; i < int(blockDim.x >> 6); ++i) m = lane[i] < m ? lane[i] : m;
+        *out = m;
+    }
+}
+
+// Cell-major host layout <-> SoA device layout (Layout, reactiveTransportKernels.h:569-576).
+__global__ void packKernel(const double* __restrict__ aos, size_t cells, int nv, size_t stride, size_t offset,
+                           double* __restrict__ soa) {
+    const size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= cells * size_t(nv)) return;
+    const size_t c = j / nv, k = j % nv;
+    soa[k * stride + offset + c] = aos[j];
+}
+__global__ void unpackKernel(const double* __restrict__ soa, size_t cells, int nv, size_t stride,
+                             double* __restrict__ aos) {
+    const size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= cells * size_t(nv)) return;
+    const size_t c = j / nv, k = j % nv;
+    aos[j] = soa[k * stride + c];
+}
+__global__ void primitiveUploadKernel(const ReactiveTransportPrimitive* __restrict__ in, size_t cells, size_t all,
+                                      double* __restrict__ prim) {
+    const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    prim[c] = in[c].rho;
+    for (int d = 0; d < 3; ++d) prim[size_t(d + 1) * all + c] = in[c].u[d];
+}
+
+// HLL coefficients of the boundary faces only, primitives rebuilt from q
+// (Cells + Faces restricted to the boundary list, same arithmetic).
+__global__ void boundaryHllKernel(const FaceDev* __restrict__ faces, const uint32_t* __restrict__ boundary, size_t nb,
+                                  const double* __restrict__ q, size_t all, int ns, const double* __restrict__ pArr,
+                                  const double* __restrict__ cArr, double waveFactor, double4* __restrict__ work) {
+    const size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= nb) return;
+    const uint32_t fi = boundary[j];
+    const FaceDev f = faces[fi];
+    double rhoL = 0, rhoR = 0;
+    for (int k = 0; k < ns; ++k) {
+        rhoL += q[size_t(k) * all + f.owner];
+        rhoR += q[size_t(k) * all + f.right];
+    }
+    double ul[3], ur[3], dot = 0;
+    for (int d = 0; d < 3; ++d) {
+        ul[d] = q[size_t(ns + d) * all + f.owner] / rhoL;
+        ur[d] = q[size_t(ns + d) * all + f.right] / rhoR;
+        dot += ur[d] * f.n[d];
+    }
+    if (f.kind == 1)
+        for (int d = 0; d < 3; ++d) ur[d] -= 2 * dot * f.n[d];
+    double unL = 0, unR = 0;
+    for (int d = 0; d < 3; ++d) {
+        unL += ul[d] * f.n[d];
+        unR += ur[d] * f.n[d];
+    }
+    const double pL = pArr[f.owner], pR = pArr[f.right];
+    const double aL = waveFactor * cArr[f.owner], aR = waveFactor * cArr[f.right];
+    const double left = dmin(0, dmin(unL - aL, unR - aR)), right = dmax(0, dmax(unL + aL, unR + aR));
+    const double invWave = 1 / (right - left);
+    double4 w;
+    w.x = right * (unL - left) * invWave;
+    w.y = left * (right - unR) * invWave;
+    w.z = (right * pL - left * pR) * invWave;
+    w.w = (right * pL * unL - left * pR * unR) * invWave;
+    work[fi] = w;
+}
+
+inline unsigned gridFor(size_t n, unsigned block) { return unsigned((n + block - 1) / block); }
+
+// ---------------------------------------------------------------- handle
+Transport::Transport(int backend, const ReactiveTransportConfig& c, const double* volumes,
+                     const ReactiveTransportFace* hostFaces, const double* fixedQ,
+                     const ReactiveTransportState* fixedStates, const ReactiveTransportOptionsV2* options,
+                     const ReactiveTransportOptionsV21* options21)
+    : cfg(c) {
+    require(backend == 0 || backend == 1, "Unknown transport backend");
+    require(backend == 1, "backend=0 (serial host operators) is not part of the MI355X transport build; use backend=1");
+    int devices = 0;
+    RAMD_HIP(hipGetDeviceCount(&devices));
+    require(devices > 0, "No HIP device");
+    device = currentDevice();
+    if (options) {
+        require(options->abiVersion == 1 && options->structBytes == sizeof(*options) && options->bridgeCells > 0,
+                "Invalid transport options ABI");
+        require(options->physicalModelHash[64] == 0 && std::strlen(options->physicalModelHash) == 64,
+                "Invalid transport model hash");
+        physicalHash = options->physicalModelHash;
+        bridgeCells = options->bridgeCells;
+        require(!options->recomputeGas, "Recompute requires ideal-gas diffusive transport");
+    }
+    if (options21) {
+        cost.blockThreads = options21->blockThreads;
+        cost.pinnedBytes = 0;
+    }
+    require(cfg.cells && cfg.species && cfg.faces, "Empty transport mesh/model");
+    require(cfg.cells <= size_t(INT64_MAX) && cfg.faces < size_t(INT64_MAX) && cfg.fixed <= size_t(INT64_MAX),
+            "Transport index overflow");
+    require(cfg.mechanical ? cfg.variables == cfg.species + 6
+                           : (cfg.variables >= cfg.species + 4 && cfg.variables <= cfg.species + 6),
+            "Wrong transport conserved-variable count");
+    require(std::isfinite(cfg.maxBytes) && cfg.maxBytes > 0, "Invalid transport memory budget");
+    require(std::isfinite(cfg.waveFactor) && cfg.waveFactor >= 1, "Invalid transport wave factor");
+    require(std::isfinite(cfg.viscosity) && std::isfinite(cfg.conductivity) && std::isfinite(cfg.diffusivity) &&
+                cfg.viscosity >= 0 && cfg.conductivity >= 0 && cfg.diffusivity >= 0,
+            "Invalid transport coefficients");
+    require(cfg.viscosity == 0 && cfg.conductivity == 0 && cfg.diffusivity == 0,
+            "Viscous, conductive and diffusive transport are not provided by the MI355X build (inviscid HLL only)");
+    require(!cfg.mechanical, "The mechanical-equilibrium closure is not provided by the MI355X build");
+    require(volumes && hostFaces, "Null transport geometry");
+    require(cfg.cells <= std::numeric_limits<size_t>::max() - cfg.fixed, "Fixed-state index overflow");
+    all = cfg.cells + cfg.fixed;
+    require(all < (size_t(1) << 31) && cfg.faces < (size_t(1) << 31),
+            "This build indexes cells and faces with 32-bit integers (< 2^31)");
+    nv = int(cfg.variables);
+    ns = int(cfg.species);
+    std::vector<uint32_t> row(cfg.cells + 1, 0);
+    std::vector<uint32_t> boundary;
+    for (size_t c2 = 0; c2 < cfg.cells; ++c2)
+        require(std::isfinite(volumes[c2]) && volumes[c2] > 0, "Invalid transport volume");
+    std::vector<FaceDev> faces(cfg.faces);
+    for (size_t i = 0; i < cfg.faces; ++i) {
+        const auto& f = hostFaces[i];
+        require(f.owner >= 0 && size_t(f.owner) < cfg.cells && f.neighbour >= -1 &&
+                    (f.neighbour < 0 || size_t(f.neighbour) < cfg.cells),
+                "Invalid transport face cell");
+        require(f.kind >= 0 && f.kind <= 3 && (f.kind == 0) == (f.neighbour >= 0), "Invalid transport face kind");
+        require(f.kind != 3 || (f.fixed >= 0 && size_t(f.fixed) < cfg.fixed && !cfg.mechanical),
+                "Invalid fixed boundary state index");
+        double norm = 0;
+        for (double component : f.normal) norm += component * component;
+        require(std::isfinite(norm) && std::abs(norm - 1) < 1e-8, "Invalid transport unit normal");
+        require(std::isfinite(f.area) && std::isfinite(f.distance) && std::isfinite(f.ownerWeight) && f.area > 0 &&
+                    f.distance > 0 && f.ownerWeight >= 0 && f.ownerWeight <= 1,
+                "Invalid transport face geometry");
+        ++row[f.owner + 1];
+        if (f.neighbour >= 0) ++row[f.neighbour + 1];
+        else boundary.push_back(uint32_t(i));
+        FaceDev& d = faces[i];
+        d.owner = int32_t(f.owner);
+        d.right = int32_t(f.neighbour >= 0 ? f.neighbour : (f.kind == 3 ? int64_t(cfg.cells) + f.fixed : f.owner));
+        d.kind = f.kind;
+        d.pad = 0;
+        for (int k = 0; k < 3; ++k) d.n[k] = f.normal[k];
+        d.area = f.area;
+        d.distance = f.distance;
+    }
+    for (size_t c2 = 0; c2 < cfg.cells; ++c2) row[c2 + 1] += row[c2];
+    std::vector<int32_t> incidence(row.back());
+    std::vector<uint32_t> next(row);
+    for (size_t i = 0; i < cfg.faces; ++i) {
+        incidence[next[hostFaces[i].owner]++] = -int32_t(i) - 1;
+        if (hostFaces[i].neighbour >= 0) incidence[next[hostFaces[i].neighbour]++] = int32_t(i) + 1;
+    }
+    nBoundary = boundary.size();
+    partitions = std::min(size_t(256), std::max(size_t(1), (nBoundary + 255) / 256));
+    profile.boundaryPartitions = partitions;
+    bridgeCells = std::min(bridgeCells, std::max(cfg.cells, cfg.fixed));
+    RAMD_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    this->faces = allocate<FaceDev>(cfg.faces);
+    invV = allocate<double>(cfg.cells);
+    rowDev = allocate<uint32_t>(row.size());
+    incidenceDev = allocate<int32_t>(incidence.size());
+    boundaryDev = allocate<uint32_t>(std::max<size_t>(1, nBoundary));
+    q = allocate<double>(all * nv);
+    initial = allocate<double>(all * nv);
+    state = allocate<ReactiveTransportState>(all);
+    prim = allocate<double>(4 * all);
+    work = allocate<double4>(cfg.faces);
+    faceSpeed = allocate<double>(cfg.faces);
+    blockMin = allocate<double>(gridFor(cfg.cells, 256) + 1);
+    boundaryPartial = allocate<double>(nv * partitions);
+    boundaryRate = allocate<double>(nv);
+    staging = allocate<double>(std::max(all, cfg.cells) * nv);
+    deviceProfile.faceWorkspaceBytes = cfg.faces * (sizeof(double4) + sizeof(double));
+    std::vector<double> inverse(cfg.cells);
+    for (size_t c2 = 0; c2 < cfg.cells; ++c2) {
+        inverse[c2] = 1 / volumes[c2];
+        require(std::isfinite(inverse[c2]), "Invalid inverse volume");
+    }
+    upload(this->faces, faces.data(), faces.size());
+    upload(invV, inverse.data(), inverse.size());
+    upload(rowDev, row.data(), row.size());
+    upload(incidenceDev, incidence.data(), incidence.size());
+    if (nBoundary) upload(boundaryDev, boundary.data(), nBoundary);
+    cartesian = detectCartesian(hostFaces, volumes, row, incidence);
+    if (cfg.fixed) {
+        require(fixedQ, "Null packed source");
+        validateStates(fixedStates, cfg.fixed);
+        upload(staging, fixedQ, cfg.fixed * nv);
+        launchPack(staging, cfg.fixed, q, cfg.cells);
+        RAMD_HIP(hipMemcpyAsync(initial, q, all * nv * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        upload(state + cfg.cells, fixedStates, cfg.fixed);
+        std::vector<double> fixedPrim(4 * cfg.fixed);
+        for (size_t c2 = 0; c2 < cfg.fixed; ++c2) {
+            double rho = 0;
+            for (int k = 0; k < ns; ++k) rho += fixedQ[c2 * nv + k];
+            require(std::isfinite(rho) && rho > 0, "Invalid fixed density");
+            fixedPrim[c2] = rho;
+            for (int d = 0; d < 3; ++d) {
+                fixedPrim[(d + 1) * cfg.fixed + c2] = fixedQ[c2 * nv + ns + d] / rho;
+                require(std::isfinite(fixedPrim[(d + 1) * cfg.fixed + c2]), "Invalid fixed velocity");
+            }
+        }
+        for (int d = 0; d < 4; ++d) upload(prim + d * all + cfg.cells, fixedPrim.data() + d * cfg.fixed, cfg.fixed);
+    }
+    finish();
+}
+
+Transport::~Transport() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : allocations) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void* Transport::allocateBytes(size_t bytes) {
+    if (!bytes) return nullptr;
+    require(double(stats.allocatedBytes) + double(bytes) <= cfg.maxBytes,
+            "Transport exceeds maxDeviceMemoryGB/workspace budget");
+    void* p = nullptr;
+    RAMD_HIP(hipMalloc(&p, bytes));
+    allocations.push_back(p);
+    stats.allocatedBytes += bytes;
+    peakBytes = std::max<uint64_t>(peakBytes, stats.allocatedBytes);
+    return p;
+}
+
+void Transport::uploadBytes(void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    require(src && dst, "Null transport input buffer");
+    RAMD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+    stats.uploadedBytes += bytes;
+    ++memcpyCalls;
+}
+void Transport::downloadBytes(void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    require(src && dst, "Null transport output buffer");
+    RAMD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+    stats.downloadedBytes += bytes;
+    ++memcpyCalls;
+}
+void Transport::finish() {
+    ++synchronizations;
+    RAMD_HIP(hipStreamSynchronize(stream));
+}
+void Transport::launched() { ++stats.kernelLaunches; }
+
+void Transport::validateStates(const ReactiveTransportState* s, size_t count) {  // reactiveTransport.cpp:537-545
+    require(!count || s, "Null transport state");
+    for (size_t c2 = 0; c2 < count; ++c2)
+        require(std::isfinite(s[c2].p) && s[c2].p > 0 && std::isfinite(s[c2].T) && s[c2].T > 0 &&
+                    std::isfinite(s[c2].rho) && s[c2].rho > 0 && std::isfinite(s[c2].sound) && s[c2].sound > 0 &&
+                    std::isfinite(s[c2].cv) && s[c2].cv > 0 && std::isfinite(s[c2].gasMass) && s[c2].gasMass >= 0 &&
+                    std::isfinite(s[c2].dilatation),
+                "Invalid recovered transport state");
+}
+
+void Transport::launchPack(const double* aos, size_t cells, double* soa, size_t offset) {
+    hipLaunchKernelGGL(packKernel, dim3(gridFor(cells * nv, 256)), dim3(256), 0, stream, aos, cells, nv, all, offset,
+                       soa);
+    RAMD_HIP(hipGetLastError());
+    launched();
+    ++cost.layoutLaunches;
+    ++cost.layoutTiles;
+}
+void Transport::launchUnpack(const double* soa, double* hostOut) {
+    hipLaunchKernelGGL(unpackKernel, dim3(gridFor(cfg.cells * nv, 256)), dim3(256), 0, stream, soa, cfg.cells, nv,
+                       all, staging);
+    RAMD_HIP(hipGetLastError());
+    launched();
+    ++cost.layoutLaunches;
+    ++cost.layoutTiles;
+    downloadBytes(hostOut, staging, cfg.cells * nv * sizeof(double));
+}
+
+void Transport::uploadQ(const double* hostQ) {
+    require(hostQ, "Null packed source");
+    upload(staging, hostQ, cfg.cells * nv);
+    launchPack(staging, cfg.cells, q, 0);
+    ++profile.conservedUploads;
+    profile.conservedUploadBytes += cfg.cells * nv * sizeof(double);
+}
+
+void Transport::uploadState(const ReactiveTransportState* states) {
+    require(states, "Null transport state");
+    if (stateShadowValid && stateShadow.size() == cfg.cells &&
+        std::memcmp(stateShadow.data(), states, cfg.cells * sizeof(ReactiveTransportState)) == 0)
+        return;
+    stateShadowValid = false;
+    validateStates(states, cfg.cells);
+    upload(state, states, cfg.cells);
+    profile.stateUploadBytes += cfg.cells * sizeof(ReactiveTransportState);
+    stateShadow.assign(states, states + cfg.cells);
+    stateShadowValid = true;
+}
+
+void Transport::primitives() {
+    hipLaunchKernelGGL(primitivesKernel, dim3(gridFor(all, 256)), dim3(256), 0, stream, q, all, ns, prim);
+    RAMD_HIP(hipGetLastError());
+    launched();
+}
+
+void Transport::prepareResident(const ReactiveTransportState* states, bool transport) {
+    uploadState(states);
+    primitives();
+    if (transport) {
+        hipLaunchKernelGGL(hllKernel, dim3(gridFor(cfg.faces, 256)), dim3(256), 0, stream, faces, cfg.faces, prim, all,
+                           state, cfg.waveFactor, work);
+        RAMD_HIP(hipGetLastError());
+        launched();
+        ++deviceProfile.transportFaceLaunches;
+    } else {
+        launchFaceSpeeds();
+    }
+}
+
+void Transport::launchFaceSpeeds() {
+    hipLaunchKernelGGL(faceSpeedKernel, dim3(gridFor(cfg.faces, 256)), dim3(256), 0, stream, faces, cfg.faces, prim,
+                       all, state, cfg.waveFactor, faceSpeed);
+    RAMD_HIP(hipGetLastError());
+    launched();
+    ++deviceProfile.cflFaceLaunches;
+}
+
+void Transport::gather() {
+    hipLaunchKernelGGL(boundaryPartialKernel, dim3(unsigned(nv * partitions)), dim3(256), 0, stream, faces, work,
+                       boundaryDev, nBoundary, partitions, q, all, ns, boundaryPartial);
+    RAMD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(boundaryFinishKernel, dim3(unsigned(nv)), dim3(256), 0, stream, boundaryPartial, partitions,
+                       boundaryRate);
+    RAMD_HIP(hipGetLastError());
+    launched();
+    launched();
+}
+
+void Transport::boundaryRatesFrom(const double* qv, const double* p, const double* c, double* out, hipStream_t s) {
+    if (nBoundary) {
+        hipLaunchKernelGGL(boundaryHllKernel, dim3(gridFor(nBoundary, 256)), dim3(256), 0, s, faces, boundaryDev,
+                           nBoundary, qv, all, ns, p, c, cfg.waveFactor, work);
+        RAMD_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(boundaryPartialKernel, dim3(unsigned(nv * partitions)), dim3(256), 0, s, faces, work,
+                       boundaryDev, nBoundary, partitions, qv, all, ns, boundaryPartial);
+    RAMD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(boundaryFinishKernel, dim3(unsigned(nv)), dim3(256), 0, s, boundaryPartial, partitions, out);
+    RAMD_HIP(hipGetLastError());
+}
+
+void Transport::stableStep(double cfl, double maximum, double* dt) {  // reactiveTransport.cpp:1445-1456
+    require(dt && std::isfinite(cfl) && cfl > 0 && cfl <= .5 && std::isfinite(maximum) && maximum > 0,
+            "Invalid transport CFL controls");
+    const unsigned blocks = gridFor(cfg.cells, 256);
+    hipLaunchKernelGGL(cellStepKernel, dim3(blocks), dim3(256), 0, stream, faces, faceSpeed, rowDev, incidenceDev, invV,
+                       cfg.cells, cfl, maximum, blockMin);
+    RAMD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(minReduceKernel, dim3(1), dim3(256), 0, stream, blockMin, size_t(blocks), blockMin + blocks);
+    RAMD_HIP(hipGetLastError());
+    launched();
+    launched();
+    downloadBytes(dt, blockMin + blocks, sizeof(double));
+    finish();
+    require(std::isfinite(*dt) && *dt > 0, "Invalid wave/diffusion timestep");
+    ++stats.stepQueries;
+}
+
+void Transport::advance(double dt, int stage, double* hostQ, double* boundary) {
+    if (cartesian.valid) {
+        cartesianAdvance(dt, stage);
+    } else {
+        hipLaunchKernelGGL(advanceKernel, dim3(gridFor(cfg.cells * nv, 256)), dim3(256), 0, stream, faces, work,
+                           rowDev, incidenceDev, invV, q, cfg.cells, all, ns, nv, dt, stage, initial);
+        RAMD_HIP(hipGetLastError());
+        launched();
+    }
+    std::swap(q, initial);
+    if (hostQ) {
+        launchUnpack(q, hostQ);
+        profile.conservedDownloadBytes += cfg.cells * nv * sizeof(double);
+    }
+    downloadBytes(boundary, boundaryRate, nv * sizeof(double));
+    finish();
+    haveInitial = stage == 0;
+    stageDt = dt;
+    ++stats.stages;
+}
+
+void Transport::rhs(double* hostRhs, double* boundary) {
+    double* out = rhsBuffer ? rhsBuffer : (rhsBuffer = allocate<double>(cfg.cells * nv));
+    hipLaunchKernelGGL(advanceKernel, dim3(gridFor(cfg.cells * nv, 256)), dim3(256), 0, stream, faces, work, rowDev,
+                       incidenceDev, invV, q, cfg.cells, all, ns, nv, 0.0, 2, out);
+    RAMD_HIP(hipGetLastError());
+    launched();
+    downloadBytes(hostRhs, out, cfg.cells * nv * sizeof(double));
+    downloadBytes(boundary, boundaryRate, nv * sizeof(double));
+    finish();
+}
+
+void Transport::uploadPrimitives(const ReactiveTransportPrimitive* primitive, const ReactiveTransportState* states) {
+    require(primitive, "Null transport primitives");  // reactiveTransport.cpp:850-858
+    for (size_t c2 = 0; c2 < cfg.cells; ++c2) {
+        require(std::isfinite(primitive[c2].rho) && primitive[c2].rho > 0, "Invalid primitive density");
+        for (double u : primitive[c2].u) require(std::isfinite(u), "Invalid primitive velocity");
+    }
+    uploadState(states);
+    upload(reinterpret_cast<double*>(staging), reinterpret_cast<const double*>(primitive), cfg.cells * 4);
+    hipLaunchKernelGGL(primitiveUploadKernel, dim3(gridFor(cfg.cells, 256)), dim3(256), 0, stream,
+                       reinterpret_cast<const ReactiveTransportPrimitive*>(staging), cfg.cells, all, prim);
+    RAMD_HIP(hipGetLastError());
+    launched();
+    profile.primitiveUploadBytes += cfg.cells * sizeof(ReactiveTransportPrimitive);
+}
+
+template <class F>
+int protect(void* handle, F f) {  // reactiveTransport.cpp:1471-1491
+    if (!handle) return -1;
+    auto& t = *static_cast<Transport*>(handle);
+    std::unique_lock<std::mutex> use(t.entry, std::defer_lock);
+    if (!use.try_lock()) return 2;
+    try {
+        RAMD_HIP(hipSetDevice(t.device));
+        f(t);
+        t.error[0] = 0;
+        return 0;
+    } catch (const std::exception& ex) {
+        try { (void)hipStreamSynchronize(t.stream); } catch (...) {}
+        t.haveInitial = false;
+        t.haveResident = false;
+        ++t.cost.failedCalls;
+        std::snprintf(t.error, sizeof(t.error), "%s", ex.what());
+        return 1;
+    }
+}
+
+int stub(void* handle, const char* name, const char* what) {
+    return protect(handle, [&](Transport&) {
+        throw std::runtime_error(std::string(name) + ": " + what + " is not provided by the MI355X transport build");
+    });
+}
+
+}  // namespace ramd
+
+using namespace ramd;
+
+extern "C" {
+
+void* reactive_transport_create(int backend, const ReactiveTransportConfig* cfg, const double* volumes,
+                                const ReactiveTransportFace* faces, const double* q, const ReactiveTransportState* states,
+                                const double* gasY, const double* gasH, char* error, size_t size) {
+    try {
+        require(cfg, "Null transport configuration");
+        require(!cfg->diffusivity || (gasY && gasH), "Missing fixed gas properties");
+        return new Transport(backend, *cfg, volumes, faces, q, states);
+    } catch (const std::exception& ex) {
+        writeError(error, size, ex.what());
+        return nullptr;
+    }
+}
+
+void* reactive_transport_create_v2(int backend, const ReactiveTransportConfig* cfg,
+                                   const ReactiveTransportOptionsV2* options, const double* volume,
+                                   const ReactiveTransportFace* faces, const double* q,
+                                   const ReactiveTransportState* states, const double* y, const double* h, char* error,
+                                   size_t size) {
+    (void)y; (void)h;
+    try {
+        require(cfg && options, "Missing v2 configuration");
+        return new Transport(backend, *cfg, volume, faces, q, states, options);
+    } catch (const std::exception& ex) {
+        writeError(error, size, ex.what());
+        return nullptr;
+    }
+}
+
+void* reactive_transport_create_v21(int backend, const ReactiveTransportConfig* cfg,
+                                    const ReactiveTransportOptionsV21* opt, const double* volume,
+                                    const ReactiveTransportFace* faces, const double* q,
+                                    const ReactiveTransportState* state, const double* y, const double* h, char* error,
+                                    size_t errorSize) {
+    (void)y; (void)h;
+    try {  // reactiveTransport.cpp:1837-1852
+        require(cfg && opt && opt->abiVersion == 1 && opt->structBytes == sizeof(*opt), "Invalid v2.1 staging options");
+        require(opt->slots == 1, "This version supports one event-owned staging slot");
+        require(opt->blockThreads == 32 || opt->blockThreads == 64 || opt->blockThreads == 128 ||
+                    opt->blockThreads == 256,
+                "Unsupported transport block size");
+        const size_t cellBytes = cfg->variables * sizeof(double);
+        require(cellBytes > 0 && opt->slotBytes >= cellBytes, "Staging budget cannot hold one cell");
+        require(opt->pinnedBudgetBytes >= opt->slotBytes, "Pinned staging budget smaller than slot budget");
+        const size_t cells = opt->bridgeCellsOverride ? opt->bridgeCellsOverride : opt->slotBytes / cellBytes;
+        require(cells > 0 && cells * cellBytes <= opt->slotBytes, "Bridge override exceeds byte budget");
+        ReactiveTransportOptionsV2 old{};
+        old.abiVersion = 1;
+        old.structBytes = sizeof(old);
+        old.bridgeCells = cells;
+        old.recomputeGas = opt->recomputeGas;
+        std::memcpy(old.physicalModelHash, opt->physicalModelHash, 65);
+        return new Transport(backend, *cfg, volume, faces, q, state, &old, opt);
+    } catch (const std::exception& ex) {
+        writeError(error, errorSize, ex.what());
+        return nullptr;
+    }
+}
+
+void reactive_transport_destroy(void* t) { delete static_cast<Transport*>(t); }
+const char* reactive_transport_error(void* t) { return t ? static_cast<Transport*>(t)->error : "Null transport"; }
+int reactive_transport_is_cuda(void* t) { return t ? 1 : 0; }
+
+int reactive_transport_stats(void* t, ReactiveTransportStats* result) {
+    return protect(t, [&](Transport& x) {
+        require(result, "Null transport statistics");
+        *result = x.stats;
+    });
+}
+
+int reactive_transport_stable_step(void* t, const double* q, const ReactiveTransportState* state, double cfl,
+                                   double maximum, double* dt) {
+    return protect(t, [&](Transport& x) {
+        require(dt && std::isfinite(cfl) && cfl > 0 && cfl <= .5 && std::isfinite(maximum) && maximum > 0,
+                "Invalid transport CFL controls");
+        x.haveResident = false;
+        x.uploadQ(q);
+        x.prepareResident(state, false);
+        x.stableStep(cfl, maximum, dt);
+    });
+}
+
+int reactive_transport_stable_step_primitives(void* t, const ReactiveTransportPrimitive* primitive,
+                                              const ReactiveTransportState* state, double cfl, double maximum,
+                                              double* dt) {
+    return protect(t, [&](Transport& x) {
+        x.uploadPrimitives(primitive, state);
+        x.launchFaceSpeeds();
+        x.stableStep(cfl, maximum, dt);
+    });
+}
+
+int reactive_transport_rhs(void* t, const double* q, const ReactiveTransportState* states, const double* gasY,
+                           const double* gasH, double* rhs, double* boundary) {
+    (void)gasY; (void)gasH;
+    return protect(t, [&](Transport& x) {
+        require(rhs && boundary, "Null transport output buffer");
+        x.haveInitial = false;
+        x.haveResident = false;
+        x.uploadQ(q);
+        x.prepareResident(states, true);
+        x.gather();
+        x.rhs(rhs, boundary);
+    });
+}
+
+int reactive_transport_stage(void* t, double* q, const ReactiveTransportState* states, const double* gasY,
+                             const double* gasH, double dt, int stage, double* boundary) {
+    (void)gasY; (void)gasH;
+    return protect(t, [&](Transport& x) {
+        require(std::isfinite(dt) && dt > 0 && (stage == 0 || stage == 1), "Invalid RK stage");
+        require(stage == 0 || (x.haveInitial && x.stageDt == dt), "RK stage 1 requires matching stage 0");
+        require(q && boundary, "Null transport output buffer");
+        x.haveResident = false;
+        x.uploadQ(q);
+        x.prepareResident(states, true);
+        x.gather();
+        x.advance(dt, stage, q, boundary);
+    });
+}
+
+int reactive_transport_upload_conserved(void* t, const double* q, uint64_t version) {
+    return protect(t, [&](Transport& x) {  // reactiveTransport.cpp:1556-1564
+        require(version, "Conserved version must be nonzero");
+        if (x.haveResident && version == x.residentVersion) return;
+        require(version > x.highestVersion, "Conserved upload version must increase");
+        x.haveInitial = false;
+        x.uploadQ(q);
+        x.finish();
+        x.residentVersion = x.highestVersion = version;
+        x.haveResident = true;
+    });
+}
+
+int reactive_transport_stage_resident(void* t, const ReactiveTransportState* states, const double* gasY,
+                                      const double* gasH, double dt, int stage, uint64_t inputVersion,
+                                      uint64_t outputVersion, double* q, double* boundary) {
+    (void)gasY; (void)gasH;
+    return protect(t, [&](Transport& x) {  // reactiveTransport.cpp:1709-1719
+        require(x.haveResident && x.residentVersion == inputVersion, "Stale resident conserved state");
+        require(outputVersion > x.highestVersion, "Output conserved version must increase");
+        require(std::isfinite(dt) && dt > 0 && (stage == 0 || stage == 1), "Invalid RK stage");
+        require(stage == 0 || (x.haveInitial && x.stageDt == dt), "RK stage 1 requires matching stage 0");
+        require(boundary, "Null RK output buffer");
+        x.prepareResident(states, true);
+        x.gather();
+        x.advance(dt, stage, q, boundary);
+        x.residentVersion = x.highestVersion = outputVersion;
+        ++x.profile.residentStages;
+    });
+}
+
+int reactive_transport_begin_attempt(void* handle, const char* hash, uint64_t id) {
+    return protect(handle, [&](Transport& t) {  // reactiveTransport.cpp:1749-1759
+        require(hash && t.physicalHash == hash, "Transport physical model hash mismatch");
+        require(!t.attemptOpen && id > t.lastAttempt, "Stale/open transport attempt");
+        t.attemptId = t.lastAttempt = id;
+        t.tokenStage = 0;
+        t.attemptOpen = true;
+        t.haveResident = t.haveInitial = false;
+    });
+}
+
+int reactive_transport_end_attempt(void* handle, uint64_t id, int commit) {
+    return protect(handle, [&](Transport& t) {  // reactiveTransport.cpp:1760-1771
+        require((commit == 0 || commit == 1) && t.attemptOpen && id == t.attemptId, "Wrong transport attempt completion");
+        require(!commit || t.tokenStage == 2, "Cannot commit incomplete transport RK");
+        t.finish();
+        t.attemptOpen = false;
+        t.haveInitial = false;
+        if (!commit) t.haveResident = false;
+    });
+}
+
+int reactive_transport_advance_resident_v2(void* handle, ReactiveTransportToken input, ReactiveTransportToken output,
+                                           const ReactiveTransportState* state, const ReactiveGasPartition* partition,
+                                           const double* y, const double* h, double dt, double* boundary) {
+    (void)partition; (void)y; (void)h;
+    return protect(handle, [&](Transport& t) {  // reactiveTransport.cpp:1772-1783
+        require(t.attemptOpen && input.attemptId == t.attemptId && output.attemptId == input.attemptId,
+                "Wrong RK attempt");
+        require(input.stageId == t.tokenStage && input.stageId < 2 && output.stageId == input.stageId + 1,
+                "Stale RK stage token");
+        require(t.haveResident && input.contentVersion == t.residentVersion &&
+                    output.contentVersion > t.highestVersion,
+                "Stale RK content token");
+        require(std::isfinite(dt) && dt > 0 && boundary, "Invalid resident RK inputs");
+        require(input.stageId == 0 || (t.haveInitial && t.stageDt == dt), "Missing matching RK stage 0");
+        t.prepareResident(state, true);
+        t.gather();
+        t.advance(dt, int(input.stageId), nullptr, boundary);
+        t.tokenStage = output.stageId;
+        t.highestVersion = t.residentVersion = output.contentVersion;
+        ++t.profile.residentStages;
+    });
+}
+
+int reactive_transport_download_conserved(void* handle, ReactiveTransportToken token, double* output) {
+    return protect(handle, [&](Transport& t) {  // reactiveTransport.cpp:1784-1788
+        require(t.attemptOpen && token.attemptId == t.attemptId && token.stageId == t.tokenStage && t.haveResident &&
+                    token.contentVersion == t.residentVersion,
+                "Stale explicit download token");
+        require(output, "Null unpack output");
+        t.launchUnpack(t.q, output);
+        t.profile.conservedDownloadBytes += t.cfg.cells * t.nv * sizeof(double);
+        t.finish();
+    });
+}
+
+int reactive_transport_memory_v2(void* handle, ReactiveTransportMemoryV2* out) {
+    return protect(handle, [&](Transport& t) {
+        require(out, "Missing memory output");
+        const auto& c = t.cfg;
+        *out = {t.stats.allocatedBytes, t.peakBytes, t.bridgeCells * c.variables * sizeof(double),
+                2 * (c.cells + c.fixed) * c.variables * sizeof(double), 0,
+                t.rhsBuffer ? c.cells * c.variables * sizeof(double) : 0, t.deviceCopies, t.synchronizations};
+    });
+}
+
+int reactive_transport_profile(void* t, ReactiveTransportProfile* result) {
+    return protect(t, [&](Transport& x) {
+        require(result, "Null transport profile");
+        *result = x.profile;
+    });
+}
+
+int reactive_transport_device_profile(void* t, ReactiveTransportDeviceProfile* result) {
+    return protect(t, [&](Transport& x) {
+        require(result, "Null device profile");
+        *result = x.deviceProfile;
+    });
+}
+
+int reactive_transport_profile_v21(void* handle, ReactiveTransportProfileV21* out) {
+    return protect(handle, [&](Transport& t) {
+        require(out, "Missing transport v2.1 profile");
+        auto result = t.cost;
+        result.memcpyCalls = t.memcpyCalls;
+        result.payloadBytes = t.stats.uploadedBytes + t.stats.downloadedBytes + t.deviceCopies;
+        result.slotCells = t.bridgeCells;
+        result.slots = 1;
+        result.deviceStagingBytes = std::max(t.all, t.cfg.cells) * t.cfg.variables * sizeof(double);
+        *out = result;
+    });
+}
+
+// ---- exported stubs (physics that is off on this path) -------------------
+int reactive_transport_set_gas_thermo(void* t, const ReactiveGasThermoSpecies*, size_t, const ReactiveGasThermoRegion*,
+                                      size_t, const int64_t*, size_t) {
+    return stub(t, "reactive_transport_set_gas_thermo", "device NASA gas thermo (diffusive transport)");
+}
+int reactive_transport_stage_resident_gas(void* t, const ReactiveTransportState*, const ReactiveGasPartition*, double,
+                                          int, uint64_t, uint64_t, double*, double*) {
+    return stub(t, "reactive_transport_stage_resident_gas", "diffusive gas transport");
+}
+int reactive_transport_rhs_gas(void* t, const double*, const ReactiveTransportState*, const ReactiveGasPartition*,
+                               double*, double*) {
+    return stub(t, "reactive_transport_rhs_gas", "diffusive gas transport");
+}
+int reactive_transport_gas_properties_resident(void* t, const ReactiveTransportState*, const ReactiveGasPartition*,
+                                               double*, double*) {
+    return stub(t, "reactive_transport_gas_properties_resident", "diffusive gas transport");
+}
+int reactive_transport_set_capillary_v1(void* t, const ReactiveCapillaryOptionsV1*) {
+    return stub(t, "reactive_transport_set_capillary_v1", "capillary transport");
+}
+int reactive_transport_capillary_geometry_v1(void* t, const double*, const double*, double*, double*, double*) {
+    return stub(t, "reactive_transport_capillary_geometry_v1", "capillary transport");
+}
+int reactive_transport_capillary_profile_v1(void* t, ReactiveCapillaryProfileV1*) {
+    return stub(t, "reactive_transport_capillary_profile_v1", "capillary transport");
+}
+int reactive_transport_replace_stage_state_v1(void* t, ReactiveTransportToken, uint64_t, const double*) {
+    return stub(t, "reactive_transport_replace_stage_state_v1", "capillary stage replacement");
+}
+int reactive_transport_capillary_resident_begin_v1(void* t, size_t, const double*, const double*, const double*,
+                                                   const void*, const double*, double, int, int*) {
+    return stub(t, "reactive_transport_capillary_resident_begin_v1", "resident capillary closure");
+}
+int reactive_transport_capillary_resident_select_v1(void* t, int, void*, int*) {
+    return stub(t, "reactive_transport_capillary_resident_select_v1", "resident capillary closure");
+}
+int reactive_transport_capillary_resident_update_v1(void* t, size_t, double*, int*) {
+    return stub(t, "reactive_transport_capillary_resident_update_v1", "resident capillary closure");
+}
+int reactive_transport_capillary_resident_relax_v1(void* t) {
+    return stub(t, "reactive_transport_capillary_resident_relax_v1", "resident capillary closure");
+}
+int reactive_transport_capillary_resident_commit_v1(void* t, void*, double*, double*, double*) {
+    return stub(t, "reactive_transport_capillary_resident_commit_v1", "resident capillary closure");
+}
+int reactive_transport_install_cartesian_v1(void* t, const void*) {
+    return stub(t, "reactive_transport_install_cartesian_v1", "Cartesian implicit-interface geometry");
+}
+int reactive_transport_cartesian_info_v1(void* t, void*) {
+    return stub(t, "reactive_transport_cartesian_info_v1", "Cartesian implicit-interface geometry");
+}
+int reactive_transport_set_implicit_v1(void* t, const void*) {
+    return stub(t, "reactive_transport_set_implicit_v1", "implicit capillary geometry");
+}
+int reactive_transport_implicit_coefficients_v1(void* t, double*, const double*, size_t, size_t*) {
+    return stub(t, "reactive_transport_implicit_coefficients_v1", "implicit capillary geometry");
+}
+int reactive_transport_geometric_diagnostic_v1(void* t, const void*, const double*, const ReactiveTransportState*,
+                                               const double*, const void*, const void*, double*, double*, void*) {
+    return stub(t, "reactive_transport_geometric_diagnostic_v1", "geometric capillary diagnostic");
+}
+int reactive_transport_set_wale_v1(void* t, const void*) {
+    return stub(t, "reactive_transport_set_wale_v1", "WALE LES");
+}
+int reactive_transport_wale_profile_v1(void* t, void*) {
+    return stub(t, "reactive_transport_wale_profile_v1", "WALE LES");
+}
+int reactive_transport_wale_primitives_v1(void* t, const ReactiveTransportPrimitive*, const ReactiveTransportState*,
+                                          double*) {
+    return stub(t, "reactive_transport_wale_primitives_v1", "WALE LES");
+}
+int reactive_transport_set_wale_scalars_v1(void* t, const void*) {
+    return stub(t, "reactive_transport_set_wale_scalars_v1", "WALE scalar closure");
+}
+int reactive_transport_wale_scalar_profile_v1(void* t, void*) {
+    return stub(t, "reactive_transport_wale_scalar_profile_v1", "WALE scalar closure");
+}
+int reactive_transport_wale_scalar_fields_v1(void* t, const double*, const double*, const double*, const double*) {
+    return stub(t, "reactive_transport_wale_scalar_fields_v1", "WALE scalar closure");
+}
+int reactive_transport_set_wale_pr_model_v1(void* t, const void*) {
+    return stub(t, "reactive_transport_set_wale_pr_model_v1", "WALE PR properties");
+}
+int reactive_transport_wale_pr_properties_v1(void* t, const void*, const double*, size_t, const void*) {
+    return stub(t, "reactive_transport_wale_pr_properties_v1", "WALE PR properties");
+}
+int reactive_transport_wale_pr_properties_v2(void* t, const void*, const double*, size_t, const void*, int) {
+    return stub(t, "reactive_transport_wale_pr_properties_v2", "WALE PR properties");
+}
+int reactive_transport_wale_pr_profile_v1(void* t, void*) {
+    return stub(t, "reactive_transport_wale_pr_profile_v1", "WALE PR properties");
+}
+void* reactive_closure_scalar_create_v1(size_t, char* error, size_t errorSize) {
+    writeError(error, errorSize,
+               "reactive_closure_scalar_create_v1: the scalar closure candidate is not provided by the MI355X build");
+    return nullptr;
+}
+void reactive_closure_scalar_destroy_v1(void*) {}
+int reactive_closure_scalar_run_v1(void*, const void*, size_t, void*, void*, char* error, size_t errorSize) {
+    writeError(error, errorSize,
+               "reactive_closure_scalar_run_v1: the scalar closure candidate is not provided by the MI355X build");
+    return 1;
+}
+
+}

@@ -1,0 +1,110 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+// Transport handle shared by transport.hip (generic face graphs, ABI) and
+// cartesian.hip (structured-box fast path) and step.hip (resident SSPRK2 step).
+#ifndef REACTIVE_AMD_TRANSPORT_INTERNAL_H
+#define REACTIVE_AMD_TRANSPORT_INTERNAL_H
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+#include "reactive_transport.h"
+
+namespace ramd {
+
+// Compact face record (56 B): right = neighbour, the owner itself for
+// extrapolated boundaries, or cells + fixed for fixed-state boundaries.
+struct FaceDev {
+    int32_t owner, right, kind, pad;
+    double n[3], area, distance;
+};
+
+// Structured N^3-like box detected from a face list: cell c = i + nx (j + ny k),
+// axis-aligned unit normals, one face area, one cell volume, inviscid
+// internal (kind 0) and extrapolated (kind 2) faces only.  The order in which
+// a cell's six faces appear in its CSR row (face id order) is a function of
+// the cell's boundary type only; slotOrder[type] packs that order (3 bits per
+// position, slots 0..5 = -x +x -y +y -z +z), type = 6 boundary flags.
+struct CartesianInfo {
+    bool valid = false;
+    int nx = 0, ny = 0, nz = 0;
+    double area = 0, inverseVolume = 0, volume = 0;
+    uint32_t slotOrder[64] = {};
+    double normal[6][2][3] = {};  // exact normal of [slot][boundary face?]
+};
+
+class Transport {
+public:
+    std::mutex entry;
+    ReactiveTransportConfig cfg;
+    size_t all = 0, nBoundary = 0, partitions = 1, bridgeCells = 256;
+    int nv = 0, ns = 0, device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocations;
+    // device arrays
+    FaceDev* faces = nullptr;
+    double* invV = nullptr;
+    uint32_t* rowDev = nullptr;
+    int32_t* incidenceDev = nullptr;
+    uint32_t* boundaryDev = nullptr;
+    double *q = nullptr, *initial = nullptr, *rhsBuffer = nullptr;
+    ReactiveTransportState* state = nullptr;
+    double* prim = nullptr;            // rho, u0, u1, u2 SoA over cells + fixed
+    double4* work = nullptr;           // HLL advectL, advectR, pressureMomentum, pressureEnergy per face
+    double* faceSpeed = nullptr;
+    double* blockMin = nullptr;
+    double *boundaryPartial = nullptr, *boundaryRate = nullptr;
+    double* staging = nullptr;         // cell-major host-layout bridge
+    double *pressure = nullptr, *sound = nullptr;  // SoA p, c_frozen for the Cartesian path
+    CartesianInfo cartesian;
+    // versions, tokens, attempts (reactiveTransport.cpp:384-393)
+    std::string physicalHash;
+    uint64_t attemptId = 0, lastAttempt = 0, tokenStage = 0, residentVersion = 0, highestVersion = 0;
+    bool attemptOpen = false, haveResident = false, haveInitial = false;
+    double stageDt = 0;
+    std::vector<ReactiveTransportState> stateShadow;
+    bool stateShadowValid = false;
+    // counters
+    ReactiveTransportStats stats{};
+    ReactiveTransportProfile profile{};
+    ReactiveTransportDeviceProfile deviceProfile{};
+    ReactiveTransportProfileV21 cost{};
+    uint64_t peakBytes = 0, deviceCopies = 0, synchronizations = 0, memcpyCalls = 0;
+    char error[1024] = {};
+
+    Transport(int backend, const ReactiveTransportConfig& c, const double* volumes, const ReactiveTransportFace* faces,
+              const double* fixedQ, const ReactiveTransportState* fixedStates,
+              const ReactiveTransportOptionsV2* options = nullptr,
+              const ReactiveTransportOptionsV21* options21 = nullptr);
+    ~Transport();
+    void* allocateBytes(size_t bytes);
+    template <class T> T* allocate(size_t n) { return static_cast<T*>(allocateBytes(n * sizeof(T))); }
+    void uploadBytes(void* dst, const void* src, size_t bytes);
+    void downloadBytes(void* dst, const void* src, size_t bytes);
+    template <class T> void upload(T* dst, const T* src, size_t n) { uploadBytes(dst, src, n * sizeof(T)); }
+    void finish();
+    void launched();
+    static void validateStates(const ReactiveTransportState* s, size_t count);
+    void launchPack(const double* aos, size_t cells, double* soa, size_t offset);
+    void launchUnpack(const double* soa, double* hostOut);
+    void uploadQ(const double* hostQ);
+    void uploadState(const ReactiveTransportState* states);
+    void primitives();
+    void prepareResident(const ReactiveTransportState* states, bool transport);
+    void launchFaceSpeeds();
+    void gather();
+    void stableStep(double cfl, double maximum, double* dt);
+    void advance(double dt, int stage, double* hostQ, double* boundary);
+    void rhs(double* hostRhs, double* boundary);
+    void uploadPrimitives(const ReactiveTransportPrimitive* primitive, const ReactiveTransportState* states);
+    // cartesian.hip
+    CartesianInfo detectCartesian(const ReactiveTransportFace* hostFaces, const double* volumes,
+                                  const std::vector<uint32_t>& row, const std::vector<int32_t>& incidence);
+    void cartesianAdvance(double dt, int stage);
+    // device-resident step support (step.hip): boundary flux rates of q
+    // (stride `all`) with HLL coefficients rebuilt from q, p, c on the fly
+    void boundaryRatesFrom(const double* qv, const double* p, const double* c, double* out, hipStream_t s);
+};
+
+}  // namespace ramd
+#endif

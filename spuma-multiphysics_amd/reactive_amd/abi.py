@@ -138,3 +138,19 @@ class ProfileV21(C.Structure):
 
 assert C.sizeof(Config) == 88 and C.sizeof(OptionsV2) == 88 and C.sizeof(OptionsV21) == 120
 assert C.sizeof(Token) == 24 and C.sizeof(TransportState) == 56 and C.sizeof(Primitive) == 32
+
+
+class StepReport(C.Structure):
+    """ReactiveAmdStepReport (include/reactive_amd.h)."""
+    _fields_ = [("abiVersion", C.c_uint32), ("structBytes", C.c_uint32), ("steps", C.c_uint64),
+                ("flashFailures", C.c_uint64 * 2), ("cflViolations", C.c_uint64), ("phaseEvaluations", C.c_uint64),
+                ("dt", C.c_double), ("recheckDt", C.c_double), ("totals", C.c_double * 32),
+                ("boundaryIntegral", C.c_double * 32), ("seconds", C.c_double), ("stageSeconds", C.c_double * 2),
+                ("flashSeconds", C.c_double * 2), ("cflSeconds", C.c_double)]
+
+    @classmethod
+    def fresh(cls):
+        r = cls()
+        r.abiVersion = 1
+        r.structBytes = C.sizeof(cls)
+        return r

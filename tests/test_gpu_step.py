@@ -1,0 +1,74 @@
+"""Device-resident SSPRK2 step (reactive_amd_step_run) against the same
+step driven on the oracle: Flow::step without chemistry
+(src/reactiveFoam/ReactiveFoam.C:961-1019)."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from helpers import OracleTransport, config, cube_case, oracle, oracle_flash
+from reactive_amd import abi, cases, lib as rlib
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_step(case, cfl):
+    q0, s0, mb = case["q"], case["states"], case["model"]
+    cfg = config(len(q0), len(case["faces"]))
+    ora = OracleTransport(cfg, case["volume"], case["faces"])
+    rc, dt = ora.stable_step(q0, cases.transport_states(s0), cfl, 1.0)
+    assert rc == 0
+    q = q0.copy()
+    bA = ora.stage(q, cases.transport_states(s0), dt, 0)
+    s1, ok1, _ = oracle_flash(mb, q[:, :1], cases.internal_energy(q, 1), s0)
+    rc, recheck = ora.stable_step(q, cases.transport_states(s1), cfl, dt)
+    bB = ora.stage(q, cases.transport_states(s1), dt, 1)
+    s2, ok2, _ = oracle_flash(mb, q[:, :1], cases.internal_energy(q, 1), s0)
+    ora.close()
+    return dict(dt=dt, recheck=recheck, q=q, s1=s1, s2=s2, ok=(ok1, ok2), bA=bA, bB=bB)
+
+
+@pytest.mark.parametrize("n", [8, 12])
+def test_resident_step_matches_oracle_step(n):
+    lib = rlib.load()
+    case = cube_case(n, seed=11 + n)
+    cfl = 0.4
+    ref = oracle_step(case, cfl)
+    err = C.create_string_buffer(1024)
+    mb = case["model"]
+    hem = lib.reactive_gpu_hem_create_v1(mb, len(mb), len(case["q"]), err, 1024)
+    cfg = config(len(case["q"]), len(case["faces"]))
+    tr = lib.reactive_transport_create(1, C.byref(cfg), rlib.ptr(case["volume"]), rlib.ptr(case["faces"]), None,
+                                       None, None, None, err, 1024)
+    step = lib.reactive_amd_step_create(tr, hem, err, 1024)
+    assert step, err.value
+    assert lib.reactive_amd_step_load(step, rlib.ptr(case["q"]), rlib.ptr(case["states"])) == 0
+    rep = abi.StepReport.fresh()
+    assert lib.reactive_amd_step_run(step, cfl, 1.0, 1, 0, C.byref(rep)) == 0, lib.reactive_amd_step_error(step)
+    q = np.zeros_like(case["q"])
+    s = np.zeros(len(q), dtype=abi.STATE_DTYPE)
+    assert lib.reactive_amd_step_read(step, rlib.ptr(q), rlib.ptr(s)) == 0
+    assert rep.dt == ref["dt"]  # same CFL arithmetic, bit for bit
+    assert rep.flashFailures[0] == int((ref["ok"][0] == 0).sum())
+    assert rep.flashFailures[1] == int((ref["ok"][1] == 0).sum())
+    # stage 1 reads flash #1's p and c_frozen, whose last bits depend on the
+    # device math library; compare each variable on its own scale
+    scaled = np.abs(q - ref["q"]) / np.abs(ref["q"]).max(0)
+    assert scaled.max() <= 1e-12, scaled.max(0)
+    for f in ("p", "T", "soundFrozen"):
+        r = np.abs(s[f] - ref["s2"][f]) / np.abs(ref["s2"][f])
+        assert r.max() <= 1e-12, (f, r.max())
+    # conservation totals: compensated device sum vs the exactly rounded sum of the same products
+    vol = case["volume"][0]
+    for k in range(5):
+        exact = math.fsum(q[:, k] * vol)
+        scale = math.fsum(np.abs(q[:, k] * vol))
+        assert abs(rep.totals[k] - exact) <= 1e-14 * scale
+    for k in range(5):
+        assert rep.boundaryIntegral[k] == .5 * ref["dt"] * (ref["bA"][k] + ref["bB"][k]) or \
+            abs(rep.boundaryIntegral[k] - .5 * ref["dt"] * (ref["bA"][k] + ref["bB"][k])) <= 1e-12 * (
+                abs(ref["bA"][k]) + abs(ref["bB"][k])) * ref["dt"]
+    lib.reactive_amd_step_destroy(step)
+    lib.reactive_transport_destroy(tr)
+    lib.reactive_gpu_hem_destroy_v1(hem)
