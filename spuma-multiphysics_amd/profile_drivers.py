@@ -1,0 +1,86 @@
+#!/usr/bin/env python3
+"""Small drivers for rocprofv3 passes (each runs in seconds):
+
+  stage N REPS   fused Cartesian SSPRK2 stages (cartesianStageKernel) on an N^3
+                 pure-N2O box through reactive_transport_stage, REPS pairs
+  flash N        one HEM flash batch (hemFlashKernel) of an N^3 box through
+                 reactive_gpu_hem_run_v1
+  copy MIB       a plain device copy kernel of MIB MiB (torch elementwise copy),
+                 to calibrate FETCH_SIZE and WRITE_SIZE against a known byte count
+
+Usage: python3 spuma-multiphysics_amd/profile_drivers.py stage 160 3
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+from reactive_amd import abi, cases, lib as rlib, model  # noqa: E402
+
+
+def stage(n, reps):
+    lib = rlib.load()
+    faces, volume, q, seeds, _ = cases.pure_n2o_cube(n, seed=1)
+    st = np.zeros(len(q), dtype=abi.TSTATE_DTYPE)
+    fluid = cases.PureFluid("N2O")
+    st["p"] = np.maximum(seeds["p"], 1e3); st["T"] = seeds["T"]; st["rho"] = q[:, 0]; st["cv"] = 800.0
+    st["sound"] = 200.0; st["gasMass"] = q[:, 0]
+    cfg = abi.Config(cells=len(q), species=1, variables=5, faces=len(faces), fixed=0, viscosity=0, conductivity=0,
+                     diffusivity=0, waveFactor=1.1, maxBytes=100e9, mechanical=0)
+    err = C.create_string_buffer(1024)
+    h = lib.reactive_transport_create(1, C.byref(cfg), rlib.ptr(volume), rlib.ptr(faces), None, None, None, None,
+                                      err, 1024)
+    assert h, err.value
+    b = np.zeros(5)
+    for _ in range(reps):
+        for s in (0, 1):
+            rc = lib.reactive_transport_stage(h, rlib.ptr(q), rlib.ptr(st), None, None, 1e-9, s, rlib.ptr(b))
+            assert rc == 0, lib.reactive_transport_error(h)
+    lib.reactive_transport_destroy(h)
+    print(f"stage: {len(q)} cells, {reps} SSPRK2 stage pairs", flush=True)
+    del fluid
+
+
+def flash(n):
+    lib = rlib.load()
+    faces, volume, q, seeds, _ = cases.pure_n2o_cube(n, seed=1)
+    m = model.model_bytes(model.build_model())
+    err = C.create_string_buffer(1024)
+    h = lib.reactive_gpu_hem_create_v1(m, len(m), len(q), err, 1024)
+    assert h, err.value
+    e = cases.internal_energy(q, 1)
+    s = seeds.copy()
+    ok = np.zeros(len(q), np.int32)
+    prof = abi.HemProfile.fresh()
+    rc = lib.reactive_gpu_hem_run_v1(h, rlib.ptr(np.ascontiguousarray(q[:, :1])), rlib.ptr(e), len(q), rlib.ptr(s),
+                                     rlib.ptr(ok), C.byref(prof), err, 1024)
+    assert rc == 0, err.value
+    print(f"flash: {len(q)} cells, {int(ok.sum())} converged, {prof.phaseEvaluations} phase evaluations, "
+          f"kernel {prof.kernelSeconds:.4f} s", flush=True)
+    lib.reactive_gpu_hem_destroy_v1(h)
+
+
+def copy(mib):
+    import torch
+    n = int(mib) << 20
+    src = torch.ones(n // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)  # one elementwise copy kernel: reads and writes n bytes
+    torch.cuda.synchronize()
+    print(f"copy: 3 x {mib} MiB device-to-device (torch elementwise copy kernel)", flush=True)
+
+
+if __name__ == "__main__":
+    what = sys.argv[1]
+    if what == "stage":
+        stage(int(sys.argv[2]), int(sys.argv[3]))
+    elif what == "flash":
+        flash(int(sys.argv[2]))
+    elif what == "copy":
+        copy(int(sys.argv[2]))
+    else:
+        raise SystemExit(__doc__)

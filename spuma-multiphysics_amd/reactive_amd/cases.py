@@ -138,16 +138,17 @@ class PureFluid:
         if bracket is None:
             return None
         a_, b_ = math.log(bracket[0]), math.log(bracket[1])
+        ga = prev[1]
         for _ in range(80):
             m = 0.5 * (a_ + b_)
             r = self.roots(T, math.exp(m))
             if r.size < 3:
                 break
             g = self.lnphi(T, math.exp(m), r[0]) - self.lnphi(T, math.exp(m), r[-1])
-            if g > 0:
-                b_ = m  # liquid fugacity above vapour: lower p
+            if (g > 0) == (ga > 0):  # same sign as the lower end: the root lies above m
+                a_, ga = m, g
             else:
-                a_ = m
+                b_ = m
         p = math.exp(0.5 * (a_ + b_))
         r = self.roots(T, p)
         return p, r[0], r[-1]

@@ -1,0 +1,60 @@
+"""The oracle restatement against the reference's own outputs (fixtures of
+tests/golden/, written by make_golden.py from the compiled reference):
+bit for bit, including the flash work counters."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from helpers import GOLDEN, OracleTransport, config, oracle, oracle_flash, REF_PATH
+
+
+def load(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+@pytest.mark.parametrize("part", ["cube", "edge"])
+def test_flash_bitwise_vs_reference(part):
+    g = load("flash_n2o.npz")
+    mb = g["model"].tobytes()
+    assert oracle().oracle_model_supported(C.create_string_buffer(mb, len(mb))) == 1
+    out, success, counters = oracle_flash(mb, g[f"{part}_q"], g[f"{part}_energy"], g[f"{part}_seeds"])
+    assert np.array_equal(success, g[f"{part}_success"])
+    assert out.tobytes() == g[f"{part}_out"].tobytes()
+    assert np.array_equal(counters, g[f"{part}_counters"])
+
+
+@pytest.mark.parametrize("name", ["cube6", "graph"])
+def test_transport_bitwise_vs_reference(name):
+    g = load("transport.npz")
+    d = {k[len(name) + 1:]: g[k] for k in g.files if k.startswith(name + "_")}
+    fq = d["fixed_q"] if len(d["fixed_q"]) else None
+    fs = d["fixed_states"] if len(d["fixed_states"]) else None
+    cfg = config(len(d["q"]), len(d["faces"]), fixed=0 if fq is None else len(fq))
+    ora = OracleTransport(cfg, d["volume"], d["faces"], fq, fs)
+    rhs, b = ora.rhs(d["q"], d["states"])
+    assert np.array_equal(rhs, d["rhs"]) and np.array_equal(b, d["boundary"])
+    rc, dt = ora.stable_step(d["q"], d["states"], 0.4, 1.0)
+    assert rc == 0 and dt == float(d["dt"])
+    q = d["q"].copy()
+    assert np.array_equal(ora.stage(q, d["states"], dt, 0), d["b0"]) and np.array_equal(q, d["q1"])
+    assert np.array_equal(ora.stage(q, d["states"], dt, 1), d["b1"]) and np.array_equal(q, d["q2"])
+    ora.close()
+
+
+@pytest.mark.skipif(not os.path.exists(REF_PATH), reason="oracle/_ref is built only where the reference is mounted")
+def test_flash_fresh_inputs_vs_reference():
+    """Fresh seeded cube, not in the fixtures: oracle == compiled reference."""
+    from helpers import cube_case
+    lib = C.CDLL(REF_PATH)
+    V = C.c_void_p
+    lib.ref_flash_batch.argtypes = [V, C.c_size_t, V, V, V, C.c_int, V, V, V]
+    case = cube_case(6, seed=99)
+    q, e, seeds, mb = case["q"][:, :1].copy(), case["energy"], case["seeds"], case["model"]
+    out = seeds.copy()
+    success = np.zeros(len(e), np.int32)
+    lib.ref_flash_batch(C.create_string_buffer(mb, len(mb)), len(e), q.ctypes.data, e.ctypes.data,
+                        seeds.ctypes.data, 1, out.ctypes.data, success.ctypes.data, None)
+    mine, msuccess, _ = oracle_flash(mb, q, e, seeds)
+    assert np.array_equal(success, msuccess) and out.tobytes() == mine.tobytes()
