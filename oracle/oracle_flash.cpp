@@ -14,9 +14,30 @@
 #include <cmath>
 #include <cstring>
 #include <vector>
+#ifdef ORACLE_TWIN
+#include "../spuma-multiphysics_amd/csrc/elementary.h"
+#endif
 
 namespace oracle {
 namespace {
+
+// Elementary functions.  The default build calls glibc, as the reference's
+// host build does; the twin build (ORACLE_TWIN) calls the exact-sequence
+// functions the device flash uses (csrc/elementary.h), so that the device
+// flash can be compared with this restatement bit for bit.
+#ifdef ORACLE_TWIN
+inline double olog(double x) { return ramd::elem::log(x); }
+inline double oexp(double x) { return ramd::elem::exp(x); }
+inline double othird(double x) { return ramd::elem::powThird(x); }
+inline double oacos(double x) { return ramd::elem::acos(x); }
+inline double ocos(double x) { return ramd::elem::cos(x); }
+#else
+inline double olog(double x) { return std::log(x); }
+inline double oexp(double x) { return std::exp(x); }
+inline double othird(double x) { return std::pow(x, 1.0 / 3.0); }
+inline double oacos(double x) { return std::acos(x); }
+inline double ocos(double x) { return std::cos(x); }
+#endif
 
 constexpr double R = 8314.46261815324;            // reactiveDevicePR.h:31
 constexpr double kSmall = 1e-300;                 // :32
@@ -50,7 +71,7 @@ struct Roots {  // reactiveDevicePR.h:88-99
 void nasa(const NasaRegion& r, double T, double& cp, double& h, double& s) {
     const double* a = r.c;
     const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
-    const double invT = 1.0 / T, logT = std::log(T);
+    const double invT = 1.0 / T, logT = olog(T);
     if (r.poly == 7) {
         cp = a[0] + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4;
         h = a[0] + .5 * a[1] * T + a[2] * T2 / 3.0 + .25 * a[3] * T3 + .2 * a[4] * T4 + a[5] * invT;
@@ -107,7 +128,7 @@ Status mixing(const Table& t, double T, const double* x, double& a, double& b, d
 }
 
 inline double cubic(double V, double bn, double cn, double dn) { return ((V + bn) * V + cn) * V + dn; }
-inline double cbrtSigned(double x) { return x < 0 ? -std::pow(-x, 1.0 / 3.0) : std::pow(x, 1.0 / 3.0); }
+inline double cbrtSigned(double x) { return x < 0 ? -othird(-x) : othird(x); }
 
 // Separated third root inside the near-repeated discriminant band (:280-307).
 bool isolatedRoot(double center, double delta, double delta2, double q, double bn, double cn, double dn, double b,
@@ -175,10 +196,10 @@ Status roots(double T, double p, double a, double b, double aa, Roots& out) {
     } else if (disc < -1e-14) {
         double arg = -.5 * q / (delta * delta * delta);
         arg = hi(-1.0, lo(1.0, arg));
-        const double theta = std::acos(arg) / 3.0;
-        raw[0] = center + 2.0 * delta * std::cos(theta);
-        raw[1] = center + 2.0 * delta * std::cos(theta + 2.0 * kPi / 3.0);
-        raw[2] = center + 2.0 * delta * std::cos(theta + 4.0 * kPi / 3.0);
+        const double theta = oacos(arg) / 3.0;
+        raw[0] = center + 2.0 * delta * ocos(theta);
+        raw[1] = center + 2.0 * delta * ocos(theta + 2.0 * kPi / 3.0);
+        raw[2] = center + 2.0 * delta * ocos(theta + 4.0 * kPi / 3.0);
         n = 3;
     } else {
         return NoRoot;
@@ -250,7 +271,7 @@ Status reportedPhase(const Table& t, double T, const double* x, double W, double
     double tmid = tc - 100.0;
     if (tmid < 0) tmid = .5 * tc;
     const double ratio = tc / tmid;
-    double pp = pc * std::exp(-.8734 * ratio * ratio - 3.4522 * ratio + 4.2918);
+    double pp = pc * oexp(-.8734 * ratio * ratio - 3.4522 * ratio + 4.2918);
     double ma, mb, maa, mda, md2a;
     Status st = mixing(t, tmid, x, ma, mb, maa, mda, md2a);
     if (st != OK) return st;
@@ -296,7 +317,7 @@ Status stateTrho(const Table& t, double T, double rho, const double* x, double W
     s.dpdV = -R * T / (vmb * vmb) + 2.0 * aa * (s.V + b) / (den * den);
     s.dpdT = R / vmb - da / den;
     if (!fin(s.dpdV) || !fin(s.dpdT) || s.dpdV >= 0) return Unstable;
-    const double logRatio = std::log((s.V + (1.0 + kSqrt2) * b) / (s.V + (1.0 - kSqrt2) * b));
+    const double logRatio = olog((s.V + (1.0 + kSqrt2) * b) / (s.V + (1.0 - kSqrt2) * b));
     const double L = logRatio / (2.0 * kSqrt2 * b);
     double h0 = 0, s0 = 0, cp0 = 0, xlogx = 0;
     double hRT[kMaxSpecies], sR[kMaxSpecies];
@@ -307,13 +328,13 @@ Status stateTrho(const Table& t, double T, double rho, const double* x, double W
         cp0 += x[k] * R * cpR;
         h0 += x[k] * R * T * hk;
         s0 += x[k] * R * sk;
-        if (x[k] > 0) xlogx += x[k] * std::log(x[k]);
+        if (x[k] > 0) xlogx += x[k] * olog(x[k]);
     }
     const double z = s.p * s.V / (R * T);
     const double hRes = R * T * (z - 1.0) + L * (T * da - aa);
-    const double sRes = R * std::log(z * (1.0 - b / s.V)) + L * da;
+    const double sRes = R * olog(z * (1.0 - b / s.V)) + L * da;
     const double hM = h0 + hRes;
-    const double sM = s0 - R * xlogx - R * std::log(s.p / t.refP) + sRes;
+    const double sM = s0 - R * xlogx - R * olog(s.p / t.refP) + sRes;
     const double dHdT = cp0 + s.V * s.dpdT - R + L * T * d2a;
     const double cpM = dHdT - (s.V + T * s.dpdT / s.dpdV) * s.dpdT;
     const double cvM = cpM + T * s.dpdT * s.dpdT / s.dpdV;
@@ -339,8 +360,8 @@ Status stateTrho(const Table& t, double T, double rho, const double* x, double W
             const double bk = t.sp[k].b;
             const double num = 2.0 * b * pp - aa * bk;
             const double RT = R * T;
-            s.mu[k] = RT * (hRT[k] - sR[k]) + RT * std::log(hi(kSmall, x[k])) + RT * std::log(s.p / t.refP) -
-                      RT * std::log(s.p * s.V / RT) + RT * std::log(s.V / vmb) + RT * bk / vmb -
+            s.mu[k] = RT * (hRT[k] - sR[k]) + RT * olog(hi(kSmall, x[k])) + RT * olog(s.p / t.refP) -
+                      RT * olog(s.p * s.V / RT) + RT * olog(s.V / vmb) + RT * bk / vmb -
                       num / denomMu * logRatio - aa * bk * s.V / denomMu2;
         }
     }
@@ -708,7 +729,7 @@ struct Flash {
             bool accepted = false;
             for (double fraction = 1; fraction > 1e-9; fraction *= .5) {
                 Evaluation trial;
-                const double pp = p * std::exp(fraction * step[0]), tt = T * std::exp(fraction * step[1]);
+                const double pp = p * oexp(fraction * step[0]), tt = T * oexp(fraction * step[1]);
                 if (evaluate(mass, pp, tt, trial)) {
                     const double next = hi(std::fabs(trial.volume - 1), std::fabs(trial.energy - energy) / scale);
                     if (next < previous * (1 - 1e-4 * fraction) || next < lo(m.vtol, m.etol)) {
@@ -736,7 +757,7 @@ struct Flash {
             mass[i] = x[j + 2] * qq;
         }
         Evaluation v;
-        if (!evaluate(mass, std::exp(x[0]), std::exp(x[1]), v, true) || !(v.state.gasMass > 0)) return false;
+        if (!evaluate(mass, oexp(x[0]), oexp(x[1]), v, true) || !(v.state.gasMass > 0)) return false;
         f[0] = v.volume - 1;
         f[1] = (v.energy - energy) / scale;
         for (int j = 0; j < n; ++j) {
@@ -778,7 +799,7 @@ struct Flash {
         const double rho = density();
         const ThermoState& g = restart ? *restart : guess;
         const double scale = hi(1e5 * rho, hi(std::fabs(energy), rho * 2000 * hi(g.T, m.Tmin)));
-        double x[4]{std::log(clamp(g.p, m.pmin, m.pmax)), std::log(clamp(g.T, m.Tmin, m.Tmax))};
+        double x[4]{olog(clamp(g.p, m.pmin, m.pmax)), olog(clamp(g.T, m.Tmin, m.Tmax))};
         for (int j = 0; j < n; ++j) {
             const int i = active[j];
             const double previous = g.liquidMass[i] / q[m.condensable[i]];
@@ -1144,4 +1165,14 @@ int oracle_nasa(const void* image, int phase, int k, double T, double* out) {
     return 0;
 }
 
+}
+
+// Elementary functions of this build, for tests: which = 0 log, 1 exp,
+// 2 x^(1/3) as pow(x, 1.0 / 3.0), 3 acos, 4 cos.
+extern "C" void oracle_elementary(int which, size_t n, const double* x, double* y) {
+    for (size_t i = 0; i < n; ++i) {
+        const double v = x[i];
+        y[i] = which == 0 ? oracle::olog(v) : which == 1 ? oracle::oexp(v) : which == 2 ? oracle::othird(v)
+             : which == 3 ? oracle::oacos(v) : oracle::ocos(v);
+    }
 }

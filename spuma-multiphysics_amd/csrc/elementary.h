@@ -1,0 +1,267 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+// Elementary functions with one definition for the device and the host.
+//
+// Every function is a fixed sequence of IEEE-754 double operations (add,
+// multiply, divide, square root, fused multiply-add through an explicit
+// fma call, exact exponent scaling), so gfx950 and an x86-64 host compiled
+// with -ffp-contract=off produce the same bits for the same argument.  The
+// flash kernels use them in place of the device math library, and the
+// oracle's twin build uses them in place of glibc: the device flash can then
+// be compared with its CPU restatement bit for bit.
+//
+// Methods (textbook argument reductions, Taylor polynomials with their
+// truncation error below 2^-56 relative on the reduced range):
+//   log   x = 2^e m, m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s = (m-1)/(m+1)
+//   exp   x = k ln2 + r, |r| <= ln2/2 (two-part ln2, r kept as a pair);
+//         exp r = 1 + r + r^2/2 + r^3 P(r), the first three terms summed
+//         with their rounding errors
+//   cbrt  Newton on c^3 = x from an exponent-third estimate; the last
+//         correction, from the cube's residual with the square's rounding
+//         error recovered by fma, is added in one rounding
+//   powThird  x^fl(1/3): fl(1/3) = 1/3 - 1/(3 2^54), so
+//         x^fl(1/3) = cbrt(x) (1 - log(x) / (3 2^54)) to within 2^-100
+//   acos  pi/2 - asin x for |x| <= 1/2; 2 asin(sqrt((1-|x|)/2)) beyond, asin
+//         by its Taylor series on |t| <= 1/2
+//   cos   x = n pi/2 + r (three-part pi/2), |r| <= pi/4, Taylor sin/cos of r;
+//         accurate for |x| < 2^20
+// Measured against 80-digit references (tests/test_elementary.py): log, exp,
+// cbrt and powThird within 1 ulp, acos and cos within 2 ulp.
+#ifndef REACTIVE_AMD_ELEMENTARY_H
+#define REACTIVE_AMD_ELEMENTARY_H
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#ifdef __HIPCC__
+#define RAMD_ELEM __host__ __device__ inline
+#else
+#define RAMD_ELEM inline
+#endif
+
+namespace ramd {
+namespace elem {
+
+RAMD_ELEM uint64_t bits(double x) {
+    uint64_t u;
+    std::memcpy(&u, &x, sizeof(u));
+    return u;
+}
+RAMD_ELEM double fromBits(uint64_t u) {
+    double x;
+    std::memcpy(&x, &u, sizeof(x));
+    return x;
+}
+RAMD_ELEM double fmaE(double a, double b, double c) { return ::fma(a, b, c); }
+// 2^k for -1022 <= k <= 1023, exactly
+RAMD_ELEM double pow2(int k) { return fromBits(uint64_t(int64_t(k) + 1023) << 52); }
+// x 2^k for any k that keeps the result normal or zero: two exact steps
+RAMD_ELEM double scale(double x, int k) {
+    if (k > 1000) return x * pow2(1000) * pow2(k - 1000);
+    if (k < -1000) return x * pow2(-1000) * pow2(k + 1000);
+    return x * pow2(k);
+}
+
+constexpr double kLn2Hi = 0x1.62e42fef00000p-1;   // 32 leading bits of ln 2
+constexpr double kLn2Lo = 7.440617110012397e-11;   // ln 2 - kLn2Hi
+constexpr double kInvLn2 = 1.4426950408889634;
+constexpr double kSqrtHalf = 0.70710678118654757;
+constexpr double kPio2Hi = 1.5707963267948966, kPio2Lo = 6.123233995736766e-17;
+constexpr double kPiHi = 3.141592653589793, kPiLo = 1.2246467991473532e-16;
+constexpr double kPio2A = 1.5707963267341256, kPio2B = 6.077100506303966e-11, kPio2C = 2.0222662487959506e-21;
+
+RAMD_ELEM double log(double x) {
+    if (!(x > 0)) return x == 0 ? -HUGE_VAL : NAN;  // also NaN in, NaN out
+    if (x == HUGE_VAL) return x;
+    uint64_t u = bits(x);
+    int e = int((u >> 52) & 0x7ff);
+    if (e == 0) {  // subnormal: renormalise
+        x *= 0x1p54;
+        u = bits(x);
+        e = int((u >> 52) & 0x7ff) - 54;
+    }
+    e -= 1023;
+    double m = fromBits((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);  // [1, 2)
+    if (m > 2 * kSqrtHalf) {
+        m *= 0.5;
+        ++e;
+    }
+    const double f = m - 1.0;  // exact
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    // R(z) = sum_k 2 z^k / (2k + 1), k = 1..13 (z <= 0.0295)
+    double r = 0.07407407407407407;
+    r = fmaE(r, z, 0.08);
+    r = fmaE(r, z, 0.08695652173913043);
+    r = fmaE(r, z, 0.09523809523809523);
+    r = fmaE(r, z, 0.10526315789473684);
+    r = fmaE(r, z, 0.11764705882352941);
+    r = fmaE(r, z, 0.13333333333333333);
+    r = fmaE(r, z, 0.15384615384615385);
+    r = fmaE(r, z, 0.18181818181818182);
+    r = fmaE(r, z, 0.2222222222222222);
+    r = fmaE(r, z, 0.2857142857142857);
+    r = fmaE(r, z, 0.4);
+    r = fmaE(r, z, 0.6666666666666666);
+    r *= z;
+    // log(1 + f) = f - hfsq + s (hfsq + R), hfsq = f^2 / 2
+    const double hfsq = 0.5 * f * f;
+    const double de = double(e);
+    return fmaE(de, kLn2Hi, f - (hfsq - fmaE(s, hfsq + r, de * kLn2Lo)));
+}
+
+RAMD_ELEM double exp(double x) {
+    if (x != x) return x;
+    if (x > 709.782712893384) return HUGE_VAL;
+    if (x < -745.1332191019412) return 0.0;
+    const double kd = ::rint(x * kInvLn2);
+    const int k = int(kd);
+    // r = x - k ln2 as rh + rl: x - k kLn2Hi is exact, k kLn2Lo is split
+    const double r1 = fmaE(-kd, kLn2Hi, x);
+    const double t = kd * kLn2Lo, te = fmaE(kd, kLn2Lo, -t);
+    const double rh = r1 - t, rl = ((r1 - rh) - t) - te;
+    // exp r = 1 + r + r^2/2 + r^3 (1/3! + r/4! + ... + r^10/13!)
+    double p = 1.6059043836821613e-10;
+    p = fmaE(p, rh, 2.08767569878681e-09);
+    p = fmaE(p, rh, 2.505210838544172e-08);
+    p = fmaE(p, rh, 2.755731922398589e-07);
+    p = fmaE(p, rh, 2.7557319223985893e-06);
+    p = fmaE(p, rh, 2.48015873015873e-05);
+    p = fmaE(p, rh, 0.0001984126984126984);
+    p = fmaE(p, rh, 0.001388888888888889);
+    p = fmaE(p, rh, 0.008333333333333333);
+    p = fmaE(p, rh, 0.041666666666666664);
+    p = fmaE(p, rh, 0.16666666666666666);
+    const double zh = rh * rh, zl = fmaE(rh, rh, -zh);  // r^2 exactly
+    // leading terms 1 + rh + zh/2 summed with their rounding errors
+    const double s1 = 1.0 + rh, e1 = (1.0 - s1) + rh;
+    const double half = 0.5 * zh;
+    const double s2 = s1 + half, e2 = (s1 - s2) + half;
+    const double tail = ((rl + rl * rh) + 0.5 * zl) + (zh * rh) * p;
+    return scale(s2 + ((e1 + e2) + tail), k);
+}
+
+// cube root of a > 0 as c + corr (c from Newton, corr its last correction)
+RAMD_ELEM void cbrtParts(double a, double& c, double& corr) {
+    int adjust = 0;
+    if (a < 0x1p-1000) {  // keep the cube and the squares normal
+        a *= 0x1p300;
+        adjust = -100;
+    } else if (a > 0x1p1000) {
+        a *= 0x1p-300;
+        adjust = 100;
+    }
+    const int e = int((bits(a) >> 52) & 0x7ff) - 1023;
+    // 2^floor(e/3) times a linear estimate of the mantissa's cube root
+    // (about 6 bits), refined by Newton (quadratic)
+    const int e3 = e >= 0 ? e / 3 : -((-e + 2) / 3);
+    const int rem = e - 3 * e3;  // 0, 1, 2
+    const double mant = fromBits((bits(a) & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+    c = (rem == 0 ? 1.0 : rem == 1 ? 1.2599210498948732 : 1.5874010519681994) * (0.75 + 0.25 * mant);
+    c = scale(c, e3);
+    for (int i = 0; i < 6; ++i) c = c - (c - a / (c * c)) / 3.0;
+    // last Newton correction from the residual c^3 - a, with the square's
+    // rounding error recovered by fma
+    const double c2 = c * c, c2e = fmaE(c, c, -c2);
+    const double residual = fmaE(c2, c, -a) + c2e * c;
+    corr = -residual / (3.0 * c2);
+    c = scale(c, adjust);
+    corr = scale(corr, adjust);
+}
+
+RAMD_ELEM double cbrt(double x) {
+    if (x == 0 || x != x || x == HUGE_VAL || x == -HUGE_VAL) return x;
+    double c, corr;
+    cbrtParts(x < 0 ? -x : x, c, corr);
+    const double r = c + corr;
+    return x < 0 ? -r : r;
+}
+
+// x^fl(1/3), the value pow(x, 1.0 / 3.0) denotes, for x >= 0
+RAMD_ELEM double powThird(double x) {
+    if (x != x || x == HUGE_VAL) return x;
+    if (!(x > 0)) return x == 0 ? 0.0 : NAN;
+    double c, corr;
+    cbrtParts(x, c, corr);
+    return c + (corr - c * (log(x) * (1.0 / 54043195528445952.0)));  // 3 * 2^54
+}
+
+RAMD_ELEM double asinSmall(double t) {  // |t| <= 1/2
+    const double z = t * t;
+    double p = 0.0017680811205154183;
+    p = fmaE(p, z, 0.0018622264064031275);
+    p = fmaE(p, z, 0.0019650336162772837);
+    p = fmaE(p, z, 0.0020776610325181676);
+    p = fmaE(p, z, 0.0022014739737101384);
+    p = fmaE(p, z, 0.002338091892111975);
+    p = fmaE(p, z, 0.0024894486782468836);
+    p = fmaE(p, z, 0.00265787063820729);
+    p = fmaE(p, z, 0.002846178401108942);
+    p = fmaE(p, z, 0.0030578216492580306);
+    p = fmaE(p, z, 0.003297059503473485);
+    p = fmaE(p, z, 0.0035692053938259347);
+    p = fmaE(p, z, 0.003880964558837669);
+    p = fmaE(p, z, 0.004240907093679363);
+    p = fmaE(p, z, 0.004660143486915096);
+    p = fmaE(p, z, 0.005153309682319905);
+    p = fmaE(p, z, 0.005740037670841924);
+    p = fmaE(p, z, 0.006447210311889649);
+    p = fmaE(p, z, 0.0073125258735988454);
+    p = fmaE(p, z, 0.008390335809616815);
+    p = fmaE(p, z, 0.009761609529194078);
+    p = fmaE(p, z, 0.011551800896139705);
+    p = fmaE(p, z, 0.01396484375);
+    p = fmaE(p, z, 0.017352764423076924);
+    p = fmaE(p, z, 0.022372159090909092);
+    p = fmaE(p, z, 0.030381944444444444);
+    p = fmaE(p, z, 0.044642857142857144);
+    p = fmaE(p, z, 0.075);
+    p = fmaE(p, z, 0.16666666666666666);
+    return fmaE(t * z, p, t);
+}
+
+RAMD_ELEM double acos(double x) {
+    if (!(x >= -1 && x <= 1)) return NAN;
+    if (x >= -0.5 && x <= 0.5) return kPio2Hi - (asinSmall(x) - kPio2Lo);
+    if (x > 0) return 2.0 * asinSmall(::sqrt(0.5 * (1.0 - x)));
+    return kPiHi - (2.0 * asinSmall(::sqrt(0.5 * (1.0 + x))) - kPiLo);
+}
+
+RAMD_ELEM double cosReduced(double r) {  // |r| <= pi/4
+    const double z = r * r;
+    double p = 4.779477332387385e-14;  // 1/16!
+    p = fmaE(p, -z, 1.1470745597729725e-11);
+    p = fmaE(p, -z, 2.08767569878681e-09);
+    p = fmaE(p, -z, 2.755731922398589e-07);
+    p = fmaE(p, -z, 2.48015873015873e-05);
+    p = fmaE(p, -z, 0.001388888888888889);
+    p = fmaE(p, -z, 0.041666666666666664);
+    // cos r = 1 - z/2 + z^2 (1/4! - z/6! + ...)
+    const double hz = 0.5 * z, w = 1.0 - hz;
+    return w + (((1.0 - w) - hz) + z * z * p);
+}
+RAMD_ELEM double sinReduced(double r) {  // |r| <= pi/4
+    const double z = r * r;
+    double p = 2.8114572543455206e-15;  // 1/17!
+    p = fmaE(p, -z, 7.647163731819816e-13);
+    p = fmaE(p, -z, 1.6059043836821613e-10);
+    p = fmaE(p, -z, 2.505210838544172e-08);
+    p = fmaE(p, -z, 2.7557319223985893e-06);
+    p = fmaE(p, -z, 0.0001984126984126984);
+    p = fmaE(p, -z, 0.008333333333333333);
+    p = fmaE(p, -z, 0.16666666666666666);
+    return fmaE(-r * z, p, r);
+}
+RAMD_ELEM double cos(double x) {
+    if (x != x || x == HUGE_VAL || x == -HUGE_VAL) return NAN;
+    const double nd = ::rint(x * 0.63661977236758138);  // 2 / pi
+    const double r = ((x - nd * kPio2A) - nd * kPio2B) - nd * kPio2C;
+    const int n = int(int64_t(nd) & 3);
+    if (n == 0) return cosReduced(r);
+    if (n == 1) return -sinReduced(r);
+    if (n == 2) return -cosReduced(r);
+    return sinReduced(r);
+}
+
+}  // namespace elem
+}  // namespace ramd
+#endif

@@ -19,6 +19,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
+#include "elementary.h"
 
 #define RHD __host__ __device__ inline
 
@@ -36,12 +37,14 @@ constexpr int kMaxRegions = 8;
 enum Status : int { kOk = 0, kBadTable, kBadInput, kBadComposition, kNoRoot, kUnavailable, kUnstable, kNonFinite };
 enum RootChoice : int { kAutoRoot = 0, kGasRoot = 1, kLiquidRoot = 2 };
 
-// ---- elementary functions (one place to choose the FP64 math library) ----
-RHD double m_log(double x) { return ::log(x); }
-RHD double m_exp(double x) { return ::exp(x); }
-RHD double m_pow(double x, double y) { return ::pow(x, y); }
-RHD double m_acos(double x) { return ::acos(x); }
-RHD double m_cos(double x) { return ::cos(x); }
+// ---- elementary functions: fixed operation sequences shared with the
+// oracle's twin build (elementary.h), so results are reproducible on the host;
+// square root is IEEE correctly rounded on both.
+RHD double m_log(double x) { return elem::log(x); }
+RHD double m_exp(double x) { return elem::exp(x); }
+RHD double m_third(double x) { return elem::powThird(x); }  // pow(x, 1.0 / 3.0)
+RHD double m_acos(double x) { return elem::acos(x); }
+RHD double m_cos(double x) { return elem::cos(x); }
 RHD double m_sqrt(double x) { return ::sqrt(x); }
 
 RHD bool fin(double x) { return x == x && (x < 0 ? -x : x) <= 1.7976931348623157e308; }
@@ -136,7 +139,7 @@ RHD int mixing(const Table<N>& t, double T, const double* x, Mix& m) {
 }
 
 RHD double cubic(double V, double bn, double cn, double dn) { return ((V + bn) * V + cn) * V + dn; }
-RHD double cbrtSigned(double x) { return x < 0 ? -m_pow(-x, 1.0 / 3.0) : m_pow(x, 1.0 / 3.0); }
+RHD double cbrtSigned(double x) { return x < 0 ? -m_third(-x) : m_third(x); }
 
 // Isolated third root in the near-repeated discriminant band (reactiveDevicePR.h:280-307).
 RHD bool isolatedRoot(double center, double delta, double delta2, double q, double bn, double cn, double dn,

@@ -40,6 +40,7 @@ __global__ __launch_bounds__(64) void stepFlashKernel(const HemModel<NS>* __rest
                                                       const uint32_t* __restrict__ order, uint32_t count,
                                                       const double* __restrict__ dtDev, ThermoOut* __restrict__ out,
                                                       double* __restrict__ pOut, double* __restrict__ cOut,
+                                                      uint16_t* __restrict__ cost,
                                                       unsigned long long* __restrict__ counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Counts n = {0, 0, 0, 0, 0, 0};
@@ -71,6 +72,7 @@ __global__ __launch_bounds__(64) void stepFlashKernel(const HemModel<NS>* __rest
             cOut[c] = g.soundFrozen;
             failed = 1;
         }
+        if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
     }
     const uint32_t sumFailed = waveSum(failed), sumPhases = waveSum(n.phases);
     if ((threadIdx.x & 63) == 0) {
@@ -79,10 +81,15 @@ __global__ __launch_bounds__(64) void stepFlashKernel(const HemModel<NS>* __rest
     }
 }
 
-// Heavy-first scheduling key (reactiveHemBucket.h:13-18) and a counting
-// sort: histogram, exclusive scan from the heaviest bucket down, scatter.
+// Heavy-first scheduling and a counting sort: histogram, exclusive scan from
+// the heaviest bucket down, scatter.  Without a cost history the key is the
+// reference's bucket (reactiveHemBucket.h:13-18): liquids, present species,
+// dominant species.  With one it is the cell's phase-evaluation count in its
+// latest flash, in bins of 8, so the 64 cells of a wave carry near-equal work
+// (the order decides nothing numerical: every cell's flash is independent).
 __global__ __launch_bounds__(256) void bucketHistogramKernel(const double* __restrict__ q, size_t stride, int ns,
-                                                             const ThermoOut* __restrict__ seeds, size_t cells,
+                                                             const ThermoOut* __restrict__ seeds,
+                                                             const uint16_t* __restrict__ cost, size_t cells,
                                                              uint8_t* __restrict__ key,
                                                              unsigned* __restrict__ count) {
     __shared__ unsigned hist[256];
@@ -90,16 +97,22 @@ __global__ __launch_bounds__(256) void bucketHistogramKernel(const double* __res
     __syncthreads();
     const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c < cells) {
-        unsigned present = 0, dominant = 0;
-        double best = q[c];
-        for (int k = 0; k < ns; ++k) {
-            const double v = q[size_t(k) * stride + c];
-            if (v > 0) ++present;
-            if (v > best) { best = v; dominant = unsigned(k); }
+        unsigned k;
+        if (cost) {
+            const unsigned bin = unsigned(cost[c]) >> 3;
+            k = bin > 255u ? 255u : bin;
+        } else {
+            unsigned present = 0, dominant = 0;
+            double best = q[c];
+            for (int s = 0; s < ns; ++s) {
+                const double v = q[size_t(s) * stride + c];
+                if (v > 0) ++present;
+                if (v > best) { best = v; dominant = unsigned(s); }
+            }
+            const int al = seeds[c].activeLiquids;
+            const unsigned liquids = al < 0 ? 0u : (al > 3 ? 3u : unsigned(al));
+            k = (liquids << 6) | ((present > 7 ? 7u : present) << 3) | (dominant > 7 ? 7u : dominant);
         }
-        const int al = seeds[c].activeLiquids;
-        const unsigned liquids = al < 0 ? 0u : (al > 3 ? 3u : unsigned(al));
-        const unsigned k = (liquids << 6) | ((present > 7 ? 7u : present) << 3) | (dominant > 7 ? 7u : dominant);
         key[c] = uint8_t(k);
         atomicAdd(&hist[k], 1u);
     }
@@ -117,12 +130,25 @@ __global__ void bucketScanKernel(unsigned* __restrict__ count, unsigned* __restr
     }
 }
 
+// Each block ranks its cells per bucket in LDS and reserves one run per
+// bucket with a single global atomic, so no bucket counter is hit by every
+// cell of the grid.
 __global__ __launch_bounds__(256) void bucketScatterKernel(const uint8_t* __restrict__ key, size_t cells,
                                                            unsigned* __restrict__ offset,
                                                            uint32_t* __restrict__ order) {
+    __shared__ unsigned local[256], base[256];
+    local[threadIdx.x] = 0;
+    __syncthreads();
     const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c >= cells) return;
-    order[atomicAdd(&offset[key[c]], 1u)] = uint32_t(c);
+    unsigned k = 0, rank = 0;
+    if (c < cells) {
+        k = key[c];
+        rank = atomicAdd(&local[k], 1u);
+    }
+    __syncthreads();
+    if (local[threadIdx.x]) base[threadIdx.x] = atomicAdd(&offset[threadIdx.x], local[threadIdx.x]);
+    __syncthreads();
+    if (c < cells) order[base[k] + rank] = uint32_t(c);
 }
 
 // Neumaier-compensated sum of q_k V over the cells: per-block partials in a
@@ -219,6 +245,8 @@ struct Step {
     ThermoOut *sA = nullptr, *sB = nullptr;
     uint8_t* key = nullptr;
     uint32_t* order = nullptr;
+    uint16_t* cost = nullptr;  // phase evaluations of each cell in its latest flash
+    bool haveCost = false;
     unsigned *bucketCount = nullptr, *bucketOffset = nullptr;
     double* blockMin = nullptr;
     double* scalars = nullptr;  // dt, recheck
@@ -246,12 +274,13 @@ struct Step {
         return static_cast<T*>(p);
     }
 
-    void flash(const double* q, double* pOut, double* cOut, ThermoOut* out, unsigned long long* counter) {
+    void flash(const double* q, double* pOut, double* cOut, ThermoOut* out, uint16_t* costOut,
+               unsigned long long* counter) {
         const unsigned blocks = blocksFor(cells, 64);
 #define RAMD_STEP_FLASH(NS_, NL_)                                                                                 \
     hipLaunchKernelGGL((stepFlashKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                             \
                        static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), scalars, out, \
-                       pOut, cOut, counter)
+                       pOut, cOut, costOut, counter)
         if (h->ns == 1 && h->nl == 0) RAMD_STEP_FLASH(1, 0);
         else if (h->ns == 1) RAMD_STEP_FLASH(1, 1);
         else if (h->nl == 0) RAMD_STEP_FLASH(2, 0);
@@ -274,7 +303,7 @@ struct Step {
         mark(m, 1);
         // heavy-first order from the step-start seeds
         hipLaunchKernelGGL(bucketHistogramKernel, dim3(blocksFor(cells, 256)), dim3(256), 0, stream, qA, all, ns, sA,
-                           cells, key, bucketCount);
+                           haveCost ? cost : nullptr, cells, key, bucketCount);
         hipLaunchKernelGGL(bucketScanKernel, dim3(1), dim3(64), 0, stream, bucketCount, bucketOffset);
         hipLaunchKernelGGL(bucketScatterKernel, dim3(blocksFor(cells, 256)), dim3(256), 0, stream, key, cells,
                            bucketOffset, order);
@@ -284,7 +313,7 @@ struct Step {
         launchCartesianStage(a, ns, nv, qA, all, pA, cA, 0.0, scalars, 0, qB, stream);
         mark(m, 3);
         // flash #1 seeded from the step-start states
-        flash(qB, pB, cB, sB, counters);
+        flash(qB, pB, cB, sB, nullptr, counters);
         mark(m, 4);
         // CFL recheck after stage 0 (maximum = dt)
         launchCartesianCfl(a, ns, qB, all, pB, cB, cfl, 0.0, scalars, blockMin, stream);
@@ -297,7 +326,8 @@ struct Step {
         launchCartesianStage(a, ns, nv, qB, all, pB, cB, 0.0, scalars, 1, qA, stream);
         mark(m, 7);
         // flash #2 from the step-start seeds, into the state buffer of stage 1
-        flash(qA, pA, cA, sB, counters + 2);
+        flash(qA, pA, cA, sB, cost, counters + 2);
+        haveCost = true;
         mark(m, 8);
         std::swap(sA, sB);
         // conservation totals and boundary integral
@@ -361,6 +391,7 @@ void* reactive_amd_step_create(void* transport, void* hem, char* error, size_t e
         s->sB = s->alloc<ThermoOut>(s->cells);
         s->key = s->alloc<uint8_t>(s->cells);
         s->order = s->alloc<uint32_t>(s->cells);
+        s->cost = s->alloc<uint16_t>(s->cells);
         s->bucketCount = s->alloc<unsigned>(256);
         s->bucketOffset = s->alloc<unsigned>(256);
         s->blockMin = s->alloc<double>((s->cells + 255) / 256 + 1);
@@ -397,6 +428,7 @@ int reactive_amd_step_load(void* step, const double* q, const ReactiveThermoStat
         RAMD_HIP(hipStreamSynchronize(s.stream));
         RAMD_HIP(hipFree(staging));
         s.loaded = true;
+        s.haveCost = false;  // new states: schedule by the reference bucket until a flash has run
     });
 }
 

@@ -1,7 +1,12 @@
 """Test helpers: package path, oracle bindings, meshes and states.
 
-The oracle (oracle/_build/liboracle.so) is the checker only; the product is
-always called through its C ABI (reactive_amd.lib)."""
+The oracle is the checker only; the product is always called through its C
+ABI (reactive_amd.lib).  Two builds of the same restatement exist:
+oracle/_build/liboracle.so calls glibc's elementary functions, as the
+reference's host build does, and equals the compiled reference bit for bit;
+oracle/_build/liboracle_twin.so calls the elementary functions the device
+uses (spuma-multiphysics_amd/csrc/elementary.h), so the device flash can be
+compared with it bit for bit."""
 import ctypes as C
 import os
 import subprocess
@@ -15,18 +20,19 @@ sys.path.insert(0, os.path.join(ROOT, "spuma-multiphysics_amd"))
 from reactive_amd import abi, cases, model  # noqa: E402
 
 ORACLE_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+TWIN_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle_twin.so")
 REF_PATH = os.path.join(ROOT, "oracle", "_ref", "libref.so")
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
-_ORACLE = None
+_ORACLE = {}
 
 
-def oracle():
-    global _ORACLE
-    if _ORACLE is None:
-        if not os.path.exists(ORACLE_PATH):
+def oracle(twin=False):
+    if twin not in _ORACLE:
+        path = TWIN_PATH if twin else ORACLE_PATH
+        if not os.path.exists(path):
             subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
-        lib = C.CDLL(ORACLE_PATH)
+        lib = C.CDLL(path)
         V = C.c_void_p
         lib.oracle_transport_create.restype = V
         lib.oracle_transport_create.argtypes = [V, V, V, V, V, C.c_char_p, C.c_size_t]
@@ -46,17 +52,19 @@ def oracle():
         lib.oracle_pr_roots.argtypes = [V, C.c_int, C.c_double, C.c_double, V, V]
         lib.oracle_solve.argtypes = [V, V, C.c_int, V]
         lib.oracle_nasa.argtypes = [V, C.c_int, C.c_int, C.c_double, V]
-        _ORACLE = lib
-    return _ORACLE
+        lib.oracle_elementary.argtypes = [C.c_int, C.c_size_t, V, V]
+        _ORACLE[twin] = lib
+    return _ORACLE[twin]
 
 
 def p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
-def oracle_flash(model_bytes, q_species, energy, seeds, threads=8):
-    """Batch UV flash on the oracle: (states, success, counters[9])."""
-    lib = oracle()
+def oracle_flash(model_bytes, q_species, energy, seeds, threads=8, twin=False):
+    """Batch UV flash on the oracle: (states, success, counters[9]); twin
+    selects the build with the device's elementary functions."""
+    lib = oracle(twin)
     n = len(energy)
     states = seeds.copy()
     success = np.zeros(n, np.int32)
