@@ -40,10 +40,25 @@ typedef struct ReactiveAmdStepReport {
     double stageSeconds[2];             /* fused transport stage 0 / stage 1 kernels */
     double flashSeconds[2];             /* flash #1 / flash #2 kernels */
     double cflSeconds;                  /* both CFL sweeps (+ their final minimum) */
+    uint64_t fluxFailures;              /* face evaluations the HLLC flux rejected (0 under HLL) */
 } ReactiveAmdStepReport;
 
 /* Flash arithmetic of a step (0 = plain FP64, the reference's arithmetic). */
 #define REACTIVE_AMD_FLASH_FP64 0
+
+/* Face flux of a transport handle (applies to reactive_transport_rhs/stage,
+ * the resident stages and the resident step): REACTIVE_AMD_FLUX_HLL (the
+ * reference's default, reactiveTransportKernels.h:343-356) or
+ * REACTIVE_AMD_FLUX_HLLC, the reference's contact-preserving flux
+ * (src/reactiveInterface/reactiveBalancedCapillary.h:55-142) with zero surface
+ * tension, which only its capillary model reaches
+ * (reactiveTransportKernels.h:358-417).  A face the HLLC flux rejects fails
+ * the stage ("HLLC face flux failed"); the resident step counts it in
+ * ReactiveAmdStepReport.fluxFailures.  Returns 0, 1 (unknown scheme), 2
+ * (concurrent entry) or -1 (null handle). */
+#define REACTIVE_AMD_FLUX_HLL 0
+#define REACTIVE_AMD_FLUX_HLLC 1
+int reactive_amd_transport_set_flux(void* transport, int scheme);
 
 void* reactive_amd_step_create(void* transport, void* hem, char* error, size_t errorSize);
 /* Host cell-major q (cells x variables) and the step-start thermo states. */

@@ -6,6 +6,11 @@
 // (src/reactiveTransport/reactiveTransportKernels.h, reactiveTransport.cpp).
 // Scope: viscosity = conductivity = diffusivity = 0, no mechanical closure,
 // no WALE or capillary; those configurations are rejected at create.
+// Face flux: HLL (reactiveTransportKernels.h:343-356), or the HLLC option
+// (scheme 1): the capillary branch's contact-preserving flux
+// (reactiveBalancedCapillary.h:55-142) with sigma = curvature = 0 and no
+// surface stress, whose coefficients replace the HLL ones
+// (reactiveTransportKernels.h:358-417).
 #include "oracle.h"
 #include <cmath>
 #include <cstdio>
@@ -24,6 +29,83 @@ void require(bool ok, const char* message) { if (!ok) throw std::runtime_error(m
 struct Prim { double rho, u[3]; };
 struct Work { double advectL, advectR, pressureMomentum, pressureEnergy; };
 
+inline bool finite(double x) { return x == x && x <= 1.7976931348623157e308 && x >= -1.7976931348623157e308; }
+
+// HLLC coefficients for one face (reactiveBalancedCapillary.h:55-142, sigma =
+// curvature = colorFace = 0, zero surface stress, so pi = p).  Inputs: side
+// densities, velocities, pressures, wave speeds a = waveFactor c, total
+// energies and the unit normal.  False where the reference's faceFlux fails.
+bool hllcFace(double rhoL, const double* uL, double pL, double aL, double EL, double rhoR, const double* uR,
+              double pR, double aR, double ER, const double* n, Work& w) {
+    // State validity (:40-46; color = 0)
+    if (!finite(rhoL) || rhoL <= 0 || !finite(pL) || pL <= 0 || !finite(aL) || aL <= 0 || !finite(EL)) return false;
+    if (!finite(rhoR) || rhoR <= 0 || !finite(pR) || pR <= 0 || !finite(aR) || aR <= 0 || !finite(ER)) return false;
+    for (int d = 0; d < 3; ++d)
+        if (!finite(uL[d]) || !finite(uR[d])) return false;
+    double n2 = 0, unL = 0, unR = 0;
+    for (int d = 0; d < 3; ++d) {
+        if (!finite(n[d])) return false;
+        n2 += n[d] * n[d];
+        unL += uL[d] * n[d];
+        unR += uR[d] * n[d];
+    }
+    if (!finite(n2) || std::fabs(n2 - 1) > 1e-10) return false;
+    const double sl = mn(0, mn(unL - aL, unR - aR)), sr = mx(0, mx(unL + aL, unR + aR));
+    const double den = rhoL * (sl - unL) - rhoR * (sr - unR);
+    if (!finite(den) || den == 0 || !finite(sl) || !finite(sr) || sr <= sl) return false;
+    const double star = (pR - pL + rhoL * unL * (sl - unL) - rhoR * unR * (sr - unR)) / den;
+    Work out{0, 0, 0, 0};
+    double mass, contact;
+    if (!finite(star) || star <= sl || star >= sr) {  // HLL fallback (:79-89)
+        const double inv = 1 / (sr - sl);
+        contact = (sr * unL - sl * unR) * inv;
+        out.advectL = sr * (unL - sl) * inv;
+        out.advectR = sl * (sr - unR) * inv;
+        mass = out.advectL * rhoL + out.advectR * rhoR;
+        out.pressureMomentum = (sr * pL - sl * pR) * inv;
+        out.pressureEnergy = (sr * pL * unL - sl * pR * unR) * inv;
+    } else {
+        const bool useLeft = star >= 0;
+        const double rho = useLeft ? rhoL : rhoR, un = useLeft ? unL : unR, wave = useLeft ? sl : sr;
+        const double pi = useLeft ? pL : pR, E = useLeft ? EL : ER;
+        const bool supersonic = (useLeft && sl >= 0) || (!useLeft && sr <= 0);
+        contact = supersonic ? un : star;
+        if (supersonic) {  // :96-101
+            mass = rho * un;
+            out.pressureMomentum = pi;
+            out.pressureEnergy = pi * un;
+        } else {  // star state (:103-110)
+            const double rhoStar = rho * (wave - un) / (wave - star);
+            const double piStar = pi + rho * (wave - un) * (star - un);
+            const double energyStar = ((wave - un) * E - pi * un + piStar * star) / (wave - star);
+            mass = rhoStar * star;
+            out.pressureMomentum = piStar + mass * (star - un);
+            out.pressureEnergy = star * (energyStar + piStar) - mass * E / rho;
+        }
+        if (useLeft) out.advectL = mass / rhoL;
+        else out.advectR = mass / rhoR;
+    }
+    // finiteness of the assembled flux (:114-140): zero capillary terms
+    const double advective = out.advectL * EL + out.advectR * ER;
+    double faceU[3], average = 0, capillaryEnergy = 0;
+    for (int d = 0; d < 3; ++d) {
+        faceU[d] = 0.5 * (uL[d] + uR[d]);
+        average += faceU[d] * n[d];
+    }
+    for (int d = 0; d < 3; ++d) faceU[d] += (contact - average) * n[d];
+    for (int d = 0; d < 3; ++d) capillaryEnergy += faceU[d] * 0.0;
+    const double total = advective + out.pressureEnergy + capillaryEnergy;
+    if (!finite(mass) || !finite(out.pressureMomentum) || !finite(out.pressureEnergy) || !finite(capillaryEnergy) ||
+        !finite(total))
+        return false;
+    for (int d = 0; d < 3; ++d) {
+        const double momentum = out.advectL * rhoL * uL[d] + out.advectR * rhoR * uR[d] + out.pressureMomentum * n[d] + 0.0;
+        if (!finite(momentum)) return false;
+    }
+    w = out;
+    return true;
+}
+
 struct Transport {
     Config cfg;
     std::vector<Face> faces;
@@ -36,6 +118,7 @@ struct Transport {
     std::vector<Work> work;
     std::vector<double> faceSpeed, step;
     size_t partitions = 1;
+    int scheme = 0;  // 0 HLL, 1 HLLC
     bool haveInitial = false;
     double stageDt = 0;
     std::string error;
@@ -158,6 +241,12 @@ struct Transport {
             w.advectR = left * (right - unR) * invWave;
             w.pressureMomentum = (right * sl.p - left * sr.p) * invWave;
             w.pressureEnergy = (right * sl.p * unL - left * sr.p * unR) * invWave;
+            if (scheme == 1) {
+                const size_t e = cfg.species + 3;
+                require(hllcFace(sl.rho, ul, sl.p, aL, q[qi(f.owner, e)], sr.rho, ur, sr.p, aR, rightValue(f, e),
+                                 f.normal, w),
+                        "HLLC face flux failed");
+            }
             work[fi] = w;
         }
     }
@@ -324,6 +413,22 @@ int oracle_transport_stable_step(void* h, const double* q, const TState* states,
         t.faceSpeeds();
         *dt = t.minimumStep(cfl, maximum);
     });
+}
+int oracle_transport_set_scheme(void* h, int scheme) {
+    return guard(h, [&](Transport& t) {
+        require(scheme == 0 || scheme == 1, "Unknown face flux scheme");
+        t.scheme = scheme;
+    });
+}
+// One HLLC face: left/right = {rho, u0, u1, u2, p, a, E}; out = {advectL,
+// advectR, pressureMomentum, pressureEnergy}.  Returns 1, or 0 on failure.
+int oracle_hllc_face(const double* left, const double* right, const double* normal, double* out) {
+    Work w{};
+    if (!hllcFace(left[0], left + 1, left[4], left[5], left[6], right[0], right + 1, right[4], right[5], right[6],
+                  normal, w))
+        return 0;
+    out[0] = w.advectL; out[1] = w.advectR; out[2] = w.pressureMomentum; out[3] = w.pressureEnergy;
+    return 1;
 }
 // Conservation totals sum_c q_ck V_c in serial cell order (ReactiveFoam.C:953-958).
 void oracle_totals(size_t cells, size_t variables, const double* q, const double* volume, double* out) {

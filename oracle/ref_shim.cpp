@@ -5,6 +5,7 @@
 // translation unit is built into oracle/_ref/libref.so by oracle/Makefile
 // (target `ref`) and is never linked into the product library.
 #include "reactiveDeviceFlash.h"
+#include "reactiveBalancedCapillary.h"
 #include <cstring>
 #include <cstdint>
 
@@ -100,6 +101,27 @@ int ref_pr_roots(const void* model, int phase, double T, double p, const double*
     out[0] = r.volume[0]; out[1] = r.volume[1]; out[2] = r.volume[2];
     out[3] = r.count; out[4] = r.signedCount; out[5] = r.gasAvailable; out[6] = r.liquidAvailable;
     return 0;
+}
+
+
+// The reference's contact-preserving face flux
+// (reactiveBalancedCapillary.h:55-142) with sigma = curvature = colorFace = 0
+// and no surface stress.  left/right = {rho, u0, u1, u2, p, a, E}; out =
+// {advectLeft, advectRight, pressureMomentum, pressureEnergy}.
+int ref_hllc_face(const double* left, const double* right, const double* normal, double* out) {
+    ReactiveBalancedCapillary::State l{}, r{};
+    l.rho = left[0]; r.rho = right[0];
+    for (int d = 0; d < 3; ++d) { l.velocity[d] = left[1 + d]; r.velocity[d] = right[1 + d]; }
+    l.pressure = left[4]; r.pressure = right[4];
+    l.sound = left[5]; r.sound = right[5];
+    l.totalEnergy = left[6]; r.totalEnergy = right[6];
+    ReactiveBalancedCapillary::Face f{};
+    for (int d = 0; d < 3; ++d) f.normal[d] = normal[d];
+    ReactiveBalancedCapillary::Flux flux{};
+    if (!ReactiveBalancedCapillary::faceFlux(l, r, f, flux)) return 0;
+    out[0] = flux.advectLeft; out[1] = flux.advectRight;
+    out[2] = flux.pressureMomentum; out[3] = flux.pressureEnergy;
+    return 1;
 }
 
 }

@@ -141,20 +141,23 @@ CartesianInfo Transport::detectCartesian(const ReactiveTransportFace* f, const d
 }
 
 __global__ void extractStateKernel(const ReactiveTransportState* __restrict__ s, size_t n, double* __restrict__ p,
-                                   double* __restrict__ c) {
+                                   double* __restrict__ c, double* __restrict__ r) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     p[i] = s[i].p;
     c[i] = s[i].sound;
+    if (r) r[i] = s[i].rho;
 }
 
 void Transport::cartesianAdvance(double dt, int stage) {
+    if (scheme == kHllc && !density) density = allocate<double>(all);
     hipLaunchKernelGGL(extractStateKernel, dim3(unsigned((cfg.cells + 255) / 256)), dim3(256), 0, stream, state,
-                       cfg.cells, pressure, sound);
+                       cfg.cells, pressure, sound, scheme == kHllc ? density : nullptr);
     RAMD_HIP(hipGetLastError());
     launched();
     CartesianArgs a = makeCartesianArgs(cartesian, cfg.waveFactor);
-    launchCartesianStage(a, ns, nv, q, all, pressure, sound, dt, nullptr, stage, initial, stream);
+    launchCartesianStage(a, ns, nv, scheme, q, all, pressure, sound, density, dt, nullptr, stage, initial,
+                         fluxFailures, stream);
     launched();
 }
 

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include "transport_internal.h"
+#include "flux_device.h"
 
 namespace ramd {
 
@@ -45,12 +46,17 @@ inline unsigned xcdGrid(long logicalBlocks) {
     return unsigned(per * kXcds);
 }
 
-template <int NS, int NV>
+// HLLC = true uses the HLLC option (flux_device.h): rArr holds the recovered
+// densities, and a face the HLLC flux rejects keeps its HLL coefficients and
+// is counted in failures (each face is seen by both its cells).
+template <int NS, int NV, bool HLLC>
 __global__ __launch_bounds__(256) void cartesianStageKernel(CartesianArgs a, const double* __restrict__ q,
                                                             size_t stride, const double* __restrict__ pArr,
-                                                            const double* __restrict__ cArr, double dtHost,
+                                                            const double* __restrict__ cArr,
+                                                            const double* __restrict__ rArr, double dtHost,
                                                             const double* __restrict__ dtDev, int mode,
-                                                            double* __restrict__ out) {
+                                                            double* __restrict__ out,
+                                                            unsigned long long* __restrict__ failures) {
     const long cells = long(a.nx) * a.ny * a.nz;
     const long blocks = (cells + 255) / 256;
     const long b = xcdBlock(blockIdx.x, blocks);
@@ -66,6 +72,8 @@ __global__ __launch_bounds__(256) void cartesianStageKernel(CartesianArgs a, con
     double qc[NV];
     for (int v = 0; v < NV; ++v) qc[v] = q[size_t(v) * stride + c];
     const double pc = pArr[c], cc = cArr[c];
+    const double rc = HLLC ? rArr[c] : 0.0;
+    uint32_t rejected = 0;
     double rate[NV];
     for (int v = 0; v < NV; ++v) rate[v] = 0;
     for (int pos = 0; pos < 6; ++pos) {
@@ -76,7 +84,7 @@ __global__ __launch_bounds__(256) void cartesianStageKernel(CartesianArgs a, con
         const bool bnd = plus ? coord == ext - 1 : coord == 0;
         const long step = ax == 0 ? 1 : (ax == 1 ? long(a.nx) : nxy);
         const double* n = a.normal[slot][bnd ? 1 : 0];
-        double qn[NV], pn = pc, cn = cc;
+        double qn[NV], pn = pc, cn = cc, rn = rc;
         if (bnd) {
             for (int v = 0; v < NV; ++v) qn[v] = qc[v];
         } else {
@@ -84,6 +92,7 @@ __global__ __launch_bounds__(256) void cartesianStageKernel(CartesianArgs a, con
             for (int v = 0; v < NV; ++v) qn[v] = q[size_t(v) * stride + nb];
             pn = pArr[nb];
             cn = cArr[nb];
+            if (HLLC) rn = rArr[nb];
         }
         // owner (L) and right (R) of this face
         const bool selfLeft = bnd || plus;
@@ -105,10 +114,22 @@ __global__ __launch_bounds__(256) void cartesianStageKernel(CartesianArgs a, con
         const double right = unL + aL > unR + aR ? unL + aL : unR + aR;
         const double wl = 0 < left ? 0 : left, wr = 0 > right ? 0 : right;
         const double invWave = 1 / (wr - wl);
-        const double advectL = wr * (unL - wl) * invWave;
-        const double advectR = wl * (wr - unR) * invWave;
-        const double pM = (wr * pL - wl * pR) * invWave;
-        const double pE = (wr * pL * unL - wl * pR * unR) * invWave;
+        double4 w;
+        w.x = wr * (unL - wl) * invWave;
+        w.y = wl * (wr - unR) * invWave;
+        w.z = (wr * pL - wl * pR) * invWave;
+        w.w = (wr * pL * unL - wl * pR * unR) * invWave;
+        if (HLLC) {
+            double uL[3], uR[3];
+            for (int d = 0; d < 3; ++d) {
+                uL[d] = qL[NS + d] / rhoL;
+                uR[d] = qR[NS + d] / rhoR;
+            }
+            if (!hllcCoefficients(selfLeft ? rc : rn, uL, pL, aL, qL[NS + 3], selfLeft ? rn : rc, uR, pR, aR,
+                                  qR[NS + 3], n, w))
+                ++rejected;
+        }
+        const double advectL = w.x, advectR = w.y, pM = w.z, pE = w.w;
         const double sign = selfLeft ? -1.0 : 1.0;
         for (int v = 0; v < NV; ++v) {
             double flux = advectL * qL[v] + advectR * qR[v];
@@ -125,6 +146,7 @@ __global__ __launch_bounds__(256) void cartesianStageKernel(CartesianArgs a, con
         if (mode == 0) out[idx] = qc[v] + dt * rate[v];
         else out[idx] = .5 * out[idx] + .5 * (qc[v] + dt * rate[v]);
     }
+    if (HLLC && rejected) atomicAdd(failures, (unsigned long long)rejected);
 }
 
 // FaceSpeeds + Step + the block level of Minimum (reactiveTransportKernels.h:
@@ -232,18 +254,22 @@ inline bool cartesianSupported(int ns, int nv) {
     return (ns == 1 && (nv == 5 || nv == 6)) || (ns == 2 && (nv == 6 || nv == 7));
 }
 
-inline void launchCartesianStage(const CartesianArgs& a, int ns, int nv, const double* q, size_t stride,
-                                 const double* p, const double* c, double dt, const double* dtDev, int mode,
-                                 double* out, hipStream_t stream) {
+inline void launchCartesianStage(const CartesianArgs& a, int ns, int nv, int scheme, const double* q, size_t stride,
+                                 const double* p, const double* c, const double* r, double dt, const double* dtDev,
+                                 int mode, double* out, unsigned long long* failures, hipStream_t stream) {
     const long cells = long(a.nx) * a.ny * a.nz;
     const unsigned grid = xcdGrid((cells + 255) / 256);
-#define RAMD_CART(NS_, NV_)                                                                                         \
-    hipLaunchKernelGGL((cartesianStageKernel<NS_, NV_>), dim3(grid), dim3(256), 0, stream, a, q, stride, p, c, dt, \
-                       dtDev, mode, out)
-    if (ns == 1 && nv == 5) RAMD_CART(1, 5);
-    else if (ns == 1 && nv == 6) RAMD_CART(1, 6);
-    else if (ns == 2 && nv == 6) RAMD_CART(2, 6);
-    else RAMD_CART(2, 7);
+#define RAMD_CART(NS_, NV_, H_)                                                                                   \
+    hipLaunchKernelGGL((cartesianStageKernel<NS_, NV_, H_>), dim3(grid), dim3(256), 0, stream, a, q, stride, p, c, \
+                       r, dt, dtDev, mode, out, failures)
+#define RAMD_CART_SCHEME(NS_, NV_)          \
+    if (scheme == kHllc) RAMD_CART(NS_, NV_, true); \
+    else RAMD_CART(NS_, NV_, false)
+    if (ns == 1 && nv == 5) { RAMD_CART_SCHEME(1, 5); }
+    else if (ns == 1 && nv == 6) { RAMD_CART_SCHEME(1, 6); }
+    else if (ns == 2 && nv == 6) { RAMD_CART_SCHEME(2, 6); }
+    else { RAMD_CART_SCHEME(2, 7); }
+#undef RAMD_CART_SCHEME
 #undef RAMD_CART
 }
 

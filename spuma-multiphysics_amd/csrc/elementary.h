@@ -15,11 +15,11 @@
 //   exp   x = k ln2 + r, |r| <= ln2/2 (two-part ln2, r kept as a pair);
 //         exp r = 1 + r + r^2/2 + r^3 P(r), the first three terms summed
 //         with their rounding errors
-//   cbrt  Newton on c^3 = x from an exponent-third estimate; the last
-//         correction, from the cube's residual with the square's rounding
-//         error recovered by fma, is added in one rounding
+//   cbrt  exponent third and a quadratic mantissa fit, one Halley step, then
+//         a Newton correction from the cube's residual (the square's rounding
+//         error recovered by fma) added in one rounding
 //   powThird  x^fl(1/3): fl(1/3) = 1/3 - 1/(3 2^54), so
-//         x^fl(1/3) = cbrt(x) (1 - log(x) / (3 2^54)) to within 2^-100
+//         x^fl(1/3) = cbrt(x) (1 - ln(x) / (3 2^54)) to within 2^-100
 //   acos  pi/2 - asin x for |x| <= 1/2; 2 asin(sqrt((1-|x|)/2)) beyond, asin
 //         by its Taylor series on |t| <= 1/2
 //   cos   x = n pi/2 + r (three-part pi/2), |r| <= pi/4, Taylor sin/cos of r;
@@ -140,32 +140,47 @@ RAMD_ELEM double exp(double x) {
     return scale(s2 + ((e1 + e2) + tail), k);
 }
 
-// cube root of a > 0 as c + corr (c from Newton, corr its last correction)
+// cube root of a > 0 as c + corr (c from Halley, corr its Newton correction)
 RAMD_ELEM void cbrtParts(double a, double& c, double& corr) {
     int adjust = 0;
-    if (a < 0x1p-1000) {  // keep the cube and the squares normal
-        a *= 0x1p300;
-        adjust = -100;
-    } else if (a > 0x1p1000) {
-        a *= 0x1p-300;
-        adjust = 100;
+    if (a < 0x1p-600) {  // keep a normal and every cube, square and error term in range
+        a *= 0x1p900;
+        adjust = -300;
+    } else if (a > 0x1p600) {
+        a *= 0x1p-900;
+        adjust = 300;
     }
     const int e = int((bits(a) >> 52) & 0x7ff) - 1023;
-    // 2^floor(e/3) times a linear estimate of the mantissa's cube root
-    // (about 6 bits), refined by Newton (quadratic)
+    // 2^floor(e/3) 2^(rem/3) times a quadratic fit of the mantissa's cube
+    // root on [1, 2) (relative error 1.2e-3)
     const int e3 = e >= 0 ? e / 3 : -((-e + 2) / 3);
     const int rem = e - 3 * e3;  // 0, 1, 2
     const double mant = fromBits((bits(a) & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
-    c = (rem == 0 ? 1.0 : rem == 1 ? 1.2599210498948732 : 1.5874010519681994) * (0.75 + 0.25 * mant);
-    c = scale(c, e3);
-    for (int i = 0; i < 6; ++i) c = c - (c - a / (c * c)) / 3.0;
-    // last Newton correction from the residual c^3 - a, with the square's
-    // rounding error recovered by fma
+    const double fit = fmaE(fmaE(-0.05930307106611154, mant, 0.4355650806935765), mant, 0.6249076548324355);
+    c = scale((rem == 0 ? 1.0 : rem == 1 ? 1.2599210498948732 : 1.5874010519681994) * fit, e3);
+    // one Halley step (cubic convergence: 1.2e-3 -> ~2e-9)
+    const double c3 = c * c * c;
+    c = c * ((c3 + 2.0 * a) / (2.0 * c3 + a));
+    // Newton correction from the residual c^3 - a, with the square's rounding
+    // error recovered by fma (quadratic: ~2e-9 -> ~4e-18)
     const double c2 = c * c, c2e = fmaE(c, c, -c2);
     const double residual = fmaE(c2, c, -a) + c2e * c;
     corr = -residual / (3.0 * c2);
     c = scale(c, adjust);
     corr = scale(corr, adjust);
+}
+
+// ln x to about 0.007, enough for the size of the 1/3-exponent correction
+RAMD_ELEM double logRough(double x) {
+    int shift = 0;
+    if (x < 0x1p-1022) {  // subnormal
+        x *= 0x1p54;
+        shift = 54;
+    }
+    const uint64_t u = bits(x);
+    const double t = fromBits((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull) - 1.0;
+    return double(int((u >> 52) & 0x7ff) - 1023 - shift) * 0.6931471805599453 +
+           t * fmaE(-0.25436922244455573, t, 0.940777422303657);
 }
 
 RAMD_ELEM double cbrt(double x) {
@@ -182,7 +197,10 @@ RAMD_ELEM double powThird(double x) {
     if (!(x > 0)) return x == 0 ? 0.0 : NAN;
     double c, corr;
     cbrtParts(x, c, corr);
-    return c + (corr - c * (log(x) * (1.0 / 54043195528445952.0)));  // 3 * 2^54
+    // x^(1/3 - d) = cbrt(x) (1 - d ln x) with d = 1/(3 2^54): |d ln x| < 2e-14,
+    // so ln x needs two digits (logRough; subnormal x only loses accuracy in
+    // a term far below the last place)
+    return c + (corr - c * (logRough(x) * (1.0 / 54043195528445952.0)));
 }
 
 RAMD_ELEM double asinSmall(double t) {  // |t| <= 1/2

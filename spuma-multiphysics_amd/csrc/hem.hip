@@ -26,23 +26,29 @@
 namespace ramd {
 
 static_assert(sizeof(ThermoOut) == sizeof(ReactiveThermoState), "state layout");
+static_assert(sizeof(HemCapillaryDev) == sizeof(ReactiveGpuHemCapillaryInputV2) &&
+                  offsetof(HemCapillaryDev, equilibrium) == offsetof(ReactiveGpuHemCapillaryInputV2, equilibrium),
+              "capillary input layout");
 static_assert(sizeof(HemOutputDev) == sizeof(ReactiveGpuHemOutputV1) &&
                   offsetof(HemOutputDev, success) == offsetof(ReactiveGpuHemOutputV1, success),
               "output layout");
 
-// mode 0: equilibrium UV flash (recover, reactiveDeviceFlash.h:564-567)
-// mode 1: frozen inventory at the seed's liquid masses (recoverCapillary with
-//         equilibrium = 0 and no interface, reactiveDeviceFlash.h:572-590)
+// Without capillary input: equilibrium UV flash (recover,
+// reactiveDeviceFlash.h:564-567).  With it (validated to color = jump = 0,
+// equilibrium = 0): frozen inventory at the seed's liquid masses
+// (recoverCapillary, reactiveDeviceFlash.h:572-590).
 template <int NS, int NL>
 __global__ __launch_bounds__(64) void hemFlashKernel(const HemModel<NS>* __restrict__ model,
                                                      const double* __restrict__ q, const double* __restrict__ energy,
                                                      const ThermoOut* __restrict__ guess,
                                                      const uint32_t* __restrict__ order, uint32_t count,
-                                                     const int* __restrict__ mode, HemOutputDev* __restrict__ out,
+                                                     const HemCapillaryDev* __restrict__ capillary,
+                                                     HemOutputDev* __restrict__ out,
                                                      unsigned long long* __restrict__ totals,
                                                      uint32_t* __restrict__ cellPhases) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Counts n = {0, 0, 0, 0, 0, 0};
+    uint32_t succeeded = 0;
     if (i < count) {
         const uint32_t c = order ? order[i] : i;
         Flash<NS, NL> f(*model, n);
@@ -55,18 +61,20 @@ __global__ __launch_bounds__(64) void hemFlashKernel(const HemModel<NS>* __restr
         Eval result;
         zeroEval(result);
         bool ok;
-        if (mode && mode[c]) ok = f.frozen(NL ? g.liquidMass[0] : 0.0, result, 0, g.liquidMass[1]);
+        if (capillary) ok = f.frozen(NL ? g.liquidMass[0] : 0.0, result, 0, g.liquidMass[1]);
         else ok = f.run(result);
         HemOutputDev& o = out[c];
         if (ok) writeOut(result, o.state);
         else std::memset(&o.state, 0, sizeof(o.state));
         o.success = ok ? 1 : 0;
         o.reserved = 0;
+        succeeded = ok ? 1u : 0u;
         if (cellPhases) cellPhases[c] = n.phases;
     }
     if (totals) {
         const uint32_t s0 = waveSum(n.phases), s1 = waveSum(n.residuals), s2 = waveSum(n.candidates);
         const uint32_t s3 = waveSum(n.stable), s4 = waveSum(n.failures), s5 = waveSum(n.fd);
+        const uint32_t s6 = waveSum(succeeded);
         if ((threadIdx.x & 63) == 0) {
             atomicAdd(&totals[0], (unsigned long long)s0);
             atomicAdd(&totals[1], (unsigned long long)s1);
@@ -74,8 +82,24 @@ __global__ __launch_bounds__(64) void hemFlashKernel(const HemModel<NS>* __restr
             atomicAdd(&totals[3], (unsigned long long)s3);
             atomicAdd(&totals[4], (unsigned long long)s4);
             atomicAdd(&totals[5], (unsigned long long)s5);
+            atomicAdd(&totals[6], (unsigned long long)s6);
         }
     }
+}
+
+// Counts the ordered cells whose capillary input this build cannot run
+// (non-finite or out-of-range fields, an interface, or equilibrium = 1).
+__global__ __launch_bounds__(256) void capillaryCheckKernel(const HemCapillaryDev* __restrict__ capillary,
+                                                            const uint32_t* __restrict__ order, uint32_t count,
+                                                            unsigned long long* __restrict__ rejected) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t bad = 0;
+    if (i < count) {
+        const HemCapillaryDev& x = capillary[order[i]];
+        bad = !(fin(x.color) && x.color == 0 && fin(x.pressureJump) && x.pressureJump == 0 && x.equilibrium == 0);
+    }
+    const uint32_t sum = waveSum(bad);
+    if ((threadIdx.x & 63) == 0 && sum) atomicAdd(rejected, (unsigned long long)sum);
 }
 
 __global__ void zeroTotalsKernel(unsigned long long* totals, int n) {
@@ -123,13 +147,13 @@ void HemHandle::ensureHostPath() {
 }
 
 void HemHandle::launch(const double* q, const double* e, const ThermoOut* g, const uint32_t* order, size_t count,
-                       const int* mode, HemOutputDev* out, uint32_t* cellPhases, hipStream_t s) {
+                       const HemCapillaryDev* capillary, HemOutputDev* out, uint32_t* cellPhases, hipStream_t s) {
     if (!count) return;
     const unsigned blocks = unsigned((count + 63) / 64);
-    hipLaunchKernelGGL(zeroTotalsKernel, dim3(1), dim3(64), 0, s, totals, 6);
+    hipLaunchKernelGGL(zeroTotalsKernel, dim3(1), dim3(64), 0, s, totals, 7);
 #define RAMD_HEM_LAUNCH(NS_, NL_)                                                                                  \
     hipLaunchKernelGGL((hemFlashKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, s,                                   \
-                       static_cast<const HemModel<NS_>*>(model), q, e, g, order, uint32_t(count), mode, out, totals, \
+                       static_cast<const HemModel<NS_>*>(model), q, e, g, order, uint32_t(count), capillary, out, totals, \
                        cellPhases)
     if (ns == 1 && nl == 0) RAMD_HEM_LAUNCH(1, 0);
     else if (ns == 1 && nl == 1) RAMD_HEM_LAUNCH(1, 1);
@@ -184,9 +208,7 @@ int runHost(HemHandle* h, const double* q, const double* energy, const ReactiveG
                 profile->structBytes == sizeof(*profile),
             "Invalid HIP HEM batch arguments");
     require(count && count <= h->capacity, "HIP HEM batch exceeds capacity");
-    std::vector<int> mode;
     if (capillary) {
-        mode.resize(count);
         for (size_t c = 0; c < count; ++c) {
             const auto& x = capillary[c];
             require(fin(x.color) && x.color >= 0 && x.color <= 1 && fin(x.pressureJump) &&
@@ -196,12 +218,12 @@ int runHost(HemHandle* h, const double* q, const double* energy, const ReactiveG
                     "Capillary interfaces (color, pressureJump != 0) are not provided by this HEM build");
             require(x.equilibrium == 0,
                     "Capillary equilibrium cells need the boundary-recovery search, not provided by this HEM build");
-            mode[c] = 1;
         }
     }
     RAMD_HIP(hipSetDevice(h->device));
     h->ensureHostPath();
-    if (capillary && !h->dmode) h->dmode = static_cast<int*>(h->allocate(h->capacity * sizeof(int)));
+    if (capillary && !h->dcap)
+        h->dcap = static_cast<HemCapillaryDev*>(h->allocate(h->capacity * sizeof(HemCapillaryDev)));
     const auto begin = std::chrono::steady_clock::now();
     // stable counting sort on the scheduling key, heavy buckets first
     std::array<size_t, 256> counts{};
@@ -220,12 +242,13 @@ int runHost(HemHandle* h, const double* q, const double* energy, const ReactiveG
     RAMD_HIP(hipMemcpyAsync(h->de, energy, count * sizeof(double), hipMemcpyHostToDevice, s));
     RAMD_HIP(hipMemcpyAsync(h->dguess, states, count * sizeof(ThermoOut), hipMemcpyHostToDevice, s));
     RAMD_HIP(hipMemcpyAsync(h->dorder, h->hostOrder.data(), count * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    if (capillary) RAMD_HIP(hipMemcpyAsync(h->dmode, mode.data(), count * sizeof(int), hipMemcpyHostToDevice, s));
+    if (capillary)
+        RAMD_HIP(hipMemcpyAsync(h->dcap, capillary, count * sizeof(HemCapillaryDev), hipMemcpyHostToDevice, s));
     RAMD_HIP(hipEventRecord(h->events[1], s));
-    h->launch(h->dq, h->de, h->dguess, h->dorder, count, capillary ? h->dmode : nullptr, h->dout, nullptr, s);
+    h->launch(h->dq, h->de, h->dguess, h->dorder, count, capillary ? h->dcap : nullptr, h->dout, nullptr, s);
     RAMD_HIP(hipEventRecord(h->events[2], s));
     RAMD_HIP(hipMemcpyAsync(h->hostOut.data(), h->dout, count * sizeof(HemOutputDev), hipMemcpyDeviceToHost, s));
-    unsigned long long totals[6] = {};
+    unsigned long long totals[7] = {};
     RAMD_HIP(hipMemcpyAsync(totals, h->totals, sizeof(totals), hipMemcpyDeviceToHost, s));
     RAMD_HIP(hipEventRecord(h->events[3], s));
     RAMD_HIP(hipEventSynchronize(h->events[3]));
@@ -330,11 +353,62 @@ int reactive_gpu_hem_run_resident_v1(void* raw, const double* q, const double* e
                                      const ReactiveGpuHemCapillaryInputV2* capillary, const uint32_t* order,
                                      size_t count, ReactiveGpuHemOutputV1* output, ReactiveGpuHemProfileV1* profile,
                                      char* error, size_t size) {
-    (void)raw; (void)q; (void)energy; (void)seed; (void)capillary; (void)order; (void)count; (void)output;
-    (void)profile;
-    writeError(error, size, "reactive_gpu_hem_run_resident_v1: the capillary resident batch needs the "
-                            "boundary-recovery search, not provided by the MI355X HEM build");
-    return 1;
+    try {
+        require(raw && q && energy && seed && capillary && order && output && profile && profile->abiVersion == 1 &&
+                    profile->structBytes == sizeof(*profile),
+                "Invalid resident HIP HEM batch arguments");
+        HemHandle* h = static_cast<HemHandle*>(raw);
+        require(count && count <= h->capacity, "Resident HIP HEM batch exceeds capacity");
+        const auto begin = std::chrono::steady_clock::now();
+        RAMD_HIP(hipSetDevice(h->device));
+        hipStream_t s = h->stream;
+        if (!h->rejected) h->rejected = static_cast<unsigned long long*>(h->allocate(sizeof(unsigned long long)));
+        // every input is device memory indexed by global cell; reject the batch
+        // before any flash runs if a cell needs what this build does not provide
+        const auto* cap = reinterpret_cast<const HemCapillaryDev*>(capillary);
+        hipLaunchKernelGGL(zeroTotalsKernel, dim3(1), dim3(64), 0, s, h->rejected, 1);
+        hipLaunchKernelGGL(capillaryCheckKernel, dim3(unsigned((count + 255) / 256)), dim3(256), 0, s, cap, order,
+                           uint32_t(count), h->rejected);
+        RAMD_HIP(hipGetLastError());
+        unsigned long long rejected = 0;
+        RAMD_HIP(hipMemcpyAsync(&rejected, h->rejected, sizeof(rejected), hipMemcpyDeviceToHost, s));
+        RAMD_HIP(hipStreamSynchronize(s));
+        require(rejected == 0, "Resident HIP HEM batch holds cells with an interface (color or pressureJump != 0), "
+                               "equilibrium = 1 or invalid capillary input; this build runs frozen-inventory cells only");
+        RAMD_HIP(hipEventRecord(h->events[0], s));
+        h->launch(q, energy, reinterpret_cast<const ThermoOut*>(seed), order, count, cap,
+                  reinterpret_cast<HemOutputDev*>(output), nullptr, s);
+        RAMD_HIP(hipEventRecord(h->events[1], s));
+        unsigned long long totals[7] = {};
+        RAMD_HIP(hipMemcpyAsync(totals, h->totals, sizeof(totals), hipMemcpyDeviceToHost, s));
+        RAMD_HIP(hipEventRecord(h->events[2], s));
+        RAMD_HIP(hipEventSynchronize(h->events[2]));
+        float kernel = 0, download = 0;
+        RAMD_HIP(hipEventElapsedTime(&kernel, h->events[0], h->events[1]));
+        RAMD_HIP(hipEventElapsedTime(&download, h->events[1], h->events[2]));
+        ReactiveGpuHemProfileV1 p{};
+        p.abiVersion = 1;
+        p.structBytes = sizeof(p);
+        p.batches = 1;
+        p.submitted = count;
+        p.succeeded = totals[6];
+        p.deviceFailures = count - totals[6];
+        p.phaseEvaluations = totals[0];
+        p.residualEvaluations = totals[1];
+        p.flashCandidates = totals[2];
+        p.stableCandidates = totals[3];
+        p.finiteDifferenceJacobians = totals[5];
+        p.transferBytes = sizeof(totals) + sizeof(rejected);
+        p.deviceBytes = h->deviceBytes;
+        p.wallSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - begin).count();
+        p.kernelSeconds = kernel / 1000.;
+        p.copySeconds = download / 1000.;
+        *profile = p;
+        return 0;
+    } catch (const std::exception& e) {
+        writeError(error, size, e.what());
+        return 1;
+    }
 }
 
 }

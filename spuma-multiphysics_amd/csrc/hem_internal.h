@@ -15,18 +15,28 @@ struct HemOutputDev {
     int success, reserved;
 };
 
+// Per-cell capillary input, byte-compatible with ReactiveGpuHemCapillaryInputV2
+// (24 B).  This build runs cells with color = pressureJump = 0 only:
+// equilibrium 0 is the frozen-inventory recovery; equilibrium 1 would need the
+// boundary-recovery search and is rejected before any flash runs.
+struct HemCapillaryDev {
+    double color, pressureJump;
+    int equilibrium, pad;
+};
+
 struct HemHandle {
     int ns = 0, nl = 0, device = 0;
     size_t capacity = 0, deviceBytes = 0, modelBytes = 0;
     void* model = nullptr;  // HemModel<ns> in device memory
-    unsigned long long* totals = nullptr;  // phases residuals candidates stable failures fd
+    unsigned long long* totals = nullptr;  // phases residuals candidates stable failures fd successes
     // host-path staging (allocated on the first host batch)
     bool hostReady = false;
     double *dq = nullptr, *de = nullptr;
     ThermoOut* dguess = nullptr;
     HemOutputDev* dout = nullptr;
     uint32_t* dorder = nullptr;
-    int* dmode = nullptr;
+    HemCapillaryDev* dcap = nullptr;      // host-path capillary staging
+    unsigned long long* rejected = nullptr;  // resident batches: unsupported capillary cells
     std::vector<uint32_t> hostOrder;
     std::vector<uint8_t> hostKey;
     std::vector<HemOutputDev> hostOut;
@@ -37,9 +47,11 @@ struct HemHandle {
     ~HemHandle();
     void* allocate(size_t bytes);
     void ensureHostPath();
-    // Flash count cells named by order (or 0..count-1) on stream s.
+    // Flash count cells named by order (or 0..count-1) on stream s; q, e,
+    // guess, capillary and out are indexed by cell.  Totals: phases,
+    // residuals, candidates, stable, failures, fd Jacobians, successes.
     void launch(const double* q, const double* e, const ThermoOut* guess, const uint32_t* order, size_t count,
-                const int* mode, HemOutputDev* out, uint32_t* cellPhases, hipStream_t s);
+                const HemCapillaryDev* capillary, HemOutputDev* out, uint32_t* cellPhases, hipStream_t s);
 };
 
 HemHandle* createHem(const void* raw, size_t bytes, size_t capacity, bool hostPath);

@@ -40,7 +40,7 @@ __global__ __launch_bounds__(64) void stepFlashKernel(const HemModel<NS>* __rest
                                                       const uint32_t* __restrict__ order, uint32_t count,
                                                       const double* __restrict__ dtDev, ThermoOut* __restrict__ out,
                                                       double* __restrict__ pOut, double* __restrict__ cOut,
-                                                      uint16_t* __restrict__ cost,
+                                                      double* __restrict__ rOut, uint16_t* __restrict__ cost,
                                                       unsigned long long* __restrict__ counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Counts n = {0, 0, 0, 0, 0, 0};
@@ -66,10 +66,12 @@ __global__ __launch_bounds__(64) void stepFlashKernel(const HemModel<NS>* __rest
             writeOut(result, out[c]);
             pOut[c] = result.p;
             cOut[c] = result.cf;
+            rOut[c] = result.rho;
         } else {  // the cell keeps its step-start state (the reference commits nothing for it)
             out[c] = g;
             pOut[c] = g.p;
             cOut[c] = g.soundFrozen;
+            rOut[c] = g.rho;
             failed = 1;
         }
         if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
@@ -224,12 +226,13 @@ __global__ void clearCountersKernel(unsigned long long* __restrict__ c, int n) {
     if (threadIdx.x < n) c[threadIdx.x] = 0;
 }
 __global__ void toThermoKernel(const ReactiveThermoState* __restrict__ in, size_t n, ThermoOut* __restrict__ out,
-                               double* __restrict__ p, double* __restrict__ c) {
+                               double* __restrict__ p, double* __restrict__ c, double* __restrict__ r) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     std::memcpy(&out[i], &in[i], sizeof(ThermoOut));
     p[i] = in[i].p;
     c[i] = in[i].soundFrozen;
+    r[i] = in[i].rho;
 }
 
 inline unsigned blocksFor(size_t n, unsigned b) { return unsigned((n + b - 1) / b); }
@@ -242,6 +245,7 @@ struct Step {
     size_t cells = 0, all = 0;
     std::vector<void*> allocations;
     double *qA = nullptr, *qB = nullptr, *pA = nullptr, *cA = nullptr, *pB = nullptr, *cB = nullptr;
+    double *rA = nullptr, *rB = nullptr;  // recovered densities (read by the HLLC flux)
     ThermoOut *sA = nullptr, *sB = nullptr;
     uint8_t* key = nullptr;
     uint32_t* order = nullptr;
@@ -252,7 +256,7 @@ struct Step {
     double* scalars = nullptr;  // dt, recheck
     double *bA = nullptr, *bB = nullptr, *boundary = nullptr, *totals = nullptr;
     double2* totalPartial = nullptr;
-    unsigned long long* counters = nullptr;  // fail1, phases1, fail2, phases2, violations
+    unsigned long long* counters = nullptr;  // fail1, phases1, fail2, phases2, violations, HLLC rejections
     hipStream_t stream = nullptr;
     hipEvent_t events[2] = {};
     std::vector<hipEvent_t> marks;  // 10 per step: cfl, stage 0, flash #1, recheck, stage 1, flash #2 brackets
@@ -274,13 +278,13 @@ struct Step {
         return static_cast<T*>(p);
     }
 
-    void flash(const double* q, double* pOut, double* cOut, ThermoOut* out, uint16_t* costOut,
+    void flash(const double* q, double* pOut, double* cOut, double* rOut, ThermoOut* out, uint16_t* costOut,
                unsigned long long* counter) {
         const unsigned blocks = blocksFor(cells, 64);
 #define RAMD_STEP_FLASH(NS_, NL_)                                                                                 \
     hipLaunchKernelGGL((stepFlashKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                             \
                        static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), scalars, out, \
-                       pOut, cOut, costOut, counter)
+                       pOut, cOut, rOut, costOut, counter)
         if (h->ns == 1 && h->nl == 0) RAMD_STEP_FLASH(1, 0);
         else if (h->ns == 1) RAMD_STEP_FLASH(1, 1);
         else if (h->nl == 0) RAMD_STEP_FLASH(2, 0);
@@ -308,12 +312,12 @@ struct Step {
         hipLaunchKernelGGL(bucketScatterKernel, dim3(blocksFor(cells, 256)), dim3(256), 0, stream, key, cells,
                            bucketOffset, order);
         // stage 0 and its boundary rate
-        t->boundaryRatesFrom(qA, pA, cA, bA, stream);
+        t->boundaryRatesFrom(qA, pA, cA, rA, bA, counters + 5, stream);
         mark(m, 2);
-        launchCartesianStage(a, ns, nv, qA, all, pA, cA, 0.0, scalars, 0, qB, stream);
+        launchCartesianStage(a, ns, nv, t->scheme, qA, all, pA, cA, rA, 0.0, scalars, 0, qB, counters + 5, stream);
         mark(m, 3);
         // flash #1 seeded from the step-start states
-        flash(qB, pB, cB, sB, nullptr, counters);
+        flash(qB, pB, cB, rB, sB, nullptr, counters);
         mark(m, 4);
         // CFL recheck after stage 0 (maximum = dt)
         launchCartesianCfl(a, ns, qB, all, pB, cB, cfl, 0.0, scalars, blockMin, stream);
@@ -321,12 +325,12 @@ struct Step {
         mark(m, 5);
         hipLaunchKernelGGL(recheckKernel, dim3(1), dim3(64), 0, stream, scalars, counters);
         // stage 1 (in place into qA, which holds q0) and its boundary rate
-        t->boundaryRatesFrom(qB, pB, cB, bB, stream);
+        t->boundaryRatesFrom(qB, pB, cB, rB, bB, counters + 5, stream);
         mark(m, 6);
-        launchCartesianStage(a, ns, nv, qB, all, pB, cB, 0.0, scalars, 1, qA, stream);
+        launchCartesianStage(a, ns, nv, t->scheme, qB, all, pB, cB, rB, 0.0, scalars, 1, qA, counters + 5, stream);
         mark(m, 7);
         // flash #2 from the step-start seeds, into the state buffer of stage 1
-        flash(qA, pA, cA, sB, cost, counters + 2);
+        flash(qA, pA, cA, rA, sB, cost, counters + 2);
         haveCost = true;
         mark(m, 8);
         std::swap(sA, sB);
@@ -387,6 +391,8 @@ void* reactive_amd_step_create(void* transport, void* hem, char* error, size_t e
         s->cA = s->alloc<double>(s->all);
         s->pB = s->alloc<double>(s->all);
         s->cB = s->alloc<double>(s->all);
+        s->rA = s->alloc<double>(s->all);
+        s->rB = s->alloc<double>(s->all);
         s->sA = s->alloc<ThermoOut>(s->cells);
         s->sB = s->alloc<ThermoOut>(s->cells);
         s->key = s->alloc<uint8_t>(s->cells);
@@ -423,7 +429,7 @@ int reactive_amd_step_load(void* step, const double* q, const ReactiveThermoStat
         RAMD_HIP(hipMemcpyAsync(staging, states, s.cells * sizeof(ReactiveThermoState), hipMemcpyHostToDevice,
                                 s.stream));
         hipLaunchKernelGGL(toThermoKernel, dim3(blocksFor(s.cells, 256)), dim3(256), 0, s.stream, staging, s.cells,
-                           s.sA, s.pA, s.cA);
+                           s.sA, s.pA, s.cA, s.rA);
         RAMD_HIP(hipGetLastError());
         RAMD_HIP(hipStreamSynchronize(s.stream));
         RAMD_HIP(hipFree(staging));
@@ -467,6 +473,7 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         r.flashFailures[1] = counters[2];
         r.phaseEvaluations = counters[1] + counters[3];
         r.cflViolations = counters[4];
+        r.fluxFailures = counters[5];
         r.dt = scalars[0];
         r.recheckDt = scalars[1];
         for (int k = 0; k < s.nv; ++k) {

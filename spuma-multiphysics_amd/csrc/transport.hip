@@ -30,6 +30,8 @@
 
 #include "hip_common.h"
 #include "transport_internal.h"
+#include "flux_device.h"
+#include "reactive_amd.h"
 #include "reactive_transport.h"
 
 namespace ramd {
@@ -52,11 +54,11 @@ __global__ void primitivesKernel(const double* __restrict__ q, size_t all, int n
     for (int d = 0; d < 3; ++d) prim[size_t(d + 1) * all + c] = q[size_t(ns + d) * all + c] / rho;
 }
 
-// velocities() (reactiveTransportKernels.h:145-149): normal components of
-// the owner and (mirrored for slip walls) right velocities.
-__device__ inline void normalVelocities(const FaceDev& f, const double* __restrict__ prim, size_t all, double& unL,
-                                        double& unR) {
-    double ul[3], ur[3], dot = 0;
+// velocities() (reactiveTransportKernels.h:145-149): the owner and (mirrored
+// for slip walls) right velocities and their normal components.
+__device__ inline void normalVelocities(const FaceDev& f, const double* __restrict__ prim, size_t all, double* ul,
+                                        double* ur, double& unL, double& unR) {
+    double dot = 0;
     for (int d = 0; d < 3; ++d) {
         ul[d] = prim[size_t(d + 1) * all + f.owner];
         ur[d] = prim[size_t(d + 1) * all + f.right];
@@ -72,15 +74,18 @@ __device__ inline void normalVelocities(const FaceDev& f, const double* __restri
     }
 }
 
-// HLL coefficients (Faces, reactiveTransportKernels.h:343-356).
+// Face coefficients (Faces, reactiveTransportKernels.h:343-356): HLL, or the
+// HLLC option (flux_device.h), which also reads the total energies from q.
+// A face the HLLC flux rejects keeps its HLL coefficients and is counted.
 __global__ void hllKernel(const FaceDev* __restrict__ faces, size_t nf, const double* __restrict__ prim, size_t all,
                           const ReactiveTransportState* __restrict__ state, double waveFactor,
-                          double4* __restrict__ work) {
+                          double4* __restrict__ work, int scheme, const double* __restrict__ q, int ns,
+                          unsigned long long* __restrict__ failures) {
     const size_t fi = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (fi >= nf) return;
     const FaceDev f = faces[fi];
-    double unL, unR;
-    normalVelocities(f, prim, all, unL, unR);
+    double ul[3], ur[3], unL, unR;
+    normalVelocities(f, prim, all, ul, ur, unL, unR);
     const double pL = state[f.owner].p, pR = state[f.right].p;
     const double aL = waveFactor * state[f.owner].sound, aR = waveFactor * state[f.right].sound;
     const double left = dmin(0, dmin(unL - aL, unR - aR)), right = dmax(0, dmax(unL + aL, unR + aR));
@@ -90,6 +95,10 @@ __global__ void hllKernel(const FaceDev* __restrict__ faces, size_t nf, const do
     w.y = left * (right - unR) * invWave;
     w.z = (right * pL - left * pR) * invWave;
     w.w = (right * pL * unL - left * pR * unR) * invWave;
+    if (scheme == kHllc &&
+        !hllcCoefficients(state[f.owner].rho, ul, pL, aL, q[size_t(ns + 3) * all + f.owner], state[f.right].rho, ur,
+                          pR, aR, q[size_t(ns + 3) * all + f.right], f.n, w))
+        atomicAdd(failures, 1ull);
     work[fi] = w;
 }
 
@@ -182,8 +191,8 @@ __global__ void faceSpeedKernel(const FaceDev* __restrict__ faces, size_t nf, co
     const size_t fi = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (fi >= nf) return;
     const FaceDev f = faces[fi];
-    double unL, unR;
-    normalVelocities(f, prim, all, unL, unR);
+    double ul[3], ur[3], unL, unR;
+    normalVelocities(f, prim, all, ul, ur, unL, unR);
     const double aL = waveFactor * state[f.owner].sound, aR = waveFactor * state[f.right].sound;
     const double left = dmin(0, dmin(unL - aL, unR - aR)), right = dmax(0, dmax(unL + aL, unR + aR));
     const double invWave = 1 / (right - left);
@@ -272,7 +281,8 @@ __global__ void primitiveUploadKernel(const ReactiveTransportPrimitive* __restri
 // (Cells + Faces restricted to the boundary list, same arithmetic).
 __global__ void boundaryHllKernel(const FaceDev* __restrict__ faces, const uint32_t* __restrict__ boundary, size_t nb,
                                   const double* __restrict__ q, size_t all, int ns, const double* __restrict__ pArr,
-                                  const double* __restrict__ cArr, double waveFactor, double4* __restrict__ work) {
+                                  const double* __restrict__ cArr, const double* __restrict__ rArr, double waveFactor,
+                                  double4* __restrict__ work, int scheme, unsigned long long* __restrict__ failures) {
     const size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= nb) return;
     const uint32_t fi = boundary[j];
@@ -304,6 +314,10 @@ __global__ void boundaryHllKernel(const FaceDev* __restrict__ faces, const uint3
     w.y = left * (right - unR) * invWave;
     w.z = (right * pL - left * pR) * invWave;
     w.w = (right * pL * unL - left * pR * unR) * invWave;
+    if (scheme == kHllc &&
+        !hllcCoefficients(rArr[f.owner], ul, pL, aL, q[size_t(ns + 3) * all + f.owner], rArr[f.right], ur, pR, aR,
+                          q[size_t(ns + 3) * all + f.right], f.n, w))
+        atomicAdd(failures, 1ull);
     work[fi] = w;
 }
 
@@ -413,6 +427,8 @@ Transport::Transport(int backend, const ReactiveTransportConfig& c, const double
     boundaryPartial = allocate<double>(nv * partitions);
     boundaryRate = allocate<double>(nv);
     staging = allocate<double>(std::max(all, cfg.cells) * nv);
+    fluxFailures = allocate<unsigned long long>(1);
+    RAMD_HIP(hipMemsetAsync(fluxFailures, 0, sizeof(unsigned long long), stream));
     deviceProfile.faceWorkspaceBytes = cfg.faces * (sizeof(double4) + sizeof(double));
     std::vector<double> inverse(cfg.cells);
     for (size_t c2 = 0; c2 < cfg.cells; ++c2) {
@@ -546,7 +562,7 @@ void Transport::prepareResident(const ReactiveTransportState* states, bool trans
     primitives();
     if (transport) {
         hipLaunchKernelGGL(hllKernel, dim3(gridFor(cfg.faces, 256)), dim3(256), 0, stream, faces, cfg.faces, prim, all,
-                           state, cfg.waveFactor, work);
+                           state, cfg.waveFactor, work, scheme, q, ns, fluxFailures);
         RAMD_HIP(hipGetLastError());
         launched();
         ++deviceProfile.transportFaceLaunches;
@@ -574,10 +590,11 @@ void Transport::gather() {
     launched();
 }
 
-void Transport::boundaryRatesFrom(const double* qv, const double* p, const double* c, double* out, hipStream_t s) {
+void Transport::boundaryRatesFrom(const double* qv, const double* p, const double* c, const double* r, double* out,
+                                  unsigned long long* failures, hipStream_t s) {
     if (nBoundary) {
         hipLaunchKernelGGL(boundaryHllKernel, dim3(gridFor(nBoundary, 256)), dim3(256), 0, s, faces, boundaryDev,
-                           nBoundary, qv, all, ns, p, c, cfg.waveFactor, work);
+                           nBoundary, qv, all, ns, p, c, r, cfg.waveFactor, work, scheme, failures);
         RAMD_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(boundaryPartialKernel, dim3(unsigned(nv * partitions)), dim3(256), 0, s, faces, work,
@@ -604,6 +621,14 @@ void Transport::stableStep(double cfl, double maximum, double* dt) {  // reactiv
     ++stats.stepQueries;
 }
 
+void Transport::checkFluxFailures() {
+    unsigned long long failed = 0;
+    RAMD_HIP(hipMemcpyAsync(&failed, fluxFailures, sizeof(failed), hipMemcpyDeviceToHost, stream));
+    RAMD_HIP(hipMemsetAsync(fluxFailures, 0, sizeof(failed), stream));
+    finish();
+    require(failed == 0, "HLLC face flux failed");  // the capillary branch's failCapillary()
+}
+
 void Transport::advance(double dt, int stage, double* hostQ, double* boundary) {
     if (cartesian.valid) {
         cartesianAdvance(dt, stage);
@@ -613,6 +638,7 @@ void Transport::advance(double dt, int stage, double* hostQ, double* boundary) {
         RAMD_HIP(hipGetLastError());
         launched();
     }
+    if (scheme == kHllc) checkFluxFailures();
     std::swap(q, initial);
     if (hostQ) {
         launchUnpack(q, hostQ);
@@ -631,6 +657,7 @@ void Transport::rhs(double* hostRhs, double* boundary) {
                        incidenceDev, invV, q, cfg.cells, all, ns, nv, 0.0, 2, out);
     RAMD_HIP(hipGetLastError());
     launched();
+    if (scheme == kHllc) checkFluxFailures();
     downloadBytes(hostRhs, out, cfg.cells * nv * sizeof(double));
     downloadBytes(boundary, boundaryRate, nv * sizeof(double));
     finish();
@@ -683,6 +710,20 @@ int stub(void* handle, const char* name, const char* what) {
 using namespace ramd;
 
 extern "C" {
+
+int reactive_amd_transport_set_flux(void* handle, int scheme) {
+    if (!handle) return -1;
+    Transport& t = *static_cast<Transport*>(handle);
+    std::unique_lock<std::mutex> lock(t.entry, std::try_to_lock);
+    if (!lock.owns_lock()) return 2;
+    if (scheme != kHll && scheme != kHllc) {
+        std::snprintf(t.error, sizeof(t.error), "Unknown face flux scheme %d", scheme);
+        return 1;
+    }
+    t.scheme = scheme;
+    t.error[0] = 0;
+    return 0;
+}
 
 void* reactive_transport_create(int backend, const ReactiveTransportConfig* cfg, const double* volumes,
                                 const ReactiveTransportFace* faces, const double* q, const ReactiveTransportState* states,

@@ -56,6 +56,9 @@ public:
     double *boundaryPartial = nullptr, *boundaryRate = nullptr;
     double* staging = nullptr;         // cell-major host-layout bridge
     double *pressure = nullptr, *sound = nullptr;  // SoA p, c_frozen for the Cartesian path
+    double* density = nullptr;                     // SoA recovered density (HLLC on the Cartesian path)
+    int scheme = 0;                                // face flux: 0 HLL, 1 HLLC
+    unsigned long long* fluxFailures = nullptr;    // faces the HLLC flux rejected since the last check
     CartesianInfo cartesian;
     // versions, tokens, attempts (reactiveTransport.cpp:384-393)
     std::string physicalHash;
@@ -102,8 +105,12 @@ public:
                                   const std::vector<uint32_t>& row, const std::vector<int32_t>& incidence);
     void cartesianAdvance(double dt, int stage);
     // device-resident step support (step.hip): boundary flux rates of q
-    // (stride `all`) with HLL coefficients rebuilt from q, p, c on the fly
-    void boundaryRatesFrom(const double* qv, const double* p, const double* c, double* out, hipStream_t s);
+    // (stride `all`) with the face coefficients rebuilt from q, p, c (and the
+    // recovered density r for HLLC) on the fly; HLLC rejections count into failures
+    void boundaryRatesFrom(const double* qv, const double* p, const double* c, const double* r, double* out,
+                           unsigned long long* failures, hipStream_t s);
+    // throws (and clears the counter) if the HLLC flux rejected a face
+    void checkFluxFailures();
 };
 
 }  // namespace ramd

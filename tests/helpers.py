@@ -53,6 +53,8 @@ def oracle(twin=False):
         lib.oracle_solve.argtypes = [V, V, C.c_int, V]
         lib.oracle_nasa.argtypes = [V, C.c_int, C.c_int, C.c_double, V]
         lib.oracle_elementary.argtypes = [C.c_int, C.c_size_t, V, V]
+        lib.oracle_transport_set_scheme.argtypes = [V, C.c_int]
+        lib.oracle_hllc_face.argtypes = [V, V, V, V]
         _ORACLE[twin] = lib
     return _ORACLE[twin]
 
@@ -76,13 +78,15 @@ def oracle_flash(model_bytes, q_species, energy, seeds, threads=8, twin=False):
 
 
 class OracleTransport:
-    def __init__(self, cfg, volume, faces, fixed_q=None, fixed_states=None):
+    def __init__(self, cfg, volume, faces, fixed_q=None, fixed_states=None, scheme=0):
+        """scheme: 0 HLL, 1 HLLC face flux."""
         self.lib = oracle()
         err = C.create_string_buffer(1024)
         self.keep = (volume, faces, fixed_q, fixed_states)
         self.h = self.lib.oracle_transport_create(C.byref(cfg), p(volume), p(faces), p(fixed_q), p(fixed_states),
                                                   err, 1024)
         assert self.h, err.value
+        assert self.lib.oracle_transport_set_scheme(self.h, scheme) == 0
         self.nv = cfg.variables
         self.nc = cfg.cells
 

@@ -70,3 +70,75 @@ def test_flash_edge_classes_bitwise():
 def test_flash_larger_cube_bitwise():
     case = cube_case(16, seed=5)
     check_bitwise(case["model"], case["q"][:, :1], case["energy"], case["seeds"])
+
+
+CAP_DTYPE = np.dtype([("color", "<f8"), ("pressureJump", "<f8"), ("equilibrium", "<i4"), ("pad", "<i4")])
+
+
+def test_resident_frozen_batch_matches_host_batch_and_oracle():
+    """reactive_gpu_hem_run_resident_v1 with device pointers and a cell order,
+    against reactive_gpu_hem_run_v2 (host arrays) and the oracle's
+    frozen-inventory recovery (reactiveDeviceFlash.h:572-590), bit for bit."""
+    import torch
+    from helpers import oracle, p
+    lib = rlib.load()
+    case = cube_case(6, seed=17)
+    mb, q, e, seeds = case["model"], np.ascontiguousarray(case["q"][:, :1]), case["energy"], case["states"]
+    n = len(e)
+    cap = np.zeros(n, CAP_DTYPE)
+    # host batch (run_v2, equilibrium = 0)
+    err = C.create_string_buffer(1024)
+    h = lib.reactive_gpu_hem_create_v1(mb, len(mb), n, err, 1024)
+    assert h, err.value
+    hs, hsucc = seeds.copy(), np.zeros(n, np.int32)
+    prof = abi.HemProfile.fresh()
+    assert lib.reactive_gpu_hem_run_v2(h, rlib.ptr(q), rlib.ptr(e), rlib.ptr(cap), n, rlib.ptr(hs), rlib.ptr(hsucc),
+                                       C.byref(prof), err, 1024) == 0, err.value
+    lib.reactive_gpu_hem_destroy_v1(h)
+    # resident batch over every other cell, in reverse order
+    order = np.arange(n - 1, -1, -2, dtype=np.uint32)
+    dev = torch.device("cuda:0")
+    dq = torch.from_numpy(q.copy()).to(dev)
+    de = torch.from_numpy(e.copy()).to(dev)
+    ds = torch.from_numpy(seeds.view(np.uint8).copy()).to(dev)
+    dc = torch.from_numpy(cap.view(np.uint8).copy()).to(dev)
+    do = torch.from_numpy(order.view(np.int32).copy()).to(dev)
+    dout = torch.zeros(n * 184, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    r = lib.reactive_gpu_hem_create_resident_v1(mb, len(mb), n, err, 1024)
+    assert r, err.value
+    rprof = abi.HemProfile.fresh()
+    rc = lib.reactive_gpu_hem_run_resident_v1(r, C.c_void_p(dq.data_ptr()), C.c_void_p(de.data_ptr()),
+                                              C.c_void_p(ds.data_ptr()), C.c_void_p(dc.data_ptr()),
+                                              C.c_void_p(do.data_ptr()), len(order), C.c_void_p(dout.data_ptr()),
+                                              C.byref(rprof), err, 1024)
+    assert rc == 0, err.value
+    out = dout.cpu().numpy().view(abi.HEM_OUTPUT_DTYPE)
+    assert rprof.submitted == len(order) and rprof.succeeded == int(out["success"][order].sum())
+    untouched = np.setdiff1d(np.arange(n), order)
+    assert not out.view(np.uint8).reshape(n, -1)[untouched].any()
+    assert np.array_equal(out["success"][order], hsucc[order])
+    good = order[hsucc[order] == 1]
+    assert out["state"][good].tobytes() == hs[good].tobytes()
+    # oracle (twin) frozen recovery, cell by cell
+    L = oracle(True)
+    img = C.create_string_buffer(mb, len(mb))
+    for c in order[:40]:
+        o = np.zeros(1, abi.STATE_DTYPE)
+        ok = L.oracle_flash_frozen(img, p(q[c]), C.c_double(e[c]), p(seeds[c:c + 1]), p(o), None)
+        assert ok == out["success"][c]
+        if ok:
+            assert o.tobytes() == out["state"][c:c + 1].tobytes()
+    # a cell asking for the equilibrium (boundary-recovery) search: rejected, nothing written
+    cap2 = cap.copy()
+    cap2["equilibrium"][order[3]] = 1
+    dc2 = torch.from_numpy(cap2.view(np.uint8).copy()).to(dev)
+    dout.zero_()
+    torch.cuda.synchronize()
+    rc = lib.reactive_gpu_hem_run_resident_v1(r, C.c_void_p(dq.data_ptr()), C.c_void_p(de.data_ptr()),
+                                              C.c_void_p(ds.data_ptr()), C.c_void_p(dc2.data_ptr()),
+                                              C.c_void_p(do.data_ptr()), len(order), C.c_void_p(dout.data_ptr()),
+                                              C.byref(rprof), err, 1024)
+    assert rc == 1 and b"frozen-inventory" in err.value
+    assert not dout.cpu().numpy().any()
+    lib.reactive_gpu_hem_destroy_v1(r)

@@ -15,10 +15,10 @@ from reactive_amd import abi, cases, lib as rlib
 pytestmark = pytest.mark.gpu
 
 
-def oracle_step(case, cfl):
+def oracle_step(case, cfl, scheme=0):
     q0, s0, mb = case["q"], case["states"], case["model"]
     cfg = config(len(q0), len(case["faces"]))
-    ora = OracleTransport(cfg, case["volume"], case["faces"])
+    ora = OracleTransport(cfg, case["volume"], case["faces"], scheme=scheme)
     rc, dt = ora.stable_step(q0, cases.transport_states(s0), cfl, 1.0)
     assert rc == 0
     q = q0.copy()
@@ -31,23 +31,25 @@ def oracle_step(case, cfl):
     return dict(dt=dt, recheck=recheck, q=q, s1=s1, s2=s2, ok=(ok1, ok2), bA=bA, bB=bB)
 
 
-@pytest.mark.parametrize("n", [8, 12])
-def test_resident_step_matches_oracle_step(n):
+@pytest.mark.parametrize("n,scheme", [(8, 0), (12, 0), (8, 1)], ids=["8-hll", "12-hll", "8-hllc"])
+def test_resident_step_matches_oracle_step(n, scheme):
     lib = rlib.load()
     case = cube_case(n, seed=11 + n)
     cfl = 0.4
-    ref = oracle_step(case, cfl)
+    ref = oracle_step(case, cfl, scheme)
     err = C.create_string_buffer(1024)
     mb = case["model"]
     hem = lib.reactive_gpu_hem_create_v1(mb, len(mb), len(case["q"]), err, 1024)
     cfg = config(len(case["q"]), len(case["faces"]))
     tr = lib.reactive_transport_create(1, C.byref(cfg), rlib.ptr(case["volume"]), rlib.ptr(case["faces"]), None,
                                        None, None, None, err, 1024)
+    assert lib.reactive_amd_transport_set_flux(tr, scheme) == 0
     step = lib.reactive_amd_step_create(tr, hem, err, 1024)
     assert step, err.value
     assert lib.reactive_amd_step_load(step, rlib.ptr(case["q"]), rlib.ptr(case["states"])) == 0
     rep = abi.StepReport.fresh()
     assert lib.reactive_amd_step_run(step, cfl, 1.0, 1, 0, C.byref(rep)) == 0, lib.reactive_amd_step_error(step)
+    assert rep.fluxFailures == 0
     q = np.zeros_like(case["q"])
     s = np.zeros(len(q), dtype=abi.STATE_DTYPE)
     assert lib.reactive_amd_step_read(step, rlib.ptr(q), rlib.ptr(s)) == 0

@@ -27,8 +27,9 @@ def gpu_rhs(lib, h, q, st, nc, nv):
     return rhs, b
 
 
+@pytest.mark.parametrize("scheme", [0, 1], ids=["hll", "hllc"])
 @pytest.mark.parametrize("n", [4, 7, 10])
-def test_cartesian_rhs_dt_stages_bitwise(n):
+def test_cartesian_rhs_dt_stages_bitwise(n, scheme):
     lib = rlib.load()
     case = cube_case(n, seed=n)
     q, st = case["q"], case["tstates"]
@@ -37,7 +38,8 @@ def test_cartesian_rhs_dt_stages_bitwise(n):
     h, err = create(lib, cfg, case["volume"], case["faces"])
     assert h, err
     assert lib.reactive_transport_is_cuda(h) == 1
-    ora = OracleTransport(cfg, case["volume"], case["faces"])
+    assert lib.reactive_amd_transport_set_flux(h, scheme) == 0
+    ora = OracleTransport(cfg, case["volume"], case["faces"], scheme=scheme)
     r_gpu, b_gpu = gpu_rhs(lib, h, q, st, nc, 5)
     r_ora, b_ora = ora.rhs(q, st)
     assert np.array_equal(r_gpu, r_ora) and np.array_equal(b_gpu, b_ora)
@@ -61,15 +63,20 @@ def test_cartesian_rhs_dt_stages_bitwise(n):
     ora.close()
 
 
+@pytest.mark.parametrize("scheme", [0, 1], ids=["hll", "hllc"])
 @pytest.mark.parametrize("seed", [1, 2, 3])
-def test_generic_graph_all_face_kinds_bitwise(seed):
+def test_generic_graph_all_face_kinds_bitwise(seed, scheme):
     lib = rlib.load()
     nc = 257
     faces, volume, q, st, fq, fst = random_graph(nc, seed)
+    if scheme == 1:  # HLLC reads the recovered density, not sum(q): make them differ
+        st = st.copy()
+        st["rho"] *= 1 + 1e-3 * np.random.default_rng(seed).standard_normal(len(st))
     cfg = config(nc, len(faces), fixed=len(fq))
     h, err = create(lib, cfg, volume, faces, fq, fst)
     assert h, err
-    ora = OracleTransport(cfg, volume, faces, fq, fst)
+    assert lib.reactive_amd_transport_set_flux(h, scheme) == 0
+    ora = OracleTransport(cfg, volume, faces, fq, fst, scheme=scheme)
     r_gpu, b_gpu = gpu_rhs(lib, h, q, st, nc, 5)
     r_ora, b_ora = ora.rhs(q, st)
     assert np.array_equal(r_gpu, r_ora) and np.array_equal(b_gpu, b_ora)
@@ -164,3 +171,16 @@ def test_error_contracts():
     assert lib.reactive_transport_set_wale_v1(h, None) == 1
     assert b"not provided" in lib.reactive_transport_error(h)
     assert lib.reactive_transport_destroy(h) is None
+
+
+def test_flux_scheme_contract():
+    lib = rlib.load()
+    case = cube_case(4, seed=2)
+    cfg = config(len(case["q"]), len(case["faces"]))
+    h, err = create(lib, cfg, case["volume"], case["faces"])
+    assert h, err
+    assert lib.reactive_amd_transport_set_flux(None, 1) == -1
+    assert lib.reactive_amd_transport_set_flux(h, 7) == 1
+    assert b"Unknown face flux scheme" in lib.reactive_transport_error(h)
+    assert lib.reactive_amd_transport_set_flux(h, 1) == 0
+    lib.reactive_transport_destroy(h)
