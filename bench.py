@@ -8,11 +8,17 @@ energy from the isochoric equilibrium at (rho, T).  One step = CFL, stage 0,
 flash, CFL recheck, stage 1, flash, conservation totals, all resident on the
 device (reactive_amd_step_run).  Inputs are resident in HBM before timing.
 
+The timed flash is the mixed-precision one (FP32 search, FP64 refinement);
+the plain FP64 path is timed on the same case for the comparison, and the
+two are compared cell by cell after one step from the same states.
+
 Prints one JSON line (the bench contract) with:
-  roofline      fused flux sweep (both SSPRK2 stages): algorithmic bytes per
-                cell per stage / event-timed kernel duration vs 8 TB/s HBM
-  roofline_flash the dominant kernel (UV flash), FP64 VALU: see DESIGN.md
-  cpu_baseline  the reference's own CPU code (oracle/_ref) on a bounded sample
+  roofline       fused flux sweep (both SSPRK2 stages): algorithmic bytes per
+                 cell per stage / event-timed kernel duration vs 8 TB/s HBM
+  roofline_flash the flash kernels: time per launch, phase evaluations
+  fp64           the plain FP64 path on the same case
+  mixed_vs_fp64  flips and relative T, p, c_f errors after one step
+  cpu_baseline   the reference's own CPU code (oracle/_ref) on a bounded sample
 """
 import argparse
 import ctypes as C
@@ -152,6 +158,58 @@ def cpu_baseline(m, q_end, seeds_start, threads, sample, seed):
                        f"{int(succ.sum())}/{len(idx)} sample flashes converged")}
 
 
+def run_steps(lib, step, q, states, cfl, steps, mode, warmup):
+    """Loads (q, states), runs `warmup` untimed steps (at least one: it also
+    records the per-cell costs that order the timed flashes), then times
+    `steps` steps.  Returns (report, wall seconds, q and states after the
+    first warmup step)."""
+    def check(rc):
+        if rc:
+            raise RuntimeError(lib.reactive_amd_step_error(step).decode())
+    check(lib.reactive_amd_step_load(step, rlib.ptr(q), rlib.ptr(states)))
+    rep = abi.StepReport.fresh()
+    check(lib.reactive_amd_step_run(step, cfl, 1.0, 1, mode, C.byref(rep)))
+    q_one = np.zeros_like(q)
+    s_one = np.zeros(len(q), dtype=abi.STATE_DTYPE)
+    check(lib.reactive_amd_step_read(step, rlib.ptr(q_one), rlib.ptr(s_one)))
+    if warmup > 1:
+        check(lib.reactive_amd_step_run(step, cfl, 1.0, warmup - 1, mode, C.byref(rep)))
+    rep = abi.StepReport.fresh()
+    t0 = time.perf_counter()
+    check(lib.reactive_amd_step_run(step, cfl, 1.0, steps, mode, C.byref(rep)))
+    wall = time.perf_counter() - t0
+    return rep, wall, q_one, s_one
+
+
+def stage_roofline(rep, cells, steps, nv=5):
+    bytes0, bytes1 = 8 * (2 * nv + 2), 8 * (3 * nv + 2)  # algorithmic bytes per cell, stage 0 / stage 1
+    t_stage = (rep.stageSeconds[0] + rep.stageSeconds[1]) / steps
+    achieved = (bytes0 + bytes1) * cells / t_stage / 1e9
+    return {"bound": "hbm", "kernel": "cartesianStageKernel (both SSPRK2 stages)", "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "bytes_per_cell": [bytes0, bytes1],
+            "avg_launch_ms": [1e3 * rep.stageSeconds[0] / steps, 1e3 * rep.stageSeconds[1] / steps]}
+
+
+def census(states0, s64, smx):
+    """Mixed vs FP64 after one step from the same states: cells that failed
+    in one flash only or chose another phase set (flips), and the relative
+    T, p error over the rest."""
+    cells = len(s64)
+    raw0 = states0.view(np.uint8).reshape(cells, -1)
+    fail64 = (s64.view(np.uint8).reshape(cells, -1) == raw0).all(1)
+    failmx = (smx.view(np.uint8).reshape(cells, -1) == raw0).all(1)
+    both = ~fail64 & ~failmx
+    flips = int((fail64 != failmx).sum() + (smx["activeLiquids"][both] != s64["activeLiquids"][both]).sum())
+    same = both & (smx["activeLiquids"] == s64["activeLiquids"])
+    out = {"flips": flips, "compared_cells": int(same.sum())}
+    for f in ("T", "p", "soundFrozen"):
+        r = np.abs(smx[f][same] - s64[f][same]) / np.abs(s64[f][same])
+        out[f] = {"max": float(r.max()), "p99": float(np.percentile(r, 99)),
+                  "share_le_1e-10": float((r <= 1e-10).mean())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,7 +218,9 @@ def main():
     ap.add_argument("--n", type=int, default=160, help="cells per edge")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--cfl", type=float, default=0.4)
-    ap.add_argument("--cpu-sample", type=int, default=12288)
+    ap.add_argument("--flash", choices=["mixed", "fp64"], default="mixed",
+                    help="timed flash arithmetic; mixed also times the FP64 path for the comparison")
+    ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     a = ap.parse_args()
@@ -182,45 +242,42 @@ def main():
     step = lib.reactive_amd_step_create(tr, hem, err, 1024)
     if not step:
         raise RuntimeError(err.value.decode())
-
-    def check(rc):
-        if rc:
-            raise RuntimeError(lib.reactive_amd_step_error(step).decode())
-
-    check(lib.reactive_amd_step_load(step, rlib.ptr(q), rlib.ptr(states)))
-    rep = abi.StepReport.fresh()
-    log(f"bench: warmup {a.warmup} step(s)")
-    check(lib.reactive_amd_step_run(step, a.cfl, 1.0, 1, 0, C.byref(rep)))  # first warmup step
-    q_one = np.zeros_like(q)  # q after one step from (q, states): the CPU sample flashes exactly this step's flash #2
-    check(lib.reactive_amd_step_read(step, rlib.ptr(q_one), None))
-    if a.warmup > 1:
-        check(lib.reactive_amd_step_run(step, a.cfl, 1.0, a.warmup - 1, 0, C.byref(rep)))
-    log(f"bench: timing {a.steps} step(s)")
-    rep = abi.StepReport.fresh()
-    t0 = time.perf_counter()
-    check(lib.reactive_amd_step_run(step, a.cfl, 1.0, a.steps, 0, C.byref(rep)))
-    wall = time.perf_counter() - t0
-    log(f"bench: {wall:.3f} s for {a.steps} steps (device {rep.seconds:.3f} s)")
+    log(f"bench: FP64 flash, {a.warmup} warmup + {a.steps} timed step(s)")
+    rep64, wall64, q_one, s64 = run_steps(lib, step, q, states, a.cfl, a.steps, 0, a.warmup)
+    log(f"bench: FP64 {wall64:.3f} s for {a.steps} steps (device {rep64.seconds:.3f} s)")
+    value64 = cells * a.steps / wall64
+    t_flash64 = (rep64.flashSeconds[0] + rep64.flashSeconds[1]) / (2 * a.steps)
+    fp64 = {"value": value64, "ms_per_step": 1e3 * wall64 / a.steps, "flash_ms": 1e3 * t_flash64,
+            "phase_evaluations_per_cell_step": rep64.phaseEvaluations / (cells * a.steps),
+            "flash_failures": [int(rep64.flashFailures[0]), int(rep64.flashFailures[1])]}
+    if a.flash == "mixed":
+        log(f"bench: mixed flash, {a.warmup} warmup + {a.steps} timed step(s)")
+        rep, wall, _, smx = run_steps(lib, step, q, states, a.cfl, a.steps, 1, a.warmup)
+        log(f"bench: mixed {wall:.3f} s for {a.steps} steps (device {rep.seconds:.3f} s)")
+        accuracy = census(states, s64, smx)
+    else:
+        rep, wall, accuracy = rep64, wall64, None
     value = cells * a.steps / wall
-    nv = 5
-    bytes0, bytes1 = 8 * (2 * nv + 2), 8 * (3 * nv + 2)  # algorithmic bytes per cell, stage 0 / stage 1
-    t_stage = (rep.stageSeconds[0] + rep.stageSeconds[1]) / a.steps
-    achieved = (bytes0 + bytes1) * cells / t_stage / 1e9
     t_flash = (rep.flashSeconds[0] + rep.flashSeconds[1]) / (2 * a.steps)
+    mixed = a.flash == "mixed"
     result = {
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "fp32+fp64" if mixed else "f64", "data": "synthetic",
         "config": {"workload": f"cube{a.n}-n2o-pure", "cells": cells, "faces": len(faces), "cfl": a.cfl,
-                   "species": ["N2O"], "liquids": ["N2O"], "flash": "fp64 (plain, reference arithmetic)",
+                   "species": ["N2O"], "liquids": ["N2O"],
+                   "flash": ("mixed: FP32 candidate search, FP64 refinement, stability and selection" if mixed
+                             else "fp64 (plain, reference arithmetic)"),
                    "seed": a.seed, "two_phase_fraction": round(info["two_phase_fraction"], 4)},
-        "roofline": {"bound": "hbm", "kernel": "cartesianStageKernel (both SSPRK2 stages)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_cell": [bytes0, bytes1],
-                     "avg_launch_ms": [1e3 * rep.stageSeconds[0] / a.steps, 1e3 * rep.stageSeconds[1] / a.steps]},
-        "roofline_flash": {"bound": "valu-fp64", "kernel": "stepFlashKernel", "avg_launch_ms": 1e3 * t_flash,
+        "roofline": stage_roofline(rep, cells, a.steps),
+        "roofline_flash": {"bound": "valu", "kernel": "stepSearchKernel + stepRefineKernel" if mixed
+                           else "stepFlashKernel", "avg_launch_ms": 1e3 * t_flash,
                            "phase_evaluations_per_cell_step": rep.phaseEvaluations / (cells * a.steps),
+                           "fp32_phase_evaluations_per_cell_step": rep.singlePhaseEvaluations / (cells * a.steps),
                            "share_of_step": (rep.flashSeconds[0] + rep.flashSeconds[1]) / rep.seconds},
+        "fp64": fp64,
+        "speedup_vs_fp64": value / value64,
+        "mixed_vs_fp64": accuracy,
         "flash_failures": [int(rep.flashFailures[0]), int(rep.flashFailures[1])],
         "cfl_violations": int(rep.cflViolations), "dt": rep.dt,
     }

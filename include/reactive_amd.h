@@ -41,10 +41,18 @@ typedef struct ReactiveAmdStepReport {
     double flashSeconds[2];             /* flash #1 / flash #2 kernels */
     double cflSeconds;                  /* both CFL sweeps (+ their final minimum) */
     uint64_t fluxFailures;              /* face evaluations the HLLC flux rejected (0 under HLL) */
+    uint64_t singlePhaseEvaluations;    /* FP32 phase evaluations of the mixed flash (phaseEvaluations: FP64) */
 } ReactiveAmdStepReport;
 
-/* Flash arithmetic of a step (0 = plain FP64, the reference's arithmetic). */
+/* Flash arithmetic of a step.  FP64: the reference's arithmetic.  MIXED: the
+ * candidate search runs in FP32 (Newton iterations, cubic solves, line
+ * searches; the cubic's near-repeated-root classification in FP64) and
+ * records each converged candidate; FP64 Newton corrections then finish them
+ * (to 1e-3 of the tolerances), and stability, maximum entropy and the
+ * equilibrium sound speed are decided in FP64.  Cells whose FP32 decisions
+ * FP64 contradicts are searched again in FP64. */
 #define REACTIVE_AMD_FLASH_FP64 0
+#define REACTIVE_AMD_FLASH_MIXED 1
 
 /* Face flux of a transport handle (applies to reactive_transport_rhs/stage,
  * the resident stages and the resident step): REACTIVE_AMD_FLUX_HLL (the

@@ -9,11 +9,21 @@ flags=(--offload-arch=gfx950 -O3 -std=c++17 -fPIC -ffp-contract=off -fno-fast-ma
        -Xclang -target-feature -Xclang -packed-fp32-ops
        -I"$here/../include" -Wall -Wno-unused-result)
 objs=()
+pids=()
 for src in "$here"/csrc/*.hip; do
   obj="$here/_build/$(basename "${src%.hip}").o"
+  rm -f "$obj"
   "$hipcc" "${flags[@]}" -c "$src" -o "$obj" 2> >(grep -v "packed-fp32-ops" >&2) &
   objs+=("$obj")
+  pids+=("$!")
 done
-wait
+failed=0
+for pid in "${pids[@]}"; do
+  wait "$pid" || failed=1
+done
+if [ "$failed" -ne 0 ]; then
+  echo "build.sh: a HIP compile failed" >&2
+  exit 1
+fi
 "$hipcc" --offload-arch=gfx950 -shared -fPIC "${objs[@]}" -o "$here/lib/libreactiveTransport.so.tmp"
 mv "$here/lib/libreactiveTransport.so.tmp" "$here/lib/libreactiveTransport.so"

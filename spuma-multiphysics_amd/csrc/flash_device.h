@@ -13,6 +13,13 @@
 // (0 or 1) are template parameters, so every per-species and per-unknown
 // array is register resident; the evaluation record keeps only the fields
 // the flash reads; the model is a compact table image in global memory.
+//
+// The scalar type R is double for the reference arithmetic.  R = float is
+// the search half of the mixed-precision flash: the same candidate search
+// with single-precision tolerances and finite-difference steps, which records
+// each converged candidate's (ln p, ln T, liquid fraction) in a CandidateSet;
+// Flash<NS, NL, double>::refine then finishes the recorded candidates with
+// FP64 Newton corrections and decides stability and maximum entropy in FP64.
 #ifndef REACTIVE_AMD_FLASH_DEVICE_H
 #define REACTIVE_AMD_FLASH_DEVICE_H
 #include "pr_device.h"
@@ -50,14 +57,17 @@ struct Counts {
 };
 
 // Evaluation record (reactiveDeviceFlash.h:64-73) for at most one liquid.
-struct Eval {
-    double p, T, rho, e, entropy, cf, ceq, volRes, enRes, chemRes;
-    double alphaGas, alphaLiq, rhoGas, rhoLiq, liqMass, gasMass, cp, cv;
+template <class R>
+struct EvalT {
+    R p, T, rho, e, entropy, cf, ceq, volRes, enRes, chemRes;
+    R alphaGas, alphaLiq, rhoGas, rhoLiq, liqMass, gasMass, cp, cv;
     int active, iterations;
-    double volume, energy, entropyDensity, Vp, VT, Ep, ET, cpDensity, muGas, muLiquid;
+    R volume, energy, entropyDensity, Vp, VT, Ep, ET, cpDensity, muGas, muLiquid;
 };
+using Eval = EvalT<double>;
 
-RD void zeroEval(Eval& v) {
+template <class R>
+RD void zeroEval(EvalT<R>& v) {
     v.p = v.T = v.rho = v.e = v.entropy = v.cf = v.ceq = v.volRes = v.enRes = v.chemRes = 0;
     v.alphaGas = v.alphaLiq = v.rhoGas = v.rhoLiq = v.liqMass = v.gasMass = v.cp = v.cv = 0;
     v.active = v.iterations = 0;
@@ -75,10 +85,29 @@ RD void writeOut(const Eval& v, ThermoOut& o) {
     o.activeLiquids = v.active; o.iterations = v.iterations;
 }
 
+// Candidates of the single-precision search, in search order.
+enum CandidateKind : int16_t { kGasCandidate = 0, kLiquidCandidate = 1, kSeedCandidate = 2, kRestartCandidate = 3 };
+struct Candidate {
+    float x[3];     // ln p, ln T, liquid fraction (0 for the gas and all-liquid candidates)
+    int16_t kind;   // CandidateKind
+    int16_t index;  // seed j, or 5 t + j for restart temperature t
+};
+constexpr int kMaxCandidates = 6;
+enum CandidateFlags : uint32_t {
+    kRestartsSearched = 1u,  // the search ran the restart temperatures
+    kOverflow = 2u,          // more distinct candidates than the set holds
+    kSeedsPruned = 4u,       // the stable-gas pruning skipped the seeds
+};
+struct CandidateSet {
+    uint32_t count, flags;
+    Candidate c[kMaxCandidates];
+};
+static_assert(sizeof(CandidateSet) == 104, "candidate set layout");
+
 // Full-pivot LU with Eigen's rank rule (reactiveDeviceFlash.h:111-139), n fixed.
-template <int n>
-RD bool solveN(const double (&m)[3][3], const double* rhs, double* x) {
-    double a[n][n], b[n], largest = 0;
+template <int n, class R>
+RD bool solveN(const R (&m)[3][3], const R* rhs, R* x) {
+    R a[n][n], b[n], largest = 0;
     int column[n];
     for (int i = 0; i < n; ++i) {
         column[i] = i;
@@ -91,30 +120,30 @@ RD bool solveN(const double (&m)[3][3], const double* rhs, double* x) {
     }
     for (int k = 0; k < n; ++k) {
         int row = k, col = k;
-        double pivot = 0;
+        R pivot = 0;
         for (int j = k; j < n; ++j)
             for (int i = k; i < n; ++i)
-                if (::fabs(a[i][j]) > pivot) { pivot = ::fabs(a[i][j]); row = i; col = j; }
+                if (ab(a[i][j]) > pivot) { pivot = ab(a[i][j]); row = i; col = j; }
         largest = hi(largest, pivot);
-        if (!(pivot > largest * n * DBL_EPSILON)) return false;
+        if (!(pivot > largest * R(n) * R(Real<R>::eps))) return false;
         if (row != k) {
-            for (int j = 0; j < n; ++j) { const double t = a[k][j]; a[k][j] = a[row][j]; a[row][j] = t; }
-            const double t = b[k]; b[k] = b[row]; b[row] = t;
+            for (int j = 0; j < n; ++j) { const R t = a[k][j]; a[k][j] = a[row][j]; a[row][j] = t; }
+            const R t = b[k]; b[k] = b[row]; b[row] = t;
         }
         if (col != k) {
-            for (int i = 0; i < n; ++i) { const double t = a[i][k]; a[i][k] = a[i][col]; a[i][col] = t; }
+            for (int i = 0; i < n; ++i) { const R t = a[i][k]; a[i][k] = a[i][col]; a[i][col] = t; }
             const int t = column[k]; column[k] = column[col]; column[col] = t;
         }
         for (int i = k + 1; i < n; ++i) {
-            const double f = a[i][k] / a[k][k];
+            const R f = a[i][k] / a[k][k];
             a[i][k] = f;
             for (int j = k + 1; j < n; ++j) a[i][j] -= f * a[k][j];
             b[i] -= f * b[k];
         }
     }
-    double z[n];
+    R z[n];
     for (int i = n - 1; i >= 0; --i) {
-        double v = b[i];
+        R v = b[i];
         for (int j = i + 1; j < n; ++j) v -= a[i][j] * z[j];
         z[i] = v / a[i][i];
         if (!fin(z[i])) return false;
@@ -125,44 +154,83 @@ RD bool solveN(const double (&m)[3][3], const double* rhs, double* x) {
     return true;
 }
 
-template <int NS, int NL>
+// Precision-dependent flash settings.  FP64: the reference's values.  FP32:
+// tolerances and finite-difference steps at single-precision rounding, and a
+// hand-over threshold: an FP32 Newton that stalls (rejected line search)
+// below it still records its point for the FP64 refinement.
+template <class R> struct FlashPrecision;
+template <> struct FlashPrecision<double> {
+    static constexpr bool single = false;
+    static constexpr double hLog = 2e-6, hFraction = 2e-5;  // reactiveDeviceFlash.h:338-373
+    static constexpr double fractionEdge = 1e-8;             // seed clamp (:385)
+    static constexpr double rootCheck = 1e-7;                // |p_EOS - p| (reactiveDevicePR.h:574-609)
+    static constexpr double densityCheck = 1e-8;             // gas probe root consistency (:264-275)
+    static constexpr double vtolFloor = 0, etolFloor = 0, mutolFloor = 0, handOver = 0;
+};
+template <> struct FlashPrecision<float> {
+    static constexpr bool single = true;
+    static constexpr float hLog = 1e-3f, hFraction = 1e-3f;
+    static constexpr float fractionEdge = 1e-6f;
+    static constexpr float rootCheck = 5e-3f;
+    static constexpr float densityCheck = 1e-4f;
+    static constexpr float vtolFloor = 3e-5f, etolFloor = 3e-5f, mutolFloor = 3e-4f, handOver = 1e-3f;
+};
+
+template <int NS, int NL, class R = double>
 struct Flash {
     static_assert(NL == 0 || NL == 1, "one pure liquid at most");
+    using P = FlashPrecision<R>;
+    using E = EvalT<R>;
+    using S = PRStateT<R>;
+    using M = MixT<R>;
     const HemModel<NS>& m;
-    double q[NS];
-    double energy;
-    double guessP, guessT, guessLiquid;
+    R q[NS];
+    R energy;
+    R guessP, guessT, guessLiquid;
     Counts& n;
-    double cacheP, cacheT;  // one unavailable liquid result (reactiveDeviceFlash.h:142-145)
+    R cacheP, cacheT;  // one unavailable liquid result (reactiveDeviceFlash.h:142-145)
     int cacheStatus;
     bool cacheVirtual;
-    double memoKey[NS];     // composition of the memoised gas heuristic (NS > 1)
+    R memoKey[NS];     // composition of the memoised gas heuristic (NS > 1)
     PhaseHeuristic memo;
     bool memoValid;
+    R Tmin, Tmax, pmin, pmax, vtol, etol, mutol, liquidTmin, liquidTc;
+    // single-precision search: where converged candidates are recorded
+    CandidateSet* record;
+    bool stalled;  // the last FP32 Newton stopped near convergence (hand-over)
 
     RD Flash(const HemModel<NS>& model, Counts& counts) : m(model), n(counts) {
         cacheP = cacheT = 0;
         cacheStatus = -1;
         cacheVirtual = false;
         memoValid = false;
+        Tmin = R(m.Tmin); Tmax = R(m.Tmax); pmin = R(m.pmin); pmax = R(m.pmax);
+        vtol = hi(R(m.vtol), R(P::vtolFloor));
+        etol = hi(R(m.etol), R(P::etolFloor));
+        mutol = hi(R(m.mutol), R(P::mutolFloor));
+        liquidTmin = R(m.liquidTmin); liquidTc = R(m.liquidTc);
+        record = nullptr;
+        stalled = false;
     }
 
-    RD double density() const {
-        double r = 0;
+    RD R density() const {
+        R r = 0;
         for (int k = 0; k < NS; ++k) r += q[k];
         return r;
     }
-    RD bool bounds(double p, double T) const {
-        return fin(p) && fin(T) && p >= m.pmin && p <= m.pmax && T >= m.Tmin && T <= m.Tmax;
+    RD bool bounds(R p, R T) const {
+        return fin(p) && fin(T) && p >= pmin && p <= pmax && T >= Tmin && T <= Tmax;
     }
     static RD bool unavailable(int s) { return s == kUnavailable || s == kNoRoot; }
 
-    RD const PhaseHeuristic& gasHeuristic(const double* x, double W, const Mix& mx) {
+    RD const PhaseHeuristic& gasHeuristic(const R* x, R W, const M& mx) {
         if (NS == 1) return m.gasFixed;
         bool hit = memoValid;
         for (int k = 0; k < NS; ++k) hit = hit && memoKey[k] == x[k];
         if (!hit) {
-            phaseHeuristic(m.gas, x, W, mx.a, mx.b, memo);
+            double xd[NS];
+            for (int k = 0; k < NS; ++k) xd[k] = double(x[k]);
+            phaseHeuristic(m.gas, xd, double(W), double(mx.a), double(mx.b), memo);
             for (int k = 0; k < NS; ++k) memoKey[k] = x[k];
             memoValid = true;
         }
@@ -170,15 +238,15 @@ struct Flash {
     }
 
     // Gas PR state at (T, p) on the gas root (evaluateTPTrusted, reactiveDevicePR.h:574-609).
-    RAMD_NOINLINE int gasTP(double T, double p, const double* Y, PRState& s, double* mu) {
+    RAMD_NOINLINE int gasTP(R T, R p, const R* Y, S& s, R* mu) {
         if (!fin(T) || !fin(p) || T <= 0 || p <= 0) return kBadInput;
-        double x[NS], W = 0;
+        R x[NS], W = 0;
         int st = massToMole(m.gas, Y, x, W);
         if (st != kOk) return st;
-        Mix mx;
+        M mx;
         st = mixing(m.gas, T, x, mx);
         if (st != kOk) return st;
-        Roots r;
+        RootsT<R> r;
         st = roots(T, p, mx.a, mx.b, mx.aa, r);
         if (st != kOk) return st;
         if (!r.gas) return kUnavailable;
@@ -186,22 +254,22 @@ struct Flash {
         if (st != kOk) return st;
         st = finishState<NS>(gasHeuristic(x, W, mx), T, s, mu);
         if (st != kOk) return st;
-        if (ab(s.p - p) > 1e-7 * hi(1.0, p)) return kNonFinite;
+        if (ab(s.p - p) > R(P::rootCheck) * hi(R(1), p)) return kNonFinite;
         s.p = p;
         if (s.phase >= 0) return kUnavailable;
         return kOk;
     }
     // Pure-liquid PR state at (T, p) on the liquid root.
-    RAMD_NOINLINE int liquidTP(double T, double p, PRState& s, double* mu) {
+    RAMD_NOINLINE int liquidTP(R T, R p, S& s, R* mu) {
         if (!fin(T) || !fin(p) || T <= 0 || p <= 0) return kBadInput;
-        const double Y[1] = {1.0};
-        double x[1], W = 0;
+        const R Y[1] = {R(1)};
+        R x[1], W = 0;
         int st = massToMole(m.liquid, Y, x, W);
         if (st != kOk) return st;
-        Mix mx;
+        M mx;
         st = mixing(m.liquid, T, x, mx);
         if (st != kOk) return st;
-        Roots r;
+        RootsT<R> r;
         st = roots(T, p, mx.a, mx.b, mx.aa, r);
         if (st != kOk) return st;
         if (!r.liquid) return kUnavailable;
@@ -209,27 +277,27 @@ struct Flash {
         if (st != kOk) return st;
         st = finishState<1>(m.liquidFixed, T, s, mu);
         if (st != kOk) return st;
-        if (ab(s.p - p) > 1e-7 * hi(1.0, p)) return kNonFinite;
+        if (ab(s.p - p) > R(P::rootCheck) * hi(R(1), p)) return kNonFinite;
         s.p = p;
         if (s.phase < 0) return kUnavailable;
         return kOk;
     }
 
     // reactiveDeviceFlash.h:157-196 for index -1 (gas) and 0 (liquid).
-    RD int phaseStatus(int index, double p, double T, const double* Y, PRState& out, double* mu) {
+    RD int phaseStatus(int index, R p, R T, const R* Y, S& out, R* mu) {
         ++n.phases;
         if (!bounds(p, T)) return kBadInput;
-        if (index >= 0 && T < m.liquidTmin) return kBadInput;
-        if (index >= 0 && T >= m.liquidTc * (1 - 1e-10)) return kUnavailable;
+        if (index >= 0 && T < liquidTmin) return kBadInput;
+        if (index >= 0 && T >= liquidTc * (R(1) - R(1e-10))) return kUnavailable;
         if (index < 0 && m.enforceBounds)
             for (int k = 0; k < NS; ++k)
-                if (Y[k] > 0 && (T < m.gas.tlo[k] || T > m.gas.thi[k])) return kBadInput;
+                if (Y[k] > 0 && (T < R(m.gas.tlo[k]) || T > R(m.gas.thi[k]))) return kBadInput;
         if (index >= 0) return liquidTP(T, p, out, mu);
         return gasTP(T, p, Y, out, mu);
     }
 
-    static RD void accumulate(Eval& v, double mass, const PRState& p, double phaseP) {  // :203-207
-        const double volume = mass / p.rho, vp = -volume * p.kappaT, vt = volume * p.alpha;
+    static RD void accumulate(E& v, R mass, const S& p, R phaseP) {  // :203-207
+        const R volume = mass / p.rho, vp = -volume * p.kappaT, vt = volume * p.alpha;
         v.volume += volume; v.energy += mass * p.e; v.entropyDensity += mass * p.s; v.Vp += vp; v.VT += vt;
         v.Ep += -v.T * vt - phaseP * vp;
         v.ET += mass * p.cp - phaseP * vt;
@@ -237,14 +305,14 @@ struct Flash {
     }
 
     // Gas + liquid mixture at common (p, T) (reactiveDeviceFlash.h:208-262).
-    RAMD_NOINLINE bool evaluate(double mass0, double p, double T, Eval& v, bool virtualLiquids) {
+    RAMD_NOINLINE bool evaluate(R mass0, R p, R T, E& v, bool virtualLiquids) {
         zeroEval(v);
         if (!bounds(p, T)) return false;
         if (NL > 0 && mass0 > 0 && cacheStatus >= 0 && p == cacheP && T == cacheT && virtualLiquids == cacheVirtual)
             return false;
         v.p = p; v.T = T; v.rho = density();
         if (!(v.rho > 0)) return false;
-        double gas[NS];
+        R gas[NS];
         for (int k = 0; k < NS; ++k) {
             if (!fin(q[k]) || q[k] < 0) return false;
             gas[k] = q[k];
@@ -255,39 +323,39 @@ struct Flash {
             gas[c] = gas[c] - mass0;
             if (!fin(gas[c]) || gas[c] < 0 || gas[c] > q[c]) return false;
         }
-        double mg = 0;
+        R mg = 0;
         for (int k = 0; k < NS; ++k) mg += gas[k];
         v.gasMass = mg;
         if (mg > 0) {
             for (int k = 0; k < NS; ++k) gas[k] /= mg;
-            PRState prop;
-            double mu[NS];
+            S prop;
+            R mu[NS];
             if (phaseStatus(-1, p, T, gas, prop, virtualLiquids ? mu : nullptr) != kOk) return false;
             accumulate(v, mg, prop, p);
             v.alphaGas = mg / prop.rho;
             v.rhoGas = prop.rho;
-            if (NL && virtualLiquids) v.muGas = mu[c] / (8314.46261815324 * T);
+            if (NL && virtualLiquids) v.muGas = mu[c] / (R(8314.46261815324) * T);
         }
         if (NL) {
             v.liqMass = mass0;
             if (mass0 > 0 || (virtualLiquids && q[c] > 0)) {
-                PRState prop;
-                double mu[1];
+                S prop;
+                R mu[1];
                 int st;
                 if (cacheStatus >= 0 && p == cacheP && T == cacheT && virtualLiquids == cacheVirtual) {
                     ++n.phases;
                     st = cacheStatus;
                 } else {
                     st = phaseStatus(0, p, T, nullptr, prop, virtualLiquids ? mu : nullptr);
-                    if (unavailable(st) && T >= m.liquidTmin && T < m.liquidTc * (1 - 1e-10)) {
+                    if (unavailable(st) && T >= liquidTmin && T < liquidTc * (R(1) - R(1e-10))) {
                         cacheP = p; cacheT = T; cacheVirtual = virtualLiquids; cacheStatus = st;
                     }
                 }
                 if (st != kOk) {
                     if (mass0 > 0 || !unavailable(st)) return false;
-                    v.muLiquid = __builtin_huge_val();
+                    v.muLiquid = R(Real<R>::big) * R(2);  // +infinity
                 } else {
-                    if (virtualLiquids) v.muLiquid = mu[0] / (8314.46261815324 * T);
+                    if (virtualLiquids) v.muLiquid = mu[0] / (R(8314.46261815324) * T);
                     if (mass0 > 0) {
                         accumulate(v, mass0, prop, p);
                         v.rhoLiq = prop.rho;
@@ -300,139 +368,144 @@ struct Flash {
         v.e = v.energy / v.rho; v.entropy = v.entropyDensity / v.rho; v.cp = v.cpDensity / v.rho;
         v.cv = (v.cpDensity + T * v.VT * v.VT / v.Vp) / v.rho;
         if (!fin(v.cv) || v.cv <= 0) return false;
-        const double specific = v.volume / v.rho, vp = v.Vp / v.rho, vt = v.VT / v.rho;
-        const double adiabatic = vp + T * vt * vt / v.cp;
+        const R specific = v.volume / v.rho, vp = v.Vp / v.rho, vt = v.VT / v.rho;
+        const R adiabatic = vp + T * vt * vt / v.cp;
         if (!(adiabatic < 0)) return false;
         v.cf = m_sqrt(-specific * specific / adiabatic);
         v.ceq = v.cf;
         return fin(v.cf) && v.cf > 0;
     }
-    RD double energyScale(const Eval& v) const { return hi(1e5 * v.rho, v.cpDensity * v.T); }
+    RD R energyScale(const E& v) const { return hi(R(1e5) * v.rho, v.cpDensity * v.T); }
 
     // Single-phase gas energy residual at fixed rho (reactiveDeviceFlash.h:264-275).
-    RAMD_NOINLINE bool gasProbe(double T, double& F, double& cv, double& p, double* cp) {
-        const double rho = density();
-        double Y[NS];
+    RAMD_NOINLINE bool gasProbe(R T, R& F, R& cv, R& p, R* cp) {
+        const R rho = density();
+        R Y[NS];
         for (int k = 0; k < NS; ++k) Y[k] = q[k] / rho;
         ++n.phases;
-        PRState prop;
+        S prop;
         {
             if (!fin(T) || !fin(rho) || T <= 0 || rho <= 0) return false;
-            double x[NS], W = 0;
+            R x[NS], W = 0;
             if (massToMole(m.gas, Y, x, W) != kOk) return false;
-            Mix mx;
+            M mx;
             if (mixing(m.gas, T, x, mx) != kOk) return false;
-            if (stateTrho(m.gas, T, rho, x, W, mx, prop, nullptr) != kOk) return false;
-            if (finishState<NS>(gasHeuristic(x, W, mx), T, prop, nullptr) != kOk) return false;
+            if (stateTrho(m.gas, T, rho, x, W, mx, prop, (R*)nullptr) != kOk) return false;
+            if (finishState<NS>(gasHeuristic(x, W, mx), T, prop, (const R*)nullptr) != kOk) return false;
         }
-        const double gasP = prop.p;
-        p = gasP + 0.0 * 0.0;
+        const R gasP = prop.p;
+        p = gasP + R(0) * R(0);
         if (!bounds(p, T)) return false;
-        PRState gas;
-        if (phaseStatus(-1, gasP, T, Y, gas, nullptr) != kOk || ::fabs(gas.rho - rho) > 1e-8 * rho) return false;
+        S gas;
+        if (phaseStatus(-1, gasP, T, Y, gas, nullptr) != kOk || ab(gas.rho - rho) > R(P::densityCheck) * rho)
+            return false;
         F = rho * prop.e - energy;
         cv = rho * prop.cv;
         if (cp) *cp = prop.cp;
         return fin(F) && fin(cv) && cv > 0;
     }
     // Bracketed scalar Newton in T (reactiveDeviceFlash.h:276-299).
-    RAMD_NOINLINE bool scalarGas(Eval& result) {
-        const double rho = density();
-        double T = clamp(guessT, m.Tmin, m.Tmax), F, cv, p, cp;
+    RAMD_NOINLINE bool scalarGas(E& result) {
+        const R rho = density();
+        R T = clamp(guessT, Tmin, Tmax), F, cv, p, cp;
         if (!gasProbe(T, F, cv, p, &cp)) return false;
-        const double scale = hi(1e5 * rho, rho * cp * T);
-        double lower = T, upper = T, fl = F, fh = F;
-        if (::fabs(F) > m.etol * scale) {
-            double radius = hi(1., .01 * T);
-            for (int j = 0; j < 24 && (fl > 0 || fh < 0); ++j, radius *= 2) {
+        const R scale = hi(R(1e5) * rho, rho * cp * T);
+        R lower = T, upper = T, fl = F, fh = F;
+        if (ab(F) > etol * scale) {
+            R radius = hi(R(1), R(.01) * T);
+            for (int j = 0; j < 24 && (fl > 0 || fh < 0); ++j, radius *= R(2)) {
                 if (fl > 0) {
-                    double candidate = hi(m.Tmin, T - radius), value, dcv, dp;
+                    R candidate = hi(Tmin, T - radius), value, dcv, dp;
                     bool found = false;
                     for (int retry = 0; retry < 8 && !found; ++retry) {
                         if (gasProbe(candidate, value, dcv, dp, nullptr)) { fl = value; lower = candidate; found = true; }
-                        else candidate = .5 * (lower + candidate);
+                        else candidate = R(.5) * (lower + candidate);
                     }
                     if (!found) return false;
                 }
                 if (fh < 0) {
-                    double candidate = lo(m.Tmax, T + radius), value, dcv, dp;
+                    R candidate = lo(Tmax, T + radius), value, dcv, dp;
                     bool found = false;
                     for (int retry = 0; retry < 8 && !found; ++retry) {
                         if (gasProbe(candidate, value, dcv, dp, nullptr)) { fh = value; upper = candidate; found = true; }
-                        else candidate = .5 * (upper + candidate);
+                        else candidate = R(.5) * (upper + candidate);
                     }
                     if (!found) return false;
                 }
-                if (::fabs(fl) <= m.etol * scale) { T = lower; if (!gasProbe(T, F, cv, p, nullptr)) return false; break; }
-                if (::fabs(fh) <= m.etol * scale) { T = upper; if (!gasProbe(T, F, cv, p, nullptr)) return false; break; }
-                if (lower == m.Tmin && upper == m.Tmax) break;
+                if (ab(fl) <= etol * scale) { T = lower; if (!gasProbe(T, F, cv, p, nullptr)) return false; break; }
+                if (ab(fh) <= etol * scale) { T = upper; if (!gasProbe(T, F, cv, p, nullptr)) return false; break; }
+                if (lower == Tmin && upper == Tmax) break;
             }
-            if (!(::fabs(F) <= m.etol * scale || (fl <= 0 && fh >= 0 && upper > lower))) return false;
+            if (!(ab(F) <= etol * scale || (fl <= 0 && fh >= 0 && upper > lower))) return false;
         }
         int it = 0;
-        for (; it < 70 && ::fabs(F) > m.etol * scale; ++it) {
+        for (; it < 70 && ab(F) > etol * scale; ++it) {
             if (F > 0) upper = T; else lower = T;
-            const double next = T - F / cv;
-            T = fin(next) && next > lower && next < upper ? next : .5 * (lower + upper);
+            const R next = T - F / cv;
+            T = fin(next) && next > lower && next < upper ? next : R(.5) * (lower + upper);
             if (!gasProbe(T, F, cv, p, nullptr)) return false;
         }
-        if (::fabs(F) > m.etol * scale) return false;
-        if (!evaluate(0.0, p, T, result, false)) return false;
-        result.volRes = ::fabs(result.volume - 1);
-        result.enRes = ::fabs(result.energy - energy) / energyScale(result);
+        if (ab(F) > etol * scale) return false;
+        if (!evaluate(R(0), p, T, result, false)) return false;
+        result.volRes = ab(result.volume - R(1));
+        result.enRes = ab(result.energy - energy) / energyScale(result);
         result.iterations = it;
-        return result.volRes <= m.vtol && result.enRes <= m.etol;
+        return result.volRes <= vtol && result.enRes <= etol;
     }
     // Fixed split, 2x2 Newton in (ln p, ln T) with line search (reactiveDeviceFlash.h:300-326).
-    RAMD_NOINLINE bool frozen(double mass0, Eval& value, int stallLimit, double mass1 = 0.0) {
+    RAMD_NOINLINE bool frozen(R mass0, E& value, int stallLimit, R mass1 = R(0)) {
+        stalled = false;
         if (!fin(energy)) return false;
         if (m.scalarRecovery && mass0 == 0 && mass1 == 0 && scalarGas(value)) return true;
-        double p = clamp(guessP, m.pmin, m.pmax), T = clamp(guessT, m.Tmin, m.Tmax);
+        R p = clamp(guessP, pmin, pmax), T = clamp(guessT, Tmin, Tmax);
         if (!evaluate(mass0, p, T, value, false)) return false;
-        int stalled = 0;
+        int stalls = 0;
         for (int it = 0; it < 70; ++it) {
-            const double scale = energyScale(value), rv = value.volume - 1, re = (value.energy - energy) / scale;
-            value.volRes = ::fabs(rv);
-            value.enRes = ::fabs(re);
+            const R scale = energyScale(value), rv = value.volume - R(1), re = (value.energy - energy) / scale;
+            value.volRes = ab(rv);
+            value.enRes = ab(re);
             value.iterations = it;
-            if (::fabs(rv) <= m.vtol && ::fabs(re) <= m.etol) return true;
-            double jac[3][3] = {};
+            if (ab(rv) <= vtol && ab(re) <= etol) return true;
+            R jac[3][3] = {};
             jac[0][0] = value.Vp * p; jac[0][1] = value.VT * T;
             jac[1][0] = value.Ep * p / scale; jac[1][1] = value.ET * T / scale;
-            double rhs[2] = {-rv, -re}, step[2];
+            R rhs[2] = {-rv, -re}, step[2];
             if (!solveN<2>(jac, rhs, step)) return false;
-            const double limit = hi(1., hi(::fabs(step[0]) / .7, ::fabs(step[1]) / .2));
+            const R limit = hi(R(1), hi(ab(step[0]) / R(.7), ab(step[1]) / R(.2)));
             step[0] /= limit; step[1] /= limit;
-            const double previous = hi(::fabs(rv), ::fabs(re));
+            const R previous = hi(ab(rv), ab(re));
             bool accepted = false;
-            for (double fraction = 1; fraction > 1e-9; fraction *= .5) {
-                Eval trial;
-                const double pp = p * m_exp(fraction * step[0]), tt = T * m_exp(fraction * step[1]);
+            for (R fraction = 1; fraction > R(1e-9); fraction *= R(.5)) {
+                E trial;
+                const R pp = p * m_exp(fraction * step[0]), tt = T * m_exp(fraction * step[1]);
                 if (evaluate(mass0, pp, tt, trial, false)) {
-                    const double next = hi(::fabs(trial.volume - 1), ::fabs(trial.energy - energy) / scale);
-                    if (next < previous * (1 - 1e-4 * fraction) || next < lo(m.vtol, m.etol)) {
+                    const R next = hi(ab(trial.volume - R(1)), ab(trial.energy - energy) / scale);
+                    if (next < previous * (R(1) - R(1e-4) * fraction) || next < lo(vtol, etol)) {
                         value = trial; p = pp; T = tt; accepted = true;
-                        stalled = fraction < 1. / 32 ? stalled + 1 : 0;
+                        stalls = fraction < R(1) / R(32) ? stalls + 1 : 0;
                         break;
                     }
                 }
             }
-            if (!accepted) return false;
-            if (stallLimit > 0 && stalled >= stallLimit &&
-                !(::fabs(value.volume - 1) <= m.vtol && ::fabs(value.energy - energy) / energyScale(value) <= m.etol))
+            if (!accepted) {
+                stalled = P::single && previous <= R(P::handOver);
+                return false;
+            }
+            if (stallLimit > 0 && stalls >= stallLimit &&
+                !(ab(value.volume - R(1)) <= vtol && ab(value.energy - energy) / energyScale(value) <= etol))
                 return false;
         }
         return false;
     }
     // Residual [V-1, (E-e)/scale, mu_l/RT - mu_g/RT] in x = [ln p, ln T, beta]
     // (reactiveDeviceFlash.h:327-337).
-    RAMD_NOINLINE bool residual(const double* x, double scale, double* f, Eval* out) {
+    RAMD_NOINLINE bool residual(const R* x, R scale, R* f, E* out) {
         ++n.residuals;
         if (!(x[2] >= 0 && x[2] < 1)) return false;
-        const double mass0 = x[2] * q[m.condensable];
-        Eval v;
+        const R mass0 = x[2] * q[m.condensable];
+        E v;
         if (!evaluate(mass0, m_exp(x[0]), m_exp(x[1]), v, true) || !(v.gasMass > 0)) return false;
-        f[0] = v.volume - 1;
+        f[0] = v.volume - R(1);
         f[1] = (v.energy - energy) / scale;
         f[2] = v.muLiquid - v.muGas;
         if (!fin(f[2])) return false;
@@ -440,13 +513,13 @@ struct Flash {
         return true;
     }
     // Central finite differences with step halving (reactiveDeviceFlash.h:338-373).
-    RAMD_NOINLINE bool jacobian(const double* x, double scale, const double* f, double (&jac)[3][3]) {
+    RAMD_NOINLINE bool jacobian(const R* x, R scale, const R* f, R (&jac)[3][3]) {
         ++n.fd;
         for (int j = 0; j < 3; ++j) {
-            double h = j < 2 ? 2e-6 : 2e-5;
+            R h = j < 2 ? R(P::hLog) : R(P::hFraction);
             bool success = false;
-            for (int attempt = 0; attempt < 12 && !success; ++attempt, h *= .5) {
-                double xp[3], xm[3], fp[3] = {}, fm[3] = {};
+            for (int attempt = 0; attempt < 12 && !success; ++attempt, h *= R(.5)) {
+                R xp[3], xm[3], fp[3] = {}, fm[3] = {};
                 for (int i = 0; i < 3; ++i) xp[i] = xm[i] = x[i];
                 xp[j] += h;
                 xm[j] -= h;
@@ -454,7 +527,7 @@ struct Flash {
                 if (!plus && !minus) continue;
                 success = true;
                 for (int i = 0; i < 3; ++i) {
-                    const double v = plus && minus ? (fp[i] - fm[i]) / (2 * h) : plus ? (fp[i] - f[i]) / h : (f[i] - fm[i]) / h;
+                    const R v = plus && minus ? (fp[i] - fm[i]) / (R(2) * h) : plus ? (fp[i] - f[i]) / h : (f[i] - fm[i]) / h;
                     jac[i][j] = v;
                     success = success && fin(v);
                 }
@@ -463,111 +536,135 @@ struct Flash {
         }
         return true;
     }
-    // Damped Newton on the one-liquid active set (reactiveDeviceFlash.h:374-448).
-    RAMD_NOINLINE bool activeFlash(double seed, Eval& value, bool restart, double restartT) {
-        ++n.candidates;
-        const double rho = density();
-        const double gT = restart ? restartT : guessT;
-        const double scale = hi(1e5 * rho, hi(::fabs(energy), rho * 2000 * hi(gT, m.Tmin)));
-        double x[3];
-        x[0] = m_log(clamp(guessP, m.pmin, m.pmax));
-        x[1] = m_log(clamp(gT, m.Tmin, m.Tmax));
-        const double previousFraction = guessLiquid / q[m.condensable];
-        x[2] = seed < 0 ? clamp(previousFraction, 1e-8, 1 - 1e-8) : seed;
-        double f[3];
+    // Start of an active-set solve: x = [ln p, ln T, seed fraction] and the
+    // energy scale (reactiveDeviceFlash.h:374-390).
+    RD void activeStart(R seed, bool restart, R restartT, R* x, R& scale) const {
+        const R rho = density();
+        const R gT = restart ? restartT : guessT;
+        scale = hi(R(1e5) * rho, hi(ab(energy), rho * R(2000) * hi(gT, Tmin)));
+        x[0] = m_log(clamp(guessP, pmin, pmax));
+        x[1] = m_log(clamp(gT, Tmin, Tmax));
+        const R previousFraction = guessLiquid / q[m.condensable];
+        x[2] = seed < 0 ? clamp(previousFraction, R(P::fractionEdge), R(1) - R(P::fractionEdge)) : seed;
+    }
+    // Damped Newton on the one-liquid active set (reactiveDeviceFlash.h:391-448)
+    // from x.  polish > 0 takes up to that many further Newton steps once the
+    // tolerances hold, while the residual is above 1e-3 of them (the FP64
+    // refinement of a single-precision candidate).  In single precision the
+    // converged point is returned without the equilibrium sound speed.
+    RAMD_NOINLINE bool activeNewton(R* x, R scale, E& value, int polish) {
+        stalled = false;
+        const R rho = density();
+        R f[3];
         if (!residual(x, scale, f, &value)) return false;
         for (int it = 0; it < 90; ++it) {
-            const double chemical = hi(0.0, ::fabs(f[2]));
-            double jac[3][3];
-            if (::fabs(f[0]) <= m.vtol && ::fabs(f[1]) <= m.etol && chemical <= m.mutol) {
-                value.volRes = ::fabs(f[0]);
-                value.enRes = ::fabs(f[1]);
+            const R chemical = hi(R(0), ab(f[2]));
+            R jac[3][3];
+            if (ab(f[0]) <= vtol && ab(f[1]) <= etol && chemical <= mutol) {
+                value.volRes = ab(f[0]);
+                value.enRes = ab(f[1]);
                 value.chemRes = chemical;
                 value.iterations = it;
-                if (!jacobian(x, scale, f, jac)) return false;
-                double rhs[3] = {-1 / rho, value.p / (rho * scale), 0.0}, response[3];
-                if (!solveN<3>(jac, rhs, response)) return false;
-                const double squared = value.p * response[0];
-                if (!fin(squared) || squared <= 0) return false;
-                value.ceq = m_sqrt(squared);
-                return value.ceq <= value.cf * (1 + 2e-3);
+                if (P::single) return true;
+                const bool tight = ab(f[0]) <= R(1e-3) * vtol && ab(f[1]) <= R(1e-3) * etol &&
+                                   chemical <= R(1e-3) * mutol;
+                if (polish > 0 && !tight) {
+                    --polish;
+                } else {
+                    if (!jacobian(x, scale, f, jac)) return false;
+                    R rhs[3] = {-R(1) / rho, value.p / (rho * scale), R(0)}, response[3];
+                    if (!solveN<3>(jac, rhs, response)) return false;
+                    const R squared = value.p * response[0];
+                    if (!fin(squared) || squared <= 0) return false;
+                    value.ceq = m_sqrt(squared);
+                    return value.ceq <= value.cf * (R(1) + R(2e-3));
+                }
             }
             if (!jacobian(x, scale, f, jac)) return false;
-            double rhs[3] = {-f[0], -f[1], -f[2]}, step[3];
+            R rhs[3] = {-f[0], -f[1], -f[2]}, step[3];
             if (!solveN<3>(jac, rhs, step)) return false;
-            double limiter = hi(1., hi(::fabs(step[0]) / .7, ::fabs(step[1]) / .18));
-            limiter = hi(limiter, ::fabs(step[2]) / .3);
+            R limiter = hi(R(1), hi(ab(step[0]) / R(.7), ab(step[1]) / R(.18)));
+            limiter = hi(limiter, ab(step[2]) / R(.3));
             for (int j = 0; j < 3; ++j) step[j] /= limiter;
             bool accepted = false;
-            const double previous = hi(hi(hi(0.0, ::fabs(f[0])), ::fabs(f[1])), ::fabs(f[2]));
-            double fraction = 1;
-            for (int index = 0; index < 30; ++index, fraction *= .5) {
-                double tx[3], tf[3];
+            const R previous = hi(hi(hi(R(0), ab(f[0])), ab(f[1])), ab(f[2]));
+            R fraction = 1;
+            for (int index = 0; index < 30; ++index, fraction *= R(.5)) {
+                R tx[3], tf[3];
                 for (int j = 0; j < 3; ++j) tx[j] = x[j] + fraction * step[j];
                 if (!(tx[2] >= 0 && tx[2] < 1)) {
                     // rejected fractions form a prefix; bisect to its end (:419-438)
                     int lower = index, upper = 30;
                     while (upper - lower > 1) {
                         const int mid = (lower + upper) / 2;
-                        const double y = x[2] + ::ldexp(1.0, -mid) * step[2];
+                        const R y = x[2] + R(::ldexp(1.0, -mid)) * step[2];
                         if (!(y >= 0 && y < 1)) lower = mid; else upper = mid;
                     }
                     n.residuals += uint32_t(upper - index);
                     if (upper == 30) break;
                     index = upper;
-                    fraction = ::ldexp(1.0, -upper);
+                    fraction = R(::ldexp(1.0, -upper));
                     for (int j = 0; j < 3; ++j) tx[j] = x[j] + fraction * step[j];
                 }
-                Eval trial;
+                E trial;
                 if (residual(tx, scale, tf, &trial) &&
-                    hi(hi(hi(0.0, ::fabs(tf[0])), ::fabs(tf[1])), ::fabs(tf[2])) < previous * (1 - 1e-4 * fraction)) {
+                    hi(hi(hi(R(0), ab(tf[0])), ab(tf[1])), ab(tf[2])) < previous * (R(1) - R(1e-4) * fraction)) {
                     for (int j = 0; j < 3; ++j) { x[j] = tx[j]; f[j] = tf[j]; }
                     value = trial;
                     accepted = true;
                     break;
                 }
             }
-            if (!accepted) return false;
+            if (!accepted) {
+                stalled = P::single && previous <= R(P::handOver);
+                return false;
+            }
         }
         return false;
     }
+    RAMD_NOINLINE bool activeFlash(R seed, E& value, bool restart, R restartT) {
+        ++n.candidates;
+        R x[3], scale;
+        activeStart(seed, restart, restartT, x, scale);
+        return activeNewton(x, scale, value, 0);
+    }
     // Stability (reactiveDeviceFlash.h:449-485) for at most one liquid.
-    RAMD_NOINLINE bool stable(Eval& value) {
-        const double p = value.p, T = value.T;
+    RAMD_NOINLINE bool stable(E& value) {
+        const R p = value.p, T = value.T;
         if (value.gasMass > 0) {
-            Eval comparison;
+            E comparison;
             if (!evaluate(value.liqMass, p, T, comparison, true)) return false;
             if (!NL || q[m.condensable] == 0) return true;
-            const double affinity = comparison.muLiquid - comparison.muGas;
+            const R affinity = comparison.muLiquid - comparison.muGas;
             if (value.liqMass > 0) {
-                if (!fin(affinity) || ::fabs(affinity) > m.mutol * 2) return false;
-            } else if (affinity < -m.mutol) return false;
+                if (!fin(affinity) || ab(affinity) > mutol * R(2)) return false;
+            } else if (affinity < -mutol) return false;
             return true;
         }
         if (!NL || !(value.liqMass > 0)) return false;
-        PRState prop;
-        double mu[1];
+        S prop;
+        R mu[1];
         if (phaseStatus(0, p, T, nullptr, prop, mu) != kOk) return false;
-        const double muL = mu[0] / (8314.46261815324 * T);
+        const R muL = mu[0] / (R(8314.46261815324) * T);
         // Tangent-plane distance of the pure vapour of the condensable (:474-485, n = 1).
-        double mole[NS], Y[NS], sum = 0;
+        R mole[NS], Y[NS], sum = 0;
         for (int k = 0; k < NS; ++k) mole[k] = 0;
         mole[m.condensable] = 1;
-        for (int k = 0; k < NS; ++k) { Y[k] = mole[k] * m.weights[k]; sum += Y[k]; }
+        for (int k = 0; k < NS; ++k) { Y[k] = mole[k] * R(m.weights[k]); sum += Y[k]; }
         for (int k = 0; k < NS; ++k) Y[k] /= sum;
-        PRState gas;
+        S gas;
         const int st = phaseStatus(-1, p, T, Y, gas, nullptr);
-        double tangent;
-        if (st != kOk) tangent = unavailable(st) ? __builtin_huge_val() : __builtin_nan("");
+        R tangent;
+        if (st != kOk) tangent = unavailable(st) ? R(Real<R>::big) * R(2) : R(0) / R(0);
         else {
-            const double reference = 0 + mole[m.condensable] * muL;
-            double W = 0;
-            for (int k = 0; k < NS; ++k) W += mole[k] * m.weights[k];
-            tangent = (gas.h - T * gas.s) * W / (8314.46261815324 * T) - reference;
+            const R reference = R(0) + mole[m.condensable] * muL;
+            R W = 0;
+            for (int k = 0; k < NS; ++k) W += mole[k] * R(m.weights[k]);
+            tangent = (gas.h - T * gas.s) * W / (R(8314.46261815324) * T) - reference;
         }
-        return tangent >= -m.mutol;
+        return tangent >= -mutol;
     }
-    RD void consider(Eval& candidate, Eval& best, bool& have) {  // :486-489
+    RD void consider(E& candidate, E& best, bool& have) {  // :486-489
         if (stable(candidate)) {
             ++n.stable;
             if (!have || candidate.entropyDensity > best.entropyDensity) { best = candidate; have = true; }
@@ -575,13 +672,21 @@ struct Flash {
             ++n.failures;
         }
     }
+    RD R restartTemperature(int t) const {  // reactiveFlashSeeds.h:13-16
+        const R lower = hi(Tmin, liquidTmin), upper = lo(Tmax, liquidTc * (R(1) - R(1e-10)));
+        const R frac = t == 0 ? R(0.02) : t == 1 ? R(0.25) : t == 2 ? R(0.5) : R(0.75);
+        return lower + (upper - lower) * frac;
+    }
+    RD bool restartRange() const {
+        return lo(Tmax, liquidTc * (R(1) - R(1e-10))) > hi(Tmin, liquidTmin);
+    }
     // Candidate search (reactiveDeviceFlash.h:490-543).
-    RAMD_NOINLINE bool search(Eval& best) {
+    RAMD_NOINLINE bool search(E& best) {
         bool have = false;
-        Eval candidate;
+        E candidate;
         bool gasStable = false;
         const bool prunable = m.stableGasPrune != 0;
-        if (frozen(0.0, candidate, prunable ? 4 : 0)) {
+        if (frozen(R(0), candidate, prunable ? 4 : 0)) {
             const uint32_t before = n.stable;
             consider(candidate, best, have);
             gasStable = n.stable > before && candidate.gasMass > 0;
@@ -591,38 +696,210 @@ struct Flash {
         if (!NL) return have;
         const bool prune = prunable && gasStable;
         const int c = m.condensable;
-        double other = 0;
+        R other = 0;
         for (int k = 0; k < NS; ++k)
             if (k != c) other += q[k];
         if (other == 0) {
             if (frozen(q[c], candidate, 0)) consider(candidate, best, have);
             else ++n.failures;
         }
-        const double seeds[5] = {-1, .5, .95, .1, .9999};
+        const R seeds[5] = {R(-1), R(.5), R(.95), R(.1), R(.9999)};
         if (!prune && q[c] > 0)
             for (int j = 0; j < 5; ++j) {
-                if (activeFlash(seeds[j], candidate, false, 0.0)) consider(candidate, best, have);
+                if (activeFlash(seeds[j], candidate, false, R(0))) consider(candidate, best, have);
                 else ++n.failures;
             }
-        if (!have && q[c] > 0) {
-            const double lower = hi(m.Tmin, m.liquidTmin), upper = lo(m.Tmax, m.liquidTc * (1 - 1e-10));
-            if (upper > lower)
-                for (int t = 0; t < 4; ++t) {
-                    const double frac = t == 0 ? 0.02 : t == 1 ? 0.25 : t == 2 ? 0.5 : 0.75;  // reactiveFlashSeeds.h:13-16
-                    const double restartT = lower + (upper - lower) * frac;
-                    for (int j = 0; j < 5; ++j) {
-                        if (activeFlash(seeds[j], candidate, true, restartT)) consider(candidate, best, have);
-                        else ++n.failures;
-                    }
+        if (!have && q[c] > 0 && restartRange())
+            for (int t = 0; t < 4; ++t) {
+                const R restartT = restartTemperature(t);
+                for (int j = 0; j < 5; ++j) {
+                    if (activeFlash(seeds[j], candidate, true, restartT)) consider(candidate, best, have);
+                    else ++n.failures;
                 }
-        }
+            }
         return have;
     }
-    RD bool run(Eval& result) {  // :544-560, boundaryRecovery = 0
+    RD bool run(E& result) {  // :544-560, boundaryRecovery = 0
         if (!fin(energy) || !(density() > 0)) return false;
         for (int k = 0; k < NS; ++k)
             if (!fin(q[k]) || q[k] < 0) return false;
         return search(result);
+    }
+
+    // ---- single-precision search (R = float): the candidate search with the
+    // converged candidates recorded instead of selected.  Stability and
+    // entropy are judged here only to decide, as the reference does, whether
+    // the restart temperatures run; the FP64 refinement judges them again.
+    RD void note(int16_t kind, int16_t index, const R* x, bool dedupe) {
+        CandidateSet& set = *record;
+        if (dedupe)
+            for (uint32_t i = 0; i < set.count; ++i) {
+                const Candidate& o = set.c[i];
+                if (o.kind == kind && ab(float(x[0]) - o.x[0]) <= 1e-4f && ab(float(x[1]) - o.x[1]) <= 1e-4f &&
+                    ab(float(x[2]) - o.x[2]) <= 1e-4f)
+                    return;
+            }
+        if (set.count == kMaxCandidates) {
+            set.flags |= kOverflow;
+            return;
+        }
+        Candidate& o = set.c[set.count++];
+        o.x[0] = float(x[0]); o.x[1] = float(x[1]); o.x[2] = float(x[2]);
+        o.kind = kind;
+        o.index = index;
+    }
+    RAMD_NOINLINE void searchRecord(CandidateSet& set) {
+        record = &set;
+        set.count = 0;
+        set.flags = 0;
+        if (!fin(energy) || !(density() > 0)) return;
+        for (int k = 0; k < NS; ++k)
+            if (!fin(q[k]) || q[k] < 0) return;
+        bool have = false;
+        E candidate, best;
+        zeroEval(best);
+        bool gasStable = false;
+        const bool prunable = m.stableGasPrune != 0;
+        const bool gasOk = frozen(R(0), candidate, prunable ? 4 : 0);
+        if (gasOk || stalled) {
+            const R x[3] = {m_log(candidate.p), m_log(candidate.T), R(0)};
+            if (gasOk) {
+                const uint32_t before = n.stable;
+                consider(candidate, best, have);
+                gasStable = n.stable > before && candidate.gasMass > 0;
+            }
+            note(kGasCandidate, 0, x, false);
+        }
+        if (!NL) return;
+        const bool prune = prunable && gasStable;
+        if (prune) set.flags |= kSeedsPruned;
+        const int c = m.condensable;
+        R other = 0;
+        for (int k = 0; k < NS; ++k)
+            if (k != c) other += q[k];
+        if (other == 0) {
+            const bool ok = frozen(q[c], candidate, 0);
+            if (ok || stalled) {
+                const R x[3] = {m_log(candidate.p), m_log(candidate.T), R(0)};
+                if (ok) consider(candidate, best, have);
+                note(kLiquidCandidate, 0, x, false);
+            }
+        }
+        const R seeds[5] = {R(-1), R(.5), R(.95), R(.1), R(.9999)};
+        if (!prune && q[c] > 0)
+            for (int j = 0; j < 5; ++j) {
+                ++n.candidates;
+                R x[3], scale;
+                activeStart(seeds[j], false, R(0), x, scale);
+                const bool ok = activeNewton(x, scale, candidate, 0);
+                if (ok || stalled) {
+                    if (ok) consider(candidate, best, have);
+                    note(kSeedCandidate, int16_t(j), x, true);
+                }
+            }
+        if (!have && q[c] > 0 && restartRange()) {
+            set.flags |= kRestartsSearched;
+            for (int t = 0; t < 4; ++t) {
+                const R restartT = restartTemperature(t);
+                for (int j = 0; j < 5; ++j) {
+                    ++n.candidates;
+                    R x[3], scale;
+                    activeStart(seeds[j], true, restartT, x, scale);
+                    const bool ok = activeNewton(x, scale, candidate, 0);
+                    if (ok || stalled) {
+                        if (ok) consider(candidate, best, have);
+                        note(kRestartCandidate, int16_t(5 * t + j), x, true);
+                    }
+                }
+            }
+        }
+    }
+
+    // ---- FP64 refinement of a recorded search (R = double): each recorded
+    // candidate is finished by the reference's own solver started from the
+    // single-precision point (plus up to two polishing Newton steps for the
+    // active-set candidates), then judged by the FP64 stability test and
+    // maximum entropy in the reference's order.  Cells whose single-precision
+    // decisions (seed pruning, whether restarts were needed) disagree with
+    // FP64, or whose set overflowed, run the plain FP64 search.
+    RAMD_NOINLINE bool finishFrozen(R mass0, const Candidate& c, E& value, int stallLimit) {
+        const R p0 = guessP, T0 = guessT;
+        guessP = m_exp(R(c.x[0]));
+        guessT = m_exp(R(c.x[1]));
+        bool ok = frozen(mass0, value, stallLimit);
+        guessP = p0;
+        guessT = T0;
+        if (!ok) ok = frozen(mass0, value, stallLimit);  // from the cell's own guess, as the reference starts
+        return ok;
+    }
+    RAMD_NOINLINE bool finishActive(const Candidate& c, E& value) {
+        ++n.candidates;
+        const bool restart = c.kind == kRestartCandidate;
+        const R seeds[5] = {R(-1), R(.5), R(.95), R(.1), R(.9999)};
+        R x[3], scale;
+        activeStart(seeds[c.index % 5], restart, restart ? restartTemperature(c.index / 5) : R(0), x, scale);
+        x[0] = R(c.x[0]);
+        x[1] = R(c.x[1]);
+        x[2] = R(c.x[2]);
+        return activeNewton(x, scale, value, 2);
+    }
+    RAMD_NOINLINE bool refine(const CandidateSet& set, E& best) {
+        if (!fin(energy) || !(density() > 0)) return false;
+        for (int k = 0; k < NS; ++k)
+            if (!fin(q[k]) || q[k] < 0) return false;
+        if (set.flags & kOverflow) return search(best);
+        bool have = false;
+        E candidate;
+        bool gasStable = false;
+        const bool prunable = m.stableGasPrune != 0;
+        uint32_t i = 0;
+        if (i < set.count && set.c[i].kind == kGasCandidate) {
+            if (finishFrozen(R(0), set.c[i], candidate, prunable ? 4 : 0)) {
+                const uint32_t before = n.stable;
+                consider(candidate, best, have);
+                gasStable = n.stable > before && candidate.gasMass > 0;
+            } else {
+                ++n.failures;
+            }
+            ++i;
+        } else {
+            ++n.failures;
+        }
+        if (!NL) return have;
+        const bool prune = prunable && gasStable;
+        if (prune != ((set.flags & kSeedsPruned) != 0)) {
+            have = false;
+            return search(best);
+        }
+        const int c = m.condensable;
+        if (i < set.count && set.c[i].kind == kLiquidCandidate) {
+            if (finishFrozen(q[c], set.c[i], candidate, 0)) consider(candidate, best, have);
+            else ++n.failures;
+            ++i;
+        }
+        for (; i < set.count && set.c[i].kind == kSeedCandidate; ++i) {
+            if (finishActive(set.c[i], candidate)) consider(candidate, best, have);
+            else ++n.failures;
+        }
+        if (!have && q[c] > 0 && restartRange()) {
+            if (!(set.flags & kRestartsSearched)) {
+                // the single-precision search found a stable candidate that FP64
+                // rejects: search the restart temperatures in FP64
+                const R seeds[5] = {R(-1), R(.5), R(.95), R(.1), R(.9999)};
+                for (int t = 0; t < 4; ++t)
+                    for (int j = 0; j < 5; ++j) {
+                        if (activeFlash(seeds[j], candidate, true, restartTemperature(t)))
+                            consider(candidate, best, have);
+                        else ++n.failures;
+                    }
+            } else {
+                for (; i < set.count; ++i) {
+                    if (finishActive(set.c[i], candidate)) consider(candidate, best, have);
+                    else ++n.failures;
+                }
+            }
+        }
+        return have;
     }
 };
 

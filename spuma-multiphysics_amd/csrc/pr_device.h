@@ -37,21 +37,42 @@ constexpr int kMaxRegions = 8;
 enum Status : int { kOk = 0, kBadTable, kBadInput, kBadComposition, kNoRoot, kUnavailable, kUnstable, kNonFinite };
 enum RootChoice : int { kAutoRoot = 0, kGasRoot = 1, kLiquidRoot = 2 };
 
-// ---- elementary functions: fixed operation sequences shared with the
-// oracle's twin build (elementary.h), so results are reproducible on the host;
-// square root is IEEE correctly rounded on both.
+// ---- elementary functions, by precision.  FP64: fixed operation sequences
+// shared with the oracle's twin build (elementary.h), so results are
+// reproducible on the host; square root is IEEE correctly rounded on both.
+// FP32 (the mixed-precision search only): the device library.
 RHD double m_log(double x) { return elem::log(x); }
 RHD double m_exp(double x) { return elem::exp(x); }
 RHD double m_third(double x) { return elem::powThird(x); }  // pow(x, 1.0 / 3.0)
 RHD double m_acos(double x) { return elem::acos(x); }
 RHD double m_cos(double x) { return elem::cos(x); }
 RHD double m_sqrt(double x) { return ::sqrt(x); }
+RHD float m_log(float x) { return ::logf(x); }
+RHD float m_exp(float x) { return ::expf(x); }
+RHD float m_third(float x) { return ::cbrtf(x); }
+RHD float m_acos(float x) { return ::acosf(x); }
+RHD float m_cos(float x) { return ::cosf(x); }
+RHD float m_sqrt(float x) { return ::sqrtf(x); }
 
-RHD bool fin(double x) { return x == x && (x < 0 ? -x : x) <= 1.7976931348623157e308; }
-RHD double lo(double a, double b) { return a < b ? a : b; }
-RHD double hi(double a, double b) { return a > b ? a : b; }
-RHD double ab(double x) { return x < 0 ? -x : x; }
-RHD double clamp(double a, double l, double h) { return lo(hi(a, l), h); }
+// Precision-dependent constants: the FP64 values are the reference's; the
+// FP32 ones scale its absolute thresholds to single precision.
+template <class R> struct Real;
+template <> struct Real<double> {
+    static constexpr double big = 1.7976931348623157e308, eps = DBL_EPSILON, tiny = 1e-300;
+    static constexpr double polish = 4e-15;    // cubic-root Newton polish (reactiveDevicePR.h:280-307)
+    static constexpr double flatness = 1e-14;  // |f'| below which the polish stops
+};
+template <> struct Real<float> {
+    static constexpr float big = 3.4028234663852886e38f, eps = FLT_EPSILON, tiny = 1e-30f;
+    static constexpr float polish = 2e-7f;
+    static constexpr float flatness = 1e-30f;
+};
+
+template <class R> RHD bool fin(R x) { return x == x && (x < 0 ? -x : x) <= Real<R>::big; }
+template <class R> RHD R lo(R a, R b) { return a < b ? a : b; }
+template <class R> RHD R hi(R a, R b) { return a > b ? a : b; }
+template <class R> RHD R ab(R x) { return x < 0 ? -x : x; }
+template <class R> RHD R clamp(R a, R l, R h) { return lo(hi(a, l), h); }
 
 struct Region { double tmin; double c[9]; int poly; };
 
@@ -71,35 +92,39 @@ struct Table {
     int fixedStatus;
 };
 
-struct PRState {  // subset of reactiveDevicePR.h:101-110 used by the flash
-    double V, rho, e, h, s, cp, cv, alpha, kappaT, sound, p, dpdT, dpdV;
+template <class R>
+struct PRStateT {  // subset of reactiveDevicePR.h:101-110 used by the flash
+    R V, rho, e, h, s, cp, cv, alpha, kappaT, sound, p, dpdT, dpdV;
     int phase;
 };
+using PRState = PRStateT<double>;
 
 // NASA7 / NASA9 (reactiveDevicePR.h:127-155).
-RHD void nasa(const Region& r, double T, double& cp, double& h, double& s) {
-    const double* a = r.c;
-    const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
-    const double invT = 1.0 / T, logT = m_log(T);
+template <class R>
+RHD void nasa(const Region& r, R T, R& cp, R& h, R& s) {
+    R a[9];
+    for (int j = 0; j < 9; ++j) a[j] = R(r.c[j]);
+    const R T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+    const R invT = R(1) / T, logT = m_log(T);
     if (r.poly == 7) {
         cp = a[0] + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4;
-        h = a[0] + .5 * a[1] * T + a[2] * T2 / 3.0 + .25 * a[3] * T3 + .2 * a[4] * T4 + a[5] * invT;
-        s = a[0] * logT + a[1] * T + .5 * a[2] * T2 + a[3] * T3 / 3.0 + .25 * a[4] * T4 + a[6];
+        h = a[0] + R(.5) * a[1] * T + a[2] * T2 / R(3) + R(.25) * a[3] * T3 + R(.2) * a[4] * T4 + a[5] * invT;
+        s = a[0] * logT + a[1] * T + R(.5) * a[2] * T2 + a[3] * T3 / R(3) + R(.25) * a[4] * T4 + a[6];
     } else {
-        const double c0 = a[0] * invT * invT, c1 = a[1] * invT, c2 = a[2], c3 = a[3] * T;
-        const double c4 = a[4] * T2, c5 = a[5] * T3, c6 = a[6] * T4;
+        const R c0 = a[0] * invT * invT, c1 = a[1] * invT, c2 = a[2], c3 = a[3] * T;
+        const R c4 = a[4] * T2, c5 = a[5] * T3, c6 = a[6] * T4;
         cp = c0 + c1 + c2 + c3 + c4 + c5 + c6;
-        h = -c0 + logT * c1 + c2 + .5 * c3 + c4 / 3.0 + .25 * c5 + .2 * c6 + a[7] * invT;
-        s = -.5 * c0 - c1 + logT * c2 + c3 + .5 * c4 + c5 / 3.0 + .25 * c6 + a[8];
+        h = -c0 + logT * c1 + c2 + R(.5) * c3 + c4 / R(3) + R(.25) * c5 + R(.2) * c6 + a[7] * invT;
+        s = -R(.5) * c0 - c1 + logT * c2 + c3 + R(.5) * c4 + c5 / R(3) + R(.25) * c6 + a[8];
     }
 }
 
 // NASA9 upper region at a shared bound, NASA7 lower (reactiveDevicePR.h:197-212).
-template <int N>
-RHD const Region& region(const Table<N>& t, int k, double T) {
+template <int N, class R>
+RHD const Region& region(const Table<N>& t, int k, R T) {
     int r = 0;
     for (int j = 1; j < t.nreg[k]; ++j) {
-        const double bound = t.region[k][j].tmin;
+        const R bound = R(t.region[k][j].tmin);
         const bool upper = t.region[k][j].poly == 9 ? T >= bound : T > bound;
         if (!upper) break;
         r = j;
@@ -107,111 +132,128 @@ RHD const Region& region(const Table<N>& t, int k, double T) {
     return t.region[k][r];
 }
 
-struct Mix { double a, b, aa, da, d2a; };
+template <class R>
+struct MixT { R a, b, aa, da, d2a; };
+using Mix = MixT<double>;
 
 // Mixing rule (reactiveDevicePR.h:214-247).
-template <int N>
-RHD int mixing(const Table<N>& t, double T, const double* x, Mix& m) {
-    double q[N], dq[N], d2q[N];
+template <int N, class R>
+RHD int mixing(const Table<N>& t, R T, const R* x, MixT<R>& m) {
+    R q[N], dq[N], d2q[N];
     m.a = 0; m.b = 0; m.aa = 0; m.da = 0; m.d2a = 0;
     for (int i = 0; i < N; ++i) {
-        const double Tc = t.Tc[i];
+        const R Tc = R(t.Tc[i]), kappa = R(t.kappa[i]);
         if (!(Tc > 0) || !fin(Tc)) return kBadTable;
-        const double sqrtTr = m_sqrt(T / Tc);
-        const double f = 1.0 + t.kappa[i] * (1.0 - sqrtTr);
-        if (ab(f) <= 1e-14) return kNonFinite;
-        const double sign = f > 0 ? 1.0 : -1.0;
+        const R sqrtTr = m_sqrt(T / Tc);
+        const R f = R(1) + kappa * (R(1) - sqrtTr);
+        if (ab(f) <= R(1e-14)) return kNonFinite;
+        const R sign = f > 0 ? R(1) : R(-1);
         q[i] = ab(f);
-        dq[i] = -sign * t.kappa[i] / (2.0 * Tc * sqrtTr);
-        d2q[i] = sign * t.kappa[i] / (4.0 * Tc * Tc * sqrtTr * sqrtTr * sqrtTr);
-        m.b += x[i] * t.b[i];
+        dq[i] = -sign * kappa / (R(2) * Tc * sqrtTr);
+        d2q[i] = sign * kappa / (R(4) * Tc * Tc * sqrtTr * sqrtTr * sqrtTr);
+        m.b += x[i] * R(t.b[i]);
     }
     for (int i = 0; i < N; ++i)
         for (int j = 0; j < N; ++j) {
-            const double w = x[i] * x[j] * t.aij[i][j];
+            const R w = x[i] * x[j] * R(t.aij[i][j]);
             m.a += w;
             m.aa += w * q[i] * q[j];
             m.da += w * (dq[i] * q[j] + q[i] * dq[j]);
-            m.d2a += w * (d2q[i] * q[j] + 2.0 * dq[i] * dq[j] + q[i] * d2q[j]);
+            m.d2a += w * (d2q[i] * q[j] + R(2) * dq[i] * dq[j] + q[i] * d2q[j]);
         }
     return fin(m.a) && fin(m.b) && fin(m.aa) && fin(m.da) && fin(m.d2a) && m.a > 0 && m.b > 0 && m.aa > 0 ? kOk
                                                                                                          : kNonFinite;
 }
 
-RHD double cubic(double V, double bn, double cn, double dn) { return ((V + bn) * V + cn) * V + dn; }
-RHD double cbrtSigned(double x) { return x < 0 ? -m_third(-x) : m_third(x); }
+template <class R> RHD R cubic(R V, R bn, R cn, R dn) { return ((V + bn) * V + cn) * V + dn; }
+template <class R> RHD R cbrtSigned(R x) { return x < 0 ? -m_third(-x) : m_third(x); }
 
 // Isolated third root in the near-repeated discriminant band (reactiveDevicePR.h:280-307).
-RHD bool isolatedRoot(double center, double delta, double delta2, double q, double bn, double cn, double dn,
-                      double b, double RTp, double aap, double& root, bool& gasSide) {
+template <class R>
+RHD bool isolatedRoot(R center, R delta, R delta2, R q, R bn, R cn, R dn, R b, R RTp, R aap, R& root,
+                      bool& gasSide) {
     if (!(delta2 > 0) || !fin(delta) || !fin(q) || q == 0) return false;
     gasSide = q < 0;
-    const double pair = center + (gasSide ? -1.0 : 1.0) * delta;
-    double x = center + (gasSide ? 2.0 : -2.0) * delta;
+    const R pair = center + (gasSide ? R(-1) : R(1)) * delta;
+    R x = center + (gasSide ? R(2) : R(-2)) * delta;
     bool converged = false;
     for (int n = 0; n < 24; ++n) {
-        const double f = cubic(x, bn, cn, dn), df = (3.0 * x + 2.0 * bn) * x + cn;
-        if (!fin(f) || !fin(df) || ab(df) <= 1e-300) return false;
-        const double step = f / df;
+        const R f = cubic(x, bn, cn, dn), df = (R(3) * x + R(2) * bn) * x + cn;
+        if (!fin(f) || !fin(df) || ab(df) <= R(Real<R>::tiny)) return false;
+        const R step = f / df;
         x -= step;
-        if (ab(step) <= 4e-15 * hi(1.0, ab(x))) { converged = true; break; }
+        if (ab(step) <= R(Real<R>::polish) * hi(R(1), ab(x))) { converged = true; break; }
     }
     if (!converged || !fin(x) || !(x > b)) return false;
-    const double f = cubic(x, bn, cn, dn);
-    const double scale = ab(x * x * x) + ab(bn * x * x) + ab(cn * x) + ab(dn);
-    if (!(ab(f) <= 64.0 * DBL_EPSILON * scale)) return false;
-    if (!(ab(x - pair) > 1e-3 * hi(ab(x), ab(pair)))) return false;
+    const R f = cubic(x, bn, cn, dn);
+    const R scale = ab(x * x * x) + ab(bn * x * x) + ab(cn * x) + ab(dn);
+    if (!(ab(f) <= R(64) * R(Real<R>::eps) * scale)) return false;
+    if (!(ab(x - pair) > R(1e-3) * hi(ab(x), ab(pair)))) return false;
     if (gasSide ? !(x > pair) : !(x < pair)) return false;
-    const double vb = x - b, den = x * x + 2.0 * b * x - b * b;
+    const R vb = x - b, den = x * x + R(2) * b * x - b * b;
     if (!(vb > 0) || !(den > 0)) return false;
-    if (!(-RTp / (vb * vb) + 2.0 * aap * (x + b) / (den * den) < 0)) return false;
+    if (!(-RTp / (vb * vb) + R(2) * aap * (x + b) / (den * den) < 0)) return false;
     root = x;
     return true;
 }
 
-struct Roots { double v0, vLast; int count, signedCount, gas, liquid; };
+template <class R>
+struct RootsT { R v0, vLast; int count, signedCount, gas, liquid; };
+using Roots = RootsT<double>;
+
+template <class R> RHD int roots(R T, R p, R a, R b, R aa, RootsT<R>& out);
 
 // Cubic root set at (T, p) from the mixture coefficients
 // (rootsFromMixingCached, reactiveDevicePR.h:701-790).  Only the smallest and
 // largest physical roots are kept: the flash selects nothing else.
-RHD int roots(double T, double p, double a, double b, double aa, Roots& out) {
+// In single precision the classification of a near-repeated root pair (the
+// discriminant within rounding of zero) is decided in double precision.
+template <class R>
+RHD int roots(R T, R p, R a, R b, R aa, RootsT<R>& out) {
     if (!fin(T) || !fin(p) || !fin(a) || !fin(b) || !fin(aa) || T <= 0 || p <= 0 || a <= 0 || b <= 0 || aa <= 0)
         return kBadInput;
-    const double b2 = b * b;
-    const double RTp = kR * T / p, aap = aa / p;
-    const double bn = b - RTp;
-    const double cn = -(2.0 * RTp * b - aap + 3.0 * b2);
-    const double dn = b2 * RTp + b2 * b - aap * b;
-    const double center = -bn / 3.0;
-    const double delta2 = (bn * bn - 3.0 * cn) / 9.0;
-    const double q = 2.0 * bn * bn * bn / 27.0 - bn * cn / 3.0 + dn;
-    const double delta = delta2 > 0 ? m_sqrt(delta2) : 0;
-    const double h = 2.0 * delta * delta2;
-    double disc = q * q - h * h;
-    if (ab(ab(h) - ab(q)) < 1e-10) {
-        if (disc > 1e-10) return kNonFinite;
+    const R b2 = b * b;
+    const R RTp = R(kR) * T / p, aap = aa / p;
+    const R bn = b - RTp;
+    const R cn = -(R(2) * RTp * b - aap + R(3) * b2);
+    const R dn = b2 * RTp + b2 * b - aap * b;
+    const R center = -bn / R(3);
+    const R delta2 = (bn * bn - R(3) * cn) / R(9);
+    const R q = R(2) * bn * bn * bn / R(27) - bn * cn / R(3) + dn;
+    const R delta = delta2 > 0 ? m_sqrt(delta2) : R(0);
+    const R h = R(2) * delta * delta2;
+    R disc = q * q - h * h;
+    if (sizeof(R) < sizeof(double) && ab(disc) <= R(1e-4) * (q * q + h * h)) {
+        RootsT<double> d;
+        const int st = roots<double>(double(T), double(p), double(a), double(b), double(aa), d);
+        out.v0 = R(d.v0); out.vLast = R(d.vLast);
+        out.count = d.count; out.signedCount = d.signedCount; out.gas = d.gas; out.liquid = d.liquid;
+        return st;
+    }
+    if (ab(ab(h) - ab(q)) < R(1e-10)) {
+        if (disc > R(1e-10)) return kNonFinite;
         disc = 0;
     }
-    double raw[3] = {0, 0, 0};
+    R raw[3] = {0, 0, 0};
     int n = 0, isolated = 0;
-    if (ab(disc) < 1e-14) {
+    if (ab(disc) < R(1e-14)) {
         bool gasSide = false;
         if (!isolatedRoot(center, delta, delta2, q, bn, cn, dn, b, RTp, aap, raw[0], gasSide)) return kNoRoot;
         n = 1;
         isolated = gasSide ? 1 : -1;
     }
     if (isolated) {
-    } else if (disc > 1e-14 || !(delta2 > 0)) {
-        const double sd = .5 * m_sqrt(hi(0.0, disc));
-        raw[0] = center + cbrtSigned(-.5 * q + sd) + cbrtSigned(-.5 * q - sd);
+    } else if (disc > R(1e-14) || !(delta2 > 0)) {
+        const R sd = R(.5) * m_sqrt(hi(R(0), disc));
+        raw[0] = center + cbrtSigned(-R(.5) * q + sd) + cbrtSigned(-R(.5) * q - sd);
         n = 1;
-    } else if (disc < -1e-14) {
-        double arg = -.5 * q / (delta * delta * delta);
-        arg = hi(-1.0, lo(1.0, arg));
-        const double theta = m_acos(arg) / 3.0;
-        raw[0] = center + 2.0 * delta * m_cos(theta);
-        raw[1] = center + 2.0 * delta * m_cos(theta + 2.0 * kPi / 3.0);
-        raw[2] = center + 2.0 * delta * m_cos(theta + 4.0 * kPi / 3.0);
+    } else if (disc < R(-1e-14)) {
+        R arg = -R(.5) * q / (delta * delta * delta);
+        arg = hi(R(-1), lo(R(1), arg));
+        const R theta = m_acos(arg) / R(3);
+        raw[0] = center + R(2) * delta * m_cos(theta);
+        raw[1] = center + R(2) * delta * m_cos(theta + R(2) * R(kPi) / R(3));
+        raw[2] = center + R(2) * delta * m_cos(theta + R(4) * R(kPi) / R(3));
         n = 3;
     } else {
         return kNoRoot;
@@ -219,16 +261,16 @@ RHD int roots(double T, double p, double a, double b, double aa, Roots& out) {
     if (!isolated)
         for (int i = 0; i < n; ++i)
             for (int it = 0; it < 12; ++it) {
-                const double res = cubic(raw[i], bn, cn, dn);
-                const double der = (3.0 * raw[i] + 2.0 * bn) * raw[i] + cn;
-                if (ab(der) <= 1e-300) break;
-                const double step = res / der;
+                const R res = cubic(raw[i], bn, cn, dn);
+                const R der = (R(3) * raw[i] + R(2) * bn) * raw[i] + cn;
+                if (ab(der) <= R(Real<R>::tiny)) break;
+                const R step = res / der;
                 raw[i] -= step;
-                if (ab(step) <= 4e-15 * hi(1.0, ab(raw[i]))) break;
+                if (ab(step) <= R(Real<R>::polish) * hi(R(1), ab(raw[i]))) break;
             }
     // ascending order, finite roots above the covolume (reactiveDevicePR.h:253-269)
     for (int i = 1; i < n; ++i) {
-        const double key = raw[i];
+        const R key = raw[i];
         int j = i - 1;
         while (j >= 0 && raw[j] > key) { raw[j + 1] = raw[j]; --j; }
         raw[j + 1] = key;
@@ -242,9 +284,9 @@ RHD int roots(double T, double p, double a, double b, double aa, Roots& out) {
     out.count = m;
     out.v0 = raw[0];
     out.vLast = raw[m - 1];
-    const double Tcrit = a * kOmegaB / (b * kOmegaA * kR);
-    const double Pcrit = kOmegaB * kR * Tcrit / b;
-    const double Vcrit = kOmegaVc * kR * Tcrit / Pcrit;
+    const R Tcrit = a * R(kOmegaB) / (b * R(kOmegaA) * R(kR));
+    const R Pcrit = R(kOmegaB) * R(kR) * Tcrit / b;
+    const R Vcrit = R(kOmegaVc) * R(kR) * Tcrit / Pcrit;
     if (isolated) {
         out.signedCount = isolated;
         out.gas = isolated > 0;
@@ -261,22 +303,22 @@ RHD int roots(double T, double p, double a, double b, double aa, Roots& out) {
     return kOk;
 }
 
-template <int N>
-RHD int massToMole(const Table<N>& t, const double* Y, double* x, double& W) {  // :310-327
-    double invW = 0, sumY = 0;
+template <int N, class R>
+RHD int massToMole(const Table<N>& t, const R* Y, R* x, R& W) {  // :310-327
+    R invW = 0, sumY = 0;
     for (int k = 0; k < N; ++k) {
         if (!fin(Y[k]) || Y[k] < 0) return kBadComposition;
         sumY += Y[k];
-        invW += Y[k] / t.W[k];
+        invW += Y[k] / R(t.W[k]);
     }
-    if (!(invW > 0) || ab(sumY - 1.0) > 1e-12) return kBadComposition;
-    W = 1.0 / invW;
-    for (int k = 0; k < N; ++k) x[k] = Y[k] * W / t.W[k];
+    if (!(invW > 0) || ab(sumY - R(1)) > (sizeof(R) < sizeof(double) ? R(1e-5) : R(1e-12))) return kBadComposition;
+    W = R(1) / invW;
+    for (int k = 0; k < N; ++k) x[k] = Y[k] * W / R(t.W[k]);
     return kOk;
 }
 
 // Composition part of Cantera's phaseState heuristic (reactiveDevicePR.h:
-// 428-471): everything but the final rho < rhoMid(T) comparison.
+// 428-471): everything but the final rho < rhoMid(T) comparison.  Always FP64.
 struct PhaseHeuristic {
     double tc, critDensity, densityDelta, tempDelta;
     int status;  // status of the tmid root loop (reported only when T < tc)
@@ -324,10 +366,11 @@ RHD void phaseHeuristic(const Table<N>& t, const double* x, double W, double mix
 }
 
 // Final comparison: rhoMid = rho_c + (T - Tc) (rho_c - rho_mid) / (Tc - Tmid).
-RHD int reportedPhase(const PhaseHeuristic& h, double T, double rho, int& phase) {
-    if (T >= h.tc) { phase = -2; return kOk; }
+template <class R>
+RHD int reportedPhase(const PhaseHeuristic& h, R T, R rho, int& phase) {
+    if (T >= R(h.tc)) { phase = -2; return kOk; }
     if (h.status != kOk) return h.status;
-    const double rhoMid = h.critDensity + (T - h.tc) * h.densityDelta / h.tempDelta;
+    const R rhoMid = R(h.critDensity) + (T - R(h.tc)) * R(h.densityDelta) / R(h.tempDelta);
     phase = rho < rhoMid ? -1 : 0;
     return fin(rhoMid) ? kOk : kNonFinite;
 }
@@ -337,73 +380,74 @@ RHD int reportedPhase(const PhaseHeuristic& h, double T, double rho, int& phase)
 // is computed for every species when mu != nullptr.
 // The phaseState comparison and the final finiteness checks follow in
 // finishState(), in the reference's order.
-template <int N>
-RHD int stateTrho(const Table<N>& t, double T, double rho, const double* x, double W, const Mix& mx,
-                  PRState& s, double* mu) {
+template <int N, class R>
+RHD int stateTrho(const Table<N>& t, R T, R rho, const R* x, R W, const MixT<R>& mx, PRStateT<R>& s, R* mu) {
     if (!fin(T) || !fin(rho) || T <= 0 || rho <= 0) return kBadInput;
-    const double b = mx.b, aa = mx.aa, da = mx.da, d2a = mx.d2a;
+    const R b = mx.b, aa = mx.aa, da = mx.da, d2a = mx.d2a;
+    const R Rg = R(kR), sq2 = R(kSqrt2);
     s.rho = rho;
     s.V = W / rho;
     if (!(s.V > b)) return kNoRoot;
-    const double den = s.V * s.V + 2.0 * s.V * b - b * b;
-    const double vmb = s.V - b;
+    const R den = s.V * s.V + R(2) * s.V * b - b * b;
+    const R vmb = s.V - b;
     if (!(den > 0) || !(vmb > 0)) return kNoRoot;
-    s.p = kR * T / vmb - aa / den;
+    s.p = Rg * T / vmb - aa / den;
     if (!fin(s.p) || s.p <= 0) return kNoRoot;
-    const double dpdV = -kR * T / (vmb * vmb) + 2.0 * aa * (s.V + b) / (den * den);
-    const double dpdT = kR / vmb - da / den;
+    const R dpdV = -Rg * T / (vmb * vmb) + R(2) * aa * (s.V + b) / (den * den);
+    const R dpdT = Rg / vmb - da / den;
     s.dpdV = dpdV;
     s.dpdT = dpdT;
     if (!fin(dpdV) || !fin(dpdT) || dpdV >= 0) return kUnstable;
-    const double logRatio = m_log((s.V + (1.0 + kSqrt2) * b) / (s.V + (1.0 - kSqrt2) * b));
-    const double L = logRatio / (2.0 * kSqrt2 * b);
-    double h0 = 0, s0 = 0, cp0 = 0, xlogx = 0;
-    double hRT[N], sR[N];
+    const R logRatio = m_log((s.V + (R(1) + sq2) * b) / (s.V + (R(1) - sq2) * b));
+    const R L = logRatio / (R(2) * sq2 * b);
+    R h0 = 0, s0 = 0, cp0 = 0, xlogx = 0;
+    R hRT[N], sR[N];
     for (int k = 0; k < N; ++k) {
-        double cpR, hk, sk;
+        R cpR, hk, sk;
         nasa(region(t, k, T), T, cpR, hk, sk);
         hRT[k] = hk; sR[k] = sk;
-        cp0 += x[k] * kR * cpR;
-        h0 += x[k] * kR * T * hk;
-        s0 += x[k] * kR * sk;
+        cp0 += x[k] * Rg * cpR;
+        h0 += x[k] * Rg * T * hk;
+        s0 += x[k] * Rg * sk;
         if (x[k] > 0) xlogx += x[k] * m_log(x[k]);
     }
-    const double z = s.p * s.V / (kR * T);
-    const double hRes = kR * T * (z - 1.0) + L * (T * da - aa);
-    const double sRes = kR * m_log(z * (1.0 - b / s.V)) + L * da;
-    const double hM = h0 + hRes;
-    const double logP = m_log(s.p / t.refP);
-    const double sM = s0 - kR * xlogx - kR * logP + sRes;
-    const double dHdT = cp0 + s.V * dpdT - kR + L * T * d2a;
-    const double cpM = dHdT - (s.V + T * dpdT / dpdV) * dpdT;
-    const double cvM = cpM + T * dpdT * dpdT / dpdV;
+    const R z = s.p * s.V / (Rg * T);
+    const R hRes = Rg * T * (z - R(1)) + L * (T * da - aa);
+    const R sRes = Rg * m_log(z * (R(1) - b / s.V)) + L * da;
+    const R hM = h0 + hRes;
+    const R logP = m_log(s.p / R(t.refP));
+    const R sM = s0 - Rg * xlogx - Rg * logP + sRes;
+    const R dHdT = cp0 + s.V * dpdT - Rg + L * T * d2a;
+    const R cpM = dHdT - (s.V + T * dpdT / dpdV) * dpdT;
+    const R cvM = cpM + T * dpdT * dpdT / dpdV;
     s.h = hM / W; s.e = (hM - s.p * s.V) / W; s.s = sM / W;
     s.cp = cpM / W; s.cv = cvM / W;
     s.alpha = -dpdT / (s.V * dpdV);
-    s.kappaT = -1.0 / (s.V * dpdV);
-    const double sound2 = s.V * s.V * (-cpM / cvM * dpdV / W);
+    s.kappaT = -R(1) / (s.V * dpdV);
+    const R sound2 = s.V * s.V * (-cpM / cvM * dpdV / W);
     if (!(s.cp > 0) || !(s.cv > 0) || !(s.kappaT > 0) || !(sound2 > 0)) return kUnstable;
     s.sound = m_sqrt(sound2);
     if (mu) {
-        const double denomMu = 2.0 * kSqrt2 * b * b;
-        const double denomMu2 = b * den;
-        const double RT = kR * T;
-        double f[N];
-        for (int i = 0; i < N; ++i) f[i] = 1.0 + t.kappa[i] * (1.0 - m_sqrt(T / t.Tc[i]));
+        const R denomMu = R(2) * sq2 * b * b;
+        const R denomMu2 = b * den;
+        const R RT = Rg * T;
+        R f[N];
+        for (int i = 0; i < N; ++i) f[i] = R(1) + R(t.kappa[i]) * (R(1) - m_sqrt(T / R(t.Tc[i])));
         for (int k = 0; k < N; ++k) {
-            double pp = 0;
-            for (int i = 0; i < N; ++i) pp += x[i] * t.aij[k][i] * ab(f[i] * f[k]);
-            const double bk = t.b[k];
-            const double num = 2.0 * b * pp - aa * bk;
-            mu[k] = RT * (hRT[k] - sR[k]) + RT * m_log(hi(kSmall, x[k])) + RT * logP - RT * m_log(s.p * s.V / RT) +
-                    RT * m_log(s.V / vmb) + RT * bk / vmb - num / denomMu * logRatio - aa * bk * s.V / denomMu2;
+            R pp = 0;
+            for (int i = 0; i < N; ++i) pp += x[i] * R(t.aij[k][i]) * ab(f[i] * f[k]);
+            const R bk = R(t.b[k]);
+            const R num = R(2) * b * pp - aa * bk;
+            mu[k] = RT * (hRT[k] - sR[k]) + RT * m_log(hi(R(Real<R>::tiny), x[k])) + RT * logP -
+                    RT * m_log(s.p * s.V / RT) + RT * m_log(s.V / vmb) + RT * bk / vmb - num / denomMu * logRatio -
+                    aa * bk * s.V / denomMu2;
         }
     }
     return kOk;
 }
 
-template <int N>
-RHD int finishState(const PhaseHeuristic& ph, double T, PRState& s, const double* mu) {
+template <int N, class R>
+RHD int finishState(const PhaseHeuristic& ph, R T, PRStateT<R>& s, const R* mu) {
     const int st = reportedPhase(ph, T, s.rho, s.phase);
     if (st != kOk) return st;
     if (!fin(s.rho) || !fin(s.e) || !fin(s.h) || !fin(s.s) || !fin(s.cp) || !fin(s.cv) || !fin(s.alpha) ||

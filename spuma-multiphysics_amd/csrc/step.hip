@@ -33,6 +33,54 @@
 namespace ramd {
 
 // ---------------------------------------------------------------- kernels
+// Loads cell c into a flash: species masses, e = rhoE - |rho u|^2 / (2 rho)
+// (ReactiveFoam.C:727-728, in FP64) and the step-start seed.
+template <class F>
+__device__ inline void loadCell(F& f, const double* __restrict__ q, size_t stride, const ThermoOut& g, uint32_t c,
+                                int ns) {
+    using R = decltype(f.energy);
+    double rho = 0, mass[4];
+    for (int k = 0; k < ns; ++k) {
+        mass[k] = q[size_t(k) * stride + c];
+        f.q[k] = R(mass[k]);
+        rho += mass[k];
+    }
+    const double m0 = q[size_t(ns) * stride + c], m1 = q[size_t(ns + 1) * stride + c];
+    const double m2 = q[size_t(ns + 2) * stride + c], E = q[size_t(ns + 3) * stride + c];
+    f.energy = R(E - .5 * (m0 * m0 + m1 * m1 + m2 * m2) / rho);
+    f.guessP = R(g.p);
+    f.guessT = R(g.T);
+    f.guessLiquid = R(g.liquidMass[0]);
+}
+
+// Commits a flash result, or keeps the step-start state of a failed cell
+// (the reference commits nothing for it); p, c_frozen and rho go to the SoA
+// arrays the next transport stage reads.
+__device__ inline void storeCell(bool ok, const Eval& result, const ThermoOut& g, uint32_t c,
+                                 ThermoOut* __restrict__ out, double* __restrict__ pOut, double* __restrict__ cOut,
+                                 double* __restrict__ rOut) {
+    if (ok) {
+        writeOut(result, out[c]);
+        pOut[c] = result.p;
+        cOut[c] = result.cf;
+        rOut[c] = result.rho;
+    } else {
+        out[c] = g;
+        pOut[c] = g.p;
+        cOut[c] = g.soundFrozen;
+        rOut[c] = g.rho;
+    }
+}
+
+__device__ inline void countFlash(uint32_t failed, uint32_t phases, unsigned long long* __restrict__ counters) {
+    const uint32_t sumFailed = waveSum(failed), sumPhases = waveSum(phases);
+    if ((threadIdx.x & 63) == 0) {
+        if (sumFailed) atomicAdd(&counters[0], (unsigned long long)sumFailed);
+        atomicAdd(&counters[1], (unsigned long long)sumPhases);
+    }
+}
+
+// Plain FP64 flash of the cells in `order`.
 template <int NS, int NL>
 __global__ __launch_bounds__(64) void stepFlashKernel(const HemModel<NS>* __restrict__ model,
                                                       const double* __restrict__ q, size_t stride,
@@ -47,40 +95,72 @@ __global__ __launch_bounds__(64) void stepFlashKernel(const HemModel<NS>* __rest
     uint32_t failed = 0;
     if (i < count && *dtDev > 0) {
         const uint32_t c = order[i];
+        const ThermoOut g = seeds[c];
         Flash<NS, NL> f(*model, n);
-        double rho = 0;
-        for (int k = 0; k < NS; ++k) {
-            f.q[k] = q[size_t(k) * stride + c];
-            rho += f.q[k];
-        }
-        const double m0 = q[size_t(NS) * stride + c], m1 = q[size_t(NS + 1) * stride + c];
-        const double m2 = q[size_t(NS + 2) * stride + c], E = q[size_t(NS + 3) * stride + c];
-        f.energy = E - .5 * (m0 * m0 + m1 * m1 + m2 * m2) / rho;
-        const ThermoOut& g = seeds[c];
-        f.guessP = g.p;
-        f.guessT = g.T;
-        f.guessLiquid = g.liquidMass[0];
+        loadCell(f, q, stride, g, c, NS);
         Eval result;
         zeroEval(result);
-        if (f.run(result)) {
-            writeOut(result, out[c]);
-            pOut[c] = result.p;
-            cOut[c] = result.cf;
-            rOut[c] = result.rho;
-        } else {  // the cell keeps its step-start state (the reference commits nothing for it)
-            out[c] = g;
-            pOut[c] = g.p;
-            cOut[c] = g.soundFrozen;
-            rOut[c] = g.rho;
-            failed = 1;
-        }
+        const bool ok = f.run(result);
+        storeCell(ok, result, g, c, out, pOut, cOut, rOut);
+        failed = ok ? 0u : 1u;
         if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
     }
-    const uint32_t sumFailed = waveSum(failed), sumPhases = waveSum(n.phases);
-    if ((threadIdx.x & 63) == 0) {
-        if (sumFailed) atomicAdd(&counters[0], (unsigned long long)sumFailed);
-        atomicAdd(&counters[1], (unsigned long long)sumPhases);
+    countFlash(failed, n.phases, counters);
+}
+
+// Mixed precision, first half: the candidate search in FP32, recording the
+// converged candidates of every cell.
+template <int NS, int NL>
+__global__ __launch_bounds__(64) void stepSearchKernel(const HemModel<NS>* __restrict__ model,
+                                                       const double* __restrict__ q, size_t stride,
+                                                       const ThermoOut* __restrict__ seeds,
+                                                       const uint32_t* __restrict__ order, uint32_t count,
+                                                       const double* __restrict__ dtDev,
+                                                       CandidateSet* __restrict__ sets, uint16_t* __restrict__ cost,
+                                                       unsigned long long* __restrict__ phases) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    Counts n = {0, 0, 0, 0, 0, 0};
+    if (i < count && *dtDev > 0) {
+        const uint32_t c = order[i];
+        Flash<NS, NL, float> f(*model, n);
+        loadCell(f, q, stride, seeds[c], c, NS);
+        CandidateSet set;
+        f.searchRecord(set);
+        sets[c] = set;
+        if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
     }
+    const uint32_t sum = waveSum(n.phases);
+    if ((threadIdx.x & 63) == 0) atomicAdd(phases, (unsigned long long)sum);
+}
+
+// Mixed precision, second half: FP64 refinement, stability and selection.
+template <int NS, int NL>
+__global__ __launch_bounds__(64) void stepRefineKernel(const HemModel<NS>* __restrict__ model,
+                                                       const double* __restrict__ q, size_t stride,
+                                                       const ThermoOut* __restrict__ seeds,
+                                                       const uint32_t* __restrict__ order, uint32_t count,
+                                                       const double* __restrict__ dtDev,
+                                                       const CandidateSet* __restrict__ sets,
+                                                       ThermoOut* __restrict__ out, double* __restrict__ pOut,
+                                                       double* __restrict__ cOut, double* __restrict__ rOut,
+                                                       uint16_t* __restrict__ cost,
+                                                       unsigned long long* __restrict__ counters) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    Counts n = {0, 0, 0, 0, 0, 0};
+    uint32_t failed = 0;
+    if (i < count && *dtDev > 0) {
+        const uint32_t c = order[i];
+        const ThermoOut g = seeds[c];
+        Flash<NS, NL> f(*model, n);
+        loadCell(f, q, stride, g, c, NS);
+        Eval result;
+        zeroEval(result);
+        const bool ok = f.refine(sets[c], result);
+        storeCell(ok, result, g, c, out, pOut, cOut, rOut);
+        failed = ok ? 0u : 1u;
+        if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
+    }
+    countFlash(failed, n.phases, counters);
 }
 
 // Heavy-first scheduling and a counting sort: histogram, exclusive scan from
@@ -249,14 +329,21 @@ struct Step {
     ThermoOut *sA = nullptr, *sB = nullptr;
     uint8_t* key = nullptr;
     uint32_t* order = nullptr;
-    uint16_t* cost = nullptr;  // phase evaluations of each cell in its latest flash
+    uint16_t* cost = nullptr;  // FP64 phase evaluations of each cell in its latest flash
     bool haveCost = false;
+    // mixed precision: FP32 search order and cost, candidate sets
+    uint32_t* orderSearch = nullptr;
+    uint16_t* costSearch = nullptr;
+    bool haveCostSearch = false;
+    CandidateSet* sets = nullptr;
+    int mode = REACTIVE_AMD_FLASH_FP64;
     unsigned *bucketCount = nullptr, *bucketOffset = nullptr;
     double* blockMin = nullptr;
     double* scalars = nullptr;  // dt, recheck
     double *bA = nullptr, *bB = nullptr, *boundary = nullptr, *totals = nullptr;
     double2* totalPartial = nullptr;
-    unsigned long long* counters = nullptr;  // fail1, phases1, fail2, phases2, violations, HLLC rejections
+    // fail1, phases1, fail2, phases2, violations, HLLC rejections, FP32 phases (mixed)
+    unsigned long long* counters = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t events[2] = {};
     std::vector<hipEvent_t> marks;  // 10 per step: cfl, stage 0, flash #1, recheck, stage 1, flash #2 brackets
@@ -278,19 +365,46 @@ struct Step {
         return static_cast<T*>(p);
     }
 
+    // Flash of every cell of q (step-start seeds sA) into out and the SoA
+    // transport inputs; costOut (may be null) receives the FP64 cost.
     void flash(const double* q, double* pOut, double* cOut, double* rOut, ThermoOut* out, uint16_t* costOut,
                unsigned long long* counter) {
         const unsigned blocks = blocksFor(cells, 64);
+        if (mode == REACTIVE_AMD_FLASH_MIXED) {
+#define RAMD_STEP_MIXED(NS_, NL_)                                                                                  \
+    hipLaunchKernelGGL((stepSearchKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                             \
+                       static_cast<const HemModel<NS_>*>(h->model), q, all, sA, orderSearch, uint32_t(cells),       \
+                       scalars, sets, costOut ? costSearch : nullptr, counters + 6);                              \
+    hipLaunchKernelGGL((stepRefineKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                             \
+                       static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), scalars,   \
+                       sets, out, pOut, cOut, rOut, costOut, counter)
+            if (h->ns == 1 && h->nl == 0) { RAMD_STEP_MIXED(1, 0); }
+            else if (h->ns == 1) { RAMD_STEP_MIXED(1, 1); }
+            else if (h->nl == 0) { RAMD_STEP_MIXED(2, 0); }
+            else { RAMD_STEP_MIXED(2, 1); }
+#undef RAMD_STEP_MIXED
+        } else {
 #define RAMD_STEP_FLASH(NS_, NL_)                                                                                 \
     hipLaunchKernelGGL((stepFlashKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                             \
                        static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), scalars, out, \
                        pOut, cOut, rOut, costOut, counter)
-        if (h->ns == 1 && h->nl == 0) RAMD_STEP_FLASH(1, 0);
-        else if (h->ns == 1) RAMD_STEP_FLASH(1, 1);
-        else if (h->nl == 0) RAMD_STEP_FLASH(2, 0);
-        else RAMD_STEP_FLASH(2, 1);
+            if (h->ns == 1 && h->nl == 0) { RAMD_STEP_FLASH(1, 0); }
+            else if (h->ns == 1) { RAMD_STEP_FLASH(1, 1); }
+            else if (h->nl == 0) { RAMD_STEP_FLASH(2, 0); }
+            else { RAMD_STEP_FLASH(2, 1); }
 #undef RAMD_STEP_FLASH
+        }
         RAMD_HIP(hipGetLastError());
+    }
+
+    // Heavy-first order of all cells: by cost when one is known, else by the
+    // reference bucket of the step-start state.
+    void sortCells(const uint16_t* byCost, uint32_t* out) {
+        hipLaunchKernelGGL(bucketHistogramKernel, dim3(blocksFor(cells, 256)), dim3(256), 0, stream, qA, all, ns, sA,
+                           byCost, cells, key, bucketCount);
+        hipLaunchKernelGGL(bucketScanKernel, dim3(1), dim3(64), 0, stream, bucketCount, bucketOffset);
+        hipLaunchKernelGGL(bucketScatterKernel, dim3(blocksFor(cells, 256)), dim3(256), 0, stream, key, cells,
+                           bucketOffset, out);
     }
 
     // One SSPRK2 step; every launch is on `stream` and reads dt from device memory.
@@ -305,12 +419,9 @@ struct Step {
         launchCartesianCfl(a, ns, qA, all, pA, cA, cfl, maximum, nullptr, blockMin, stream);
         hipLaunchKernelGGL(finishCflKernel, dim3(1), dim3(256), 0, stream, blockMin, size_t(nb), scalars);
         mark(m, 1);
-        // heavy-first order from the step-start seeds
-        hipLaunchKernelGGL(bucketHistogramKernel, dim3(blocksFor(cells, 256)), dim3(256), 0, stream, qA, all, ns, sA,
-                           haveCost ? cost : nullptr, cells, key, bucketCount);
-        hipLaunchKernelGGL(bucketScanKernel, dim3(1), dim3(64), 0, stream, bucketCount, bucketOffset);
-        hipLaunchKernelGGL(bucketScatterKernel, dim3(blocksFor(cells, 256)), dim3(256), 0, stream, key, cells,
-                           bucketOffset, order);
+        // heavy-first orders from the latest costs (or the step-start seeds)
+        sortCells(haveCost ? cost : nullptr, order);
+        if (mode == REACTIVE_AMD_FLASH_MIXED) sortCells(haveCostSearch ? costSearch : nullptr, orderSearch);
         // stage 0 and its boundary rate
         t->boundaryRatesFrom(qA, pA, cA, rA, bA, counters + 5, stream);
         mark(m, 2);
@@ -332,6 +443,7 @@ struct Step {
         // flash #2 from the step-start seeds, into the state buffer of stage 1
         flash(qA, pA, cA, rA, sB, cost, counters + 2);
         haveCost = true;
+        if (mode == REACTIVE_AMD_FLASH_MIXED) haveCostSearch = true;
         mark(m, 8);
         std::swap(sA, sB);
         // conservation totals and boundary integral
@@ -398,6 +510,8 @@ void* reactive_amd_step_create(void* transport, void* hem, char* error, size_t e
         s->key = s->alloc<uint8_t>(s->cells);
         s->order = s->alloc<uint32_t>(s->cells);
         s->cost = s->alloc<uint16_t>(s->cells);
+        s->orderSearch = s->alloc<uint32_t>(s->cells);
+        s->costSearch = s->alloc<uint16_t>(s->cells);
         s->bucketCount = s->alloc<unsigned>(256);
         s->bucketOffset = s->alloc<unsigned>(256);
         s->blockMin = s->alloc<double>((s->cells + 255) / 256 + 1);
@@ -435,6 +549,7 @@ int reactive_amd_step_load(void* step, const double* q, const ReactiveThermoStat
         RAMD_HIP(hipFree(staging));
         s.loaded = true;
         s.haveCost = false;  // new states: schedule by the reference bucket until a flash has run
+        s.haveCostSearch = false;
     });
 }
 
@@ -446,7 +561,10 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         require(std::isfinite(cfl) && cfl > 0 && cfl <= .5 && std::isfinite(maximum) && maximum > 0,
                 "Invalid transport CFL controls");
         require(steps >= 1, "At least one step");
-        require(flashMode == REACTIVE_AMD_FLASH_FP64, "Unknown flash arithmetic mode");
+        require(flashMode == REACTIVE_AMD_FLASH_FP64 || flashMode == REACTIVE_AMD_FLASH_MIXED,
+                "Unknown flash arithmetic mode");
+        if (flashMode == REACTIVE_AMD_FLASH_MIXED && !s.sets) s.sets = s.alloc<CandidateSet>(s.cells);
+        s.mode = flashMode;
         hipLaunchKernelGGL(clearCountersKernel, dim3(1), dim3(64), 0, s.stream, s.counters, 8);
         while (s.marks.size() < size_t(steps) * 9) {
             hipEvent_t e;
@@ -474,6 +592,7 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         r.phaseEvaluations = counters[1] + counters[3];
         r.cflViolations = counters[4];
         r.fluxFailures = counters[5];
+        r.singlePhaseEvaluations = counters[6];
         r.dt = scalars[0];
         r.recheckDt = scalars[1];
         for (int k = 0; k < s.nv; ++k) {

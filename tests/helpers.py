@@ -175,3 +175,41 @@ def random_graph(nc, seed, ns=1, fixed=3):
     st["sound"] = rng.uniform(150, 400, nc + fixed)
     st["gasMass"] = rho
     return f, volume, q[:nc].copy(), st[:nc].copy(), q[nc:].copy(), st[nc:].copy()
+
+
+class Device:
+    """Device buffers through the HIP runtime the library itself links
+    (libamdhip64 from /opt/rocm), for entry points that take device pointers."""
+
+    def __init__(self):
+        import glob
+        path = next(iter(sorted(glob.glob("/opt/rocm/lib/libamdhip64.so*"))), "libamdhip64.so")
+        self.hip = C.CDLL(path)
+        V = C.c_void_p
+        self.hip.hipMalloc.argtypes = [C.POINTER(V), C.c_size_t]
+        self.hip.hipFree.argtypes = [V]
+        self.hip.hipMemcpy.argtypes = [V, V, C.c_size_t, C.c_int]
+        self.hip.hipDeviceSynchronize.argtypes = []
+        self.buffers = []
+
+    def put(self, array):
+        a = np.ascontiguousarray(array)
+        ptr = C.c_void_p()
+        assert self.hip.hipMalloc(C.byref(ptr), max(1, a.nbytes)) == 0
+        self.buffers.append(ptr)
+        assert self.hip.hipMemcpy(ptr, p(a), a.nbytes, 1) == 0  # host to device
+        return ptr
+
+    def zeros(self, nbytes):
+        return self.put(np.zeros(nbytes, np.uint8))
+
+    def get(self, ptr, like):
+        out = np.empty_like(like)
+        assert self.hip.hipDeviceSynchronize() == 0
+        assert self.hip.hipMemcpy(p(out), ptr, out.nbytes, 2) == 0  # device to host
+        return out
+
+    def close(self):
+        for b in self.buffers:
+            self.hip.hipFree(b)
+        self.buffers = []

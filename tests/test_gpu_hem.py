@@ -79,8 +79,7 @@ def test_resident_frozen_batch_matches_host_batch_and_oracle():
     """reactive_gpu_hem_run_resident_v1 with device pointers and a cell order,
     against reactive_gpu_hem_run_v2 (host arrays) and the oracle's
     frozen-inventory recovery (reactiveDeviceFlash.h:572-590), bit for bit."""
-    import torch
-    from helpers import oracle, p
+    from helpers import Device, oracle, p
     lib = rlib.load()
     case = cube_case(6, seed=17)
     mb, q, e, seeds = case["model"], np.ascontiguousarray(case["q"][:, :1]), case["energy"], case["states"]
@@ -97,23 +96,16 @@ def test_resident_frozen_batch_matches_host_batch_and_oracle():
     lib.reactive_gpu_hem_destroy_v1(h)
     # resident batch over every other cell, in reverse order
     order = np.arange(n - 1, -1, -2, dtype=np.uint32)
-    dev = torch.device("cuda:0")
-    dq = torch.from_numpy(q.copy()).to(dev)
-    de = torch.from_numpy(e.copy()).to(dev)
-    ds = torch.from_numpy(seeds.view(np.uint8).copy()).to(dev)
-    dc = torch.from_numpy(cap.view(np.uint8).copy()).to(dev)
-    do = torch.from_numpy(order.view(np.int32).copy()).to(dev)
-    dout = torch.zeros(n * 184, dtype=torch.uint8, device=dev)
-    torch.cuda.synchronize()
+    dev = Device()
+    dq, de, ds, dc, do = dev.put(q), dev.put(e), dev.put(seeds), dev.put(cap), dev.put(order)
+    blank = np.zeros(n, abi.HEM_OUTPUT_DTYPE)
+    dout = dev.put(blank)
     r = lib.reactive_gpu_hem_create_resident_v1(mb, len(mb), n, err, 1024)
     assert r, err.value
     rprof = abi.HemProfile.fresh()
-    rc = lib.reactive_gpu_hem_run_resident_v1(r, C.c_void_p(dq.data_ptr()), C.c_void_p(de.data_ptr()),
-                                              C.c_void_p(ds.data_ptr()), C.c_void_p(dc.data_ptr()),
-                                              C.c_void_p(do.data_ptr()), len(order), C.c_void_p(dout.data_ptr()),
-                                              C.byref(rprof), err, 1024)
+    rc = lib.reactive_gpu_hem_run_resident_v1(r, dq, de, ds, dc, do, len(order), dout, C.byref(rprof), err, 1024)
     assert rc == 0, err.value
-    out = dout.cpu().numpy().view(abi.HEM_OUTPUT_DTYPE)
+    out = dev.get(dout, blank)
     assert rprof.submitted == len(order) and rprof.succeeded == int(out["success"][order].sum())
     untouched = np.setdiff1d(np.arange(n), order)
     assert not out.view(np.uint8).reshape(n, -1)[untouched].any()
@@ -132,13 +124,10 @@ def test_resident_frozen_batch_matches_host_batch_and_oracle():
     # a cell asking for the equilibrium (boundary-recovery) search: rejected, nothing written
     cap2 = cap.copy()
     cap2["equilibrium"][order[3]] = 1
-    dc2 = torch.from_numpy(cap2.view(np.uint8).copy()).to(dev)
-    dout.zero_()
-    torch.cuda.synchronize()
-    rc = lib.reactive_gpu_hem_run_resident_v1(r, C.c_void_p(dq.data_ptr()), C.c_void_p(de.data_ptr()),
-                                              C.c_void_p(ds.data_ptr()), C.c_void_p(dc2.data_ptr()),
-                                              C.c_void_p(do.data_ptr()), len(order), C.c_void_p(dout.data_ptr()),
-                                              C.byref(rprof), err, 1024)
+    dc2 = dev.put(cap2)
+    dout2 = dev.put(blank)
+    rc = lib.reactive_gpu_hem_run_resident_v1(r, dq, de, ds, dc2, do, len(order), dout2, C.byref(rprof), err, 1024)
     assert rc == 1 and b"frozen-inventory" in err.value
-    assert not dout.cpu().numpy().any()
+    assert not dev.get(dout2, blank).view(np.uint8).any()
+    dev.close()
     lib.reactive_gpu_hem_destroy_v1(r)

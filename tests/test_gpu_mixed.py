@@ -1,0 +1,91 @@
+"""Mixed-precision flash (REACTIVE_AMD_FLASH_MIXED) against the plain FP64
+flash of the same resident step, and both against the converged root.
+
+The FP64 flash stops at the first Newton iterate inside vtol = etol = 1e-9,
+mutol = 1e-7; in compressed-liquid and two-phase cells that iterate lies up to
+~1e-8 (p) from the converged root of the same equations.  The mixed flash
+refines each candidate to 1e-3 of those tolerances.  Each step's second flash
+is therefore compared with the converged root of its own inputs: the oracle
+rerun with tolerances 1e-3 as tight (vtol = etol = 1e-12, mutol = 1e-10),
+seeded as the step seeds it.  Gates: no flips (cells where the two flashes
+fail differently or choose another phase set) beyond 0.2 %, mixed within
+1e-10 (T, p) of the converged root for 99 % of cells and within 1e-8 for all,
+and mixed within the FP64 flash's own distance to its root plus 1e-10."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from helpers import config, cube_case, oracle_flash
+from reactive_amd import abi, cases, lib as rlib, model
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = {"vtol": 1e-12, "etol": 1e-12, "mutol": 1e-10}
+
+
+def one_step(lib, case, mode):
+    err = C.create_string_buffer(1024)
+    mb = case["model"]
+    hem = lib.reactive_gpu_hem_create_v1(mb, len(mb), len(case["q"]), err, 1024)
+    cfg = config(len(case["q"]), len(case["faces"]))
+    tr = lib.reactive_transport_create(1, C.byref(cfg), rlib.ptr(case["volume"]), rlib.ptr(case["faces"]), None,
+                                       None, None, None, err, 1024)
+    step = lib.reactive_amd_step_create(tr, hem, err, 1024)
+    assert step, err.value
+    assert lib.reactive_amd_step_load(step, rlib.ptr(case["q"]), rlib.ptr(case["states"])) == 0
+    rep = abi.StepReport.fresh()
+    assert lib.reactive_amd_step_run(step, 0.4, 1.0, 1, mode, C.byref(rep)) == 0, lib.reactive_amd_step_error(step)
+    q = np.zeros_like(case["q"])
+    s = np.zeros(len(q), dtype=abi.STATE_DTYPE)
+    assert lib.reactive_amd_step_read(step, rlib.ptr(q), rlib.ptr(s)) == 0
+    lib.reactive_amd_step_destroy(step)
+    lib.reactive_transport_destroy(tr)
+    lib.reactive_gpu_hem_destroy_v1(hem)
+    return q, s, rep
+
+
+def failed(s, start):
+    n = len(s)
+    return (s.view(np.uint8).reshape(n, -1) == start.view(np.uint8).reshape(n, -1)).all(1)
+
+
+def rel(a, b):
+    return np.abs(a - b) / np.abs(b)
+
+
+@pytest.mark.parametrize("n", [16])
+def test_mixed_step_against_fp64_and_converged_root(n):
+    lib = rlib.load()
+    case = cube_case(n, seed=40 + n)
+    q64, s64, r64 = one_step(lib, case, 0)
+    qmx, smx, rmx = one_step(lib, case, 1)
+    cells = len(q64)
+    assert rmx.dt == r64.dt
+    assert rmx.singlePhaseEvaluations > 0
+    f64, fmx = failed(s64, case["states"]), failed(smx, case["states"])
+    both = ~f64 & ~fmx
+    flips = int((f64 != fmx).sum() + (smx["activeLiquids"][both] != s64["activeLiquids"][both]).sum())
+    tight = model.model_bytes(model.build_model(limits=TIGHT))
+    roots = {}
+    for name, q in (("fp64", q64), ("mixed", qmx)):
+        r, ok, _ = oracle_flash(tight, q[:, :1], cases.internal_energy(q, 1), case["states"])
+        roots[name] = (r, ok == 1)
+    report = {}
+    for name, s, fl in (("fp64", s64, f64), ("mixed", smx, fmx)):
+        r, ok = roots[name]
+        use = ~fl & ok & (s["activeLiquids"] == r["activeLiquids"])
+        report[name] = {f: rel(s[f][use], r[f][use]) for f in ("T", "p")}
+    print(f"\ncells {cells}, flips {flips}, FP64 failures {int(f64.sum())}, mixed failures {int(fmx.sum())}")
+    for name, d in report.items():
+        print(f"{name} vs converged root:", {f: (float(v.max()), float(np.percentile(v, 99)),
+                                                 float((v <= 1e-10).mean())) for f, v in d.items()})
+    same = both & (smx["activeLiquids"] == s64["activeLiquids"])
+    print("mixed vs FP64:", {f: float(rel(smx[f][same], s64[f][same]).max()) for f in ("T", "p")})
+    print("phase evaluations FP64 / mixed FP64 / mixed FP32:", r64.phaseEvaluations, rmx.phaseEvaluations,
+          rmx.singlePhaseEvaluations)
+    assert flips <= max(2, cells // 500)
+    for f in ("T", "p"):
+        v = report["mixed"][f]
+        assert (v <= 1e-10).mean() >= 0.99, (f, float((v <= 1e-10).mean()))
+        assert v.max() <= 1e-8, (f, float(v.max()))
