@@ -181,14 +181,55 @@ def run_steps(lib, step, q, states, cfl, steps, mode, warmup):
     return rep, wall, q_one, s_one
 
 
-def stage_roofline(rep, cells, steps, nv=5):
+def stage_roofline(rep, cells, steps, n, nv=5):
     bytes0, bytes1 = 8 * (2 * nv + 2), 8 * (3 * nv + 2)  # algorithmic bytes per cell, stage 0 / stage 1
     t_stage = (rep.stageSeconds[0] + rep.stageSeconds[1]) / steps
     achieved = (bytes0 + bytes1) * cells / t_stage / 1e9
     return {"bound": "hbm", "kernel": "cartesianStageKernel (both SSPRK2 stages)", "achieved": achieved,
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": STAGE_TRAFFIC_160 if n == 160 and nv == 5 else None,
             "bytes_per_cell": [bytes0, bytes1],
             "avg_launch_ms": [1e3 * rep.stageSeconds[0] / steps, 1e3 * rep.stageSeconds[1] / steps]}
+
+
+# HBM bytes per launch of the fused stage kernel at 160^3 pure N2O, stage 0 and
+# stage 1, from rocprofv3 FETCH_SIZE and WRITE_SIZE (each in its own run,
+# scaled by a plain copy kernel's counts): profiles/cartesianStage160_traffic.txt
+STAGE_TRAFFIC_160 = [420.3e6, 583.7e6]
+TIGHT = {"vtol": 1e-12, "etol": 1e-12, "mutol": 1e-10}
+
+
+def converged_roots(lib, q, seeds):
+    """The FP64 flash of (q, seeds) rerun on the device with tolerances 1e-3
+    as tight: the converged root each flash's result is compared with."""
+    tight = model.model_bytes(model.build_model(limits=TIGHT))
+    err = C.create_string_buffer(1024)
+    cells = len(q)
+    hem = lib.reactive_gpu_hem_create_v1(tight, len(tight), cells, err, 1024)
+    if not hem:
+        raise RuntimeError(err.value.decode())
+    out = seeds.copy()
+    ok = np.zeros(cells, np.int32)
+    prof = abi.HemProfile.fresh()
+    rc = lib.reactive_gpu_hem_run_v1(hem, rlib.ptr(np.ascontiguousarray(q[:, :1])),
+                                     rlib.ptr(cases.internal_energy(q, 1)), cells, rlib.ptr(out), rlib.ptr(ok),
+                                     C.byref(prof), err, 1024)
+    lib.reactive_gpu_hem_destroy_v1(hem)
+    if rc:
+        raise RuntimeError(err.value.decode())
+    return out, ok == 1
+
+
+def distance(s, states0, root, ok):
+    cells = len(s)
+    fail = (s.view(np.uint8).reshape(cells, -1) == states0.view(np.uint8).reshape(cells, -1)).all(1)
+    use = ~fail & ok & (s["activeLiquids"] == root["activeLiquids"])
+    out = {"compared_cells": int(use.sum())}
+    for f in ("T", "p"):
+        r = np.abs(s[f][use] - root[f][use]) / np.abs(root[f][use])
+        out[f] = {"max": float(r.max()), "p99": float(np.percentile(r, 99)),
+                  "share_le_1e-10": float((r <= 1e-10).mean())}
+    return out
 
 
 def census(states0, s64, smx):
@@ -244,6 +285,7 @@ def main():
         raise RuntimeError(err.value.decode())
     log(f"bench: FP64 flash, {a.warmup} warmup + {a.steps} timed step(s)")
     rep64, wall64, q_one, s64 = run_steps(lib, step, q, states, a.cfl, a.steps, 0, a.warmup)
+    q64_one = q_one
     log(f"bench: FP64 {wall64:.3f} s for {a.steps} steps (device {rep64.seconds:.3f} s)")
     value64 = cells * a.steps / wall64
     t_flash64 = (rep64.flashSeconds[0] + rep64.flashSeconds[1]) / (2 * a.steps)
@@ -252,9 +294,14 @@ def main():
             "flash_failures": [int(rep64.flashFailures[0]), int(rep64.flashFailures[1])]}
     if a.flash == "mixed":
         log(f"bench: mixed flash, {a.warmup} warmup + {a.steps} timed step(s)")
-        rep, wall, _, smx = run_steps(lib, step, q, states, a.cfl, a.steps, 1, a.warmup)
+        rep, wall, qmx_one, smx = run_steps(lib, step, q, states, a.cfl, a.steps, 1, a.warmup)
         log(f"bench: mixed {wall:.3f} s for {a.steps} steps (device {rep.seconds:.3f} s)")
         accuracy = census(states, s64, smx)
+        log("bench: converged roots of both steps' second flash")
+        root, ok = converged_roots(lib, qmx_one, states)
+        accuracy["mixed_vs_converged_root"] = distance(smx, states, root, ok)
+        root, ok = converged_roots(lib, q64_one, states)
+        accuracy["fp64_vs_converged_root"] = distance(s64, states, root, ok)
     else:
         rep, wall, accuracy = rep64, wall64, None
     value = cells * a.steps / wall
@@ -269,11 +316,12 @@ def main():
                    "flash": ("mixed: FP32 candidate search, FP64 refinement, stability and selection" if mixed
                              else "fp64 (plain, reference arithmetic)"),
                    "seed": a.seed, "two_phase_fraction": round(info["two_phase_fraction"], 4)},
-        "roofline": stage_roofline(rep, cells, a.steps),
+        "roofline": stage_roofline(rep, cells, a.steps, a.n),
         "roofline_flash": {"bound": "valu", "kernel": "stepSearchKernel + stepRefineKernel" if mixed
                            else "stepFlashKernel", "avg_launch_ms": 1e3 * t_flash,
                            "phase_evaluations_per_cell_step": rep.phaseEvaluations / (cells * a.steps),
                            "fp32_phase_evaluations_per_cell_step": rep.singlePhaseEvaluations / (cells * a.steps),
+                           "search_ms": 1e3 * rep.searchSeconds / (2 * a.steps),
                            "share_of_step": (rep.flashSeconds[0] + rep.flashSeconds[1]) / rep.seconds},
         "fp64": fp64,
         "speedup_vs_fp64": value / value64,

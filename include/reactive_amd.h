@@ -42,6 +42,7 @@ typedef struct ReactiveAmdStepReport {
     double cflSeconds;                  /* both CFL sweeps (+ their final minimum) */
     uint64_t fluxFailures;              /* face evaluations the HLLC flux rejected (0 under HLL) */
     uint64_t singlePhaseEvaluations;    /* FP32 phase evaluations of the mixed flash (phaseEvaluations: FP64) */
+    double searchSeconds;               /* mixed flash: the FP32 search kernels' share of flashSeconds */
 } ReactiveAmdStepReport;
 
 /* Flash arithmetic of a step.  FP64: the reference's arithmetic.  MIXED: the

@@ -316,6 +316,7 @@ __global__ void toThermoKernel(const ReactiveThermoState* __restrict__ in, size_
 }
 
 inline unsigned blocksFor(size_t n, unsigned b) { return unsigned((n + b - 1) / b); }
+constexpr int kMarks = 11;  // events per step
 
 // ---------------------------------------------------------------- handle
 struct Step {
@@ -346,7 +347,9 @@ struct Step {
     unsigned long long* counters = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t events[2] = {};
-    std::vector<hipEvent_t> marks;  // 10 per step: cfl, stage 0, flash #1, recheck, stage 1, flash #2 brackets
+    // 11 per step: 9 in sequence bracketing cfl, stage 0, flash #1, recheck, stage 1,
+    // flash #2, then the end of each flash's FP32 search (mixed mode)
+    std::vector<hipEvent_t> marks;
     bool loaded = false;
     char error[1024] = {};
 
@@ -368,13 +371,14 @@ struct Step {
     // Flash of every cell of q (step-start seeds sA) into out and the SoA
     // transport inputs; costOut (may be null) receives the FP64 cost.
     void flash(const double* q, double* pOut, double* cOut, double* rOut, ThermoOut* out, uint16_t* costOut,
-               unsigned long long* counter) {
+               unsigned long long* counter, hipEvent_t split) {
         const unsigned blocks = blocksFor(cells, 64);
         if (mode == REACTIVE_AMD_FLASH_MIXED) {
 #define RAMD_STEP_MIXED(NS_, NL_)                                                                                  \
     hipLaunchKernelGGL((stepSearchKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                             \
                        static_cast<const HemModel<NS_>*>(h->model), q, all, sA, orderSearch, uint32_t(cells),       \
                        scalars, sets, costOut ? costSearch : nullptr, counters + 6);                              \
+    if (split) RAMD_HIP(hipEventRecord(split, stream));                                                            \
     hipLaunchKernelGGL((stepRefineKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                             \
                        static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), scalars,   \
                        sets, out, pOut, cOut, rOut, costOut, counter)
@@ -428,7 +432,7 @@ struct Step {
         launchCartesianStage(a, ns, nv, t->scheme, qA, all, pA, cA, rA, 0.0, scalars, 0, qB, counters + 5, stream);
         mark(m, 3);
         // flash #1 seeded from the step-start states
-        flash(qB, pB, cB, rB, sB, nullptr, counters);
+        flash(qB, pB, cB, rB, sB, nullptr, counters, m ? m[9] : nullptr);
         mark(m, 4);
         // CFL recheck after stage 0 (maximum = dt)
         launchCartesianCfl(a, ns, qB, all, pB, cB, cfl, 0.0, scalars, blockMin, stream);
@@ -441,7 +445,7 @@ struct Step {
         launchCartesianStage(a, ns, nv, t->scheme, qB, all, pB, cB, rB, 0.0, scalars, 1, qA, counters + 5, stream);
         mark(m, 7);
         // flash #2 from the step-start seeds, into the state buffer of stage 1
-        flash(qA, pA, cA, rA, sB, cost, counters + 2);
+        flash(qA, pA, cA, rA, sB, cost, counters + 2, m ? m[10] : nullptr);
         haveCost = true;
         if (mode == REACTIVE_AMD_FLASH_MIXED) haveCostSearch = true;
         mark(m, 8);
@@ -566,13 +570,13 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         if (flashMode == REACTIVE_AMD_FLASH_MIXED && !s.sets) s.sets = s.alloc<CandidateSet>(s.cells);
         s.mode = flashMode;
         hipLaunchKernelGGL(clearCountersKernel, dim3(1), dim3(64), 0, s.stream, s.counters, 8);
-        while (s.marks.size() < size_t(steps) * 9) {
+        while (s.marks.size() < size_t(steps) * kMarks) {
             hipEvent_t e;
             RAMD_HIP(hipEventCreate(&e));
             s.marks.push_back(e);
         }
         RAMD_HIP(hipEventRecord(s.events[0], s.stream));
-        for (int i = 0; i < steps; ++i) s.one(cfl, maximum, s.marks.data() + size_t(i) * 9);
+        for (int i = 0; i < steps; ++i) s.one(cfl, maximum, s.marks.data() + size_t(i) * kMarks);
         RAMD_HIP(hipEventRecord(s.events[1], s.stream));
         unsigned long long counters[8];
         double scalars[4], totals[REACTIVE_AMD_MAX_VARIABLES], boundary[REACTIVE_AMD_MAX_VARIABLES];
@@ -601,9 +605,15 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         }
         r.seconds = ms / 1000.0;
         for (int i = 0; i < steps; ++i) {
-            const hipEvent_t* m = s.marks.data() + size_t(i) * 9;
+            const hipEvent_t* m = s.marks.data() + size_t(i) * kMarks;
             float d[8];
             for (int j = 0; j < 8; ++j) RAMD_HIP(hipEventElapsedTime(&d[j], m[j], m[j + 1]));
+            if (s.mode == REACTIVE_AMD_FLASH_MIXED) {
+                float a = 0, b = 0;
+                RAMD_HIP(hipEventElapsedTime(&a, m[3], m[9]));
+                RAMD_HIP(hipEventElapsedTime(&b, m[7], m[10]));
+                r.searchSeconds += (a + b) / 1000.0;
+            }
             r.cflSeconds += (d[0] + d[4]) / 1000.0;
             r.stageSeconds[0] += d[2] / 1000.0;
             r.flashSeconds[0] += d[3] / 1000.0;

@@ -7,6 +7,11 @@
                  reactive_gpu_hem_run_v1
   copy MIB       a plain device copy kernel of MIB MiB (torch elementwise copy),
                  to calibrate FETCH_SIZE and WRITE_SIZE against a known byte count
+  uniform N T RHO MODE STEPS
+                 resident SSPRK2 steps (MODE 0 FP64, 1 mixed) on an N^3 box whose
+                 cells all hold the same state (T, RHO, at rest): every lane of a
+                 flash wave does the same work, so VALU counters divided by the
+                 phase evaluations give the operations per phase evaluation
 
 Usage: python3 spuma-multiphysics_amd/profile_drivers.py stage 160 3
 """
@@ -63,6 +68,46 @@ def flash(n):
     lib.reactive_gpu_hem_destroy_v1(h)
 
 
+def uniform(n, T, rho, mode, steps):
+    lib = rlib.load()
+    faces, volume = cases.cartesian_faces(n, 0.08 / n)
+    nc = n ** 3
+    e, pr, liquid, dome = cases.equilibrium_states(np.array([T]), np.array([rho]))
+    q = np.zeros((nc, 5))
+    q[:, 0] = rho
+    q[:, 4] = rho * e[0]
+    mb = model.model_bytes(model.build_model())
+    err = C.create_string_buffer(1024)
+    hem = lib.reactive_gpu_hem_create_v1(mb, len(mb), nc, err, 1024)
+    assert hem, err.value
+    seed = np.zeros(1, dtype=abi.STATE_DTYPE)
+    seed["p"] = pr; seed["T"] = T; seed["rho"] = rho; seed["liquidMass"][:, 0] = liquid
+    seed["activeLiquids"] = int(liquid[0] > 0)
+    ok = np.zeros(1, np.int32)
+    prof = abi.HemProfile.fresh()
+    qe = np.array([rho * e[0]])
+    assert lib.reactive_gpu_hem_run_v1(hem, rlib.ptr(np.array([[rho]])), rlib.ptr(qe), 1, rlib.ptr(seed),
+                                       rlib.ptr(ok), C.byref(prof), err, 1024) == 0 and ok[0] == 1, err.value
+    states = np.repeat(seed, nc)
+    cfg = abi.Config(cells=nc, species=1, variables=5, faces=len(faces), fixed=0, viscosity=0, conductivity=0,
+                     diffusivity=0, waveFactor=1.1, maxBytes=100e9, mechanical=0)
+    tr = lib.reactive_transport_create(1, C.byref(cfg), rlib.ptr(volume), rlib.ptr(faces), None, None, None, None,
+                                       err, 1024)
+    step = lib.reactive_amd_step_create(tr, hem, err, 1024)
+    assert step, err.value
+    assert lib.reactive_amd_step_load(step, rlib.ptr(q), rlib.ptr(states)) == 0
+    rep = abi.StepReport.fresh()
+    assert lib.reactive_amd_step_run(step, 0.4, 1.0, steps, mode, C.byref(rep)) == 0, \
+        lib.reactive_amd_step_error(step)
+    print(f"uniform: {nc} cells at T={T} rho={rho}, mode {mode}, {steps} steps: FP64 phase evaluations "
+          f"{rep.phaseEvaluations}, FP32 phase evaluations {rep.singlePhaseEvaluations}, flash failures "
+          f"{rep.flashFailures[0]}/{rep.flashFailures[1]}, flash {rep.flashSeconds[0] + rep.flashSeconds[1]:.4f} s, "
+          f"search {rep.searchSeconds:.4f} s", flush=True)
+    lib.reactive_amd_step_destroy(step)
+    lib.reactive_transport_destroy(tr)
+    lib.reactive_gpu_hem_destroy_v1(hem)
+
+
 def copy(mib):
     import torch
     n = int(mib) << 20
@@ -82,5 +127,7 @@ if __name__ == "__main__":
         flash(int(sys.argv[2]))
     elif what == "copy":
         copy(int(sys.argv[2]))
+    elif what == "uniform":
+        uniform(int(sys.argv[2]), float(sys.argv[3]), float(sys.argv[4]), int(sys.argv[5]), int(sys.argv[6]))
     else:
         raise SystemExit(__doc__)

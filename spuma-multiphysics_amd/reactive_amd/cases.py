@@ -154,6 +154,51 @@ class PureFluid:
         return p, r[0], r[-1]
 
 
+class MixtureFluid:
+    """Single-phase Peng-Robinson gas mixture (van der Waals mixing with
+    a_ij = sqrt(a_i a_j)) and NASA9 ideal parts, in numpy: the internal
+    energy and pressure at (T, rho, Y) that seed a synthetic mixture state."""
+
+    def __init__(self, names=("N2O", "N2")):
+        self.species = [PureFluid(n) for n in names]
+        self.W = np.array([f.W for f in self.species])
+
+    def _mix(self, T, Y):
+        Y = np.atleast_2d(Y)
+        inv = Y / self.W
+        W = 1.0 / inv.sum(1)
+        x = inv * W[:, None]
+        q, dq = [], []
+        for f in self.species:
+            sq = np.sqrt(T / f.Tc)
+            g = 1.0 + f.kappa * (1.0 - sq)
+            q.append(np.abs(g))
+            dq.append(-np.sign(g) * f.kappa / (2.0 * f.Tc * sq))
+        aa = np.zeros_like(T)
+        da = np.zeros_like(T)
+        for i, fi in enumerate(self.species):
+            for j, fj in enumerate(self.species):
+                w = x[:, i] * x[:, j] * math.sqrt(fi.a * fj.a)
+                aa += w * q[i] * q[j]
+                da += w * (dq[i] * q[j] + q[i] * dq[j])
+        b = sum(x[:, k] * f.b for k, f in enumerate(self.species))
+        return x, W, aa, da, b
+
+    def energy(self, T, rho, Y):
+        T = np.asarray(T, float)
+        x, W, aa, da, b = self._mix(T, Y)
+        V = W / rho
+        L = np.log((V + (1 + SQ2) * b) / (V + (1 - SQ2) * b)) / (2 * SQ2 * b)
+        h0 = sum(x[:, k] * f.h_ideal(T) for k, f in enumerate(self.species))
+        return (h0 - R * T + L * (T * da - aa)) / W
+
+    def pressure(self, T, rho, Y):
+        T = np.asarray(T, float)
+        x, W, aa, da, b = self._mix(T, Y)
+        V = W / rho
+        return R * T / (V - b) - aa / (V * V + 2 * b * V - b * b)
+
+
 _SAT_CACHE = {}
 
 
@@ -214,6 +259,37 @@ def pure_n2o_cube(n, seed=0, edge=0.08, tlo=180.0, thi=300.0, rlo=1.0, rhi=1000.
     seeds["activeLiquids"] = (liquid > 0).astype(np.int32)
     faces, volume = cartesian_faces(n, edge / n)
     return faces, volume, q, seeds, {"two_phase_fraction": float(dome.mean()), "seed": seed, "n": n}
+
+
+def binary_cube(n, seed=0, edge=0.08, ymax=0.3, tlo=150.0, thi=320.0, rlo=0.5, rhi=1000.0, sigma_u=20.0):
+    """Seeded binary N2O/N2 cube: Y_N2 ~ U(0, ymax), T ~ U(tlo, thi), rho
+    log-uniform, u_d ~ N(0, sigma_u).  The internal energy is the
+    single-phase PR mixture value at (T, rho, Y): the flash then finds the
+    equilibrium (gas mixture + liquid N2O) with that energy.  Returns (faces,
+    volume, q[nc, 6] = [rho_N2O, rho_N2, rho u, rho E], seeds, info)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nc = n ** 3
+    yn2 = rng.uniform(0.0, ymax, nc)
+    T = rng.uniform(tlo, thi, nc)
+    rho = np.exp(rng.uniform(math.log(rlo), math.log(rhi), nc))
+    u = rng.normal(0.0, sigma_u, (nc, 3))
+    Y = np.stack([1 - yn2, yn2], 1)
+    mix = MixtureFluid(("N2O", "N2"))
+    e = mix.energy(T, rho, Y)
+    pr = mix.pressure(T, rho, Y)
+    q = np.empty((nc, 6))
+    q[:, 0] = rho * Y[:, 0]
+    q[:, 1] = rho * Y[:, 1]
+    q[:, 2:5] = rho[:, None] * u
+    q[:, 5] = rho * (e + 0.5 * (u * u).sum(1))
+    seeds = np.zeros(nc, dtype=abi.STATE_DTYPE)
+    seeds["p"] = np.clip(pr, 2e3, 4e7)
+    seeds["T"] = T
+    seeds["rho"] = rho
+    seeds["e"] = e
+    seeds["gasMass"] = rho
+    faces, volume = cartesian_faces(n, edge / n)
+    return faces, volume, q, seeds, {"seed": seed, "n": n}
 
 
 def internal_energy(q, ns):

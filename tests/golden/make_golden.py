@@ -132,6 +132,14 @@ def main():
     np.savez_compressed(os.path.join(HERE, "transport.npz"),
                         **{f"{n}_{k}": v for n, d in sets.items() for k, v in d.items()})
     print("flash cube success", int(success.sum()), "/", len(e), "edge success", int(esuccess.sum()), "/", len(ee))
+    # binary N2O/N2 gas mixture with liquid N2O: a seeded 8^3 cube
+    mb2 = model.model_bytes(model.build_model(gas_species=("N2O", "N2")))
+    _, _, q2, seeds2, _ = cases.binary_cube(8, seed=3)
+    e2 = cases.internal_energy(q2, 2)
+    out2, success2, counters2 = flash_set(lib, mb2, q2[:, :2], e2, seeds2)
+    np.savez_compressed(os.path.join(HERE, "flash_binary.npz"), model=np.frombuffer(mb2, np.uint8),
+                        q=q2[:, :2], energy=e2, seeds=seeds2, out=out2, success=success2, counters=counters2)
+    print("binary success", int(success2.sum()), "/", len(e2))
 
 
 if __name__ == "__main__":

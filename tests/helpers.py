@@ -117,6 +117,23 @@ def config(nc, nf, ns=1, nv=None, fixed=0, wave=1.1, max_bytes=64e9):
                       conductivity=0, diffusivity=0, waveFactor=wave, maxBytes=max_bytes, mechanical=0)
 
 
+def binary_case(n, seed=1, threads=8):
+    """Binary N2O/N2 cube (cases.binary_cube) with flash-consistent states;
+    failed cells take a converged cell's conserved values and state."""
+    mb = model.model_bytes(model.build_model(gas_species=("N2O", "N2")))
+    faces, volume, q, seeds, info = cases.binary_cube(n, seed=seed)
+    e = cases.internal_energy(q, 2)
+    states, success, _ = oracle_flash(mb, q[:, :2], e, seeds, threads)
+    good = np.where(success == 1)[0]
+    bad = np.where(success == 0)[0]
+    if bad.size:
+        q[bad] = q[good[0]]
+        states[bad] = states[good[0]]
+        e = cases.internal_energy(q, 2)
+    return dict(model=mb, faces=faces, volume=volume, q=q, seeds=seeds, states=states, success=success,
+                tstates=cases.transport_states(states), energy=e, ns=2)
+
+
 def cube_case(n, seed=1, threads=8):
     """Pure-N2O cube with flash-consistent transport states (oracle flash of
     the seeds); failed cells borrow a neighbour's state."""
@@ -130,7 +147,7 @@ def cube_case(n, seed=1, threads=8):
     if bad.size:
         states[bad] = states[good[0]]
     return dict(model=mb, faces=faces, volume=volume, q=q, seeds=seeds, states=states, success=success,
-                tstates=cases.transport_states(states), energy=e)
+                tstates=cases.transport_states(states), energy=e, ns=1)
 
 
 def random_graph(nc, seed, ns=1, fixed=3):

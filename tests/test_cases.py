@@ -1,7 +1,10 @@
 """Synthetic case generator: mesh topology and state invariants."""
+import ctypes as C
+
 import numpy as np
 
-from reactive_amd import cases
+from helpers import oracle, p
+from reactive_amd import cases, model
 
 
 def test_cartesian_mesh_counts_and_orientation():
@@ -26,3 +29,30 @@ def test_equilibrium_states_are_consistent():
     e = cases.internal_energy(q, 1)
     assert np.allclose(e, q[:, 0] * seeds["e"], rtol=1e-12)
     assert (seeds["liquidMass"][:, 0] <= q[:, 0]).all()
+
+
+def test_mixture_energy_matches_the_oracle_pr_state():
+    """The numpy PR mixture energy that seeds binary cases equals the oracle's
+    single-phase PR state at (T, rho, Y) (oracle_pr_trho, reactiveDevicePR.h:840-930)."""
+    mb = model.model_bytes(model.build_model(gas_species=("N2O", "N2")))
+    img = C.create_string_buffer(mb, len(mb))
+    mix = cases.MixtureFluid(("N2O", "N2"))
+    rng = np.random.default_rng(1)
+    checked = 0
+    for _ in range(200):
+        T, rho, y = rng.uniform(200, 320), np.exp(rng.uniform(np.log(0.5), np.log(60))), rng.uniform(0, .3)
+        Y = np.array([1 - y, y])
+        out = np.zeros(40)
+        if oracle().oracle_pr_trho(img, 0, T, rho, p(Y), p(out)) == 0:
+            checked += 1
+            assert abs(mix.energy(np.array([T]), rho, Y)[0] - out[5]) <= 4e-15 * abs(out[5])
+            assert abs(mix.pressure(np.array([T]), rho, Y)[0] - out[1]) <= 1e-13 * abs(out[1])
+    assert checked > 150
+
+
+def test_binary_cube_composition():
+    faces, volume, q, seeds, info = cases.binary_cube(5, seed=2)
+    rho = q[:, :2].sum(1)
+    assert (q[:, 1] / rho <= 0.3).all() and (q[:, 1] >= 0).all()
+    e = cases.internal_energy(q, 2)
+    assert np.allclose(e, rho * seeds["e"], rtol=1e-12)

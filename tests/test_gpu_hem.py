@@ -67,6 +67,12 @@ def test_flash_edge_classes_bitwise():
     check_bitwise(g["model"].tobytes(), g["edge_q"], g["edge_energy"], g["edge_seeds"])
 
 
+def test_flash_binary_bitwise():
+    """Binary N2O/N2 gas with liquid N2O: the two-species flash (golden inputs)."""
+    g = np.load(os.path.join(GOLDEN, "flash_binary.npz"))
+    check_bitwise(g["model"].tobytes(), g["q"], g["energy"], g["seeds"])
+
+
 def test_flash_larger_cube_bitwise():
     case = cube_case(16, seed=5)
     check_bitwise(case["model"], case["q"][:, :1], case["energy"], case["seeds"])

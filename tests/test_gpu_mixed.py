@@ -16,7 +16,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from helpers import config, cube_case, oracle_flash
+from helpers import binary_case, config, cube_case, oracle_flash
 from reactive_amd import abi, cases, lib as rlib, model
 
 pytestmark = pytest.mark.gpu
@@ -28,7 +28,7 @@ def one_step(lib, case, mode):
     err = C.create_string_buffer(1024)
     mb = case["model"]
     hem = lib.reactive_gpu_hem_create_v1(mb, len(mb), len(case["q"]), err, 1024)
-    cfg = config(len(case["q"]), len(case["faces"]))
+    cfg = config(len(case["q"]), len(case["faces"]), ns=case["ns"])
     tr = lib.reactive_transport_create(1, C.byref(cfg), rlib.ptr(case["volume"]), rlib.ptr(case["faces"]), None,
                                        None, None, None, err, 1024)
     step = lib.reactive_amd_step_create(tr, hem, err, 1024)
@@ -54,10 +54,11 @@ def rel(a, b):
     return np.abs(a - b) / np.abs(b)
 
 
-@pytest.mark.parametrize("n", [16])
-def test_mixed_step_against_fp64_and_converged_root(n):
+@pytest.mark.parametrize("binary", [False, True], ids=["n2o", "n2o-n2"])
+def test_mixed_step_against_fp64_and_converged_root(binary):
     lib = rlib.load()
-    case = cube_case(n, seed=40 + n)
+    case = binary_case(12, seed=8) if binary else cube_case(16, seed=56)
+    ns = case["ns"]
     q64, s64, r64 = one_step(lib, case, 0)
     qmx, smx, rmx = one_step(lib, case, 1)
     cells = len(q64)
@@ -66,10 +67,11 @@ def test_mixed_step_against_fp64_and_converged_root(n):
     f64, fmx = failed(s64, case["states"]), failed(smx, case["states"])
     both = ~f64 & ~fmx
     flips = int((f64 != fmx).sum() + (smx["activeLiquids"][both] != s64["activeLiquids"][both]).sum())
-    tight = model.model_bytes(model.build_model(limits=TIGHT))
+    species = ("N2O", "N2") if ns == 2 else ("N2O",)
+    tight = model.model_bytes(model.build_model(gas_species=species, limits=TIGHT))
     roots = {}
     for name, q in (("fp64", q64), ("mixed", qmx)):
-        r, ok, _ = oracle_flash(tight, q[:, :1], cases.internal_energy(q, 1), case["states"])
+        r, ok, _ = oracle_flash(tight, q[:, :ns], cases.internal_energy(q, ns), case["states"])
         roots[name] = (r, ok == 1)
     report = {}
     for name, s, fl in (("fp64", s64, f64), ("mixed", smx, fmx)):

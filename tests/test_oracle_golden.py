@@ -25,6 +25,17 @@ def test_flash_bitwise_vs_reference(part):
     assert np.array_equal(counters, g[f"{part}_counters"])
 
 
+def test_binary_flash_bitwise_vs_reference():
+    """Binary N2O/N2 gas with liquid N2O (the two-species flash)."""
+    g = load("flash_binary.npz")
+    mb = g["model"].tobytes()
+    assert oracle().oracle_model_supported(C.create_string_buffer(mb, len(mb))) == 1
+    out, success, counters = oracle_flash(mb, g["q"], g["energy"], g["seeds"])
+    assert np.array_equal(success, g["success"])
+    assert out.tobytes() == g["out"].tobytes()
+    assert np.array_equal(counters, g["counters"])
+
+
 @pytest.mark.parametrize("name", ["cube6", "graph"])
 def test_transport_bitwise_vs_reference(name):
     g = load("transport.npz")
