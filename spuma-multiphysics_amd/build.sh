@@ -1,6 +1,7 @@
 #!/usr/bin/env bash
 # Builds lib/libreactiveTransport.so (HIP, gfx950) from csrc/.
-# Rounding policy: IEEE FP64, no contraction (-ffp-contract=off), no fast-math.
+# Rounding policy: IEEE FP64, no contraction (-ffp-contract=off), no fast-math;
+# search.hip (the FP32 candidate search only) adds -freciprocal-math -fapprox-func.
 set -euo pipefail
 here="$(cd -- "$(dirname -- "${BASH_SOURCE[0]}")" && pwd)"
 hipcc="${HIPCC:-/opt/rocm/bin/hipcc}"
@@ -13,7 +14,10 @@ pids=()
 for src in "$here"/csrc/*.hip; do
   obj="$here/_build/$(basename "${src%.hip}").o"
   rm -f "$obj"
-  "$hipcc" "${flags[@]}" -c "$src" -o "$obj" 2> >(grep -v "packed-fp32-ops" >&2) &
+  extra=()
+  # the FP32 candidate search may use approximate reciprocals and transcendentals
+  if [ "$(basename "$src")" = "search.hip" ]; then extra=(-freciprocal-math -fapprox-func); fi
+  "$hipcc" "${flags[@]}" "${extra[@]}" -c "$src" -o "$obj" 2> >(grep -v "packed-fp32-ops" >&2) &
   objs+=("$obj")
   pids+=("$!")
 done

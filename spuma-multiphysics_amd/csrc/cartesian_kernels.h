@@ -149,6 +149,198 @@ __global__ __launch_bounds__(256) void cartesianStageKernel(CartesianArgs a, con
     if (HLLC && rejected) atomicAdd(failures, (unsigned long long)rejected);
 }
 
+// ---- tiled stage: every face computed once -------------------------------
+//
+// A workgroup owns a kTileX x kTileY tile of (i, j) columns over kChunkZ
+// cells in k and marches up in k.  The +x and +y faces are computed by the
+// cell on their minus side and shared through LDS, the +z face by the
+// column's own thread (it is the next cell's -z face); a tile's -x / -y edge
+// cells compute their -x / -y faces themselves, from the neighbour tile's
+// cells.  Velocities are computed once per cell.  Each face is evaluated
+// with the per-cell kernel's arithmetic and every cell sums its six faces in
+// its CSR order, so the result is the same bit for bit.
+constexpr int kTileX = 32, kTileY = 8, kChunkZ = 16;
+
+template <int NS, int NV, bool HLLC>
+struct TileCell {
+    double q[NV], p, c, rho, r, u[3];  // r: recovered density (HLLC only)
+};
+
+template <int NS, int NV, bool HLLC>
+__device__ inline void loadTileCell(TileCell<NS, NV, HLLC>& t, const double* __restrict__ q, size_t stride,
+                                    const double* __restrict__ pArr, const double* __restrict__ cArr,
+                                    const double* __restrict__ rArr, long c) {
+    for (int v = 0; v < NV; ++v) t.q[v] = q[size_t(v) * stride + c];
+    t.p = pArr[c];
+    t.c = cArr[c];
+    t.r = HLLC ? rArr[c] : 0.0;
+    t.rho = 0;
+    for (int s = 0; s < NS; ++s) t.rho += t.q[s];
+    for (int d = 0; d < 3; ++d) t.u[d] = t.q[NS + d] / t.rho;
+}
+
+// flux * area of one face, L = owner side, R = right side (the per-cell
+// kernel's expressions, velocities precomputed)
+template <int NS, int NV, bool HLLC>
+__device__ inline void tileFace(const CartesianArgs& a, const TileCell<NS, NV, HLLC>& L,
+                                const TileCell<NS, NV, HLLC>& R, const double* n, double* F, uint32_t& rejected) {
+    double unL = 0, unR = 0;
+    for (int d = 0; d < 3; ++d) {
+        unL += L.u[d] * n[d];
+        unR += R.u[d] * n[d];
+    }
+    const double aL = a.waveFactor * L.c, aR = a.waveFactor * R.c;
+    const double left = unL - aL < unR - aR ? unL - aL : unR - aR;
+    const double right = unL + aL > unR + aR ? unL + aL : unR + aR;
+    const double wl = 0 < left ? 0 : left, wr = 0 > right ? 0 : right;
+    const double invWave = 1 / (wr - wl);
+    double4 w;
+    w.x = wr * (unL - wl) * invWave;
+    w.y = wl * (wr - unR) * invWave;
+    w.z = (wr * L.p - wl * R.p) * invWave;
+    w.w = (wr * L.p * unL - wl * R.p * unR) * invWave;
+    if (HLLC && !hllcCoefficients(L.r, L.u, L.p, aL, L.q[NS + 3], R.r, R.u, R.p, aR, R.q[NS + 3], n, w)) ++rejected;
+    for (int v = 0; v < NV; ++v) {
+        double flux = w.x * L.q[v] + w.y * R.q[v];
+        if (v >= NS && v < NS + 3) flux += (w.z - 0.0) * n[v - NS];
+        if (v == NS + 3) flux += w.w;
+        if (v < NS) flux += 0.0;
+        else if (v < NS + 3) flux -= 0.0;
+        else if (v == NS + 3) flux += 0.0;
+        F[v] = flux * a.area;
+    }
+}
+
+template <int NS, int NV, bool HLLC>
+__global__ __launch_bounds__(kTileX * kTileY) void cartesianTiledStageKernel(
+    CartesianArgs a, const double* __restrict__ q, size_t stride, const double* __restrict__ pArr,
+    const double* __restrict__ cArr, const double* __restrict__ rArr, double dtHost, const double* __restrict__ dtDev,
+    int mode, double* __restrict__ out, unsigned long long* __restrict__ failures) {
+    using Cell = TileCell<NS, NV, HLLC>;
+    __shared__ Cell shared[kTileY][kTileX];
+    __shared__ double faceX[kTileY][kTileX][NV], faceY[kTileY][kTileX][NV];
+    const int tilesX = (a.nx + kTileX - 1) / kTileX, tilesY = (a.ny + kTileY - 1) / kTileY;
+    const int chunks = (a.nz + kChunkZ - 1) / kChunkZ;
+    const long blocks = long(tilesX) * tilesY * chunks;
+    const long b = xcdBlock(blockIdx.x, blocks);
+    if (b >= blocks) return;  // uniform per workgroup
+    const double dt = dtDev ? *dtDev : dtHost;
+    if (!(dt > 0)) return;  // uniform: an invalid device-side CFL step leaves q untouched
+    const int bx = int(b % tilesX), by = int((b / tilesX) % tilesY), bz = int(b / (long(tilesX) * tilesY));
+    const int tx = threadIdx.x % kTileX, ty = threadIdx.x / kTileX;
+    const int i = bx * kTileX + tx, j = by * kTileY + ty;
+    const bool active = i < a.nx && j < a.ny;
+    const int k0 = bz * kChunkZ, k1 = min(a.nz, k0 + kChunkZ);
+    const long nxy = long(a.nx) * a.ny;
+    uint32_t rejected = 0;
+    Cell cur, nxt;
+    double fzMinus[NV];
+    if (active) {
+        const long c0 = long(i) + long(a.nx) * j + nxy * k0;
+        loadTileCell(cur, q, stride, pArr, cArr, rArr, c0);
+        if (k0 == 0) {
+            tileFace(a, cur, cur, a.normal[4][1], fzMinus, rejected);
+        } else {
+            Cell below;
+            loadTileCell(below, q, stride, pArr, cArr, rArr, c0 - nxy);
+            tileFace(a, below, cur, a.normal[5][0], fzMinus, rejected);
+        }
+    }
+    for (int k = k0; k < k1; ++k) {
+        const long c = long(i) + long(a.nx) * j + nxy * k;
+        if (active) shared[ty][tx] = cur;
+        __syncthreads();
+        double fxMinus[NV], fyMinus[NV], fzPlus[NV];
+        if (active) {
+            double F[NV];
+            // +x face (this cell is its owner)
+            if (i == a.nx - 1) {
+                tileFace(a, cur, cur, a.normal[1][1], F, rejected);
+            } else if (tx + 1 < kTileX) {
+                tileFace(a, cur, shared[ty][tx + 1], a.normal[1][0], F, rejected);
+            } else {
+                Cell nb;
+                loadTileCell(nb, q, stride, pArr, cArr, rArr, c + 1);
+                tileFace(a, cur, nb, a.normal[1][0], F, rejected);
+            }
+            for (int v = 0; v < NV; ++v) faceX[ty][tx][v] = F[v];
+            // +y face
+            if (j == a.ny - 1) {
+                tileFace(a, cur, cur, a.normal[3][1], F, rejected);
+            } else if (ty + 1 < kTileY) {
+                tileFace(a, cur, shared[ty + 1][tx], a.normal[3][0], F, rejected);
+            } else {
+                Cell nb;
+                loadTileCell(nb, q, stride, pArr, cArr, rArr, c + a.nx);
+                tileFace(a, cur, nb, a.normal[3][0], F, rejected);
+            }
+            for (int v = 0; v < NV; ++v) faceY[ty][tx][v] = F[v];
+            // +z face; the next cell of the column stays in registers
+            if (k == a.nz - 1) {
+                tileFace(a, cur, cur, a.normal[5][1], fzPlus, rejected);
+            } else {
+                loadTileCell(nxt, q, stride, pArr, cArr, rArr, c + nxy);
+                tileFace(a, cur, nxt, a.normal[5][0], fzPlus, rejected);
+            }
+            // -x / -y faces: domain boundary, or a tile edge (neighbour in another tile)
+            if (i == 0) {
+                tileFace(a, cur, cur, a.normal[0][1], fxMinus, rejected);
+            } else if (tx == 0) {
+                Cell nb;
+                loadTileCell(nb, q, stride, pArr, cArr, rArr, c - 1);
+                tileFace(a, nb, cur, a.normal[1][0], fxMinus, rejected);
+            }
+            if (j == 0) {
+                tileFace(a, cur, cur, a.normal[2][1], fyMinus, rejected);
+            } else if (ty == 0) {
+                Cell nb;
+                loadTileCell(nb, q, stride, pArr, cArr, rArr, c - a.nx);
+                tileFace(a, nb, cur, a.normal[3][0], fyMinus, rejected);
+            }
+        }
+        __syncthreads();
+        if (active) {
+            if (i > 0 && tx > 0)
+                for (int v = 0; v < NV; ++v) fxMinus[v] = faceX[ty][tx - 1][v];
+            if (j > 0 && ty > 0)
+                for (int v = 0; v < NV; ++v) fyMinus[v] = faceY[ty - 1][tx][v];
+            const int type = (i == 0) | (i == a.nx - 1) << 1 | (j == 0) << 2 | (j == a.ny - 1) << 3 | (k == 0) << 4 |
+                             (k == a.nz - 1) << 5;
+            const uint32_t order = a.slotOrder[type];
+            double rate[NV];
+            for (int v = 0; v < NV; ++v) rate[v] = 0;
+            for (int pos = 0; pos < 6; ++pos) {
+                const int slot = int((order >> (3 * pos)) & 7u);
+                const int ax = slot >> 1, plus = slot & 1;
+                const int coord = ax == 0 ? i : (ax == 1 ? j : k);
+                const int ext = ax == 0 ? a.nx : (ax == 1 ? a.ny : a.nz);
+                const bool bnd = plus ? coord == ext - 1 : coord == 0;
+                const double sign = bnd || plus ? -1.0 : 1.0;
+                for (int v = 0; v < NV; ++v) {
+                    const double F = slot == 0 ? fxMinus[v] : slot == 1 ? faceX[ty][tx][v] : slot == 2 ? fyMinus[v]
+                                     : slot == 3 ? faceY[ty][tx][v] : slot == 4 ? fzMinus[v] : fzPlus[v];
+                    rate[v] += sign * F * a.invV;
+                }
+            }
+            for (int v = 0; v < NV; ++v) {
+                const size_t idx = size_t(v) * stride + c;
+                if (mode == 0) out[idx] = cur.q[v] + dt * rate[v];
+                else out[idx] = .5 * out[idx] + .5 * (cur.q[v] + dt * rate[v]);
+            }
+            for (int v = 0; v < NV; ++v) fzMinus[v] = fzPlus[v];
+            cur = nxt;
+        }
+        __syncthreads();
+    }
+    if (HLLC && rejected) atomicAdd(failures, (unsigned long long)rejected);
+}
+
+inline unsigned tiledGrid(const CartesianArgs& a) {
+    const long tiles = long((a.nx + kTileX - 1) / kTileX) * ((a.ny + kTileY - 1) / kTileY) *
+                       ((a.nz + kChunkZ - 1) / kChunkZ);
+    return xcdGrid(tiles);
+}
+
 // FaceSpeeds + Step + the block level of Minimum (reactiveTransportKernels.h:
 // 315-342, 529-550; reactiveTransport.cpp:222-232) on the structured box.
 // Face speeds are rebuilt per cell and summed in CSR order; invalid cell
@@ -257,12 +449,11 @@ inline bool cartesianSupported(int ns, int nv) {
 inline void launchCartesianStage(const CartesianArgs& a, int ns, int nv, int scheme, const double* q, size_t stride,
                                  const double* p, const double* c, const double* r, double dt, const double* dtDev,
                                  int mode, double* out, unsigned long long* failures, hipStream_t stream) {
-    const long cells = long(a.nx) * a.ny * a.nz;
-    const unsigned grid = xcdGrid((cells + 255) / 256);
-#define RAMD_CART(NS_, NV_, H_)                                                                                   \
-    hipLaunchKernelGGL((cartesianStageKernel<NS_, NV_, H_>), dim3(grid), dim3(256), 0, stream, a, q, stride, p, c, \
-                       r, dt, dtDev, mode, out, failures)
-#define RAMD_CART_SCHEME(NS_, NV_)          \
+    const unsigned grid = tiledGrid(a);
+#define RAMD_CART(NS_, NV_, H_)                                                                                    \
+    hipLaunchKernelGGL((cartesianTiledStageKernel<NS_, NV_, H_>), dim3(grid), dim3(kTileX * kTileY), 0, stream, a, \
+                       q, stride, p, c, r, dt, dtDev, mode, out, failures)
+#define RAMD_CART_SCHEME(NS_, NV_)                  \
     if (scheme == kHllc) RAMD_CART(NS_, NV_, true); \
     else RAMD_CART(NS_, NV_, false)
     if (ns == 1 && nv == 5) { RAMD_CART_SCHEME(1, 5); }

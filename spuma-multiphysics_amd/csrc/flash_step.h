@@ -1,0 +1,66 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+// Per-cell helpers shared by the resident step's flash kernels (step.hip)
+// and the single-precision search kernel (search.hip).
+#ifndef REACTIVE_AMD_FLASH_STEP_H
+#define REACTIVE_AMD_FLASH_STEP_H
+#include <hip/hip_runtime.h>
+#include "hip_common.h"
+#include "flash_device.h"
+
+namespace ramd {
+
+// Loads cell c into a flash: species masses, e = rhoE - |rho u|^2 / (2 rho)
+// (ReactiveFoam.C:727-728, in FP64) and the step-start seed.
+template <class F>
+__device__ inline void loadCell(F& f, const double* __restrict__ q, size_t stride, const ThermoOut& g, uint32_t c,
+                                int ns) {
+    using R = decltype(f.energy);
+    double rho = 0, mass[4];
+    for (int k = 0; k < ns; ++k) {
+        mass[k] = q[size_t(k) * stride + c];
+        f.q[k] = R(mass[k]);
+        rho += mass[k];
+    }
+    const double m0 = q[size_t(ns) * stride + c], m1 = q[size_t(ns + 1) * stride + c];
+    const double m2 = q[size_t(ns + 2) * stride + c], E = q[size_t(ns + 3) * stride + c];
+    f.energy = R(E - .5 * (m0 * m0 + m1 * m1 + m2 * m2) / rho);
+    f.guessP = R(g.p);
+    f.guessT = R(g.T);
+    f.guessLiquid = R(g.liquidMass[0]);
+}
+
+// Commits a flash result, or keeps the step-start state of a failed cell
+// (the reference commits nothing for it); p, c_frozen and rho go to the SoA
+// arrays the next transport stage reads.
+__device__ inline void storeCell(bool ok, const Eval& result, const ThermoOut& g, uint32_t c,
+                                 ThermoOut* __restrict__ out, double* __restrict__ pOut, double* __restrict__ cOut,
+                                 double* __restrict__ rOut) {
+    if (ok) {
+        writeOut(result, out[c]);
+        pOut[c] = result.p;
+        cOut[c] = result.cf;
+        rOut[c] = result.rho;
+    } else {
+        out[c] = g;
+        pOut[c] = g.p;
+        cOut[c] = g.soundFrozen;
+        rOut[c] = g.rho;
+    }
+}
+
+__device__ inline void countFlash(uint32_t failed, uint32_t phases, unsigned long long* __restrict__ counters) {
+    const uint32_t sumFailed = waveSum(failed), sumPhases = waveSum(phases);
+    if ((threadIdx.x & 63) == 0) {
+        if (sumFailed) atomicAdd(&counters[0], (unsigned long long)sumFailed);
+        atomicAdd(&counters[1], (unsigned long long)sumPhases);
+    }
+}
+
+// Single-precision candidate search of the mixed flash (search.hip): one
+// launch over the cells in `order`, candidate sets and FP32 costs by cell.
+void launchSearch(int ns, int nl, const void* model, const double* q, size_t stride, const ThermoOut* seeds,
+                  const uint32_t* order, uint32_t count, const double* dtDev, CandidateSet* sets, uint16_t* cost,
+                  unsigned long long* phases, hipStream_t stream);
+
+}  // namespace ramd
+#endif

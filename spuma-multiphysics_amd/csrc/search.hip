@@ -1,0 +1,59 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+// Single-precision candidate search of the mixed flash.  This file alone is
+// compiled with -freciprocal-math -fapprox-func (build.sh): the FP32 search
+// only chooses and seeds candidates, every accepted result is recomputed by
+// the FP64 refinement, so it may use the hardware reciprocal and
+// transcendental approximations.  Non-finite values keep their IEEE meaning
+// (no finite-math assumption), so the search's NaN and range checks hold.
+#include <hip/hip_runtime.h>
+#include "flash_step.h"
+
+// Minimum waves per SIMD the register allocation of the search kernel targets.
+#ifndef RAMD_SEARCH_MIN_WAVES
+#define RAMD_SEARCH_MIN_WAVES 1
+#endif
+
+namespace ramd {
+
+// Mixed precision, first half: the candidate search in FP32, recording the
+// converged candidates of every cell.
+template <int NS, int NL>
+__global__ __launch_bounds__(64, RAMD_SEARCH_MIN_WAVES) void stepSearchKernel(const HemModel<NS>* __restrict__ model,
+                                                       const double* __restrict__ q, size_t stride,
+                                                       const ThermoOut* __restrict__ seeds,
+                                                       const uint32_t* __restrict__ order, uint32_t count,
+                                                       const double* __restrict__ dtDev,
+                                                       CandidateSet* __restrict__ sets, uint16_t* __restrict__ cost,
+                                                       unsigned long long* __restrict__ phases) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    Counts n = {0, 0, 0, 0, 0, 0};
+    if (i < count && *dtDev > 0) {
+        const uint32_t c = order[i];
+        Flash<NS, NL, float> f(*model, n);
+        loadCell(f, q, stride, seeds[c], c, NS);
+        CandidateSet set;
+        f.searchRecord(set);
+        sets[c] = set;
+        if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
+    }
+    const uint32_t sum = waveSum(n.phases);
+    if ((threadIdx.x & 63) == 0) atomicAdd(phases, (unsigned long long)sum);
+}
+
+void launchSearch(int ns, int nl, const void* model, const double* q, size_t stride, const ThermoOut* seeds,
+                  const uint32_t* order, uint32_t count, const double* dtDev, CandidateSet* sets, uint16_t* cost,
+                  unsigned long long* phases, hipStream_t stream) {
+    const unsigned blocks = unsigned((size_t(count) + 63) / 64);
+#define RAMD_SEARCH(NS_, NL_)                                                                                     \
+    hipLaunchKernelGGL((stepSearchKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                            \
+                       static_cast<const HemModel<NS_>*>(model), q, stride, seeds, order, count, dtDev, sets, cost, \
+                       phases)
+    if (ns == 1 && nl == 0) { RAMD_SEARCH(1, 0); }
+    else if (ns == 1) { RAMD_SEARCH(1, 1); }
+    else if (nl == 0) { RAMD_SEARCH(2, 0); }
+    else { RAMD_SEARCH(2, 1); }
+#undef RAMD_SEARCH
+    RAMD_HIP(hipGetLastError());
+}
+
+}  // namespace ramd

@@ -28,58 +28,12 @@
 #include "hem_internal.h"
 #include "transport_internal.h"
 #include "cartesian_kernels.h"
+#include "flash_step.h"
 #include "reactive_amd.h"
 
 namespace ramd {
 
 // ---------------------------------------------------------------- kernels
-// Loads cell c into a flash: species masses, e = rhoE - |rho u|^2 / (2 rho)
-// (ReactiveFoam.C:727-728, in FP64) and the step-start seed.
-template <class F>
-__device__ inline void loadCell(F& f, const double* __restrict__ q, size_t stride, const ThermoOut& g, uint32_t c,
-                                int ns) {
-    using R = decltype(f.energy);
-    double rho = 0, mass[4];
-    for (int k = 0; k < ns; ++k) {
-        mass[k] = q[size_t(k) * stride + c];
-        f.q[k] = R(mass[k]);
-        rho += mass[k];
-    }
-    const double m0 = q[size_t(ns) * stride + c], m1 = q[size_t(ns + 1) * stride + c];
-    const double m2 = q[size_t(ns + 2) * stride + c], E = q[size_t(ns + 3) * stride + c];
-    f.energy = R(E - .5 * (m0 * m0 + m1 * m1 + m2 * m2) / rho);
-    f.guessP = R(g.p);
-    f.guessT = R(g.T);
-    f.guessLiquid = R(g.liquidMass[0]);
-}
-
-// Commits a flash result, or keeps the step-start state of a failed cell
-// (the reference commits nothing for it); p, c_frozen and rho go to the SoA
-// arrays the next transport stage reads.
-__device__ inline void storeCell(bool ok, const Eval& result, const ThermoOut& g, uint32_t c,
-                                 ThermoOut* __restrict__ out, double* __restrict__ pOut, double* __restrict__ cOut,
-                                 double* __restrict__ rOut) {
-    if (ok) {
-        writeOut(result, out[c]);
-        pOut[c] = result.p;
-        cOut[c] = result.cf;
-        rOut[c] = result.rho;
-    } else {
-        out[c] = g;
-        pOut[c] = g.p;
-        cOut[c] = g.soundFrozen;
-        rOut[c] = g.rho;
-    }
-}
-
-__device__ inline void countFlash(uint32_t failed, uint32_t phases, unsigned long long* __restrict__ counters) {
-    const uint32_t sumFailed = waveSum(failed), sumPhases = waveSum(phases);
-    if ((threadIdx.x & 63) == 0) {
-        if (sumFailed) atomicAdd(&counters[0], (unsigned long long)sumFailed);
-        atomicAdd(&counters[1], (unsigned long long)sumPhases);
-    }
-}
-
 // Plain FP64 flash of the cells in `order`.
 template <int NS, int NL>
 __global__ __launch_bounds__(64) void stepFlashKernel(const HemModel<NS>* __restrict__ model,
@@ -106,31 +60,6 @@ __global__ __launch_bounds__(64) void stepFlashKernel(const HemModel<NS>* __rest
         if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
     }
     countFlash(failed, n.phases, counters);
-}
-
-// Mixed precision, first half: the candidate search in FP32, recording the
-// converged candidates of every cell.
-template <int NS, int NL>
-__global__ __launch_bounds__(64) void stepSearchKernel(const HemModel<NS>* __restrict__ model,
-                                                       const double* __restrict__ q, size_t stride,
-                                                       const ThermoOut* __restrict__ seeds,
-                                                       const uint32_t* __restrict__ order, uint32_t count,
-                                                       const double* __restrict__ dtDev,
-                                                       CandidateSet* __restrict__ sets, uint16_t* __restrict__ cost,
-                                                       unsigned long long* __restrict__ phases) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    Counts n = {0, 0, 0, 0, 0, 0};
-    if (i < count && *dtDev > 0) {
-        const uint32_t c = order[i];
-        Flash<NS, NL, float> f(*model, n);
-        loadCell(f, q, stride, seeds[c], c, NS);
-        CandidateSet set;
-        f.searchRecord(set);
-        sets[c] = set;
-        if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
-    }
-    const uint32_t sum = waveSum(n.phases);
-    if ((threadIdx.x & 63) == 0) atomicAdd(phases, (unsigned long long)sum);
 }
 
 // Mixed precision, second half: FP64 refinement, stability and selection.
@@ -375,9 +304,8 @@ struct Step {
         const unsigned blocks = blocksFor(cells, 64);
         if (mode == REACTIVE_AMD_FLASH_MIXED) {
 #define RAMD_STEP_MIXED(NS_, NL_)                                                                                  \
-    hipLaunchKernelGGL((stepSearchKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                             \
-                       static_cast<const HemModel<NS_>*>(h->model), q, all, sA, orderSearch, uint32_t(cells),       \
-                       scalars, sets, costOut ? costSearch : nullptr, counters + 6);                              \
+    launchSearch(NS_, NL_, h->model, q, all, sA, orderSearch, uint32_t(cells), scalars, sets,                     \
+                 costOut ? costSearch : nullptr, counters + 6, stream);                                            \
     if (split) RAMD_HIP(hipEventRecord(split, stream));                                                            \
     hipLaunchKernelGGL((stepRefineKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                             \
                        static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), scalars,   \

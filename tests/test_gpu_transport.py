@@ -28,7 +28,7 @@ def gpu_rhs(lib, h, q, st, nc, nv):
 
 
 @pytest.mark.parametrize("scheme", [0, 1], ids=["hll", "hllc"])
-@pytest.mark.parametrize("n", [4, 7, 10])
+@pytest.mark.parametrize("n", [4, 7, 10, 34])
 def test_cartesian_rhs_dt_stages_bitwise(n, scheme):
     lib = rlib.load()
     case = cube_case(n, seed=n)
