@@ -54,6 +54,11 @@ typedef struct ReactiveAmdStepReport {
  * FP64 contradicts are searched again in FP64. */
 #define REACTIVE_AMD_FLASH_FP64 0
 #define REACTIVE_AMD_FLASH_MIXED 1
+/* OR-ed into flashMode: replay each step from a captured HIP graph (after a
+ * first step on the stream that records the flash scheduling costs).  The
+ * report then holds totals and the call's device time only, no per-kernel
+ * times. */
+#define REACTIVE_AMD_STEP_GRAPH 0x100
 
 /* Face flux of a transport handle (applies to reactive_transport_rhs/stage,
  * the resident stages and the resident step): REACTIVE_AMD_FLUX_HLL (the

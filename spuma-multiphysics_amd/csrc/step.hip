@@ -279,11 +279,19 @@ struct Step {
     // 11 per step: 9 in sequence bracketing cfl, stage 0, flash #1, recheck, stage 1,
     // flash #2, then the end of each flash's FP32 search (mixed mode)
     std::vector<hipEvent_t> marks;
+    // captured steps: graph[parity] for the current (sA, sB) assignment, with
+    // the settings it was captured for
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graphMode[2] = {-1, -1}, graphScheme[2] = {-1, -1};
+    double graphCfl[2] = {0, 0}, graphMax[2] = {0, 0};
+    int parity = 0;  // flips with every step (the seed and result buffers swap)
     bool loaded = false;
     char error[1024] = {};
 
     ~Step() {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (auto g : graph)
+            if (g) (void)hipGraphExecDestroy(g);
         for (auto e : events)
             if (e) (void)hipEventDestroy(e);
         for (auto e : marks) (void)hipEventDestroy(e);
@@ -378,6 +386,7 @@ struct Step {
         if (mode == REACTIVE_AMD_FLASH_MIXED) haveCostSearch = true;
         mark(m, 8);
         std::swap(sA, sB);
+        parity ^= 1;
         // conservation totals and boundary integral
         hipLaunchKernelGGL(totalsPartialKernel, dim3(kTotalBlocks), dim3(256), 0, stream, qA, all, cells, nv,
                            t->cartesian.volume, totalPartial);
@@ -386,6 +395,42 @@ struct Step {
         RAMD_HIP(hipGetLastError());
     }
 };
+
+// One step from a captured graph.  The first step after a load runs on the
+// stream (it records the costs that order the flashes); later steps replay the
+// graph captured for the current buffer parity, recaptured when the flash
+// arithmetic, flux scheme, CFL number or maximum step changes.  The step
+// launches only kernels (no memory-set or copy nodes).
+void graphStep(Step& s, double cfl, double maximum) {
+    if (!s.haveCost || (s.mode == REACTIVE_AMD_FLASH_MIXED && !s.haveCostSearch)) {
+        s.one(cfl, maximum, nullptr);
+        return;
+    }
+    const int p = s.parity;
+    if (!s.graph[p] || s.graphMode[p] != s.mode || s.graphScheme[p] != s.t->scheme || s.graphCfl[p] != cfl ||
+        s.graphMax[p] != maximum) {
+        if (s.graph[p]) {
+            RAMD_HIP(hipGraphExecDestroy(s.graph[p]));
+            s.graph[p] = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        RAMD_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+        s.one(cfl, maximum, nullptr);  // recorded, not run: undo its host-side buffer swap
+        std::swap(s.sA, s.sB);
+        s.parity ^= 1;
+        RAMD_HIP(hipStreamEndCapture(s.stream, &g));
+        const hipError_t st = hipGraphInstantiate(&s.graph[p], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        RAMD_HIP(st);
+        s.graphMode[p] = s.mode;
+        s.graphScheme[p] = s.t->scheme;
+        s.graphCfl[p] = cfl;
+        s.graphMax[p] = maximum;
+    }
+    RAMD_HIP(hipGraphLaunch(s.graph[p], s.stream));
+    std::swap(s.sA, s.sB);
+    s.parity ^= 1;
+}
 
 template <class F>
 int guard(void* handle, F f) {
@@ -493,6 +538,8 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         require(std::isfinite(cfl) && cfl > 0 && cfl <= .5 && std::isfinite(maximum) && maximum > 0,
                 "Invalid transport CFL controls");
         require(steps >= 1, "At least one step");
+        const bool useGraph = (flashMode & REACTIVE_AMD_STEP_GRAPH) != 0;
+        flashMode &= ~REACTIVE_AMD_STEP_GRAPH;
         require(flashMode == REACTIVE_AMD_FLASH_FP64 || flashMode == REACTIVE_AMD_FLASH_MIXED,
                 "Unknown flash arithmetic mode");
         if (flashMode == REACTIVE_AMD_FLASH_MIXED && !s.sets) s.sets = s.alloc<CandidateSet>(s.cells);
@@ -504,7 +551,10 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
             s.marks.push_back(e);
         }
         RAMD_HIP(hipEventRecord(s.events[0], s.stream));
-        for (int i = 0; i < steps; ++i) s.one(cfl, maximum, s.marks.data() + size_t(i) * kMarks);
+        for (int i = 0; i < steps; ++i) {
+            if (useGraph) graphStep(s, cfl, maximum);
+            else s.one(cfl, maximum, s.marks.data() + size_t(i) * kMarks);
+        }
         RAMD_HIP(hipEventRecord(s.events[1], s.stream));
         unsigned long long counters[8];
         double scalars[4], totals[REACTIVE_AMD_MAX_VARIABLES], boundary[REACTIVE_AMD_MAX_VARIABLES];
@@ -532,7 +582,7 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
             r.boundaryIntegral[k] = boundary[k];
         }
         r.seconds = ms / 1000.0;
-        for (int i = 0; i < steps; ++i) {
+        for (int i = 0; i < steps && !useGraph; ++i) {
             const hipEvent_t* m = s.marks.data() + size_t(i) * kMarks;
             float d[8];
             for (int j = 0; j < 8; ++j) RAMD_HIP(hipEventElapsedTime(&d[j], m[j], m[j + 1]));

@@ -105,3 +105,41 @@ def test_two_steps_cost_ordered():
     lib.reactive_amd_step_destroy(step)
     lib.reactive_transport_destroy(tr)
     lib.reactive_gpu_hem_destroy_v1(hem)
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["fp64", "mixed"])
+def test_graph_replays_match_stream_steps(mode):
+    """Steps replayed from captured HIP graphs (REACTIVE_AMD_STEP_GRAPH) equal
+    the same steps launched on the stream, bit for bit, replay after replay,
+    with host reads (synchronisation and copies) between the replays."""
+    lib = rlib.load()
+    case = cube_case(10, seed=77)
+    runs = {}
+    for graph in (False, True):
+        err = C.create_string_buffer(1024)
+        mb = case["model"]
+        hem = lib.reactive_gpu_hem_create_v1(mb, len(mb), len(case["q"]), err, 1024)
+        cfg = config(len(case["q"]), len(case["faces"]))
+        tr = lib.reactive_transport_create(1, C.byref(cfg), rlib.ptr(case["volume"]), rlib.ptr(case["faces"]),
+                                           None, None, None, None, err, 1024)
+        step = lib.reactive_amd_step_create(tr, hem, err, 1024)
+        assert step, err.value
+        assert lib.reactive_amd_step_load(step, rlib.ptr(case["q"]), rlib.ptr(case["states"])) == 0
+        out = []
+        for i in range(5):
+            rep = abi.StepReport.fresh()
+            flag = mode | (0x100 if graph else 0)
+            assert lib.reactive_amd_step_run(step, 0.4, 1.0, 1 if i < 3 else 2, flag, C.byref(rep)) == 0, \
+                lib.reactive_amd_step_error(step)
+            q = np.zeros_like(case["q"])
+            s = np.zeros(len(q), dtype=abi.STATE_DTYPE)
+            assert lib.reactive_amd_step_read(step, rlib.ptr(q), rlib.ptr(s)) == 0
+            out.append((q, s, rep.dt, tuple(rep.totals[:5])))
+        runs[graph] = out
+        lib.reactive_amd_step_destroy(step)
+        lib.reactive_transport_destroy(tr)
+        lib.reactive_gpu_hem_destroy_v1(hem)
+    for (qe, se, dte, te), (qg, sg, dtg, tg) in zip(runs[False], runs[True]):
+        assert np.array_equal(qe, qg)
+        assert se.tobytes() == sg.tobytes()
+        assert dte == dtg and te == tg
