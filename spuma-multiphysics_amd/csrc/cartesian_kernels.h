@@ -159,7 +159,13 @@ __global__ __launch_bounds__(256) void cartesianStageKernel(CartesianArgs a, con
 // cells.  Velocities are computed once per cell.  Each face is evaluated
 // with the per-cell kernel's arithmetic and every cell sums its six faces in
 // its CSR order, so the result is the same bit for bit.
-constexpr int kTileX = 32, kTileY = 8, kChunkZ = 16;
+#ifndef RAMD_TILE_Z
+#define RAMD_TILE_Z 4
+#endif
+#ifndef RAMD_STAGE_TILED
+#define RAMD_STAGE_TILED 1
+#endif
+constexpr int kTileX = 32, kTileY = 8, kChunkZ = RAMD_TILE_Z;
 
 template <int NS, int NV, bool HLLC>
 struct TileCell {
@@ -449,10 +455,18 @@ inline bool cartesianSupported(int ns, int nv) {
 inline void launchCartesianStage(const CartesianArgs& a, int ns, int nv, int scheme, const double* q, size_t stride,
                                  const double* p, const double* c, const double* r, double dt, const double* dtDev,
                                  int mode, double* out, unsigned long long* failures, hipStream_t stream) {
+#if RAMD_STAGE_TILED
     const unsigned grid = tiledGrid(a);
 #define RAMD_CART(NS_, NV_, H_)                                                                                    \
     hipLaunchKernelGGL((cartesianTiledStageKernel<NS_, NV_, H_>), dim3(grid), dim3(kTileX * kTileY), 0, stream, a, \
                        q, stride, p, c, r, dt, dtDev, mode, out, failures)
+#else
+    const long cells = long(a.nx) * a.ny * a.nz;
+    const unsigned grid = xcdGrid((cells + 255) / 256);
+#define RAMD_CART(NS_, NV_, H_)                                                                                   \
+    hipLaunchKernelGGL((cartesianStageKernel<NS_, NV_, H_>), dim3(grid), dim3(256), 0, stream, a, q, stride, p, c, \
+                       r, dt, dtDev, mode, out, failures)
+#endif
 #define RAMD_CART_SCHEME(NS_, NV_)                  \
     if (scheme == kHllc) RAMD_CART(NS_, NV_, true); \
     else RAMD_CART(NS_, NV_, false)

@@ -405,7 +405,7 @@ struct Flash {
         return fin(F) && fin(cv) && cv > 0;
     }
     // Bracketed scalar Newton in T (reactiveDeviceFlash.h:276-299).
-    RAMD_NOINLINE bool scalarGas(E& result) {
+    RAMD_NOINLINE bool scalarGas(E& result, int polish = 0) {
         const R rho = density();
         R T = clamp(guessT, Tmin, Tmax), F, cv, p, cp;
         if (!gasProbe(T, F, cv, p, &cp)) return false;
@@ -446,6 +446,16 @@ struct Flash {
             if (!gasProbe(T, F, cv, p, nullptr)) return false;
         }
         if (ab(F) > etol * scale) return false;
+        // refinement only: further Newton steps toward 1e-3 of the tolerance;
+        // a failed probe keeps the converged point
+        for (int extra = 0; extra < polish && ab(F) > R(1e-3) * etol * scale; ++extra) {
+            if (F > 0) upper = T; else lower = T;
+            const R next = T - F / cv;
+            const R trialT = fin(next) && next > lower && next < upper ? next : R(.5) * (lower + upper);
+            R trialF, trialCv, trialP;
+            if (!gasProbe(trialT, trialF, trialCv, trialP, nullptr)) break;
+            T = trialT; F = trialF; cv = trialCv; p = trialP;
+        }
         if (!evaluate(R(0), p, T, result, false)) return false;
         result.volRes = ab(result.volume - R(1));
         result.enRes = ab(result.energy - energy) / energyScale(result);
@@ -453,24 +463,35 @@ struct Flash {
         return result.volRes <= vtol && result.enRes <= etol;
     }
     // Fixed split, 2x2 Newton in (ln p, ln T) with line search (reactiveDeviceFlash.h:300-326).
-    RAMD_NOINLINE bool frozen(R mass0, E& value, int stallLimit, R mass1 = R(0)) {
+    // polish > 0 (FP64 refinement only): up to that many further Newton steps
+    // once the tolerances hold, while the residual is above 1e-3 of them; a
+    // rejected polishing step keeps the converged point.
+    RAMD_NOINLINE bool frozen(R mass0, E& value, int stallLimit, R mass1 = R(0), int polish = 0) {
         stalled = false;
         if (!fin(energy)) return false;
-        if (m.scalarRecovery && mass0 == 0 && mass1 == 0 && scalarGas(value)) return true;
+        if (m.scalarRecovery && mass0 == 0 && mass1 == 0 && scalarGas(value, polish)) return true;
         R p = clamp(guessP, pmin, pmax), T = clamp(guessT, Tmin, Tmax);
         if (!evaluate(mass0, p, T, value, false)) return false;
         int stalls = 0;
+        bool polishing = false;
         for (int it = 0; it < 70; ++it) {
             const R scale = energyScale(value), rv = value.volume - R(1), re = (value.energy - energy) / scale;
             value.volRes = ab(rv);
             value.enRes = ab(re);
             value.iterations = it;
-            if (ab(rv) <= vtol && ab(re) <= etol) return true;
+            if (ab(rv) <= vtol && ab(re) <= etol) {
+                if (polish > 0 && !(ab(rv) <= R(1e-3) * vtol && ab(re) <= R(1e-3) * etol)) {
+                    --polish;
+                    polishing = true;
+                } else {
+                    return true;
+                }
+            }
             R jac[3][3] = {};
             jac[0][0] = value.Vp * p; jac[0][1] = value.VT * T;
             jac[1][0] = value.Ep * p / scale; jac[1][1] = value.ET * T / scale;
             R rhs[2] = {-rv, -re}, step[2];
-            if (!solveN<2>(jac, rhs, step)) return false;
+            if (!solveN<2>(jac, rhs, step)) return polishing;
             const R limit = hi(R(1), hi(ab(step[0]) / R(.7), ab(step[1]) / R(.2)));
             step[0] /= limit; step[1] /= limit;
             const R previous = hi(ab(rv), ab(re));
@@ -489,13 +510,13 @@ struct Flash {
             }
             if (!accepted) {
                 stalled = P::single && previous <= R(P::handOver);
-                return false;
+                return polishing;
             }
             if (stallLimit > 0 && stalls >= stallLimit &&
                 !(ab(value.volume - R(1)) <= vtol && ab(value.energy - energy) / energyScale(value) <= etol))
                 return false;
         }
-        return false;
+        return polishing;
     }
     // Residual [V-1, (E-e)/scale, mu_l/RT - mu_g/RT] in x = [ln p, ln T, beta]
     // (reactiveDeviceFlash.h:327-337).
@@ -552,11 +573,24 @@ struct Flash {
     // tolerances hold, while the residual is above 1e-3 of them (the FP64
     // refinement of a single-precision candidate).  In single precision the
     // converged point is returned without the equilibrium sound speed.
+    // Converged point: the equilibrium sound speed from the Jacobian's response
+    // to a volume/energy perturbation (reactiveDeviceFlash.h:400-409).
+    RAMD_NOINLINE bool activeFinish(const R* x, R scale, const R* f, E& value) {
+        const R rho = density();
+        R jac[3][3];
+        if (!jacobian(x, scale, f, jac)) return false;
+        R rhs[3] = {-R(1) / rho, value.p / (rho * scale), R(0)}, response[3];
+        if (!solveN<3>(jac, rhs, response)) return false;
+        const R squared = value.p * response[0];
+        if (!fin(squared) || squared <= 0) return false;
+        value.ceq = m_sqrt(squared);
+        return value.ceq <= value.cf * (R(1) + R(2e-3));
+    }
     RAMD_NOINLINE bool activeNewton(R* x, R scale, E& value, int polish) {
         stalled = false;
-        const R rho = density();
         R f[3];
         if (!residual(x, scale, f, &value)) return false;
+        bool polishing = false;  // converged already; a failed polishing step keeps the point
         for (int it = 0; it < 90; ++it) {
             const R chemical = hi(R(0), ab(f[2]));
             R jac[3][3];
@@ -570,19 +604,14 @@ struct Flash {
                                    chemical <= R(1e-3) * mutol;
                 if (polish > 0 && !tight) {
                     --polish;
+                    polishing = true;
                 } else {
-                    if (!jacobian(x, scale, f, jac)) return false;
-                    R rhs[3] = {-R(1) / rho, value.p / (rho * scale), R(0)}, response[3];
-                    if (!solveN<3>(jac, rhs, response)) return false;
-                    const R squared = value.p * response[0];
-                    if (!fin(squared) || squared <= 0) return false;
-                    value.ceq = m_sqrt(squared);
-                    return value.ceq <= value.cf * (R(1) + R(2e-3));
+                    return activeFinish(x, scale, f, value);
                 }
             }
-            if (!jacobian(x, scale, f, jac)) return false;
+            if (!jacobian(x, scale, f, jac)) return polishing && activeFinish(x, scale, f, value);
             R rhs[3] = {-f[0], -f[1], -f[2]}, step[3];
-            if (!solveN<3>(jac, rhs, step)) return false;
+            if (!solveN<3>(jac, rhs, step)) return polishing && activeFinish(x, scale, f, value);
             R limiter = hi(R(1), hi(ab(step[0]) / R(.7), ab(step[1]) / R(.18)));
             limiter = hi(limiter, ab(step[2]) / R(.3));
             for (int j = 0; j < 3; ++j) step[j] /= limiter;
@@ -617,10 +646,10 @@ struct Flash {
             }
             if (!accepted) {
                 stalled = P::single && previous <= R(P::handOver);
-                return false;
+                return polishing && activeFinish(x, scale, f, value);
             }
         }
-        return false;
+        return polishing && activeFinish(x, scale, f, value);
     }
     RAMD_NOINLINE bool activeFlash(R seed, E& value, bool restart, R restartT) {
         ++n.candidates;
@@ -826,7 +855,7 @@ struct Flash {
         const R p0 = guessP, T0 = guessT;
         guessP = m_exp(R(c.x[0]));
         guessT = m_exp(R(c.x[1]));
-        bool ok = frozen(mass0, value, stallLimit);
+        bool ok = frozen(mass0, value, stallLimit, R(0), 2);
         guessP = p0;
         guessT = T0;
         if (!ok) ok = frozen(mass0, value, stallLimit);  // from the cell's own guess, as the reference starts

@@ -7,7 +7,34 @@
 #include "hip_common.h"
 #include "flash_device.h"
 
+// Tuning knobs (compile-time): the model image in LDS, and the minimum waves
+// per SIMD the FP64 flash kernels' register allocation targets.
+#ifndef RAMD_MODEL_LDS
+#define RAMD_MODEL_LDS 1
+#endif
+#ifndef RAMD_FLASH_MIN_WAVES
+#define RAMD_FLASH_MIN_WAVES 2
+#endif
+
 namespace ramd {
+
+// The workgroup's copy of the model image: tables are read from LDS rather
+// than global memory.  Every thread of the workgroup must call this (it
+// synchronises the workgroup).
+template <int NS>
+__device__ inline const HemModel<NS>& workgroupModel(const HemModel<NS>* __restrict__ global) {
+#if RAMD_MODEL_LDS
+    static_assert(sizeof(HemModel<NS>) % 4 == 0, "model image in 4-byte words");
+    __shared__ HemModel<NS> local;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(global);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&local);
+    for (uint32_t w = threadIdx.x; w < sizeof(HemModel<NS>) / 4; w += blockDim.x) dst[w] = src[w];
+    __syncthreads();
+    return local;
+#else
+    return *global;
+#endif
+}
 
 // Loads cell c into a flash: species masses, e = rhoE - |rho u|^2 / (2 rho)
 // (ReactiveFoam.C:727-728, in FP64) and the step-start seed.

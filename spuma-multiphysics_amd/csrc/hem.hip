@@ -21,6 +21,7 @@
 #include "flash_device.h"
 #include "model_image.h"
 #include "hem_internal.h"
+#include "flash_step.h"
 #include "reactive_gpu_hem.h"
 
 namespace ramd {
@@ -38,7 +39,7 @@ static_assert(sizeof(HemOutputDev) == sizeof(ReactiveGpuHemOutputV1) &&
 // equilibrium = 0): frozen inventory at the seed's liquid masses
 // (recoverCapillary, reactiveDeviceFlash.h:572-590).
 template <int NS, int NL>
-__global__ __launch_bounds__(64) void hemFlashKernel(const HemModel<NS>* __restrict__ model,
+__global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void hemFlashKernel(const HemModel<NS>* __restrict__ globalModel,
                                                      const double* __restrict__ q, const double* __restrict__ energy,
                                                      const ThermoOut* __restrict__ guess,
                                                      const uint32_t* __restrict__ order, uint32_t count,
@@ -46,12 +47,13 @@ __global__ __launch_bounds__(64) void hemFlashKernel(const HemModel<NS>* __restr
                                                      HemOutputDev* __restrict__ out,
                                                      unsigned long long* __restrict__ totals,
                                                      uint32_t* __restrict__ cellPhases) {
+    const HemModel<NS>& model = workgroupModel(globalModel);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Counts n = {0, 0, 0, 0, 0, 0};
     uint32_t succeeded = 0;
     if (i < count) {
         const uint32_t c = order ? order[i] : i;
-        Flash<NS, NL> f(*model, n);
+        Flash<NS, NL> f(model, n);
         for (int k = 0; k < NS; ++k) f.q[k] = q[size_t(c) * NS + k];
         f.energy = energy[c];
         const ThermoOut& g = guess[c];

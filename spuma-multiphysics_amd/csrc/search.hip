@@ -10,7 +10,7 @@
 
 // Minimum waves per SIMD the register allocation of the search kernel targets.
 #ifndef RAMD_SEARCH_MIN_WAVES
-#define RAMD_SEARCH_MIN_WAVES 1
+#define RAMD_SEARCH_MIN_WAVES 4
 #endif
 
 namespace ramd {
@@ -18,18 +18,19 @@ namespace ramd {
 // Mixed precision, first half: the candidate search in FP32, recording the
 // converged candidates of every cell.
 template <int NS, int NL>
-__global__ __launch_bounds__(64, RAMD_SEARCH_MIN_WAVES) void stepSearchKernel(const HemModel<NS>* __restrict__ model,
+__global__ __launch_bounds__(64, RAMD_SEARCH_MIN_WAVES) void stepSearchKernel(const HemModel<NS>* __restrict__ globalModel,
                                                        const double* __restrict__ q, size_t stride,
                                                        const ThermoOut* __restrict__ seeds,
                                                        const uint32_t* __restrict__ order, uint32_t count,
                                                        const double* __restrict__ dtDev,
                                                        CandidateSet* __restrict__ sets, uint16_t* __restrict__ cost,
                                                        unsigned long long* __restrict__ phases) {
+    const HemModel<NS>& model = workgroupModel(globalModel);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Counts n = {0, 0, 0, 0, 0, 0};
     if (i < count && *dtDev > 0) {
         const uint32_t c = order[i];
-        Flash<NS, NL, float> f(*model, n);
+        Flash<NS, NL, float> f(model, n);
         loadCell(f, q, stride, seeds[c], c, NS);
         CandidateSet set;
         f.searchRecord(set);

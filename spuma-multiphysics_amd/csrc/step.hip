@@ -36,7 +36,7 @@ namespace ramd {
 // ---------------------------------------------------------------- kernels
 // Plain FP64 flash of the cells in `order`.
 template <int NS, int NL>
-__global__ __launch_bounds__(64) void stepFlashKernel(const HemModel<NS>* __restrict__ model,
+__global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void stepFlashKernel(const HemModel<NS>* __restrict__ globalModel,
                                                       const double* __restrict__ q, size_t stride,
                                                       const ThermoOut* __restrict__ seeds,
                                                       const uint32_t* __restrict__ order, uint32_t count,
@@ -44,13 +44,14 @@ __global__ __launch_bounds__(64) void stepFlashKernel(const HemModel<NS>* __rest
                                                       double* __restrict__ pOut, double* __restrict__ cOut,
                                                       double* __restrict__ rOut, uint16_t* __restrict__ cost,
                                                       unsigned long long* __restrict__ counters) {
+    const HemModel<NS>& model = workgroupModel(globalModel);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Counts n = {0, 0, 0, 0, 0, 0};
     uint32_t failed = 0;
     if (i < count && *dtDev > 0) {
         const uint32_t c = order[i];
         const ThermoOut g = seeds[c];
-        Flash<NS, NL> f(*model, n);
+        Flash<NS, NL> f(model, n);
         loadCell(f, q, stride, g, c, NS);
         Eval result;
         zeroEval(result);
@@ -64,7 +65,7 @@ __global__ __launch_bounds__(64) void stepFlashKernel(const HemModel<NS>* __rest
 
 // Mixed precision, second half: FP64 refinement, stability and selection.
 template <int NS, int NL>
-__global__ __launch_bounds__(64) void stepRefineKernel(const HemModel<NS>* __restrict__ model,
+__global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void stepRefineKernel(const HemModel<NS>* __restrict__ globalModel,
                                                        const double* __restrict__ q, size_t stride,
                                                        const ThermoOut* __restrict__ seeds,
                                                        const uint32_t* __restrict__ order, uint32_t count,
@@ -74,13 +75,14 @@ __global__ __launch_bounds__(64) void stepRefineKernel(const HemModel<NS>* __res
                                                        double* __restrict__ cOut, double* __restrict__ rOut,
                                                        uint16_t* __restrict__ cost,
                                                        unsigned long long* __restrict__ counters) {
+    const HemModel<NS>& model = workgroupModel(globalModel);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Counts n = {0, 0, 0, 0, 0, 0};
     uint32_t failed = 0;
     if (i < count && *dtDev > 0) {
         const uint32_t c = order[i];
         const ThermoOut g = seeds[c];
-        Flash<NS, NL> f(*model, n);
+        Flash<NS, NL> f(model, n);
         loadCell(f, q, stride, g, c, NS);
         Eval result;
         zeroEval(result);
