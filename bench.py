@@ -185,7 +185,7 @@ def stage_roofline(rep, cells, steps, n, nv=5):
     bytes0, bytes1 = 8 * (2 * nv + 2), 8 * (3 * nv + 2)  # algorithmic bytes per cell, stage 0 / stage 1
     t_stage = (rep.stageSeconds[0] + rep.stageSeconds[1]) / steps
     achieved = (bytes0 + bytes1) * cells / t_stage / 1e9
-    return {"bound": "hbm", "kernel": "cartesianStageKernel (both SSPRK2 stages)", "achieved": achieved,
+    return {"bound": "hbm", "kernel": "cartesianTiledStageKernel (both SSPRK2 stages)", "achieved": achieved,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": STAGE_TRAFFIC_160 if n == 160 and nv == 5 else None,
             "bytes_per_cell": [bytes0, bytes1],
@@ -197,6 +197,12 @@ def stage_roofline(rep, cells, steps, n, nv=5):
 # scaled by a plain copy kernel's counts): profiles/cartesianStage160_traffic.txt
 STAGE_TRAFFIC_160 = [420.3e6, 583.7e6]
 TIGHT = {"vtol": 1e-12, "etol": 1e-12, "mutol": 1e-10}
+# VALU FLOPs per phase evaluation (ADD, MUL, TRANS = 1, FMA = 2), measured with
+# rocprofv3 on divergence-free uniform cells, mean of a two-phase cell and a
+# gas cell: profiles/flash_valu_uniform32.txt.  The flash roofline's achieved
+# rate is phase evaluations x this figure / event-timed kernel time.
+FLOP_PER_PHASE = {"fp64_flash": (1453 + 1860) / 2, "fp32_search": (414 + 502) / 2, "fp64_refine": (1845 + 1361) / 2}
+FP32_PEAK_TFLOPS = 157.3
 
 
 def converged_roots(lib, q, seeds):
@@ -229,6 +235,35 @@ def distance(s, states0, root, ok):
         r = np.abs(s[f][use] - root[f][use]) / np.abs(root[f][use])
         out[f] = {"max": float(r.max()), "p99": float(np.percentile(r, 99)),
                   "share_le_1e-10": float((r <= 1e-10).mean())}
+    return out
+
+
+def flash_roofline(rep, rep64, cells, steps, mixed):
+    """VALU rate of the flash kernels against the vector peaks (see FLOP_PER_PHASE)."""
+    flashes = 2 * steps
+    out = {"bound": "valu", "unit": "TFLOP/s", "flop_source": "profiles/flash_valu_uniform32.txt"}
+    t64 = (rep64.flashSeconds[0] + rep64.flashSeconds[1]) / flashes
+    a64 = rep64.phaseEvaluations / flashes * FLOP_PER_PHASE["fp64_flash"] / t64 / 1e12
+    out["fp64_flash"] = {"kernel": "stepFlashKernel", "avg_launch_ms": 1e3 * t64, "achieved": a64,
+                         "peak": FP64_PEAK_TFLOPS, "frac": a64 / FP64_PEAK_TFLOPS,
+                         "phase_evaluations_per_cell": rep64.phaseEvaluations / (cells * flashes)}
+    if mixed:
+        t = (rep.flashSeconds[0] + rep.flashSeconds[1]) / flashes
+        ts = rep.searchSeconds / flashes
+        tr = t - ts
+        a32 = rep.singlePhaseEvaluations / flashes * FLOP_PER_PHASE["fp32_search"] / ts / 1e12
+        ar = rep.phaseEvaluations / flashes * FLOP_PER_PHASE["fp64_refine"] / tr / 1e12
+        out["avg_launch_ms"] = 1e3 * t
+        out["fp32_search"] = {"kernel": "stepSearchKernel", "avg_launch_ms": 1e3 * ts, "achieved": a32,
+                              "peak": FP32_PEAK_TFLOPS, "frac": a32 / FP32_PEAK_TFLOPS,
+                              "phase_evaluations_per_cell": rep.singlePhaseEvaluations / (cells * flashes)}
+        out["fp64_refine"] = {"kernel": "stepRefineKernel", "avg_launch_ms": 1e3 * tr, "achieved": ar,
+                              "peak": FP64_PEAK_TFLOPS, "frac": ar / FP64_PEAK_TFLOPS,
+                              "phase_evaluations_per_cell": rep.phaseEvaluations / (cells * flashes)}
+        out["share_of_step"] = (rep.flashSeconds[0] + rep.flashSeconds[1]) / rep.seconds
+    else:
+        out["avg_launch_ms"] = 1e3 * t64
+        out["share_of_step"] = (rep64.flashSeconds[0] + rep64.flashSeconds[1]) / rep64.seconds
     return out
 
 
@@ -317,12 +352,7 @@ def main():
                              else "fp64 (plain, reference arithmetic)"),
                    "seed": a.seed, "two_phase_fraction": round(info["two_phase_fraction"], 4)},
         "roofline": stage_roofline(rep, cells, a.steps, a.n),
-        "roofline_flash": {"bound": "valu", "kernel": "stepSearchKernel + stepRefineKernel" if mixed
-                           else "stepFlashKernel", "avg_launch_ms": 1e3 * t_flash,
-                           "phase_evaluations_per_cell_step": rep.phaseEvaluations / (cells * a.steps),
-                           "fp32_phase_evaluations_per_cell_step": rep.singlePhaseEvaluations / (cells * a.steps),
-                           "search_ms": 1e3 * rep.searchSeconds / (2 * a.steps),
-                           "share_of_step": (rep.flashSeconds[0] + rep.flashSeconds[1]) / rep.seconds},
+        "roofline_flash": flash_roofline(rep, rep64, cells, a.steps, mixed),
         "fp64": fp64,
         "speedup_vs_fp64": value / value64,
         "mixed_vs_fp64": accuracy,

@@ -56,6 +56,19 @@ struct Counts {
     uint32_t phases, residuals, candidates, stable, failures, fd;
 };
 
+// Per-phase properties the single-precision search's analytic Jacobian
+// reads (specific volume and energy, molar enthalpy / RT, compressibility);
+// empty in FP64, whose Jacobian is the reference's finite difference.
+template <class R>
+struct PhaseParts {
+    RD void clear() {}
+};
+template <>
+struct PhaseParts<float> {
+    float vG, vL, eG, eL, hG, hL, zG, zL;
+    RD void clear() { vG = vL = eG = eL = hG = hL = zG = zL = 0; }
+};
+
 // Evaluation record (reactiveDeviceFlash.h:64-73) for at most one liquid.
 template <class R>
 struct EvalT {
@@ -63,6 +76,7 @@ struct EvalT {
     R alphaGas, alphaLiq, rhoGas, rhoLiq, liqMass, gasMass, cp, cv;
     int active, iterations;
     R volume, energy, entropyDensity, Vp, VT, Ep, ET, cpDensity, muGas, muLiquid;
+    PhaseParts<R> parts;
 };
 using Eval = EvalT<double>;
 
@@ -72,6 +86,7 @@ RD void zeroEval(EvalT<R>& v) {
     v.alphaGas = v.alphaLiq = v.rhoGas = v.rhoLiq = v.liqMass = v.gasMass = v.cp = v.cv = 0;
     v.active = v.iterations = 0;
     v.volume = v.energy = v.entropyDensity = v.Vp = v.VT = v.Ep = v.ET = v.cpDensity = v.muGas = v.muLiquid = 0;
+    v.parts.clear();
 }
 
 RD void writeOut(const Eval& v, ThermoOut& o) {
@@ -304,6 +319,36 @@ struct Flash {
         v.cpDensity += mass * p.cp;
     }
 
+    // Specific volume and energy, molar enthalpy / RT and compressibility of
+    // one phase state (single-precision search only).
+    static RD void recordPhase(const S& s, R p, R T, R& v, R& e, R& h, R& z) {
+        const R RT = R(8314.46261815324) * T;
+        v = R(1) / s.rho;
+        e = s.e;
+        h = s.h * s.V * s.rho / RT;  // molar weight = V rho
+        z = p * s.V / RT;
+    }
+
+    // Analytic Jacobian of the one-liquid residual for a single gas species
+    // (single-precision search): with the gas of the condensable's own
+    // species, the volume and energy rows are the (p, T) derivatives the
+    // evaluation accumulates plus the change of one unit of mass from gas to
+    // liquid, and the chemical-potential row follows from
+    // d(mu/RT)/d ln p = Z and d(mu/RT)/d ln T = -h/RT (Gibbs-Helmholtz); mu
+    // does not depend on the phase amounts.
+    RD void analyticJacobian(const E& v, R scale, R (&jac)[3][3]) const {
+        const R qc = q[m.condensable];
+        jac[0][0] = v.Vp * v.p;
+        jac[0][1] = v.VT * v.T;
+        jac[0][2] = qc * (v.parts.vL - v.parts.vG);
+        jac[1][0] = v.Ep * v.p / scale;
+        jac[1][1] = v.ET * v.T / scale;
+        jac[1][2] = qc * (v.parts.eL - v.parts.eG) / scale;
+        jac[2][0] = v.parts.zL - v.parts.zG;
+        jac[2][1] = -(v.parts.hL - v.parts.hG);
+        jac[2][2] = 0;
+    }
+
     // Gas + liquid mixture at common (p, T) (reactiveDeviceFlash.h:208-262).
     RAMD_NOINLINE bool evaluate(R mass0, R p, R T, E& v, bool virtualLiquids) {
         zeroEval(v);
@@ -334,6 +379,7 @@ struct Flash {
             accumulate(v, mg, prop, p);
             v.alphaGas = mg / prop.rho;
             v.rhoGas = prop.rho;
+            if constexpr (P::single) recordPhase(prop, p, T, v.parts.vG, v.parts.eG, v.parts.hG, v.parts.zG);
             if (NL && virtualLiquids) v.muGas = mu[c] / (R(8314.46261815324) * T);
         }
         if (NL) {
@@ -356,6 +402,9 @@ struct Flash {
                     v.muLiquid = R(Real<R>::big) * R(2);  // +infinity
                 } else {
                     if (virtualLiquids) v.muLiquid = mu[0] / (R(8314.46261815324) * T);
+                    if constexpr (P::single)
+                        if (cacheStatus < 0 || !(p == cacheP && T == cacheT && virtualLiquids == cacheVirtual))
+                            recordPhase(prop, p, T, v.parts.vL, v.parts.eL, v.parts.hL, v.parts.zL);
                     if (mass0 > 0) {
                         accumulate(v, mass0, prop, p);
                         v.rhoLiq = prop.rho;
@@ -609,7 +658,11 @@ struct Flash {
                     return activeFinish(x, scale, f, value);
                 }
             }
-            if (!jacobian(x, scale, f, jac)) return polishing && activeFinish(x, scale, f, value);
+            if constexpr (P::single && NS == 1) {
+                analyticJacobian(value, scale, jac);
+            } else {
+                if (!jacobian(x, scale, f, jac)) return polishing && activeFinish(x, scale, f, value);
+            }
             R rhs[3] = {-f[0], -f[1], -f[2]}, step[3];
             if (!solveN<3>(jac, rhs, step)) return polishing && activeFinish(x, scale, f, value);
             R limiter = hi(R(1), hi(ab(step[0]) / R(.7), ab(step[1]) / R(.18)));

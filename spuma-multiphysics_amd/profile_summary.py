@@ -13,10 +13,11 @@ from collections import defaultdict
 
 def trace(db):
     con = sqlite3.connect(db)
+    # the top_kernels view reports durations in microseconds
     rows = con.execute("select name, total_calls, total_duration, average, percentage from top_kernels").fetchall()
     print(f"{'kernel':72s} {'calls':>6s} {'total ms':>12s} {'avg us':>12s} {'%':>7s}")
     for name, calls, total, avg, pct in rows:
-        print(f"{name[:72]:72s} {calls:6d} {total / 1e6:12.3f} {avg / 1e3:12.3f} {pct:7.2f}")
+        print(f"{name[:72]:72s} {calls:6d} {total / 1e3:12.3f} {avg:12.3f} {pct:7.2f}")
 
 
 def pmc(db, pattern=None):
