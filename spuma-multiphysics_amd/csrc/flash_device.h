@@ -56,32 +56,33 @@ struct Counts {
     uint32_t phases, residuals, candidates, stable, failures, fd;
 };
 
-// Per-phase properties the single-precision search's analytic Jacobian
-// reads (specific volume and energy, molar enthalpy / RT, compressibility);
-// empty in FP64, whose Jacobian is the reference's finite difference.
-template <class R>
+// Per-phase properties the analytic Jacobian reads (specific volume and
+// energy, molar enthalpy / RT, compressibility); empty when the flash uses
+// only the reference's finite-difference Jacobian.
+template <class R, bool A>
 struct PhaseParts {
     RD void clear() {}
 };
-template <>
-struct PhaseParts<float> {
-    float vG, vL, eG, eL, hG, hL, zG, zL;
+template <class R>
+struct PhaseParts<R, true> {
+    R vG, vL, eG, eL, hG, hL, zG, zL;
     RD void clear() { vG = vL = eG = eL = hG = hL = zG = zL = 0; }
 };
 
-// Evaluation record (reactiveDeviceFlash.h:64-73) for at most one liquid.
-template <class R>
+// Evaluation record (reactiveDeviceFlash.h:64-73) for at most one liquid;
+// A adds the per-phase parts of the analytic Jacobian.
+template <class R, bool A = false>
 struct EvalT {
     R p, T, rho, e, entropy, cf, ceq, volRes, enRes, chemRes;
     R alphaGas, alphaLiq, rhoGas, rhoLiq, liqMass, gasMass, cp, cv;
     int active, iterations;
     R volume, energy, entropyDensity, Vp, VT, Ep, ET, cpDensity, muGas, muLiquid;
-    PhaseParts<R> parts;
+    PhaseParts<R, A> parts;
 };
 using Eval = EvalT<double>;
 
-template <class R>
-RD void zeroEval(EvalT<R>& v) {
+template <class R, bool A>
+RD void zeroEval(EvalT<R, A>& v) {
     v.p = v.T = v.rho = v.e = v.entropy = v.cf = v.ceq = v.volRes = v.enRes = v.chemRes = 0;
     v.alphaGas = v.alphaLiq = v.rhoGas = v.rhoLiq = v.liqMass = v.gasMass = v.cp = v.cv = 0;
     v.active = v.iterations = 0;
@@ -89,7 +90,8 @@ RD void zeroEval(EvalT<R>& v) {
     v.parts.clear();
 }
 
-RD void writeOut(const Eval& v, ThermoOut& o) {
+template <bool A>
+RD void writeOut(const EvalT<double, A>& v, ThermoOut& o) {
     o.p = v.p; o.T = v.T; o.rho = v.rho; o.e = v.e; o.entropy = v.entropy;
     o.soundFrozen = v.cf; o.soundEquilibrium = v.ceq;
     o.volumeResidual = v.volRes; o.energyResidual = v.enRes; o.chemicalResidual = v.chemRes;
@@ -191,11 +193,15 @@ template <> struct FlashPrecision<float> {
     static constexpr float vtolFloor = 3e-5f, etolFloor = 3e-5f, mutolFloor = 3e-4f, handOver = 1e-3f;
 };
 
-template <int NS, int NL, class R = double>
+// A (analytic): the active-set Newton steps of a single-species gas use the
+// analytic Jacobian; the equilibrium sound speed keeps the reference's
+// finite-difference Jacobian.  On for the FP32 search and the FP64
+// refinement of the mixed flash, off for the reference-arithmetic flash.
+template <int NS, int NL, class R = double, bool A = FlashPrecision<R>::single>
 struct Flash {
     static_assert(NL == 0 || NL == 1, "one pure liquid at most");
     using P = FlashPrecision<R>;
-    using E = EvalT<R>;
+    using E = EvalT<R, A>;
     using S = PRStateT<R>;
     using M = MixT<R>;
     const HemModel<NS>& m;
@@ -320,7 +326,7 @@ struct Flash {
     }
 
     // Specific volume and energy, molar enthalpy / RT and compressibility of
-    // one phase state (single-precision search only).
+    // one phase state (analytic-Jacobian flashes only).
     static RD void recordPhase(const S& s, R p, R T, R& v, R& e, R& h, R& z) {
         const R RT = R(8314.46261815324) * T;
         v = R(1) / s.rho;
@@ -330,7 +336,7 @@ struct Flash {
     }
 
     // Analytic Jacobian of the one-liquid residual for a single gas species
-    // (single-precision search): with the gas of the condensable's own
+    // (mixed flash): with the gas of the condensable's own
     // species, the volume and energy rows are the (p, T) derivatives the
     // evaluation accumulates plus the change of one unit of mass from gas to
     // liquid, and the chemical-potential row follows from
@@ -379,7 +385,7 @@ struct Flash {
             accumulate(v, mg, prop, p);
             v.alphaGas = mg / prop.rho;
             v.rhoGas = prop.rho;
-            if constexpr (P::single) recordPhase(prop, p, T, v.parts.vG, v.parts.eG, v.parts.hG, v.parts.zG);
+            if constexpr (A) recordPhase(prop, p, T, v.parts.vG, v.parts.eG, v.parts.hG, v.parts.zG);
             if (NL && virtualLiquids) v.muGas = mu[c] / (R(8314.46261815324) * T);
         }
         if (NL) {
@@ -402,9 +408,7 @@ struct Flash {
                     v.muLiquid = R(Real<R>::big) * R(2);  // +infinity
                 } else {
                     if (virtualLiquids) v.muLiquid = mu[0] / (R(8314.46261815324) * T);
-                    if constexpr (P::single)
-                        if (cacheStatus < 0 || !(p == cacheP && T == cacheT && virtualLiquids == cacheVirtual))
-                            recordPhase(prop, p, T, v.parts.vL, v.parts.eL, v.parts.hL, v.parts.zL);
+                    if constexpr (A) recordPhase(prop, p, T, v.parts.vL, v.parts.eL, v.parts.hL, v.parts.zL);
                     if (mass0 > 0) {
                         accumulate(v, mass0, prop, p);
                         v.rhoLiq = prop.rho;
@@ -658,7 +662,7 @@ struct Flash {
                     return activeFinish(x, scale, f, value);
                 }
             }
-            if constexpr (P::single && NS == 1) {
+            if constexpr (A && NS == 1) {
                 analyticJacobian(value, scale, jac);
             } else {
                 if (!jacobian(x, scale, f, jac)) return polishing && activeFinish(x, scale, f, value);
@@ -904,7 +908,31 @@ struct Flash {
     // maximum entropy in the reference's order.  Cells whose single-precision
     // decisions (seed pruning, whether restarts were needed) disagree with
     // FP64, or whose set overflowed, run the plain FP64 search.
+    // Gas candidate of a scalar-recovery model: Newton in T at fixed density
+    // from the FP32 temperature, to 1e-3 of the energy tolerance, with the
+    // scalar recovery's acceptance test (reactiveDeviceFlash.h:276-299).
+    RAMD_NOINLINE bool gasPolish(R T, E& result) {
+        const R rho = density();
+        T = clamp(T, Tmin, Tmax);
+        R F, cv, p, cp;
+        if (!gasProbe(T, F, cv, p, &cp)) return false;
+        const R scale = hi(R(1e5) * rho, rho * cp * T);
+        int it = 0;
+        for (; it < 8 && ab(F) > R(1e-3) * etol * scale; ++it) {
+            const R next = T - F / cv;
+            if (!fin(next) || next < Tmin || next > Tmax) return false;
+            T = next;
+            if (!gasProbe(T, F, cv, p, nullptr)) return false;
+        }
+        if (ab(F) > etol * scale) return false;
+        if (!evaluate(R(0), p, T, result, false)) return false;
+        result.volRes = ab(result.volume - R(1));
+        result.enRes = ab(result.energy - energy) / energyScale(result);
+        result.iterations = it;
+        return result.volRes <= vtol && result.enRes <= etol;
+    }
     RAMD_NOINLINE bool finishFrozen(R mass0, const Candidate& c, E& value, int stallLimit) {
+        if (mass0 == 0 && m.scalarRecovery && gasPolish(m_exp(R(c.x[1])), value)) return true;
         const R p0 = guessP, T0 = guessT;
         guessP = m_exp(R(c.x[0]));
         guessT = m_exp(R(c.x[1]));

@@ -82,9 +82,9 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void stepRefineKernel(con
     if (i < count && *dtDev > 0) {
         const uint32_t c = order[i];
         const ThermoOut g = seeds[c];
-        Flash<NS, NL> f(model, n);
+        Flash<NS, NL, double, true> f(model, n);
         loadCell(f, q, stride, g, c, NS);
-        Eval result;
+        EvalT<double, true> result;
         zeroEval(result);
         const bool ok = f.refine(sets[c], result);
         storeCell(ok, result, g, c, out, pOut, cOut, rOut);
