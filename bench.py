@@ -185,7 +185,7 @@ def stage_roofline(rep, cells, steps, n, nv=5):
     bytes0, bytes1 = 8 * (2 * nv + 2), 8 * (3 * nv + 2)  # algorithmic bytes per cell, stage 0 / stage 1
     t_stage = (rep.stageSeconds[0] + rep.stageSeconds[1]) / steps
     achieved = (bytes0 + bytes1) * cells / t_stage / 1e9
-    return {"bound": "hbm", "kernel": "cartesianTiledStageKernel (both SSPRK2 stages)", "achieved": achieved,
+    return {"bound": "hbm", "kernel": "batchTiledStageKernel (both SSPRK2 stages)", "achieved": achieved,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": STAGE_TRAFFIC_160 if n == 160 and nv == 5 else None,
             "bytes_per_cell": [bytes0, bytes1],
@@ -195,7 +195,7 @@ def stage_roofline(rep, cells, steps, n, nv=5):
 # HBM bytes per launch of the fused stage kernel at 160^3 pure N2O, stage 0 and
 # stage 1, from rocprofv3 FETCH_SIZE and WRITE_SIZE (each in its own run,
 # scaled by a plain copy kernel's counts): profiles/cartesianStage160_traffic.txt
-STAGE_TRAFFIC_160 = [420.3e6, 583.7e6]
+STAGE_TRAFFIC_160 = [471.0e6, 641.1e6]
 TIGHT = {"vtol": 1e-12, "etol": 1e-12, "mutol": 1e-10}
 # VALU FLOPs per phase evaluation (ADD, MUL, TRANS = 1, FMA = 2), measured with
 # rocprofv3 on divergence-free uniform cells, mean of a two-phase cell and a
@@ -257,9 +257,10 @@ def flash_roofline(rep, rep64, cells, steps, mixed):
         out["fp32_search"] = {"kernel": "stepSearchKernel", "avg_launch_ms": 1e3 * ts, "achieved": a32,
                               "peak": FP32_PEAK_TFLOPS, "frac": a32 / FP32_PEAK_TFLOPS,
                               "phase_evaluations_per_cell": rep.singlePhaseEvaluations / (cells * flashes)}
-        out["fp64_refine"] = {"kernel": "stepRefineKernel", "avg_launch_ms": 1e3 * tr, "achieved": ar,
-                              "peak": FP64_PEAK_TFLOPS, "frac": ar / FP64_PEAK_TFLOPS,
-                              "phase_evaluations_per_cell": rep.phaseEvaluations / (cells * flashes)}
+        out["fp64_refine"] = {"kernel": "stepRefineKernel (+ stepRefineQueueKernel)", "avg_launch_ms": 1e3 * tr,
+                              "achieved": ar, "peak": FP64_PEAK_TFLOPS, "frac": ar / FP64_PEAK_TFLOPS,
+                              "phase_evaluations_per_cell": rep.phaseEvaluations / (cells * flashes),
+                              "queued_cells_per_flash": rep.queuedCells / flashes}
         out["share_of_step"] = (rep.flashSeconds[0] + rep.flashSeconds[1]) / rep.seconds
     else:
         out["avg_launch_ms"] = 1e3 * t64

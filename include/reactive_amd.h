@@ -43,6 +43,8 @@ typedef struct ReactiveAmdStepReport {
     uint64_t fluxFailures;              /* face evaluations the HLLC flux rejected (0 under HLL) */
     uint64_t singlePhaseEvaluations;    /* FP32 phase evaluations of the mixed flash (phaseEvaluations: FP64) */
     double searchSeconds;               /* mixed flash: the FP32 search kernels' share of flashSeconds */
+    uint64_t queuedCells;               /* mixed flash: cells whose FP64 refinement needed a full FP64 search
+                                           (or the FP64 restart search), run from a compacted queue */
 } ReactiveAmdStepReport;
 
 /* Flash arithmetic of a step.  FP64: the reference's arithmetic.  MIXED: the
@@ -84,6 +86,26 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
                           ReactiveAmdStepReport* report);
 /* Copies the current q (cell-major) and states to host memory. */
 int reactive_amd_step_read(void* step, double* q, ReactiveThermoState* states);
+
+/* Batch of independent cases: `cases` structured-box transports (each created
+ * by reactive_transport_create* on its own mesh, with its own cell volume,
+ * face area and wave factor, the same species and flux scheme, and no
+ * fixed-state slots) stepped together with one HEM handle.  The returned step
+ * takes the cases' cells concatenated in case order in reactive_amd_step_load
+ * / reactive_amd_step_read; reactive_amd_step_run advances every case by its
+ * own CFL dt: one flux launch per SSPRK2 stage and one CFL sweep for all
+ * cases, one flash over every cell (the boundary-flux integrals are launched
+ * per case), and each case's results equal those of the case stepped alone,
+ * bit for bit.  The run's
+ * report then holds the smallest dt and recheck step and the totals and
+ * boundary integrals summed over the cases; the run fails if any case's dt
+ * is invalid.  Serves the reference's uneven multi-case workloads (no
+ * counterpart in the reference ABI, whose driver steps one case per
+ * handle). */
+void* reactive_amd_step_create_batch(void* const* transports, int cases, void* hem, char* error, size_t errorSize);
+/* Case `box` of the last run: dt, recheck step, CFL-recheck violations,
+ * totals and boundary integral (other fields zero). */
+int reactive_amd_step_report_box(void* step, int box, ReactiveAmdStepReport* report);
 const char* reactive_amd_step_error(void* step);
 void reactive_amd_step_destroy(void* step);
 

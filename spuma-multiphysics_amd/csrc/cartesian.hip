@@ -2,15 +2,16 @@
 // Structured-box fast path of the transport stage.
 //
 // When the caller's face list is an axis-aligned nx*ny*nz box (uniform face
-// area and cell volume, internal + extrapolated faces), one fused kernel does
-// Cells + Faces + Rhs + Advance per cell: each thread rebuilds the HLL
-// coefficients of its six faces from the cell and neighbour states, so q, p
-// and c_frozen are streamed once per stage instead of through per-face work
-// records.  Every face is evaluated by both of its cells with identical
-// inputs and operations, so the two contributions are exact negatives, and
-// each cell adds its six contributions in its CSR (face-id) order, given by
-// the slot-order table of its boundary type.  The result is bitwise the
-// generic path's (reactiveTransportKernels.h:343-356, 502-526, 551-560).
+// area and cell volume, internal faces plus boundary faces of any kind:
+// extrapolated, slip wall, fixed state), one fused kernel does Cells + Faces
+// + Rhs + Advance: the HLL coefficients of every face are rebuilt from the
+// two adjacent states (a boundary face's right state as View builds it), so
+// q, p and c_frozen are streamed once per stage instead of through per-face
+// work records.  Each face is evaluated once and its flux added to both
+// cells with opposite signs, and each cell adds its six contributions in its
+// CSR (face-id) order, given by the slot-order table of its boundary type.
+// The result is bitwise the generic path's (reactiveTransportKernels.h:
+// 134-149, 343-356, 502-526, 551-560).
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstring>
@@ -33,7 +34,7 @@ CartesianInfo Transport::detectCartesian(const ReactiveTransportFace* f, const d
     std::vector<int8_t> axis(nf), side(nf);
     long nx = 0, nxy = 0;
     for (size_t i = 0; i < nf; ++i) {
-        if (f[i].kind != 0 && f[i].kind != 2) return info;
+        if (f[i].kind < 0 || f[i].kind > 3 || (f[i].kind == 0) != (f[i].neighbour >= 0)) return info;
         if (std::memcmp(&f[i].area, &f[0].area, sizeof(double))) return info;
         int a = -1, s = 0;
         for (int d = 0; d < 3; ++d) {
@@ -62,6 +63,9 @@ CartesianInfo Transport::detectCartesian(const ReactiveTransportFace* f, const d
     if (NZ < 2) return info;
     // slot face ids per cell: 0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z
     std::vector<int32_t> slot(6 * nc, -1);
+    // boundary kinds by side plane: -1 extrapolated, -2 slip wall, j fixed state j
+    std::vector<int32_t> kinds(size_t(boundaryPlanes(NX, NY, NZ)), -1);
+    bool special = false;
     const long strideOf[3] = {1, nx, nxy};
     const int extent[3] = {NX, NY, NZ};
     // exact normal bits of each slot class: [slot][boundary]
@@ -71,7 +75,7 @@ CartesianInfo Transport::detectCartesian(const ReactiveTransportFace* f, const d
         const int a = axis[i];
         const long o = f[i].owner;
         const int coord[3] = {int(o % nx), int((o / nx) % (nxy / nx)), int(o / nxy)};
-        int s0, bnd = f[i].kind == 2;
+        int s0, bnd = f[i].kind != 0;
         if (!bnd) {
             if (coord[a] >= extent[a] - 1) return info;  // no wrap-around
             s0 = 2 * a + 1;
@@ -85,6 +89,13 @@ CartesianInfo Transport::detectCartesian(const ReactiveTransportFace* f, const d
             s0 = 2 * a + (side[i] > 0 ? 1 : 0);
             if (slot[6 * o + s0] >= 0) return info;
             slot[6 * o + s0] = int32_t(i);
+            if (f[i].kind != 2) {
+                if (f[i].kind == 3 && (f[i].fixed < 0 || size_t(f[i].fixed) >= cfg.fixed || f[i].fixed > INT32_MAX))
+                    return info;
+                kinds[size_t(boundaryPlane(NX, NY, NZ, s0, coord[0], coord[1], coord[2]))] =
+                    f[i].kind == 1 ? -2 : int32_t(f[i].fixed);
+                special = true;
+            }
         }
         // an internal face is the owner's + slot and the neighbour's - slot
         const int classes[2] = {s0, bnd ? s0 : 2 * a};
@@ -135,6 +146,23 @@ CartesianInfo Transport::detectCartesian(const ReactiveTransportFace* f, const d
     info.inverseVolume = 1 / volumes[0];
     info.volume = volumes[0];
     info.valid = true;
+    info.fixedBase = (long long)cfg.cells;
+    if (special) {
+        info.boundaryKind = allocate<int32_t>(kinds.size());
+        RAMD_HIP(hipMemcpy(info.boundaryKind, kinds.data(), kinds.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    BoxDev box;
+    box.a = makeCartesianArgs(info, cfg.waveFactor);
+    box.offset = 0;
+    box.cells = nc;
+    box.volume = info.volume;
+    info.stageBlocks = tileBlocks(box.a);
+    info.cflBlocks = long((nc + 255) / 256);
+    const long long starts[4] = {0, info.stageBlocks, 0, info.cflBlocks};
+    info.box = allocate<BoxDev>(1);
+    info.starts = allocate<long long>(4);
+    RAMD_HIP(hipMemcpy(info.box, &box, sizeof(box), hipMemcpyHostToDevice));
+    RAMD_HIP(hipMemcpy(info.starts, starts, sizeof(starts), hipMemcpyHostToDevice));
     pressure = allocate<double>(all);
     sound = allocate<double>(all);
     return info;
@@ -151,13 +179,13 @@ __global__ void extractStateKernel(const ReactiveTransportState* __restrict__ s,
 
 void Transport::cartesianAdvance(double dt, int stage) {
     if (scheme == kHllc && !density) density = allocate<double>(all);
-    hipLaunchKernelGGL(extractStateKernel, dim3(unsigned((cfg.cells + 255) / 256)), dim3(256), 0, stream, state,
-                       cfg.cells, pressure, sound, scheme == kHllc ? density : nullptr);
+    // p, c_frozen (and rho) of the cells and of the fixed states
+    hipLaunchKernelGGL(extractStateKernel, dim3(unsigned((all + 255) / 256)), dim3(256), 0, stream, state, all,
+                       pressure, sound, scheme == kHllc ? density : nullptr);
     RAMD_HIP(hipGetLastError());
     launched();
-    CartesianArgs a = makeCartesianArgs(cartesian, cfg.waveFactor);
-    launchCartesianStage(a, ns, nv, scheme, q, all, pressure, sound, density, dt, nullptr, stage, initial,
-                         fluxFailures, stream);
+    launchBatchStage(ns, nv, scheme, cartesian.box, cartesian.starts, 1, cartesian.stageBlocks, q, all, pressure,
+                     sound, density, dt, nullptr, stage, initial, fluxFailures, stream);
     launched();
 }
 

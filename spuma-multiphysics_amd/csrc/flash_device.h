@@ -953,11 +953,18 @@ struct Flash {
         x[2] = R(c.x[2]);
         return activeNewton(x, scale, value, 2);
     }
-    RAMD_NOINLINE bool refine(const CandidateSet& set, E& best) {
+    // deferred (when given): instead of running the plain FP64 search or the
+    // FP64 restart search inline, the refinement stops with *deferred = true;
+    // the caller reruns such cells with deferred == nullptr in a launch of
+    // their own, so the long searches do not hold up the other lanes of a wave.
+    RAMD_NOINLINE bool refine(const CandidateSet& set, E& best, bool* deferred = nullptr) {
         if (!fin(energy) || !(density() > 0)) return false;
         for (int k = 0; k < NS; ++k)
             if (!fin(q[k]) || q[k] < 0) return false;
-        if (set.flags & kOverflow) return search(best);
+        if (set.flags & kOverflow) {
+            if (deferred) { *deferred = true; return false; }
+            return search(best);
+        }
         bool have = false;
         E candidate;
         bool gasStable = false;
@@ -979,6 +986,7 @@ struct Flash {
         const bool prune = prunable && gasStable;
         if (prune != ((set.flags & kSeedsPruned) != 0)) {
             have = false;
+            if (deferred) { *deferred = true; return false; }
             return search(best);
         }
         const int c = m.condensable;
@@ -993,6 +1001,7 @@ struct Flash {
         }
         if (!have && q[c] > 0 && restartRange()) {
             if (!(set.flags & kRestartsSearched)) {
+                if (deferred) { *deferred = true; return false; }
                 // the single-precision search found a stable candidate that FP64
                 // rejects: search the restart temperatures in FP64
                 const R seeds[5] = {R(-1), R(.5), R(.95), R(.1), R(.9999)};

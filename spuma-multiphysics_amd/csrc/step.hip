@@ -74,7 +74,8 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void stepRefineKernel(con
                                                        ThermoOut* __restrict__ out, double* __restrict__ pOut,
                                                        double* __restrict__ cOut, double* __restrict__ rOut,
                                                        uint16_t* __restrict__ cost,
-                                                       unsigned long long* __restrict__ counters) {
+                                                       unsigned long long* __restrict__ counters,
+                                                       uint32_t* __restrict__ queue, unsigned* __restrict__ queueCount) {
     const HemModel<NS>& model = workgroupModel(globalModel);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Counts n = {0, 0, 0, 0, 0, 0};
@@ -86,12 +87,58 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void stepRefineKernel(con
         loadCell(f, q, stride, g, c, NS);
         EvalT<double, true> result;
         zeroEval(result);
-        const bool ok = f.refine(sets[c], result);
-        storeCell(ok, result, g, c, out, pOut, cOut, rOut);
-        failed = ok ? 0u : 1u;
-        if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
+        bool deferred = false;
+        const bool ok = f.refine(sets[c], result, queue ? &deferred : nullptr);
+        if (deferred) {
+            queue[atomicAdd(queueCount, 1u)] = c;
+        } else {
+            storeCell(ok, result, g, c, out, pOut, cOut, rOut);
+            failed = ok ? 0u : 1u;
+            if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
+        }
     }
     countFlash(failed, n.phases, counters);
+}
+
+// The cells the refinement deferred (full FP64 search or FP64 restarts),
+// refined again without deferral: a persistent grid that sweeps the queue, so
+// the long searches of a few cells share waves instead of stalling the
+// waves of the bulk.  queueCount is read on the device (the launch does not
+// depend on it).
+template <int NS, int NL>
+__global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void stepRefineQueueKernel(
+    const HemModel<NS>* __restrict__ globalModel, const double* __restrict__ q, size_t stride,
+    const ThermoOut* __restrict__ seeds, const uint32_t* __restrict__ queue, const unsigned* __restrict__ queueCount,
+    const double* __restrict__ dtDev, const CandidateSet* __restrict__ sets, ThermoOut* __restrict__ out,
+    double* __restrict__ pOut, double* __restrict__ cOut, double* __restrict__ rOut, uint16_t* __restrict__ cost,
+    unsigned long long* __restrict__ counters) {
+    const unsigned count = *queueCount;
+    if (blockIdx.x * 64u >= count) return;  // uniform per workgroup: usually every workgroup leaves here
+    const HemModel<NS>& model = workgroupModel(globalModel);
+    for (unsigned base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u) {  // uniform per workgroup
+        const unsigned i = base + threadIdx.x;
+        Counts n = {0, 0, 0, 0, 0, 0};
+        uint32_t failed = 0;
+        if (i < count && *dtDev > 0) {
+            const uint32_t c = queue[i];
+            const ThermoOut g = seeds[c];
+            Flash<NS, NL, double, true> f(model, n);
+            loadCell(f, q, stride, g, c, NS);
+            EvalT<double, true> result;
+            zeroEval(result);
+            const bool ok = f.refine(sets[c], result);
+            storeCell(ok, result, g, c, out, pOut, cOut, rOut);
+            failed = ok ? 0u : 1u;
+            if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
+        }
+        countFlash(failed, n.phases, counters);
+    }
+}
+__global__ void clearQueueKernel(unsigned* __restrict__ count) {
+    if (threadIdx.x == 0) *count = 0;
+}
+__global__ void queueTotalKernel(const unsigned* __restrict__ count, unsigned long long* __restrict__ total) {
+    if (threadIdx.x == 0) atomicAdd(total, (unsigned long long)*count);
 }
 
 // Heavy-first scheduling and a counting sort: histogram, exclusive scan from
@@ -164,6 +211,144 @@ __global__ __launch_bounds__(256) void bucketScatterKernel(const uint8_t* __rest
     if (c < cells) order[base[k] + rank] = uint32_t(c);
 }
 
+// ---- thermodynamic-state order ---------------------------------------------
+//
+// Cells whose (ln rho, e) fall in the same cell of a 64 x 64 grid spanning
+// the flash input's range are flashed by the same waves: their searches take
+// the same branches with similar iteration counts, so the lanes of a wave
+// stay converged (cost order alone groups cells by total work, whose paths
+// still differ).  The order decides nothing numerical.
+#ifndef RAMD_STATE_ORDER
+#define RAMD_STATE_ORDER 1
+#endif
+constexpr int kStateSide = 64, kStateBins = kStateSide * kStateSide, kStateBlocks = 1024;
+
+__device__ inline unsigned long long orderedBits(double x) {
+    const unsigned long long u = __double_as_longlong(x);
+    return (u >> 63) ? ~u : (u | (1ull << 63));
+}
+__device__ inline double fromOrdered(unsigned long long u) {
+    return __longlong_as_double((u >> 63) ? (u & ~(1ull << 63)) : ~u);
+}
+// ln rho and specific internal energy of cell c (non-finite values map to the range ends)
+__device__ inline void cellState(const double* __restrict__ q, size_t stride, int ns, size_t c, double& lr,
+                                 double& e) {
+    double rho = 0;
+    for (int k = 0; k < ns; ++k) rho += q[size_t(k) * stride + c];
+    const double m0 = q[size_t(ns) * stride + c], m1 = q[size_t(ns + 1) * stride + c];
+    const double m2 = q[size_t(ns + 2) * stride + c], E = q[size_t(ns + 3) * stride + c];
+    lr = rho > 0 ? ::log(rho) : -1e300;
+    e = (E - .5 * (m0 * m0 + m1 * m1 + m2 * m2) / rho) / rho;
+    if (!isfinite(lr)) lr = -1e300;
+    if (!isfinite(e)) e = -1e300;
+}
+// range[0..3] = ordered min ln rho, max ln rho, min e, max e
+__global__ void stateRangeInitKernel(unsigned long long* __restrict__ range) {
+    if (threadIdx.x < 4) range[threadIdx.x] = (threadIdx.x & 1) ? 0ull : ~0ull;
+}
+__device__ inline unsigned long long waveMinU(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+__global__ __launch_bounds__(256) void stateRangeKernel(const double* __restrict__ q, size_t stride, int ns,
+                                                        size_t cells, unsigned long long* __restrict__ range) {
+    // min of (lr, -lr, e, -e) in ordered bits: entries 1 and 3 hold ~max
+    unsigned long long v[4] = {~0ull, ~0ull, ~0ull, ~0ull};
+    for (size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x; c < cells; c += size_t(gridDim.x) * blockDim.x) {
+        double lr, e;
+        cellState(q, stride, ns, c, lr, e);
+        if (lr > -1e300 && e > -1e300) {
+            const unsigned long long a = orderedBits(lr), b = orderedBits(e);
+            v[0] = a < v[0] ? a : v[0];
+            v[1] = ~a < v[1] ? ~a : v[1];
+            v[2] = b < v[2] ? b : v[2];
+            v[3] = ~b < v[3] ? ~b : v[3];
+        }
+    }
+    __shared__ unsigned long long lane[4][4];
+    for (int j = 0; j < 4; ++j) {
+        v[j] = waveMinU(v[j]);
+        if ((threadIdx.x & 63) == 0) lane[j][threadIdx.x >> 6] = v[j];
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int j = threadIdx.x;
+        unsigned long long m = lane[j][0];
+        for (int w = 1; w < 4; ++w) m = lane[j][w] < m ? lane[j][w] : m;
+        if (j & 1) atomicMax(&range[j], ~m);
+        else atomicMin(&range[j], m);
+    }
+}
+__device__ inline unsigned stateBin(double x, double lo, double hi) {
+    if (!(hi > lo)) return 0;
+    const double t = (x - lo) / (hi - lo) * kStateSide;
+    return t <= 0 ? 0u : (t >= kStateSide - 1 ? unsigned(kStateSide - 1) : unsigned(t));
+}
+// bin of every cell (key) and the bin counts; grid-stride over kStateBlocks blocks
+__global__ __launch_bounds__(256) void stateKeyKernel(const double* __restrict__ q, size_t stride, int ns,
+                                                      size_t cells, const unsigned long long* __restrict__ range,
+                                                      uint16_t* __restrict__ key, unsigned* __restrict__ count) {
+    __shared__ unsigned hist[kStateBins];
+    for (int b = threadIdx.x; b < kStateBins; b += blockDim.x) hist[b] = 0;
+    __syncthreads();
+    const double lo0 = fromOrdered(range[0]), hi0 = fromOrdered(range[1]);
+    const double lo1 = fromOrdered(range[2]), hi1 = fromOrdered(range[3]);
+    for (size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x; c < cells; c += size_t(gridDim.x) * blockDim.x) {
+        double lr, e;
+        cellState(q, stride, ns, c, lr, e);
+        const unsigned k = stateBin(lr, lo0, hi0) * kStateSide + stateBin(e, lo1, hi1);
+        key[c] = uint16_t(k);
+        atomicAdd(&hist[k], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kStateBins; b += blockDim.x)
+        if (hist[b]) atomicAdd(&count[b], hist[b]);
+}
+// exclusive scan of the bin counts into offset; clears count (one block of 256)
+__global__ __launch_bounds__(256) void stateScanKernel(unsigned* __restrict__ count, unsigned* __restrict__ offset) {
+    constexpr int per = kStateBins / 256;
+    __shared__ unsigned part[256];
+    unsigned sum = 0;
+    for (int j = 0; j < per; ++j) sum += count[threadIdx.x * per + j];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned run = 0;
+        for (int t = 0; t < 256; ++t) {
+            const unsigned v = part[t];
+            part[t] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    unsigned run = part[threadIdx.x];
+    for (int j = 0; j < per; ++j) {
+        const int b = threadIdx.x * per + j;
+        offset[b] = run;
+        run += count[b];
+        count[b] = 0;
+    }
+}
+// each block reserves one run per bin for the cells of its grid-stride set
+__global__ __launch_bounds__(256) void stateScatterKernel(const uint16_t* __restrict__ key, size_t cells,
+                                                          unsigned* __restrict__ offset,
+                                                          uint32_t* __restrict__ order) {
+    __shared__ unsigned local[kStateBins];
+    for (int b = threadIdx.x; b < kStateBins; b += blockDim.x) local[b] = 0;
+    __syncthreads();
+    for (size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x; c < cells; c += size_t(gridDim.x) * blockDim.x)
+        atomicAdd(&local[key[c]], 1u);
+    __syncthreads();
+    for (int b = threadIdx.x; b < kStateBins; b += blockDim.x)
+        if (local[b]) local[b] = atomicAdd(&offset[b], local[b]);
+    __syncthreads();
+    for (size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x; c < cells; c += size_t(gridDim.x) * blockDim.x)
+        order[atomicAdd(&local[key[c]], 1u)] = uint32_t(c);
+}
+
 // Neumaier-compensated sum of q_k V over the cells: per-block partials in a
 // fixed order, then one block merges them in block order (deterministic).
 __device__ inline void twoSumAdd(double& s, double& comp, double x) {
@@ -172,12 +357,14 @@ __device__ inline void twoSumAdd(double& s, double& comp, double x) {
     s = t;
 }
 constexpr int kTotalBlocks = 512;
-__global__ __launch_bounds__(256) void totalsPartialKernel(const double* __restrict__ q, size_t stride, size_t cells,
-                                                           int nv, double volume, double2* __restrict__ partial) {
+// Partial sums of block `block` of kTotalBlocks (each box of a batch is summed
+// by its own kTotalBlocks blocks, as when it runs alone).
+__device__ inline void totalsPartial(const double* __restrict__ q, size_t stride, size_t cells, int nv, double volume,
+                                     unsigned block, double2* __restrict__ partial) {
     __shared__ double ls[256], lc[256];
     for (int k = 0; k < nv; ++k) {
         double s = 0, comp = 0;
-        for (size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x; c < cells; c += size_t(gridDim.x) * blockDim.x)
+        for (size_t c = size_t(block) * blockDim.x + threadIdx.x; c < cells; c += size_t(kTotalBlocks) * blockDim.x)
             twoSumAdd(s, comp, q[size_t(k) * stride + c] * volume);
         ls[threadIdx.x] = s;
         lc[threadIdx.x] = comp;
@@ -191,25 +378,42 @@ __global__ __launch_bounds__(256) void totalsPartialKernel(const double* __restr
             }
             __syncthreads();
         }
-        if (threadIdx.x == 0) partial[size_t(k) * gridDim.x + blockIdx.x] = make_double2(ls[0], lc[0]);
+        if (threadIdx.x == 0) partial[size_t(k) * kTotalBlocks + block] = make_double2(ls[0], lc[0]);
         __syncthreads();
     }
 }
+__global__ __launch_bounds__(256) void totalsPartialKernel(const double* __restrict__ q, size_t stride, size_t cells,
+                                                           int nv, double volume, double2* __restrict__ partial) {
+    totalsPartial(q, stride, cells, nv, volume, blockIdx.x, partial);
+}
+// grid (kTotalBlocks, boxes): box blockIdx.y into partial + nv kTotalBlocks blockIdx.y
+__global__ __launch_bounds__(256) void batchTotalsPartialKernel(const BoxDev* __restrict__ boxes,
+                                                                const double* __restrict__ q, size_t stride, int nv,
+                                                                double2* __restrict__ partial) {
+    const BoxDev& box = boxes[blockIdx.y];
+    totalsPartial(q + box.offset, stride, box.cells, nv, box.volume, blockIdx.x,
+                  partial + size_t(nv) * kTotalBlocks * blockIdx.y);
+}
+// one block per box (thread k: variable k)
 __global__ void totalsFinishKernel(const double2* __restrict__ partial, int blocks, int nv, double* __restrict__ out) {
     const int k = threadIdx.x;
     if (k >= nv) return;
+    partial += size_t(nv) * blocks * blockIdx.x;
     double s = 0, comp = 0;
     for (int b = 0; b < blocks; ++b) {
         comp += partial[size_t(k) * blocks + b].y;
         twoSumAdd(s, comp, partial[size_t(k) * blocks + b].x);
     }
-    out[k] = s + comp;
+    out[size_t(nv) * blockIdx.x + k] = s + comp;
 }
 
-// scalars: [0] dt, [1] recheck dt, [2] cfl-violation counter (as double), [3] max dt of this call
-__global__ void finishCflKernel(const double* __restrict__ blockMin, size_t n, double* __restrict__ out) {
+// The CFL minimum of box blockIdx.x over its blocks [start[i], start[i + 1]):
+// scalars[4 i + slot] (-1 if invalid).
+__global__ void batchFinishCflKernel(const double* __restrict__ blockMin, const long long* __restrict__ start,
+                                     double* __restrict__ scalars, int slot) {
+    const int i = blockIdx.x;
     double v = __builtin_huge_val();
-    for (size_t i = threadIdx.x; i < n; i += blockDim.x) v = blockMin[i] < v ? blockMin[i] : v;
+    for (long long b = start[i] + threadIdx.x; b < start[i + 1]; b += blockDim.x) v = blockMin[b] < v ? blockMin[b] : v;
     __shared__ double lane[4];
     v = waveMin(v);
     if ((threadIdx.x & 63) == 0) lane[threadIdx.x >> 6] = v;
@@ -217,24 +421,44 @@ __global__ void finishCflKernel(const double* __restrict__ blockMin, size_t n, d
     if (threadIdx.x == 0) {
         double m = lane[0];
         for (int w = 1; w < int(blockDim.x >> 6); ++w) m = lane[w] < m ? lane[w] : m;
-        *out = isfinite(m) && m > 0 ? m : -1;
+        scalars[4 * i + slot] = isfinite(m) && m > 0 ? m : -1;
     }
 }
-__global__ void recheckKernel(double* __restrict__ scalars, unsigned long long* __restrict__ counters) {
+// The flash gate of a batch: the smallest dt of its boxes (-1 if any is invalid).
+__global__ void batchGateKernel(const double* __restrict__ scalars, int count, double* __restrict__ gate) {
     if (threadIdx.x) return;
-    const double dt = scalars[0], stable = scalars[1];
-    if (!(dt > 0) || !(dt <= stable * (1 + 1e-10))) atomicAdd(&counters[4], 1ull);
+    double g = __builtin_huge_val();
+    for (int i = 0; i < count; ++i) {
+        const double dt = scalars[4 * i];
+        g = dt > 0 && dt < g ? dt : (dt > 0 ? g : -1.0);
+        if (g < 0) break;
+    }
+    *gate = g;
 }
+// thread i: box i (violations may be null)
+__global__ void recheckKernel(double* __restrict__ scalars, int count, unsigned long long* __restrict__ counters,
+                              unsigned long long* __restrict__ violations) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= count) return;
+    const double dt = scalars[4 * i], stable = scalars[4 * i + 1];
+    if (!(dt > 0) || !(dt <= stable * (1 + 1e-10))) {
+        atomicAdd(&counters[4], 1ull);
+        if (violations) atomicAdd(&violations[i], 1ull);
+    }
+}
+// thread (i, k) = (blockIdx.x, threadIdx.x)
 __global__ void boundaryIntegralKernel(const double* __restrict__ bA, const double* __restrict__ bB,
                                       const double* __restrict__ scalars, int nv, double* __restrict__ out) {
     const int k = threadIdx.x;
-    if (k < nv) out[k] = .5 * scalars[0] * (bA[k] + bB[k]);
+    const size_t j = size_t(nv) * blockIdx.x + k;
+    if (k < nv) out[j] = .5 * scalars[4 * blockIdx.x] * (bA[j] + bB[j]);
 }
 __global__ void setScalarKernel(double* __restrict__ dst, double value) {
     if (threadIdx.x == 0) *dst = value;
 }
 __global__ void clearCountersKernel(unsigned long long* __restrict__ c, int n) {
-    if (threadIdx.x < n) c[threadIdx.x] = 0;
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) c[i] = 0;
 }
 __global__ void toThermoKernel(const ReactiveThermoState* __restrict__ in, size_t n, ThermoOut* __restrict__ out,
                                double* __restrict__ p, double* __restrict__ c, double* __restrict__ r) {
@@ -246,15 +470,43 @@ __global__ void toThermoKernel(const ReactiveThermoState* __restrict__ in, size_
     r[i] = in[i].rho;
 }
 
+// Fixed-state slots [cells, all) of the transport (reservoir q and its p,
+// c_frozen, rho) into both step buffers; no stage or flash writes them.
+__global__ void copyFixedKernel(const double* __restrict__ tq, const ReactiveTransportState* __restrict__ ts,
+                                size_t cells, size_t all, int nv, double* __restrict__ qA, double* __restrict__ qB,
+                                double* __restrict__ pA, double* __restrict__ cA, double* __restrict__ rA,
+                                double* __restrict__ pB, double* __restrict__ cB, double* __restrict__ rB) {
+    const size_t c = cells + size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= all) return;
+    for (int k = 0; k < nv; ++k) qA[size_t(k) * all + c] = qB[size_t(k) * all + c] = tq[size_t(k) * all + c];
+    pA[c] = pB[c] = ts[c].p;
+    cA[c] = cB[c] = ts[c].sound;
+    rA[c] = rB[c] = ts[c].rho;
+}
+
 inline unsigned blocksFor(size_t n, unsigned b) { return unsigned((n + b - 1) / b); }
+// Deferred refinements go through a compacted queue (knob, compile time).
+#ifndef RAMD_REFINE_QUEUE
+#define RAMD_REFINE_QUEUE 1
+#endif
+constexpr unsigned kQueueBlocks = 2048;  // persistent grid of the queue kernel: 8 waves per CU
 constexpr int kMarks = 11;  // events per step
 
 // ---------------------------------------------------------------- handle
 struct Step {
-    Transport* t = nullptr;
+    Transport* t = nullptr;  // the first box's transport (flux scheme, wave factor)
     HemHandle* h = nullptr;
     int ns = 0, nv = 0, device = 0;
     size_t cells = 0, all = 0;
+    // boxes: one per independent case (a single-box step has one); box i
+    // owns cells [offset, offset + cells) of every array
+    int count = 1;
+    std::vector<Transport*> ts;
+    std::vector<BoxDev> boxHost;
+    BoxDev* boxes = nullptr;
+    long long *stageStart = nullptr, *cflStart = nullptr;
+    long stageBlocks = 0, cflBlocks = 0;
+    unsigned long long* violations = nullptr;  // per box: failed CFL rechecks
     std::vector<void*> allocations;
     double *qA = nullptr, *qB = nullptr, *pA = nullptr, *cA = nullptr, *pB = nullptr, *cB = nullptr;
     double *rA = nullptr, *rB = nullptr;  // recovered densities (read by the HLLC flux)
@@ -268,13 +520,19 @@ struct Step {
     uint16_t* costSearch = nullptr;
     bool haveCostSearch = false;
     CandidateSet* sets = nullptr;
+    uint32_t* queue = nullptr;  // cells the refinement deferred, and their count
+    unsigned* queueCount = nullptr;
     int mode = REACTIVE_AMD_FLASH_FP64;
     unsigned *bucketCount = nullptr, *bucketOffset = nullptr;
+    // state order: bin keys, bin counts (kept zero between uses) and offsets, range
+    uint16_t* stateKey = nullptr;
+    unsigned *stateCount = nullptr, *stateOffset = nullptr;
+    unsigned long long* stateRange = nullptr;
     double* blockMin = nullptr;
-    double* scalars = nullptr;  // dt, recheck
-    double *bA = nullptr, *bB = nullptr, *boundary = nullptr, *totals = nullptr;
+    double* scalars = nullptr;  // per box [4 i]: dt, recheck; [4 count]: flash gate of a batch
+    double *bA = nullptr, *bB = nullptr, *boundary = nullptr, *totals = nullptr;  // nv per box
     double2* totalPartial = nullptr;
-    // fail1, phases1, fail2, phases2, violations, HLLC rejections, FP32 phases (mixed)
+    // fail1, phases1, fail2, phases2, violations, HLLC rejections, FP32 phases, queued cells (mixed)
     unsigned long long* counters = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t events[2] = {};
@@ -287,6 +545,7 @@ struct Step {
     int graphMode[2] = {-1, -1}, graphScheme[2] = {-1, -1};
     double graphCfl[2] = {0, 0}, graphMax[2] = {0, 0};
     int parity = 0;  // flips with every step (the seed and result buffers swap)
+    uint64_t lastSteps = 0;
     bool loaded = false;
     char error[1024] = {};
 
@@ -312,14 +571,26 @@ struct Step {
     void flash(const double* q, double* pOut, double* cOut, double* rOut, ThermoOut* out, uint16_t* costOut,
                unsigned long long* counter, hipEvent_t split) {
         const unsigned blocks = blocksFor(cells, 64);
+#if RAMD_STATE_ORDER
+        stateOrder(q, order);
+        uint32_t* orderSearch = order;
+#endif
         if (mode == REACTIVE_AMD_FLASH_MIXED) {
 #define RAMD_STEP_MIXED(NS_, NL_)                                                                                  \
-    launchSearch(NS_, NL_, h->model, q, all, sA, orderSearch, uint32_t(cells), scalars, sets,                     \
+    launchSearch(NS_, NL_, h->model, q, all, sA, orderSearch, uint32_t(cells), gate(), sets,                      \
                  costOut ? costSearch : nullptr, counters + 6, stream);                                            \
     if (split) RAMD_HIP(hipEventRecord(split, stream));                                                            \
+    if (RAMD_REFINE_QUEUE) hipLaunchKernelGGL(clearQueueKernel, dim3(1), dim3(64), 0, stream, queueCount);          \
     hipLaunchKernelGGL((stepRefineKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                             \
-                       static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), scalars,   \
-                       sets, out, pOut, cOut, rOut, costOut, counter)
+                       static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), gate(),    \
+                       sets, out, pOut, cOut, rOut, costOut, counter, RAMD_REFINE_QUEUE ? queue : nullptr,         \
+                       queueCount);                                                                                \
+    if (RAMD_REFINE_QUEUE) {                                                                                       \
+        hipLaunchKernelGGL((stepRefineQueueKernel<NS_, NL_>), dim3(kQueueBlocks), dim3(64), 0, stream,              \
+                           static_cast<const HemModel<NS_>*>(h->model), q, all, sA, queue, queueCount, gate(),      \
+                           sets, out, pOut, cOut, rOut, costOut, counter);                                          \
+        hipLaunchKernelGGL(queueTotalKernel, dim3(1), dim3(64), 0, stream, queueCount, counters + 7);             \
+    }
             if (h->ns == 1 && h->nl == 0) { RAMD_STEP_MIXED(1, 0); }
             else if (h->ns == 1) { RAMD_STEP_MIXED(1, 1); }
             else if (h->nl == 0) { RAMD_STEP_MIXED(2, 0); }
@@ -328,7 +599,7 @@ struct Step {
         } else {
 #define RAMD_STEP_FLASH(NS_, NL_)                                                                                 \
     hipLaunchKernelGGL((stepFlashKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                             \
-                       static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), scalars, out, \
+                       static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), gate(), out, \
                        pOut, cOut, rOut, costOut, counter)
             if (h->ns == 1 && h->nl == 0) { RAMD_STEP_FLASH(1, 0); }
             else if (h->ns == 1) { RAMD_STEP_FLASH(1, 1); }
@@ -337,6 +608,43 @@ struct Step {
 #undef RAMD_STEP_FLASH
         }
         RAMD_HIP(hipGetLastError());
+    }
+
+    // The flash runs when every box has a valid dt.
+    const double* gate() const { return count > 1 ? scalars + 4 * count : scalars; }
+
+    // CFL sweep of q into scalars[4 i + slot]; maxDev: the maximum step is
+    // each box's current dt (the recheck).
+    void cflSweep(const double* q, const double* p, const double* c, double cfl, double maxHost, bool maxDev,
+                  int slot) {
+        (void)p;
+        launchBatchCfl(ns, boxes, cflStart, count, cflBlocks, q, all, c, cfl, maxHost, maxDev ? scalars : nullptr,
+                       blockMin, stream);
+        hipLaunchKernelGGL(batchFinishCflKernel, dim3(unsigned(count)), dim3(256), 0, stream, blockMin, cflStart,
+                           scalars, slot);
+        if (count > 1 && slot == 0)
+            hipLaunchKernelGGL(batchGateKernel, dim3(1), dim3(64), 0, stream, scalars, count, scalars + 4 * count);
+    }
+    void stage(const double* q, const double* p, const double* c, const double* r, int which, double* out) {
+        launchBatchStage(ns, nv, t->scheme, boxes, stageStart, count, stageBlocks, q, all, p, c, r, 0.0, scalars,
+                         which, out, counters + 5, stream);
+    }
+    void boundaryRates(const double* q, const double* p, const double* c, const double* r, double* out) {
+        for (int i = 0; i < count; ++i) {
+            const size_t o = boxHost[i].offset;
+            ts[i]->boundaryRatesFrom(q + o, p + o, c + o, r + o, out + size_t(nv) * i, counters + 5, stream, all);
+        }
+    }
+
+    // Order of the cells of q by thermodynamic state (into out).
+    void stateOrder(const double* q, uint32_t* out) {
+        hipLaunchKernelGGL(stateRangeInitKernel, dim3(1), dim3(64), 0, stream, stateRange);
+        hipLaunchKernelGGL(stateRangeKernel, dim3(kStateBlocks), dim3(256), 0, stream, q, all, ns, cells, stateRange);
+        hipLaunchKernelGGL(stateKeyKernel, dim3(kStateBlocks), dim3(256), 0, stream, q, all, ns, cells, stateRange,
+                           stateKey, stateCount);
+        hipLaunchKernelGGL(stateScanKernel, dim3(1), dim3(256), 0, stream, stateCount, stateOffset);
+        hipLaunchKernelGGL(stateScatterKernel, dim3(kStateBlocks), dim3(256), 0, stream, stateKey, cells, stateOffset,
+                           out);
     }
 
     // Heavy-first order of all cells: by cost when one is known, else by the
@@ -354,33 +662,32 @@ struct Step {
         if (m) RAMD_HIP(hipEventRecord(m[i], stream));
     }
     void one(double cfl, double maximum, hipEvent_t* m) {
-        const CartesianArgs a = makeCartesianArgs(t->cartesian, t->cfg.waveFactor);
-        const unsigned nb = unsigned((cells + 255) / 256);
         // CFL on the step-start state
         mark(m, 0);
-        launchCartesianCfl(a, ns, qA, all, pA, cA, cfl, maximum, nullptr, blockMin, stream);
-        hipLaunchKernelGGL(finishCflKernel, dim3(1), dim3(256), 0, stream, blockMin, size_t(nb), scalars);
+        cflSweep(qA, pA, cA, cfl, maximum, false, 0);
         mark(m, 1);
+#if !RAMD_STATE_ORDER
         // heavy-first orders from the latest costs (or the step-start seeds)
         sortCells(haveCost ? cost : nullptr, order);
         if (mode == REACTIVE_AMD_FLASH_MIXED) sortCells(haveCostSearch ? costSearch : nullptr, orderSearch);
+#endif
         // stage 0 and its boundary rate
-        t->boundaryRatesFrom(qA, pA, cA, rA, bA, counters + 5, stream);
+        boundaryRates(qA, pA, cA, rA, bA);
         mark(m, 2);
-        launchCartesianStage(a, ns, nv, t->scheme, qA, all, pA, cA, rA, 0.0, scalars, 0, qB, counters + 5, stream);
+        stage(qA, pA, cA, rA, 0, qB);
         mark(m, 3);
         // flash #1 seeded from the step-start states
         flash(qB, pB, cB, rB, sB, nullptr, counters, m ? m[9] : nullptr);
         mark(m, 4);
         // CFL recheck after stage 0 (maximum = dt)
-        launchCartesianCfl(a, ns, qB, all, pB, cB, cfl, 0.0, scalars, blockMin, stream);
-        hipLaunchKernelGGL(finishCflKernel, dim3(1), dim3(256), 0, stream, blockMin, size_t(nb), scalars + 1);
+        cflSweep(qB, pB, cB, cfl, 0.0, true, 1);
         mark(m, 5);
-        hipLaunchKernelGGL(recheckKernel, dim3(1), dim3(64), 0, stream, scalars, counters);
+        hipLaunchKernelGGL(recheckKernel, dim3(blocksFor(size_t(count), 64)), dim3(64), 0, stream, scalars, count,
+                           counters, violations);
         // stage 1 (in place into qA, which holds q0) and its boundary rate
-        t->boundaryRatesFrom(qB, pB, cB, rB, bB, counters + 5, stream);
+        boundaryRates(qB, pB, cB, rB, bB);
         mark(m, 6);
-        launchCartesianStage(a, ns, nv, t->scheme, qB, all, pB, cB, rB, 0.0, scalars, 1, qA, counters + 5, stream);
+        stage(qB, pB, cB, rB, 1, qA);
         mark(m, 7);
         // flash #2 from the step-start seeds, into the state buffer of stage 1
         flash(qA, pA, cA, rA, sB, cost, counters + 2, m ? m[10] : nullptr);
@@ -389,11 +696,17 @@ struct Step {
         mark(m, 8);
         std::swap(sA, sB);
         parity ^= 1;
-        // conservation totals and boundary integral
-        hipLaunchKernelGGL(totalsPartialKernel, dim3(kTotalBlocks), dim3(256), 0, stream, qA, all, cells, nv,
-                           t->cartesian.volume, totalPartial);
-        hipLaunchKernelGGL(totalsFinishKernel, dim3(1), dim3(64), 0, stream, totalPartial, kTotalBlocks, nv, totals);
-        hipLaunchKernelGGL(boundaryIntegralKernel, dim3(1), dim3(64), 0, stream, bA, bB, scalars, nv, boundary);
+        // conservation totals and boundary integral of every box
+        if (count == 1)
+            hipLaunchKernelGGL(totalsPartialKernel, dim3(kTotalBlocks), dim3(256), 0, stream, qA, all, cells, nv,
+                               t->cartesian.volume, totalPartial);
+        else
+            hipLaunchKernelGGL(batchTotalsPartialKernel, dim3(kTotalBlocks, unsigned(count)), dim3(256), 0, stream,
+                               boxes, qA, all, nv, totalPartial);
+        hipLaunchKernelGGL(totalsFinishKernel, dim3(unsigned(count)), dim3(64), 0, stream, totalPartial, kTotalBlocks,
+                           nv, totals);
+        hipLaunchKernelGGL(boundaryIntegralKernel, dim3(unsigned(count)), dim3(64), 0, stream, bA, bB, scalars, nv,
+                           boundary);
         RAMD_HIP(hipGetLastError());
     }
 };
@@ -456,54 +769,123 @@ using namespace ramd;
 
 extern "C" {
 
+}  // extern "C"
+
+namespace ramd {
+
+// A step over the boxes of `list` (one or more independent cases), all on the
+// HEM handle's device with its species.
+static Step* createStep(const std::vector<Transport*>& list, HemHandle* hem) {
+    require(!list.empty() && hem, "Null transport or HEM handle");
+    std::unique_ptr<Step> s(new Step);
+    s->t = list[0];
+    s->h = hem;
+    s->ts = list;
+    s->count = int(list.size());
+    size_t total = 0;
+    for (Transport* t : list) {
+        require(t, "Null transport handle");
+        require(t->cartesian.valid, "The resident step needs a structured-box transport mesh");
+        require(t->device == hem->device, "Transport and HEM handles are on different devices");
+        require(size_t(hem->ns) == t->cfg.species && t->cfg.variables == t->cfg.species + 4,
+                "Transport and HEM species differ, or the transport carries liquid inventories");
+        require(t->cfg.variables == list[0]->cfg.variables, "The boxes of a batch carry different variables");
+        require(list.size() == 1 || t->all == t->cfg.cells, "A batched box carries fixed-state slots");
+        total += t->cfg.cells;
+    }
+    require(list[0]->cfg.variables <= REACTIVE_AMD_MAX_VARIABLES, "Too many conserved variables");
+    s->device = hem->device;
+    RAMD_HIP(hipSetDevice(s->device));
+    s->ns = int(list[0]->cfg.species);
+    s->nv = int(list[0]->cfg.variables);
+    s->cells = total;
+    s->all = list.size() == 1 ? list[0]->all : total;
+    require(s->cells <= UINT32_MAX, "Too many cells");
+    RAMD_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    for (auto& e : s->events) RAMD_HIP(hipEventCreate(&e));
+    // box table and the block ranges of the batched launches
+    std::vector<long long> stageStart(1, 0), cflStart(1, 0);
+    size_t offset = 0;
+    for (Transport* t : list) {
+        BoxDev b;
+        b.a = makeCartesianArgs(t->cartesian, t->cfg.waveFactor);
+        b.offset = offset;
+        b.cells = t->cfg.cells;
+        b.volume = t->cartesian.volume;
+        s->boxHost.push_back(b);
+        stageStart.push_back(stageStart.back() + tileBlocks(b.a));
+        cflStart.push_back(cflStart.back() + long((b.cells + 255) / 256));
+        offset += t->cfg.cells;
+    }
+    s->stageBlocks = long(stageStart.back());
+    s->cflBlocks = long(cflStart.back());
+    const int n = s->count;
+    s->qA = s->alloc<double>(s->all * s->nv);
+    s->qB = s->alloc<double>(s->all * s->nv);
+    s->pA = s->alloc<double>(s->all);
+    s->cA = s->alloc<double>(s->all);
+    s->pB = s->alloc<double>(s->all);
+    s->cB = s->alloc<double>(s->all);
+    s->rA = s->alloc<double>(s->all);
+    s->rB = s->alloc<double>(s->all);
+    s->sA = s->alloc<ThermoOut>(s->cells);
+    s->sB = s->alloc<ThermoOut>(s->cells);
+    s->key = s->alloc<uint8_t>(s->cells);
+    s->order = s->alloc<uint32_t>(s->cells);
+    s->cost = s->alloc<uint16_t>(s->cells);
+    s->orderSearch = s->alloc<uint32_t>(s->cells);
+    s->costSearch = s->alloc<uint16_t>(s->cells);
+    s->bucketCount = s->alloc<unsigned>(256);
+    s->bucketOffset = s->alloc<unsigned>(256);
+    s->stateKey = s->alloc<uint16_t>(s->cells);
+    s->stateCount = s->alloc<unsigned>(kStateBins);
+    s->stateOffset = s->alloc<unsigned>(kStateBins);
+    s->stateRange = s->alloc<unsigned long long>(4);
+    s->blockMin = s->alloc<double>(size_t(s->cflBlocks) + 1);
+    s->scalars = s->alloc<double>(4 * size_t(n) + 1);
+    s->bA = s->alloc<double>(size_t(s->nv) * n);
+    s->bB = s->alloc<double>(size_t(s->nv) * n);
+    s->boundary = s->alloc<double>(size_t(s->nv) * n);
+    s->totals = s->alloc<double>(size_t(s->nv) * n);
+    s->totalPartial = s->alloc<double2>(size_t(s->nv) * kTotalBlocks * n);
+    s->counters = s->alloc<unsigned long long>(8);
+    s->violations = s->alloc<unsigned long long>(size_t(n));
+    s->boxes = s->alloc<BoxDev>(size_t(n));
+    s->stageStart = s->alloc<long long>(size_t(n) + 1);
+    s->cflStart = s->alloc<long long>(size_t(n) + 1);
+    RAMD_HIP(hipMemcpyAsync(s->boxes, s->boxHost.data(), n * sizeof(BoxDev), hipMemcpyHostToDevice, s->stream));
+    RAMD_HIP(hipMemcpyAsync(s->stageStart, stageStart.data(), (n + 1) * sizeof(long long), hipMemcpyHostToDevice,
+                            s->stream));
+    RAMD_HIP(hipMemcpyAsync(s->cflStart, cflStart.data(), (n + 1) * sizeof(long long), hipMemcpyHostToDevice,
+                            s->stream));
+    RAMD_HIP(hipMemsetAsync(s->bucketCount, 0, 256 * sizeof(unsigned), s->stream));
+    RAMD_HIP(hipMemsetAsync(s->stateCount, 0, kStateBins * sizeof(unsigned), s->stream));
+    RAMD_HIP(hipMemsetAsync(s->violations, 0, n * sizeof(unsigned long long), s->stream));
+    RAMD_HIP(hipStreamSynchronize(s->stream));
+    return s.release();
+}
+
+}  // namespace ramd
+
+extern "C" {
+
 void* reactive_amd_step_create(void* transport, void* hem, char* error, size_t errorSize) {
     try {
-        require(transport && hem, "Null transport or HEM handle");
-        std::unique_ptr<Step> s(new Step);
-        s->t = static_cast<Transport*>(transport);
-        s->h = static_cast<HemHandle*>(hem);
-        require(s->t->cartesian.valid, "The resident step needs a structured-box transport mesh");
-        require(s->t->device == s->h->device, "Transport and HEM handles are on different devices");
-        require(size_t(s->h->ns) == s->t->cfg.species && s->t->cfg.variables == s->t->cfg.species + 4,
-                "Transport and HEM species differ, or the transport carries liquid inventories");
-        require(s->t->cfg.variables <= REACTIVE_AMD_MAX_VARIABLES, "Too many conserved variables");
-        s->device = s->t->device;
-        RAMD_HIP(hipSetDevice(s->device));
-        s->ns = int(s->t->cfg.species);
-        s->nv = int(s->t->cfg.variables);
-        s->cells = s->t->cfg.cells;
-        s->all = s->t->all;
-        require(s->cells <= UINT32_MAX, "Too many cells");
-        RAMD_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-        for (auto& e : s->events) RAMD_HIP(hipEventCreate(&e));
-        s->qA = s->alloc<double>(s->all * s->nv);
-        s->qB = s->alloc<double>(s->all * s->nv);
-        s->pA = s->alloc<double>(s->all);
-        s->cA = s->alloc<double>(s->all);
-        s->pB = s->alloc<double>(s->all);
-        s->cB = s->alloc<double>(s->all);
-        s->rA = s->alloc<double>(s->all);
-        s->rB = s->alloc<double>(s->all);
-        s->sA = s->alloc<ThermoOut>(s->cells);
-        s->sB = s->alloc<ThermoOut>(s->cells);
-        s->key = s->alloc<uint8_t>(s->cells);
-        s->order = s->alloc<uint32_t>(s->cells);
-        s->cost = s->alloc<uint16_t>(s->cells);
-        s->orderSearch = s->alloc<uint32_t>(s->cells);
-        s->costSearch = s->alloc<uint16_t>(s->cells);
-        s->bucketCount = s->alloc<unsigned>(256);
-        s->bucketOffset = s->alloc<unsigned>(256);
-        s->blockMin = s->alloc<double>((s->cells + 255) / 256 + 1);
-        s->scalars = s->alloc<double>(4);
-        s->bA = s->alloc<double>(s->nv);
-        s->bB = s->alloc<double>(s->nv);
-        s->boundary = s->alloc<double>(s->nv);
-        s->totals = s->alloc<double>(s->nv);
-        s->totalPartial = s->alloc<double2>(size_t(s->nv) * kTotalBlocks);
-        s->counters = s->alloc<unsigned long long>(8);
-        RAMD_HIP(hipMemsetAsync(s->bucketCount, 0, 256 * sizeof(unsigned), s->stream));
-        RAMD_HIP(hipStreamSynchronize(s->stream));
-        return s.release();
+        return createStep({static_cast<Transport*>(transport)}, static_cast<HemHandle*>(hem));
+    } catch (const std::exception& e) {
+        writeError(error, errorSize, e.what());
+        return nullptr;
+    }
+}
+
+void* reactive_amd_step_create_batch(void* const* transports, int cases, void* hem, char* error, size_t errorSize) {
+    try {
+        require(transports && cases >= 1, "A batch needs at least one transport handle");
+        std::vector<Transport*> list;
+        for (int i = 0; i < cases; ++i) list.push_back(static_cast<Transport*>(transports[i]));
+        for (Transport* t : list)
+            require(t && t->scheme == list[0]->scheme, "The boxes of a batch use different face fluxes");
+        return createStep(list, static_cast<HemHandle*>(hem));
     } catch (const std::exception& e) {
         writeError(error, errorSize, e.what());
         return nullptr;
@@ -524,6 +906,11 @@ int reactive_amd_step_load(void* step, const double* q, const ReactiveThermoStat
         hipLaunchKernelGGL(toThermoKernel, dim3(blocksFor(s.cells, 256)), dim3(256), 0, s.stream, staging, s.cells,
                            s.sA, s.pA, s.cA, s.rA);
         RAMD_HIP(hipGetLastError());
+        if (s.all > s.cells) {
+            hipLaunchKernelGGL(copyFixedKernel, dim3(blocksFor(s.all - s.cells, 64)), dim3(64), 0, s.stream, s.t->q,
+                               s.t->state, s.cells, s.all, s.nv, s.qA, s.qB, s.pA, s.cA, s.rA, s.pB, s.cB, s.rB);
+            RAMD_HIP(hipGetLastError());
+        }
         RAMD_HIP(hipStreamSynchronize(s.stream));
         RAMD_HIP(hipFree(staging));
         s.loaded = true;
@@ -544,9 +931,15 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         flashMode &= ~REACTIVE_AMD_STEP_GRAPH;
         require(flashMode == REACTIVE_AMD_FLASH_FP64 || flashMode == REACTIVE_AMD_FLASH_MIXED,
                 "Unknown flash arithmetic mode");
-        if (flashMode == REACTIVE_AMD_FLASH_MIXED && !s.sets) s.sets = s.alloc<CandidateSet>(s.cells);
+        if (flashMode == REACTIVE_AMD_FLASH_MIXED && !s.sets) {
+            s.sets = s.alloc<CandidateSet>(s.cells);
+            s.queue = s.alloc<uint32_t>(s.cells);
+            s.queueCount = s.alloc<unsigned>(1);
+        }
         s.mode = flashMode;
         hipLaunchKernelGGL(clearCountersKernel, dim3(1), dim3(64), 0, s.stream, s.counters, 8);
+        hipLaunchKernelGGL(clearCountersKernel, dim3(blocksFor(size_t(s.count), 64)), dim3(64), 0, s.stream,
+                           s.violations, s.count);
         while (s.marks.size() < size_t(steps) * kMarks) {
             hipEvent_t e;
             RAMD_HIP(hipEventCreate(&e));
@@ -559,12 +952,17 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         }
         RAMD_HIP(hipEventRecord(s.events[1], s.stream));
         unsigned long long counters[8];
-        double scalars[4], totals[REACTIVE_AMD_MAX_VARIABLES], boundary[REACTIVE_AMD_MAX_VARIABLES];
+        const size_t n = size_t(s.count);
+        std::vector<double> scalars(4 * n + 1), totals(s.nv * n), boundary(s.nv * n);
         RAMD_HIP(hipMemcpyAsync(counters, s.counters, sizeof(counters), hipMemcpyDeviceToHost, s.stream));
-        RAMD_HIP(hipMemcpyAsync(scalars, s.scalars, sizeof(scalars), hipMemcpyDeviceToHost, s.stream));
-        RAMD_HIP(hipMemcpyAsync(totals, s.totals, s.nv * sizeof(double), hipMemcpyDeviceToHost, s.stream));
-        RAMD_HIP(hipMemcpyAsync(boundary, s.boundary, s.nv * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+        RAMD_HIP(hipMemcpyAsync(scalars.data(), s.scalars, scalars.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                s.stream));
+        RAMD_HIP(hipMemcpyAsync(totals.data(), s.totals, totals.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                s.stream));
+        RAMD_HIP(hipMemcpyAsync(boundary.data(), s.boundary, boundary.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                s.stream));
         RAMD_HIP(hipStreamSynchronize(s.stream));
+        s.lastSteps = uint64_t(steps);
         float ms = 0;
         RAMD_HIP(hipEventElapsedTime(&ms, s.events[0], s.events[1]));
         ReactiveAmdStepReport r{};
@@ -577,12 +975,20 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         r.cflViolations = counters[4];
         r.fluxFailures = counters[5];
         r.singlePhaseEvaluations = counters[6];
+        r.queuedCells = counters[7];
+        // a batch reports its smallest dt and recheck step and the sums over its
+        // boxes (box order); reactive_amd_step_report_box gives each box
         r.dt = scalars[0];
         r.recheckDt = scalars[1];
-        for (int k = 0; k < s.nv; ++k) {
-            r.totals[k] = totals[k];
-            r.boundaryIntegral[k] = boundary[k];
+        for (size_t i = 1; i < n; ++i) {
+            r.dt = scalars[4 * i] < r.dt || !(scalars[4 * i] > 0) ? scalars[4 * i] : r.dt;
+            r.recheckDt = scalars[4 * i + 1] < r.recheckDt ? scalars[4 * i + 1] : r.recheckDt;
         }
+        for (size_t i = 0; i < n; ++i)
+            for (int k = 0; k < s.nv; ++k) {
+                r.totals[k] += totals[s.nv * i + k];
+                r.boundaryIntegral[k] += boundary[s.nv * i + k];
+            }
         r.seconds = ms / 1000.0;
         for (int i = 0; i < steps && !useGraph; ++i) {
             const hipEvent_t* m = s.marks.data() + size_t(i) * kMarks;
@@ -601,7 +1007,36 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
             r.flashSeconds[1] += d[7] / 1000.0;
         }
         *report = r;
-        require(r.dt > 0, "Invalid wave/diffusion timestep");
+        for (size_t i = 0; i < n; ++i) require(scalars[4 * i] > 0, "Invalid wave/diffusion timestep");
+    });
+}
+
+int reactive_amd_step_report_box(void* step, int box, ReactiveAmdStepReport* report) {
+    return guard(step, [&](Step& s) {
+        require(report && report->abiVersion == 1 && report->structBytes == sizeof(*report), "Invalid step report ABI");
+        require(box >= 0 && box < s.count, "Box index out of range");
+        double scalars[2];
+        unsigned long long violations = 0;
+        std::vector<double> totals(s.nv), boundary(s.nv);
+        RAMD_HIP(hipMemcpyAsync(scalars, s.scalars + 4 * box, sizeof(scalars), hipMemcpyDeviceToHost, s.stream));
+        RAMD_HIP(hipMemcpyAsync(&violations, s.violations + box, sizeof(violations), hipMemcpyDeviceToHost, s.stream));
+        RAMD_HIP(hipMemcpyAsync(totals.data(), s.totals + size_t(s.nv) * box, s.nv * sizeof(double),
+                                hipMemcpyDeviceToHost, s.stream));
+        RAMD_HIP(hipMemcpyAsync(boundary.data(), s.boundary + size_t(s.nv) * box, s.nv * sizeof(double),
+                                hipMemcpyDeviceToHost, s.stream));
+        RAMD_HIP(hipStreamSynchronize(s.stream));
+        ReactiveAmdStepReport r{};
+        r.abiVersion = 1;
+        r.structBytes = sizeof(r);
+        r.steps = s.lastSteps;
+        r.cflViolations = violations;
+        r.dt = scalars[0];
+        r.recheckDt = scalars[1];
+        for (int k = 0; k < s.nv; ++k) {
+            r.totals[k] = totals[k];
+            r.boundaryIntegral[k] = boundary[k];
+        }
+        *report = r;
     });
 }
 

@@ -591,14 +591,14 @@ void Transport::gather() {
 }
 
 void Transport::boundaryRatesFrom(const double* qv, const double* p, const double* c, const double* r, double* out,
-                                  unsigned long long* failures, hipStream_t s) {
+                                  unsigned long long* failures, hipStream_t s, size_t stride) {
     if (nBoundary) {
         hipLaunchKernelGGL(boundaryHllKernel, dim3(gridFor(nBoundary, 256)), dim3(256), 0, s, faces, boundaryDev,
-                           nBoundary, qv, all, ns, p, c, r, cfg.waveFactor, work, scheme, failures);
+                           nBoundary, qv, stride, ns, p, c, r, cfg.waveFactor, work, scheme, failures);
         RAMD_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(boundaryPartialKernel, dim3(unsigned(nv * partitions)), dim3(256), 0, s, faces, work,
-                       boundaryDev, nBoundary, partitions, qv, all, ns, boundaryPartial);
+                       boundaryDev, nBoundary, partitions, qv, stride, ns, boundaryPartial);
     RAMD_HIP(hipGetLastError());
     hipLaunchKernelGGL(boundaryFinishKernel, dim3(unsigned(nv)), dim3(256), 0, s, boundaryPartial, partitions, out);
     RAMD_HIP(hipGetLastError());

@@ -19,18 +19,45 @@ struct FaceDev {
     double n[3], area, distance;
 };
 
+// Device-side description of one structured box (see CartesianInfo).
+struct CartesianArgs {
+    int nx, ny, nz;
+    double area, invV, waveFactor;
+    uint32_t slotOrder[64];
+    double normal[6][2][3];
+    const int32_t* boundaryKind;  // CartesianInfo::boundaryKind (null: all extrapolated)
+    long long fixedBase;
+};
+// A box of a launch table: its cells are [offset, offset + cells) of the arrays.
+struct BoxDev {
+    CartesianArgs a;
+    unsigned long long offset, cells;
+    double volume;
+};
+
 // Structured N^3-like box detected from a face list: cell c = i + nx (j + ny k),
 // axis-aligned unit normals, one face area, one cell volume, inviscid
-// internal (kind 0) and extrapolated (kind 2) faces only.  The order in which
-// a cell's six faces appear in its CSR row (face id order) is a function of
-// the cell's boundary type only; slotOrder[type] packs that order (3 bits per
-// position, slots 0..5 = -x +x -y +y -z +z), type = 6 boundary flags.
+// internal (kind 0) faces and boundary faces of any kind: extrapolated (2),
+// slip wall (1) or fixed state (3).  The order in which a cell's six faces
+// appear in its CSR row (face id order) is a function of the cell's boundary
+// type only; slotOrder[type] packs that order (3 bits per position, slots
+// 0..5 = -x +x -y +y -z +z), type = 6 boundary flags.
 struct CartesianInfo {
     bool valid = false;
     int nx = 0, ny = 0, nz = 0;
     double area = 0, inverseVolume = 0, volume = 0;
     uint32_t slotOrder[64] = {};
     double normal[6][2][3] = {};  // exact normal of [slot][boundary face?]
+    // kind of every boundary face by side plane (boundaryPlane): -1
+    // extrapolated, -2 slip wall, j >= 0 fixed state j; null when all are
+    // extrapolated
+    int32_t* boundaryKind = nullptr;
+    long long fixedBase = 0;  // array index of fixed state 0 (cells)
+    // one-box launch table of the transport's own stages: box, and the
+    // stage / CFL block starts {0, stageBlocks, 0, cflBlocks}
+    BoxDev* box = nullptr;
+    long long* starts = nullptr;
+    long stageBlocks = 0, cflBlocks = 0;
 };
 
 class Transport {
@@ -105,10 +132,11 @@ public:
                                   const std::vector<uint32_t>& row, const std::vector<int32_t>& incidence);
     void cartesianAdvance(double dt, int stage);
     // device-resident step support (step.hip): boundary flux rates of q
-    // (stride `all`) with the face coefficients rebuilt from q, p, c (and the
+    // (variable stride `stride`, `all` unless the arrays hold a batch of
+    // boxes) with the face coefficients rebuilt from q, p, c (and the
     // recovered density r for HLLC) on the fly; HLLC rejections count into failures
     void boundaryRatesFrom(const double* qv, const double* p, const double* c, const double* r, double* out,
-                           unsigned long long* failures, hipStream_t s);
+                           unsigned long long* failures, hipStream_t s, size_t stride);
     // throws (and clears the counter) if the HLLC flux rejected a face
     void checkFluxFailures();
 };

@@ -1,10 +1,13 @@
 #!/usr/bin/env python3
 """Small drivers for rocprofv3 passes (each runs in seconds):
 
-  stage N REPS   fused Cartesian SSPRK2 stages (cartesianStageKernel) on an N^3
+  stage N REPS   fused Cartesian SSPRK2 stages (batchTiledStageKernel) on an N^3
                  pure-N2O box through reactive_transport_stage, REPS pairs
   flash N        one HEM flash batch (hemFlashKernel) of an N^3 box through
                  reactive_gpu_hem_run_v1
+  cube N MODE STEPS
+                 resident SSPRK2 steps (MODE 0 FP64, 1 mixed) on the seeded
+                 pure-N2O N^3 box of the benchmark
   copy MIB       a plain device copy kernel of MIB MiB (torch elementwise copy),
                  to calibrate FETCH_SIZE and WRITE_SIZE against a known byte count
   uniform N T RHO MODE STEPS
@@ -108,6 +111,45 @@ def uniform(n, T, rho, mode, steps):
     lib.reactive_gpu_hem_destroy_v1(hem)
 
 
+def cube(n, mode, steps):
+    """Resident SSPRK2 steps on the seeded pure-N2O N^3 box of the benchmark."""
+    lib = rlib.load()
+    faces, volume, q, seeds, _ = cases.pure_n2o_cube(n, seed=2026)
+    nc = len(q)
+    mb = model.model_bytes(model.build_model())
+    err = C.create_string_buffer(1024)
+    hem = lib.reactive_gpu_hem_create_v1(mb, len(mb), nc, err, 1024)
+    assert hem, err.value
+    e = cases.internal_energy(q, 1)
+    states = seeds.copy()
+    ok = np.zeros(nc, np.int32)
+    prof = abi.HemProfile.fresh()
+    assert lib.reactive_gpu_hem_run_v1(hem, rlib.ptr(np.ascontiguousarray(q[:, :1])), rlib.ptr(e), nc,
+                                       rlib.ptr(states), rlib.ptr(ok), C.byref(prof), err, 1024) == 0, err.value
+    bad = np.where(ok == 0)[0]
+    if bad.size:  # failed synthetic cells copy a converged neighbour
+        good = np.where(ok == 1)[0]
+        src = good[np.searchsorted(good, bad).clip(0, good.size - 1)]
+        q[bad] = q[src]
+        states[bad] = states[src]
+    cfg = abi.Config(cells=nc, species=1, variables=5, faces=len(faces), fixed=0, viscosity=0, conductivity=0,
+                     diffusivity=0, waveFactor=1.1, maxBytes=100e9, mechanical=0)
+    tr = lib.reactive_transport_create(1, C.byref(cfg), rlib.ptr(volume), rlib.ptr(faces), None, None, None, None,
+                                       err, 1024)
+    step = lib.reactive_amd_step_create(tr, hem, err, 1024)
+    assert step, err.value
+    assert lib.reactive_amd_step_load(step, rlib.ptr(q), rlib.ptr(states)) == 0
+    rep = abi.StepReport.fresh()
+    assert lib.reactive_amd_step_run(step, 0.4, 1.0, steps, mode, C.byref(rep)) == 0, \
+        lib.reactive_amd_step_error(step)
+    print(f"cube: {nc} cells, mode {mode}, {steps} steps: FP64 phase evaluations {rep.phaseEvaluations}, "
+          f"FP32 phase evaluations {rep.singlePhaseEvaluations}, flash {rep.flashSeconds[0] + rep.flashSeconds[1]:.4f} s, "
+          f"search {rep.searchSeconds:.4f} s", flush=True)
+    lib.reactive_amd_step_destroy(step)
+    lib.reactive_transport_destroy(tr)
+    lib.reactive_gpu_hem_destroy_v1(hem)
+
+
 def copy(mib):
     import torch
     n = int(mib) << 20
@@ -127,6 +169,8 @@ if __name__ == "__main__":
         flash(int(sys.argv[2]))
     elif what == "copy":
         copy(int(sys.argv[2]))
+    elif what == "cube":
+        cube(int(sys.argv[2]), int(sys.argv[3]), int(sys.argv[4]))
     elif what == "uniform":
         uniform(int(sys.argv[2]), float(sys.argv[3]), float(sys.argv[4]), int(sys.argv[5]), int(sys.argv[6]))
     else:

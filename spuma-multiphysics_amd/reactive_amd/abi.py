@@ -147,7 +147,8 @@ class StepReport(C.Structure):
                 ("dt", C.c_double), ("recheckDt", C.c_double), ("totals", C.c_double * 32),
                 ("boundaryIntegral", C.c_double * 32), ("seconds", C.c_double), ("stageSeconds", C.c_double * 2),
                 ("flashSeconds", C.c_double * 2), ("cflSeconds", C.c_double), ("fluxFailures", C.c_uint64),
-                ("singlePhaseEvaluations", C.c_uint64), ("searchSeconds", C.c_double)]
+                ("singlePhaseEvaluations", C.c_uint64), ("searchSeconds", C.c_double),
+                ("queuedCells", C.c_uint64)]
 
     @classmethod
     def fresh(cls):

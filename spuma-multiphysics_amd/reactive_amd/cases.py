@@ -261,6 +261,69 @@ def pure_n2o_cube(n, seed=0, edge=0.08, tlo=180.0, thi=300.0, rlo=1.0, rhi=1000.
     return faces, volume, q, seeds, {"two_phase_fraction": float(dome.mean()), "seed": seed, "n": n}
 
 
+def jet_cube(n, seed=0, edge=None, spacing=0.001, radius=0.0025, rho_jet=880.0, T_jet=293.15, speed=30.0,
+             slip_sides=()):
+    """pure_n2o_cube with two liquid-N2O jets entering through fixed-state
+    (kind 3) boundary patches: jet 0 through the -x wall along +x, jet 1
+    through the -y wall along +y, both centred at mid-height of the box (90
+    degrees apart, axes meeting at the box centre), circular of `radius`.
+    Each jet is a reservoir state (rho_jet, T_jet, |u| = speed along its
+    axis).  `slip_sides` lists boundary sides (0..5 = -x +x -y +y -z +z)
+    whose remaining faces become slip walls (kind 1).  The reservoir density
+    is 880 kg/m^3: PR puts 293.15 K, 900 kg/m^3 at 14.87 MPa (inside pmax =
+    50 MPa), but the reference flash finds no stable state for that liquid
+    above about 885 kg/m^3; 880 kg/m^3 (12.97 MPa) is the closest round value
+    it accepts.  Returns (faces,
+    volume, q, seeds, fixed_q[2, 5], fixed_seeds[2], info); the fixed states'
+    p, c_frozen come from a flash of fixed_q seeded by fixed_seeds."""
+    edge = spacing * n if edge is None else edge
+    faces, volume, q, seeds, info = pure_n2o_cube(n, seed=seed, edge=edge)
+    h = edge / n
+    own = faces["owner"]
+    i, j, k = own % n, (own // n) % n, own // (n * n)
+    normal = faces["normal"]
+    bnd = faces["neighbour"] < 0
+    side = np.full(len(faces), -1)
+    for s_, (ax, sign) in enumerate([(0, -1), (0, 1), (1, -1), (1, 1), (2, -1), (2, 1)]):
+        side[bnd & (normal[:, ax] == sign)] = s_
+    centre = 0.5 * edge
+    yz = ((j + .5) * h - centre) ** 2 + ((k + .5) * h - centre) ** 2
+    xz = ((i + .5) * h - centre) ** 2 + ((k + .5) * h - centre) ** 2
+    jet0 = (side == 0) & (yz <= radius ** 2)
+    jet1 = (side == 2) & (xz <= radius ** 2)
+    for s_ in slip_sides:
+        faces["kind"][(side == s_) & ~jet0 & ~jet1] = 1
+    faces["kind"][jet0 | jet1] = 3
+    faces["fixed"][jet0] = 0
+    faces["fixed"][jet1] = 1
+    fluid = PureFluid("N2O")
+    e_jet = fluid.energy(T_jet, rho_jet)
+    p_jet = fluid.pressure(T_jet, fluid.W / rho_jet)
+    fixed_q = np.zeros((2, 5))
+    for f_, axis in ((0, 0), (1, 1)):
+        u = np.zeros(3)
+        u[axis] = speed
+        fixed_q[f_, 0] = rho_jet
+        fixed_q[f_, 1:4] = rho_jet * u
+        fixed_q[f_, 4] = rho_jet * (e_jet + 0.5 * speed ** 2)
+    fixed_seeds = np.zeros(2, dtype=abi.STATE_DTYPE)
+    fixed_seeds["p"] = p_jet; fixed_seeds["T"] = T_jet; fixed_seeds["rho"] = rho_jet; fixed_seeds["e"] = e_jet
+    fixed_seeds["liquidMass"][:, 0] = rho_jet
+    fixed_seeds["activeLiquids"] = 1
+    info.update(jet_faces=(int(jet0.sum()), int(jet1.sum())), jet_pressure=float(p_jet))
+    return faces, volume, q, seeds, fixed_q, fixed_seeds, info
+
+
+def multi_case(count, seed=0, edges=(16, 24, 32, 40, 48), spacing=0.002):
+    """Uneven batch of independent pure-N2O cubes: edge lengths (cells) drawn
+    uniformly from `edges`, cell size `spacing` (2 mm, the 40^3 / 80 mm case),
+    each cube seeded like pure_n2o_cube with its own seed.  Returns a list of
+    pure_n2o_cube tuples."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = rng.choice(np.asarray(edges), size=count)
+    return [pure_n2o_cube(int(n), seed=seed * 1000 + i, edge=spacing * int(n)) for i, n in enumerate(sizes)]
+
+
 def binary_cube(n, seed=0, edge=0.08, ymax=0.3, tlo=150.0, thi=320.0, rlo=0.5, rhi=1000.0, sigma_u=20.0):
     """Seeded binary N2O/N2 cube: Y_N2 ~ U(0, ymax), T ~ U(tlo, thi), rho
     log-uniform, u_d ~ N(0, sigma_u).  The internal energy is the

@@ -54,6 +54,8 @@ _SIGNATURES = {
     "reactive_amd_step_load": (C.c_int, [P, P, P]),
     "reactive_amd_step_run": (C.c_int, [P, C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(StepReport)]),
     "reactive_amd_step_read": (C.c_int, [P, P, P]),
+    "reactive_amd_step_create_batch": (P, [C.POINTER(C.c_void_p), C.c_int, P, C.c_char_p, C.c_size_t]),
+    "reactive_amd_step_report_box": (C.c_int, [P, C.c_int, C.POINTER(StepReport)]),
     "reactive_amd_step_error": (C.c_char_p, [P]),
     "reactive_amd_step_destroy": (None, [P]),
 }

@@ -150,6 +150,26 @@ def cube_case(n, seed=1, threads=8):
                 tstates=cases.transport_states(states), energy=e, ns=1)
 
 
+def jet_case(n, seed=1, slip_sides=(), threads=8):
+    """cube_case with the two fixed-state liquid jets of cases.jet_cube (and
+    optional slip-wall sides); the reservoir states are the oracle flash of
+    their conserved values."""
+    m = model.build_model()
+    mb = model.model_bytes(m)
+    faces, volume, q, seeds, fq, fseeds, info = cases.jet_cube(n, seed=seed, slip_sides=slip_sides)
+    e = cases.internal_energy(q, 1)
+    states, success, _ = oracle_flash(mb, q[:, :1], e, seeds, threads)
+    good = np.where(success == 1)[0]
+    bad = np.where(success == 0)[0]
+    if bad.size:
+        states[bad] = states[good[0]]
+    fstates, fok, _ = oracle_flash(mb, fq[:, :1], cases.internal_energy(fq, 1), fseeds, 1)
+    assert fok.all(), "jet reservoir state does not flash"
+    return dict(model=mb, faces=faces, volume=volume, q=q, seeds=seeds, states=states, success=success,
+                tstates=cases.transport_states(states), energy=e, ns=1, fixed_q=fq, fixed_states=fstates,
+                fixed_tstates=cases.transport_states(fstates), info=info)
+
+
 def random_graph(nc, seed, ns=1, fixed=3):
     """Oblique face graph with all four face kinds: internal faces between
     random cell pairs, slip walls, extrapolated and fixed-state boundaries

@@ -16,7 +16,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from helpers import binary_case, config, cube_case, oracle_flash
+from helpers import binary_case, config, cube_case, jet_case, oracle_flash
 from reactive_amd import abi, cases, lib as rlib, model
 
 pytestmark = pytest.mark.gpu
@@ -28,9 +28,10 @@ def one_step(lib, case, mode):
     err = C.create_string_buffer(1024)
     mb = case["model"]
     hem = lib.reactive_gpu_hem_create_v1(mb, len(mb), len(case["q"]), err, 1024)
-    cfg = config(len(case["q"]), len(case["faces"]), ns=case["ns"])
-    tr = lib.reactive_transport_create(1, C.byref(cfg), rlib.ptr(case["volume"]), rlib.ptr(case["faces"]), None,
-                                       None, None, None, err, 1024)
+    fq = case.get("fixed_q")
+    cfg = config(len(case["q"]), len(case["faces"]), ns=case["ns"], fixed=0 if fq is None else len(fq))
+    tr = lib.reactive_transport_create(1, C.byref(cfg), rlib.ptr(case["volume"]), rlib.ptr(case["faces"]),
+                                       rlib.ptr(fq), rlib.ptr(case.get("fixed_tstates")), None, None, err, 1024)
     step = lib.reactive_amd_step_create(tr, hem, err, 1024)
     assert step, err.value
     assert lib.reactive_amd_step_load(step, rlib.ptr(case["q"]), rlib.ptr(case["states"])) == 0
@@ -54,10 +55,27 @@ def rel(a, b):
     return np.abs(a - b) / np.abs(b)
 
 
-@pytest.mark.parametrize("binary", [False, True], ids=["n2o", "n2o-n2"])
-def test_mixed_step_against_fp64_and_converged_root(binary):
+# per-field report: T, p, vapour volume and mass fractions, liquid mass, wave speeds
+FIELDS = ("T", "p", "alphaGas", "gasFraction", "liquidMass", "soundFrozen", "soundEquilibrium")
+
+
+def field(s, name):
+    if name == "gasFraction":
+        return s["gasMass"] / s["rho"]
+    if name == "liquidMass":
+        return s["liquidMass"][:, 0]
+    return s[name]
+
+
+@pytest.mark.parametrize("kind", ["n2o", "n2o-n2", "jets"])
+def test_mixed_step_against_fp64_and_converged_root(kind):
     lib = rlib.load()
-    case = binary_case(12, seed=8) if binary else cube_case(16, seed=56)
+    if kind == "n2o-n2":
+        case = binary_case(12, seed=8)
+    elif kind == "jets":
+        case = jet_case(16, seed=57)
+    else:
+        case = cube_case(16, seed=56)
     ns = case["ns"]
     q64, s64, r64 = one_step(lib, case, 0)
     qmx, smx, rmx = one_step(lib, case, 1)
@@ -83,7 +101,13 @@ def test_mixed_step_against_fp64_and_converged_root(binary):
         print(f"{name} vs converged root:", {f: (float(v.max()), float(np.percentile(v, 99)),
                                                  float((v <= 1e-10).mean())) for f, v in d.items()})
     same = both & (smx["activeLiquids"] == s64["activeLiquids"])
-    print("mixed vs FP64:", {f: float(rel(smx[f][same], s64[f][same]).max()) for f in ("T", "p")})
+    errors = {}
+    for f in FIELDS:  # absolute for fractions and liquid mass (zero in gas cells), relative otherwise
+        a, b = field(smx, f)[same], field(s64, f)[same]
+        d = np.abs(a - b) if f in ("alphaGas", "gasFraction") else (
+            np.abs(a - b) / np.maximum(np.abs(b), 1e-300) if f != "liquidMass" else np.abs(a - b) / field(s64, "rho")[same])
+        errors[f] = float(d.max())
+    print("mixed vs FP64 (max; fractions absolute, liquid mass relative to rho):", errors)
     print("phase evaluations FP64 / mixed FP64 / mixed FP32:", r64.phaseEvaluations, rmx.phaseEvaluations,
           rmx.singlePhaseEvaluations)
     assert flips <= max(2, cells // 500)

@@ -9,7 +9,7 @@ import math
 import numpy as np
 import pytest
 
-from helpers import OracleTransport, binary_case, config, cube_case, oracle, oracle_flash
+from helpers import OracleTransport, binary_case, config, cube_case, jet_case, oracle, oracle_flash
 from reactive_amd import abi, cases, lib as rlib
 
 pytestmark = pytest.mark.gpu
@@ -17,8 +17,9 @@ pytestmark = pytest.mark.gpu
 
 def oracle_step(case, cfl, scheme=0):
     q0, s0, mb, ns = case["q"], case["states"], case["model"], case["ns"]
-    cfg = config(len(q0), len(case["faces"]), ns=ns)
-    ora = OracleTransport(cfg, case["volume"], case["faces"], scheme=scheme)
+    fq = case.get("fixed_q")
+    cfg = config(len(q0), len(case["faces"]), ns=ns, fixed=0 if fq is None else len(fq))
+    ora = OracleTransport(cfg, case["volume"], case["faces"], fq, case.get("fixed_tstates"), scheme=scheme)
     rc, dt = ora.stable_step(q0, cases.transport_states(s0), cfl, 1.0)
     assert rc == 0
     q = q0.copy()
@@ -31,19 +32,26 @@ def oracle_step(case, cfl, scheme=0):
     return dict(dt=dt, recheck=recheck, q=q, s1=s1, s2=s2, ok=(ok1, ok2), bA=bA, bB=bB)
 
 
-@pytest.mark.parametrize("n,scheme,binary", [(8, 0, False), (12, 0, False), (8, 1, False), (8, 0, True)],
-                         ids=["8-hll", "12-hll", "8-hllc", "8-binary"])
-def test_resident_step_matches_oracle_step(n, scheme, binary):
+@pytest.mark.parametrize("n,scheme,kind", [(8, 0, "pure"), (12, 0, "pure"), (8, 1, "pure"), (8, 0, "binary"),
+                                           (12, 0, "jets"), (12, 1, "jets-slip")],
+                         ids=["8-hll", "12-hll", "8-hllc", "8-binary", "12-jets", "12-jets-slip-hllc"])
+def test_resident_step_matches_oracle_step(n, scheme, kind):
     lib = rlib.load()
-    case = binary_case(n, seed=5) if binary else cube_case(n, seed=11 + n)
+    if kind == "binary":
+        case = binary_case(n, seed=5)
+    elif kind.startswith("jets"):
+        case = jet_case(n, seed=31, slip_sides=(4, 5) if kind == "jets-slip" else ())
+    else:
+        case = cube_case(n, seed=11 + n)
     cfl = 0.4
     ref = oracle_step(case, cfl, scheme)
     err = C.create_string_buffer(1024)
     mb = case["model"]
     hem = lib.reactive_gpu_hem_create_v1(mb, len(mb), len(case["q"]), err, 1024)
-    cfg = config(len(case["q"]), len(case["faces"]), ns=case["ns"])
-    tr = lib.reactive_transport_create(1, C.byref(cfg), rlib.ptr(case["volume"]), rlib.ptr(case["faces"]), None,
-                                       None, None, None, err, 1024)
+    fq = case.get("fixed_q")
+    cfg = config(len(case["q"]), len(case["faces"]), ns=case["ns"], fixed=0 if fq is None else len(fq))
+    tr = lib.reactive_transport_create(1, C.byref(cfg), rlib.ptr(case["volume"]), rlib.ptr(case["faces"]),
+                                       rlib.ptr(fq), rlib.ptr(case.get("fixed_tstates")), None, None, err, 1024)
     assert lib.reactive_amd_transport_set_flux(tr, scheme) == 0
     step = lib.reactive_amd_step_create(tr, hem, err, 1024)
     assert step, err.value

@@ -7,7 +7,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from helpers import OracleTransport, config, cube_case, random_graph
+from helpers import OracleTransport, config, cube_case, jet_case, random_graph
 from reactive_amd import abi, lib as rlib
 
 pytestmark = pytest.mark.gpu
@@ -59,6 +59,40 @@ def test_cartesian_rhs_dt_stages_bitwise(n, scheme):
     total = (r_gpu * case["volume"][:, None]).sum(0) + b_gpu
     scale = np.abs(r_gpu * case["volume"][:, None]).sum(0) + np.abs(b_gpu) + 1e-300
     assert np.all(np.abs(total) <= 2e-11 * scale)
+    lib.reactive_transport_destroy(h)
+    ora.close()
+
+
+@pytest.mark.parametrize("scheme", [0, 1], ids=["hll", "hllc"])
+@pytest.mark.parametrize("slip", [(), (4, 5)], ids=["jets", "jets-slip"])
+def test_cartesian_boundary_kinds_bitwise(slip, scheme):
+    """Structured box with fixed-state jet inlets (kind 3) and slip walls
+    (kind 1) on the fast path: rhs, dt and both stages bit for bit."""
+    lib = rlib.load()
+    case = jet_case(12, seed=21, slip_sides=slip)
+    q, st = case["q"], case["tstates"]
+    nc = len(q)
+    fq, fst = case["fixed_q"], case["fixed_tstates"]
+    cfg = config(nc, len(case["faces"]), fixed=len(fq))
+    h, err = create(lib, cfg, case["volume"], case["faces"], fq, fst)
+    assert h, err
+    assert lib.reactive_amd_transport_set_flux(h, scheme) == 0
+    ora = OracleTransport(cfg, case["volume"], case["faces"], fq, fst, scheme=scheme)
+    r_gpu, b_gpu = gpu_rhs(lib, h, q, st, nc, 5)
+    r_ora, b_ora = ora.rhs(q, st)
+    assert np.array_equal(r_gpu, r_ora) and np.array_equal(b_gpu, b_ora)
+    dt = C.c_double()
+    assert lib.reactive_transport_stable_step(h, rlib.ptr(q), rlib.ptr(st), 0.4, 1.0, C.byref(dt)) == 0
+    rc, dt_ora = ora.stable_step(q, st, 0.4, 1.0)
+    assert rc == 0 and dt.value == dt_ora
+    qg, qo = q.copy(), q.copy()
+    for stage in (0, 1):
+        bg = np.zeros(5)
+        assert lib.reactive_transport_stage(h, rlib.ptr(qg), rlib.ptr(st), None, None, dt.value, stage,
+                                            rlib.ptr(bg)) == 0, lib.reactive_transport_error(h)
+        bo = ora.stage(qo, st, dt_ora, stage)
+        assert np.array_equal(qg, qo), f"stage {stage}: max diff {np.abs(qg - qo).max()}"
+        assert np.array_equal(bg, bo)
     lib.reactive_transport_destroy(h)
     ora.close()
 
