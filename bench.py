@@ -168,7 +168,8 @@ def run_steps(lib, step, q, states, cfl, steps, mode, warmup):
             raise RuntimeError(lib.reactive_amd_step_error(step).decode())
     check(lib.reactive_amd_step_load(step, rlib.ptr(q), rlib.ptr(states)))
     rep = abi.StepReport.fresh()
-    check(lib.reactive_amd_step_run(step, cfl, 1.0, 1, mode, C.byref(rep)))
+    # the first (untimed) step also autotunes the launch shapes on this state
+    check(lib.reactive_amd_step_run(step, cfl, 1.0, 1, mode | AUTOTUNE, C.byref(rep)))
     q_one = np.zeros_like(q)
     s_one = np.zeros(len(q), dtype=abi.STATE_DTYPE)
     check(lib.reactive_amd_step_read(step, rlib.ptr(q_one), rlib.ptr(s_one)))
@@ -195,8 +196,9 @@ def stage_roofline(rep, cells, steps, n, nv=5):
 # HBM bytes per launch of the fused stage kernel at 160^3 pure N2O, stage 0 and
 # stage 1, from rocprofv3 FETCH_SIZE and WRITE_SIZE (each in its own run,
 # scaled by a plain copy kernel's counts): profiles/cartesianStage160_traffic.txt
-STAGE_TRAFFIC_160 = [471.0e6, 641.1e6]
+STAGE_TRAFFIC_160 = [469.8e6, 706.5e6]
 TIGHT = {"vtol": 1e-12, "etol": 1e-12, "mutol": 1e-10}
+AUTOTUNE = 0x200  # REACTIVE_AMD_STEP_AUTOTUNE
 # VALU FLOPs per phase evaluation (ADD, MUL, TRANS = 1, FMA = 2), measured with
 # rocprofv3 on divergence-free uniform cells, mean of a two-phase cell and a
 # gas cell: profiles/flash_valu_uniform32.txt.  The flash roofline's achieved
@@ -352,6 +354,9 @@ def main():
                    "flash": ("mixed: FP32 candidate search, FP64 refinement, stability and selection" if mixed
                              else "fp64 (plain, reference arithmetic)"),
                    "seed": a.seed, "two_phase_fraction": round(info["two_phase_fraction"], 4)},
+        "launch_shapes": {"stage_tile_depth": int(rep.tileDepth), "flash_block": int(rep.flashBlock),
+                          "state_bins_per_axis": int(rep.stateBins), "fp64_flash_block": int(rep64.flashBlock),
+                          "fp64_state_bins_per_axis": int(rep64.stateBins), "source": "autotuned on the first step"},
         "roofline": stage_roofline(rep, cells, a.steps, a.n),
         "roofline_flash": flash_roofline(rep, rep64, cells, a.steps, mixed),
         "fp64": fp64,

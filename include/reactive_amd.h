@@ -20,6 +20,7 @@
 #include <stddef.h>
 #include <stdint.h>
 #include "reactive_gpu_hem.h"
+#include "reactive_transport.h"
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -45,6 +46,10 @@ typedef struct ReactiveAmdStepReport {
     double searchSeconds;               /* mixed flash: the FP32 search kernels' share of flashSeconds */
     uint64_t queuedCells;               /* mixed flash: cells whose FP64 refinement needed a full FP64 search
                                            (or the FP64 restart search), run from a compacted queue */
+    uint32_t tileDepth;                 /* launch shapes in use: cells per column of a stage workgroup, */
+    uint32_t flashBlock;                /* lanes per flash workgroup, */
+    uint32_t stateBins;                 /* state-order bins per axis (see REACTIVE_AMD_STEP_AUTOTUNE) */
+    uint32_t reserved;
 } ReactiveAmdStepReport;
 
 /* Flash arithmetic of a step.  FP64: the reference's arithmetic.  MIXED: the
@@ -61,6 +66,13 @@ typedef struct ReactiveAmdStepReport {
  * report then holds totals and the call's device time only, no per-kernel
  * times. */
 #define REACTIVE_AMD_STEP_GRAPH 0x100
+/* OR-ed into flashMode: before its steps, the first run with this flag (per
+ * flash arithmetic) times the candidate launch shapes on the loaded state and
+ * keeps the fastest for the handle's later runs: stage chunk depth 2 / 4 / 8 cells, flash
+ * workgroups of 64 / 128 / 256 lanes, 32 / 64 / 128 state-order bins per
+ * axis (the choices are reported in ReactiveAmdStepReport; nothing the
+ * trials compute is kept). */
+#define REACTIVE_AMD_STEP_AUTOTUNE 0x200
 
 /* Face flux of a transport handle (applies to reactive_transport_rhs/stage,
  * the resident stages and the resident step): REACTIVE_AMD_FLUX_HLL (the
@@ -108,6 +120,35 @@ void* reactive_amd_step_create_batch(void* const* transports, int cases, void* h
 int reactive_amd_step_report_box(void* step, int box, ReactiveAmdStepReport* report);
 const char* reactive_amd_step_error(void* step);
 void reactive_amd_step_destroy(void* step);
+
+/* ---- Adjoint of a step's pieces (no counterpart in the reference ABI; for
+ * gradients of one step's output with respect to the PR parameters and the
+ * reservoir states, behind reactive_amd/adjoint.py's torch.autograd.Function).
+ *
+ * Flash backward by the implicit-function theorem at the converged states of
+ * reactive_gpu_hem_run_v1 (one gas species): given y_bar = (p_bar, c_bar) per
+ * cell (adjoints of p and c_frozen), returns q_bar (count x 5, cell-major, the
+ * adjoint of the conserved inputs rho, rho u, rho E) and theta_bar[2] = the
+ * adjoint of (ln a-scale, ln b-scale), summed over the cells.  `scaled` holds
+ * four HEM handles created from the same model with the PR a (then b) of every
+ * species multiplied by 1 + relStep, 1 - relStep (a then b).  Cells with
+ * success[c] == 0 (may be null) contribute nothing; cells whose local system is
+ * singular are skipped and counted in *skipped.  Host arrays.  Returns 0 or 1
+ * (message in `error`). */
+int reactive_amd_flash_adjoint(void* hem, void* const* scaled, double relStep, const double* q, int nv,
+                               const ReactiveThermoState* states, const int* success, size_t count,
+                               const double* pBar, const double* cBar, double* qBar, double* thetaBar,
+                               uint64_t* skipped, char* error, size_t errorSize);
+/* Transposed HLL stage rates of a structured-box transport (one species):
+ * given rate_bar (cells x 5, cell-major: the adjoint of the rates L the stage
+ * adds dt times, reactive_transport_rhs), the cells' transport states and the
+ * handle's own reservoir states, returns the adjoints of each cell's q
+ * (cells x 5), p and c_frozen (cells), and of each reservoir's (q[5], p,
+ * c_frozen) in fixedBar (fixed x 7).  Host arrays.  Returns 0, 1
+ * (reactive_transport_error), 2 (concurrent entry) or -1 (null handle). */
+int reactive_amd_transport_stage_adjoint(void* transport, const double* q, const ReactiveTransportState* states,
+                                         const double* rateBar, double* qBar, double* pBar, double* cBar,
+                                         double* fixedBar);
 
 #ifdef __cplusplus
 }

@@ -71,10 +71,13 @@ inline unsigned xcdGrid(long logicalBlocks) {
 // cells.  Velocities are computed once per cell.  Each face is evaluated
 // with the generic path's arithmetic and every cell sums its six faces in
 // its CSR order, so the result is the same bit for bit.
+// Cells per column a workgroup marches (the chunk depth CZ): 2, 4 or 8,
+// chosen per box by the step's autotuner; RAMD_TILE_Z is the default.
 #ifndef RAMD_TILE_Z
 #define RAMD_TILE_Z 4
 #endif
 constexpr int kTileX = 32, kTileY = 8, kChunkZ = RAMD_TILE_Z;
+constexpr int kChunkDepths[3] = {2, 4, 8};
 
 template <int NS, int NV, bool HLLC>
 struct TileCell {
@@ -169,13 +172,13 @@ struct TileShared {
     double faceX[kTileY][kTileX][NV], faceY[kTileY][kTileX][NV];
 };
 
-inline __host__ __device__ long tileBlocks(const CartesianArgs& a) {
-    return long((a.nx + kTileX - 1) / kTileX) * ((a.ny + kTileY - 1) / kTileY) * ((a.nz + kChunkZ - 1) / kChunkZ);
+inline __host__ __device__ long tileBlocks(const CartesianArgs& a, int cz = kChunkZ) {
+    return long((a.nx + kTileX - 1) / kTileX) * ((a.ny + kTileY - 1) / kTileY) * ((a.nz + cz - 1) / cz);
 }
 
 // Logical tile block b (< tileBlocks(a)) of one box; every thread of the
 // workgroup calls it (it synchronises the workgroup).
-template <int NS, int NV, bool HLLC>
+template <int NS, int NV, bool HLLC, int CZ = kChunkZ>
 __device__ inline void tiledStageBlock(const CartesianArgs& a, long b, TileShared<NS, NV, HLLC>& lds,
                                        const double* __restrict__ q, size_t stride, const double* __restrict__ pArr,
                                        const double* __restrict__ cArr, const double* __restrict__ rArr, double dt,
@@ -189,7 +192,7 @@ __device__ inline void tiledStageBlock(const CartesianArgs& a, long b, TileShare
     const int tx = threadIdx.x % kTileX, ty = threadIdx.x / kTileX;
     const int i = bx * kTileX + tx, j = by * kTileY + ty;
     const bool active = i < a.nx && j < a.ny;
-    const int k0 = bz * kChunkZ, k1 = min(a.nz, k0 + kChunkZ);
+    const int k0 = bz * CZ, k1 = min(a.nz, k0 + CZ);
     const long nxy = long(a.nx) * a.ny;
     Cell cur, nxt;
     double fzMinus[NV];
@@ -208,10 +211,7 @@ __device__ inline void tiledStageBlock(const CartesianArgs& a, long b, TileShare
     }
     for (int k = k0; k < k1; ++k) {
         const long c = long(i) + long(a.nx) * j + nxy * k;
-        // the next cell of the column is loaded before this step's face work,
-        // so its latency overlaps the work
-        if (active && k + 1 < a.nz) loadTileRaw(nxt, q, stride, pArr, cArr, rArr, c + nxy);
-        double base[NV];  // stage 1: q0 of this cell, loaded early as well
+        double base[NV];  // stage 1: q0 of this cell, loaded before the face work so its latency overlaps it
         if (active && mode != 0)
             for (int v = 0; v < NV; ++v) base[v] = out[size_t(v) * stride + c];
         if (active) shared[ty][tx] = cur;
@@ -254,7 +254,7 @@ __device__ inline void tiledStageBlock(const CartesianArgs& a, long b, TileShare
                               rArr);
                 tileFace(a, cur, ghost, a.normal[5][1], fzPlus, rejected);
             } else {
-                deriveTileCell(nxt);
+                loadTileCell(nxt, q, stride, pArr, cArr, rArr, c + nxy);
                 tileFace(a, cur, nxt, a.normal[5][0], fzPlus, rejected);
             }
             // -x / -y faces: domain boundary, or a tile edge (neighbour in another tile)
@@ -465,7 +465,7 @@ __global__ __launch_bounds__(256) void batchCflKernel(const BoxDev* __restrict__
 // dtDev is null; a box whose dt is invalid is left untouched).  Three
 // workgroups per CU fit the LDS at nv = 5 and the registers are held to
 // three waves per SIMD for them; wider states run two.
-template <int NS, int NV, bool HLLC>
+template <int NS, int NV, bool HLLC, int CZ>
 __global__ __launch_bounds__(kTileX * kTileY, NV <= 5 ? 3 : 2) void batchTiledStageKernel(
     const BoxDev* __restrict__ boxes, const long long* __restrict__ start, int count, const double* __restrict__ q,
     size_t stride, const double* __restrict__ pArr, const double* __restrict__ cArr, const double* __restrict__ rArr,
@@ -481,7 +481,7 @@ __global__ __launch_bounds__(kTileX * kTileY, NV <= 5 ? 3 : 2) void batchTiledSt
     if (!(dt > 0)) return;  // uniform per workgroup
     const size_t o = box.offset;
     uint32_t rejected = 0;
-    tiledStageBlock<NS, NV, HLLC>(box.a, b - start[i], lds, q + o, stride, pArr + o, cArr + o,
+    tiledStageBlock<NS, NV, HLLC, CZ>(box.a, b - start[i], lds, q + o, stride, pArr + o, cArr + o,
                                   HLLC ? rArr + o : nullptr, dt, mode, out + o, rejected);
     if (HLLC && rejected) atomicAdd(failures, (unsigned long long)rejected);
 }
@@ -499,15 +499,22 @@ inline void launchBatchCfl(int ns, const BoxDev* boxes, const long long* start, 
                            cfl, maxHost, maxDev, blockMin);
 }
 
-// One SSPRK2 stage of every box (blocks = start[count] of the stage table).
+// One SSPRK2 stage of every box (blocks = start[count] of the stage table
+// built for chunk depth cz, one of kChunkDepths).
 inline void launchBatchStage(int ns, int nv, int scheme, const BoxDev* boxes, const long long* start, int count,
                              long blocks, const double* q, size_t stride, const double* p, const double* c,
                              const double* r, double dtHost, const double* dtDev, int mode, double* out,
-                             unsigned long long* failures, hipStream_t stream) {
+                             unsigned long long* failures, hipStream_t stream, int cz = kChunkZ) {
     const unsigned grid = xcdGrid(blocks);
-#define RAMD_BATCH(NS_, NV_, H_)                                                                                  \
-    hipLaunchKernelGGL((batchTiledStageKernel<NS_, NV_, H_>), dim3(grid), dim3(kTileX * kTileY), 0, stream, boxes, \
-                       start, count, q, stride, p, c, r, dtHost, dtDev, mode, out, failures)
+#define RAMD_BATCH_Z(NS_, NV_, H_, Z_)                                                                             \
+    hipLaunchKernelGGL((batchTiledStageKernel<NS_, NV_, H_, Z_>), dim3(grid), dim3(kTileX * kTileY), 0, stream,    \
+                       boxes, start, count, q, stride, p, c, r, dtHost, dtDev, mode, out, failures)
+#define RAMD_BATCH(NS_, NV_, H_)                          \
+    do {                                                  \
+        if (cz == 2) RAMD_BATCH_Z(NS_, NV_, H_, 2);       \
+        else if (cz == 8) RAMD_BATCH_Z(NS_, NV_, H_, 8);  \
+        else RAMD_BATCH_Z(NS_, NV_, H_, 4);               \
+    } while (0)
 #define RAMD_BATCH_SCHEME(NS_, NV_)                  \
     if (scheme == kHllc) RAMD_BATCH(NS_, NV_, true); \
     else RAMD_BATCH(NS_, NV_, false)
@@ -517,6 +524,7 @@ inline void launchBatchStage(int ns, int nv, int scheme, const BoxDev* boxes, co
     else { RAMD_BATCH_SCHEME(2, 7); }
 #undef RAMD_BATCH_SCHEME
 #undef RAMD_BATCH
+#undef RAMD_BATCH_Z
 }
 
 }  // namespace ramd

@@ -18,7 +18,7 @@ namespace ramd {
 // Mixed precision, first half: the candidate search in FP32, recording the
 // converged candidates of every cell.
 template <int NS, int NL>
-__global__ __launch_bounds__(64, RAMD_SEARCH_MIN_WAVES) void stepSearchKernel(const HemModel<NS>* __restrict__ globalModel,
+__global__ __launch_bounds__(256, RAMD_SEARCH_MIN_WAVES) void stepSearchKernel(const HemModel<NS>* __restrict__ globalModel,
                                                        const double* __restrict__ q, size_t stride,
                                                        const ThermoOut* __restrict__ seeds,
                                                        const uint32_t* __restrict__ order, uint32_t count,
@@ -43,10 +43,10 @@ __global__ __launch_bounds__(64, RAMD_SEARCH_MIN_WAVES) void stepSearchKernel(co
 
 void launchSearch(int ns, int nl, const void* model, const double* q, size_t stride, const ThermoOut* seeds,
                   const uint32_t* order, uint32_t count, const double* dtDev, CandidateSet* sets, uint16_t* cost,
-                  unsigned long long* phases, hipStream_t stream) {
-    const unsigned blocks = unsigned((size_t(count) + 63) / 64);
+                  unsigned long long* phases, hipStream_t stream, unsigned block) {
+    const unsigned blocks = unsigned((size_t(count) + block - 1) / block);
 #define RAMD_SEARCH(NS_, NL_)                                                                                     \
-    hipLaunchKernelGGL((stepSearchKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                            \
+    hipLaunchKernelGGL((stepSearchKernel<NS_, NL_>), dim3(blocks), dim3(block), 0, stream,                         \
                        static_cast<const HemModel<NS_>*>(model), q, stride, seeds, order, count, dtDev, sets, cost, \
                        phases)
     if (ns == 1 && nl == 0) { RAMD_SEARCH(1, 0); }

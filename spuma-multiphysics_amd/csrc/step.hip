@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <chrono>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -36,7 +37,7 @@ namespace ramd {
 // ---------------------------------------------------------------- kernels
 // Plain FP64 flash of the cells in `order`.
 template <int NS, int NL>
-__global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void stepFlashKernel(const HemModel<NS>* __restrict__ globalModel,
+__global__ __launch_bounds__(256, RAMD_FLASH_MIN_WAVES) void stepFlashKernel(const HemModel<NS>* __restrict__ globalModel,
                                                       const double* __restrict__ q, size_t stride,
                                                       const ThermoOut* __restrict__ seeds,
                                                       const uint32_t* __restrict__ order, uint32_t count,
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void stepFlashKernel(cons
 
 // Mixed precision, second half: FP64 refinement, stability and selection.
 template <int NS, int NL>
-__global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void stepRefineKernel(const HemModel<NS>* __restrict__ globalModel,
+__global__ __launch_bounds__(256, RAMD_FLASH_MIN_WAVES) void stepRefineKernel(const HemModel<NS>* __restrict__ globalModel,
                                                        const double* __restrict__ q, size_t stride,
                                                        const ThermoOut* __restrict__ seeds,
                                                        const uint32_t* __restrict__ order, uint32_t count,
@@ -213,15 +214,17 @@ __global__ __launch_bounds__(256) void bucketScatterKernel(const uint8_t* __rest
 
 // ---- thermodynamic-state order ---------------------------------------------
 //
-// Cells whose (ln rho, e) fall in the same cell of a 64 x 64 grid spanning
-// the flash input's range are flashed by the same waves: their searches take
+// Cells whose (ln rho, e) fall in the same cell of a side x side grid
+// spanning the flash input's range are flashed by the same waves: their searches take
 // the same branches with similar iteration counts, so the lanes of a wave
 // stay converged (cost order alone groups cells by total work, whose paths
 // still differ).  The order decides nothing numerical.
 #ifndef RAMD_STATE_ORDER
 #define RAMD_STATE_ORDER 1
 #endif
-constexpr int kStateSide = 64, kStateBins = kStateSide * kStateSide, kStateBlocks = 1024;
+// side: bins per axis, 32, 64 (default) or 128, chosen by the autotuner
+constexpr int kStateSides[3] = {32, 64, 128};
+constexpr int kStateMaxBins = 128 * 128, kStateBlocks = 1024;
 
 __device__ inline unsigned long long orderedBits(double x) {
     const unsigned long long u = __double_as_longlong(x);
@@ -282,37 +285,41 @@ __global__ __launch_bounds__(256) void stateRangeKernel(const double* __restrict
         else atomicMin(&range[j], m);
     }
 }
-__device__ inline unsigned stateBin(double x, double lo, double hi) {
+__device__ inline unsigned stateBin(double x, double lo, double hi, int side) {
     if (!(hi > lo)) return 0;
-    const double t = (x - lo) / (hi - lo) * kStateSide;
-    return t <= 0 ? 0u : (t >= kStateSide - 1 ? unsigned(kStateSide - 1) : unsigned(t));
+    const double t = (x - lo) / (hi - lo) * side;
+    return t <= 0 ? 0u : (t >= side - 1 ? unsigned(side - 1) : unsigned(t));
 }
 // bin of every cell (key) and the bin counts; grid-stride over kStateBlocks blocks
 __global__ __launch_bounds__(256) void stateKeyKernel(const double* __restrict__ q, size_t stride, int ns,
                                                       size_t cells, const unsigned long long* __restrict__ range,
-                                                      uint16_t* __restrict__ key, unsigned* __restrict__ count) {
-    __shared__ unsigned hist[kStateBins];
-    for (int b = threadIdx.x; b < kStateBins; b += blockDim.x) hist[b] = 0;
+                                                      int side, uint16_t* __restrict__ key,
+                                                      unsigned* __restrict__ count) {
+    __shared__ unsigned hist[kStateMaxBins];
+    const int bins = side * side;
+    for (int b = threadIdx.x; b < bins; b += blockDim.x) hist[b] = 0;
     __syncthreads();
     const double lo0 = fromOrdered(range[0]), hi0 = fromOrdered(range[1]);
     const double lo1 = fromOrdered(range[2]), hi1 = fromOrdered(range[3]);
     for (size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x; c < cells; c += size_t(gridDim.x) * blockDim.x) {
         double lr, e;
         cellState(q, stride, ns, c, lr, e);
-        const unsigned k = stateBin(lr, lo0, hi0) * kStateSide + stateBin(e, lo1, hi1);
+        const unsigned k = stateBin(lr, lo0, hi0, side) * side + stateBin(e, lo1, hi1, side);
         key[c] = uint16_t(k);
         atomicAdd(&hist[k], 1u);
     }
     __syncthreads();
-    for (int b = threadIdx.x; b < kStateBins; b += blockDim.x)
+    for (int b = threadIdx.x; b < bins; b += blockDim.x)
         if (hist[b]) atomicAdd(&count[b], hist[b]);
 }
 // exclusive scan of the bin counts into offset; clears count (one block of 256)
-__global__ __launch_bounds__(256) void stateScanKernel(unsigned* __restrict__ count, unsigned* __restrict__ offset) {
-    constexpr int per = kStateBins / 256;
+__global__ __launch_bounds__(256) void stateScanKernel(unsigned* __restrict__ count, unsigned* __restrict__ offset,
+                                                       int bins) {
+    const int per = (bins + 255) / 256;
     __shared__ unsigned part[256];
     unsigned sum = 0;
-    for (int j = 0; j < per; ++j) sum += count[threadIdx.x * per + j];
+    for (int j = 0; j < per; ++j)
+        if (threadIdx.x * per + j < bins) sum += count[threadIdx.x * per + j];
     part[threadIdx.x] = sum;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -327,22 +334,23 @@ __global__ __launch_bounds__(256) void stateScanKernel(unsigned* __restrict__ co
     unsigned run = part[threadIdx.x];
     for (int j = 0; j < per; ++j) {
         const int b = threadIdx.x * per + j;
+        if (b >= bins) break;
         offset[b] = run;
         run += count[b];
         count[b] = 0;
     }
 }
 // each block reserves one run per bin for the cells of its grid-stride set
-__global__ __launch_bounds__(256) void stateScatterKernel(const uint16_t* __restrict__ key, size_t cells,
+__global__ __launch_bounds__(256) void stateScatterKernel(const uint16_t* __restrict__ key, size_t cells, int bins,
                                                           unsigned* __restrict__ offset,
                                                           uint32_t* __restrict__ order) {
-    __shared__ unsigned local[kStateBins];
-    for (int b = threadIdx.x; b < kStateBins; b += blockDim.x) local[b] = 0;
+    __shared__ unsigned local[kStateMaxBins];
+    for (int b = threadIdx.x; b < bins; b += blockDim.x) local[b] = 0;
     __syncthreads();
     for (size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x; c < cells; c += size_t(gridDim.x) * blockDim.x)
         atomicAdd(&local[key[c]], 1u);
     __syncthreads();
-    for (int b = threadIdx.x; b < kStateBins; b += blockDim.x)
+    for (int b = threadIdx.x; b < bins; b += blockDim.x)
         if (local[b]) local[b] = atomicAdd(&offset[b], local[b]);
     __syncthreads();
     for (size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x; c < cells; c += size_t(gridDim.x) * blockDim.x)
@@ -506,6 +514,16 @@ struct Step {
     BoxDev* boxes = nullptr;
     long long *stageStart = nullptr, *cflStart = nullptr;
     long stageBlocks = 0, cflBlocks = 0;
+    // launch shapes (the autotuner's choices): stage chunk depth, flash
+    // workgroup size, state-order bins per axis; one stage table per depth
+    int tileZ = kChunkZ, stateSide = 64;
+    unsigned flashBlock = 64;
+    int stateSideOf[2] = {64, 64};        // per flash arithmetic (FP64, mixed)
+    unsigned flashBlockOf[2] = {64, 64};
+    bool tunedOf[2] = {false, false};
+    long long* stageStarts[3] = {nullptr, nullptr, nullptr};
+    long stageBlocksZ[3] = {0, 0, 0};
+    hipEvent_t tuneEvents[2] = {};
     unsigned long long* violations = nullptr;  // per box: failed CFL rechecks
     std::vector<void*> allocations;
     double *qA = nullptr, *qB = nullptr, *pA = nullptr, *cA = nullptr, *pB = nullptr, *cB = nullptr;
@@ -532,8 +550,10 @@ struct Step {
     double* scalars = nullptr;  // per box [4 i]: dt, recheck; [4 count]: flash gate of a batch
     double *bA = nullptr, *bB = nullptr, *boundary = nullptr, *totals = nullptr;  // nv per box
     double2* totalPartial = nullptr;
-    // fail1, phases1, fail2, phases2, violations, HLLC rejections, FP32 phases, queued cells (mixed)
+    // fail1, phases1, fail2, phases2, violations, HLLC rejections, FP32 phases, queued cells (mixed);
+    // [8, 16): the autotuner's trial flashes
     unsigned long long* counters = nullptr;
+    unsigned long long* flashCounters = nullptr;  // where flash() counts: counters, or counters + 8 while tuning
     hipStream_t stream = nullptr;
     hipEvent_t events[2] = {};
     // 11 per step: 9 in sequence bracketing cfl, stage 0, flash #1, recheck, stage 1,
@@ -555,6 +575,8 @@ struct Step {
             if (g) (void)hipGraphExecDestroy(g);
         for (auto e : events)
             if (e) (void)hipEventDestroy(e);
+        for (auto e : tuneEvents)
+            if (e) (void)hipEventDestroy(e);
         for (auto e : marks) (void)hipEventDestroy(e);
         for (void* p : allocations) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
@@ -570,7 +592,8 @@ struct Step {
     // transport inputs; costOut (may be null) receives the FP64 cost.
     void flash(const double* q, double* pOut, double* cOut, double* rOut, ThermoOut* out, uint16_t* costOut,
                unsigned long long* counter, hipEvent_t split) {
-        const unsigned blocks = blocksFor(cells, 64);
+        const unsigned blocks = blocksFor(cells, flashBlock);
+        unsigned long long* counters = flashCounters;
 #if RAMD_STATE_ORDER
         stateOrder(q, order);
         uint32_t* orderSearch = order;
@@ -578,10 +601,10 @@ struct Step {
         if (mode == REACTIVE_AMD_FLASH_MIXED) {
 #define RAMD_STEP_MIXED(NS_, NL_)                                                                                  \
     launchSearch(NS_, NL_, h->model, q, all, sA, orderSearch, uint32_t(cells), gate(), sets,                      \
-                 costOut ? costSearch : nullptr, counters + 6, stream);                                            \
+                 costOut ? costSearch : nullptr, counters + 6, stream, flashBlock);                                \
     if (split) RAMD_HIP(hipEventRecord(split, stream));                                                            \
     if (RAMD_REFINE_QUEUE) hipLaunchKernelGGL(clearQueueKernel, dim3(1), dim3(64), 0, stream, queueCount);          \
-    hipLaunchKernelGGL((stepRefineKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                             \
+    hipLaunchKernelGGL((stepRefineKernel<NS_, NL_>), dim3(blocks), dim3(flashBlock), 0, stream,                     \
                        static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), gate(),    \
                        sets, out, pOut, cOut, rOut, costOut, counter, RAMD_REFINE_QUEUE ? queue : nullptr,         \
                        queueCount);                                                                                \
@@ -598,7 +621,7 @@ struct Step {
 #undef RAMD_STEP_MIXED
         } else {
 #define RAMD_STEP_FLASH(NS_, NL_)                                                                                 \
-    hipLaunchKernelGGL((stepFlashKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                             \
+    hipLaunchKernelGGL((stepFlashKernel<NS_, NL_>), dim3(blocks), dim3(flashBlock), 0, stream,                     \
                        static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), gate(), out, \
                        pOut, cOut, rOut, costOut, counter)
             if (h->ns == 1 && h->nl == 0) { RAMD_STEP_FLASH(1, 0); }
@@ -626,8 +649,9 @@ struct Step {
             hipLaunchKernelGGL(batchGateKernel, dim3(1), dim3(64), 0, stream, scalars, count, scalars + 4 * count);
     }
     void stage(const double* q, const double* p, const double* c, const double* r, int which, double* out) {
-        launchBatchStage(ns, nv, t->scheme, boxes, stageStart, count, stageBlocks, q, all, p, c, r, 0.0, scalars,
-                         which, out, counters + 5, stream);
+        const int z = tileZ == 2 ? 0 : (tileZ == 8 ? 2 : 1);
+        launchBatchStage(ns, nv, t->scheme, boxes, stageStarts[z], count, stageBlocksZ[z], q, all, p, c, r, 0.0,
+                         scalars, which, out, counters + 5, stream, tileZ);
     }
     void boundaryRates(const double* q, const double* p, const double* c, const double* r, double* out) {
         for (int i = 0; i < count; ++i) {
@@ -641,10 +665,66 @@ struct Step {
         hipLaunchKernelGGL(stateRangeInitKernel, dim3(1), dim3(64), 0, stream, stateRange);
         hipLaunchKernelGGL(stateRangeKernel, dim3(kStateBlocks), dim3(256), 0, stream, q, all, ns, cells, stateRange);
         hipLaunchKernelGGL(stateKeyKernel, dim3(kStateBlocks), dim3(256), 0, stream, q, all, ns, cells, stateRange,
-                           stateKey, stateCount);
-        hipLaunchKernelGGL(stateScanKernel, dim3(1), dim3(256), 0, stream, stateCount, stateOffset);
-        hipLaunchKernelGGL(stateScatterKernel, dim3(kStateBlocks), dim3(256), 0, stream, stateKey, cells, stateOffset,
-                           out);
+                           stateSide, stateKey, stateCount);
+        hipLaunchKernelGGL(stateScanKernel, dim3(1), dim3(256), 0, stream, stateCount, stateOffset,
+                           stateSide * stateSide);
+        hipLaunchKernelGGL(stateScatterKernel, dim3(kStateBlocks), dim3(256), 0, stream, stateKey, cells,
+                           stateSide * stateSide, stateOffset, out);
+    }
+
+    // Autotuner: times each candidate launch shape on the loaded state (the
+    // step-start q and seeds) and keeps the fastest.  Stage: chunk depth 2,
+    // 4, 8 (stage 0 from q0, three launches each after a warm-up).  Flash (the
+    // current arithmetic, into the scratch state buffers): state-order bins
+    // per axis 32, 64, 128 with 64-lane workgroups, then workgroups of 64,
+    // 128, 256 lanes with the chosen bins.  Nothing it computes is kept.
+    float timeIt(const std::function<void()>& work, int reps) {
+        RAMD_HIP(hipEventRecord(tuneEvents[0], stream));
+        for (int r = 0; r < reps; ++r) work();
+        RAMD_HIP(hipEventRecord(tuneEvents[1], stream));
+        RAMD_HIP(hipEventSynchronize(tuneEvents[1]));
+        float ms = 0;
+        RAMD_HIP(hipEventElapsedTime(&ms, tuneEvents[0], tuneEvents[1]));
+        return ms / reps;
+    }
+    void tune(double cfl, double maximum) {
+        cflSweep(qA, pA, cA, cfl, maximum, false, 0);  // a valid dt for the trial stages
+        float best = 0;
+        int bestZ = tileZ;
+        for (int z : kChunkDepths) {
+            tileZ = z;
+            stage(qA, pA, cA, rA, 0, qB);
+            const float ms = timeIt([&] { stage(qA, pA, cA, rA, 0, qB); }, 3);
+            if (z == kChunkDepths[0] || ms < best) { best = ms; bestZ = z; }
+        }
+        tileZ = bestZ;
+        flashCounters = counters + 8;
+#if !RAMD_STATE_ORDER
+        sortCells(haveCost ? cost : nullptr, order);
+        if (mode == REACTIVE_AMD_FLASH_MIXED) sortCells(haveCostSearch ? costSearch : nullptr, orderSearch);
+#endif
+        auto trial = [&] { flash(qA, pB, cB, rB, sB, nullptr, flashCounters, nullptr); };
+        int bestSide = stateSide;
+        flashBlock = 64;
+        for (int side : kStateSides) {
+            stateSide = side;
+            const float ms = timeIt(trial, 1);
+            if (side == kStateSides[0] || ms < best) { best = ms; bestSide = side; }
+        }
+        stateSide = bestSide;
+        unsigned bestBlock = 64;
+        for (unsigned b : {64u, 128u, 256u}) {
+            flashBlock = b;
+            const float ms = b == 64u ? best : timeIt(trial, 1);
+            if (ms < best || b == 64u) { best = ms; bestBlock = b; }
+        }
+        flashBlock = bestBlock;
+        flashCounters = counters;
+        for (auto& g : graph)  // captured with the old shapes
+            if (g) { RAMD_HIP(hipGraphExecDestroy(g)); g = nullptr; }
+        stateSideOf[mode] = stateSide;
+        flashBlockOf[mode] = flashBlock;
+        tunedOf[mode] = true;
     }
 
     // Heavy-first order of all cells: by cost when one is known, else by the
@@ -804,7 +884,8 @@ static Step* createStep(const std::vector<Transport*>& list, HemHandle* hem) {
     RAMD_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     for (auto& e : s->events) RAMD_HIP(hipEventCreate(&e));
     // box table and the block ranges of the batched launches
-    std::vector<long long> stageStart(1, 0), cflStart(1, 0);
+    std::vector<long long> stageStart(1, 0), cflStart(1, 0), startsZ[3];
+    for (auto& v : startsZ) v.assign(1, 0);
     size_t offset = 0;
     for (Transport* t : list) {
         BoxDev b;
@@ -814,6 +895,7 @@ static Step* createStep(const std::vector<Transport*>& list, HemHandle* hem) {
         b.volume = t->cartesian.volume;
         s->boxHost.push_back(b);
         stageStart.push_back(stageStart.back() + tileBlocks(b.a));
+        for (int z = 0; z < 3; ++z) startsZ[z].push_back(startsZ[z].back() + tileBlocks(b.a, kChunkDepths[z]));
         cflStart.push_back(cflStart.back() + long((b.cells + 255) / 256));
         offset += t->cfg.cells;
     }
@@ -838,8 +920,8 @@ static Step* createStep(const std::vector<Transport*>& list, HemHandle* hem) {
     s->bucketCount = s->alloc<unsigned>(256);
     s->bucketOffset = s->alloc<unsigned>(256);
     s->stateKey = s->alloc<uint16_t>(s->cells);
-    s->stateCount = s->alloc<unsigned>(kStateBins);
-    s->stateOffset = s->alloc<unsigned>(kStateBins);
+    s->stateCount = s->alloc<unsigned>(kStateMaxBins);
+    s->stateOffset = s->alloc<unsigned>(kStateMaxBins);
     s->stateRange = s->alloc<unsigned long long>(4);
     s->blockMin = s->alloc<double>(size_t(s->cflBlocks) + 1);
     s->scalars = s->alloc<double>(4 * size_t(n) + 1);
@@ -848,7 +930,15 @@ static Step* createStep(const std::vector<Transport*>& list, HemHandle* hem) {
     s->boundary = s->alloc<double>(size_t(s->nv) * n);
     s->totals = s->alloc<double>(size_t(s->nv) * n);
     s->totalPartial = s->alloc<double2>(size_t(s->nv) * kTotalBlocks * n);
-    s->counters = s->alloc<unsigned long long>(8);
+    s->counters = s->alloc<unsigned long long>(16);
+    s->flashCounters = s->counters;
+    for (int z = 0; z < 3; ++z) {
+        s->stageStarts[z] = s->alloc<long long>(size_t(n) + 1);
+        s->stageBlocksZ[z] = long(startsZ[z].back());
+        RAMD_HIP(hipMemcpyAsync(s->stageStarts[z], startsZ[z].data(), (n + 1) * sizeof(long long),
+                                hipMemcpyHostToDevice, s->stream));
+    }
+    for (auto& e : s->tuneEvents) RAMD_HIP(hipEventCreate(&e));
     s->violations = s->alloc<unsigned long long>(size_t(n));
     s->boxes = s->alloc<BoxDev>(size_t(n));
     s->stageStart = s->alloc<long long>(size_t(n) + 1);
@@ -859,7 +949,7 @@ static Step* createStep(const std::vector<Transport*>& list, HemHandle* hem) {
     RAMD_HIP(hipMemcpyAsync(s->cflStart, cflStart.data(), (n + 1) * sizeof(long long), hipMemcpyHostToDevice,
                             s->stream));
     RAMD_HIP(hipMemsetAsync(s->bucketCount, 0, 256 * sizeof(unsigned), s->stream));
-    RAMD_HIP(hipMemsetAsync(s->stateCount, 0, kStateBins * sizeof(unsigned), s->stream));
+    RAMD_HIP(hipMemsetAsync(s->stateCount, 0, kStateMaxBins * sizeof(unsigned), s->stream));
     RAMD_HIP(hipMemsetAsync(s->violations, 0, n * sizeof(unsigned long long), s->stream));
     RAMD_HIP(hipStreamSynchronize(s->stream));
     return s.release();
@@ -928,7 +1018,8 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
                 "Invalid transport CFL controls");
         require(steps >= 1, "At least one step");
         const bool useGraph = (flashMode & REACTIVE_AMD_STEP_GRAPH) != 0;
-        flashMode &= ~REACTIVE_AMD_STEP_GRAPH;
+        const bool autotune = (flashMode & REACTIVE_AMD_STEP_AUTOTUNE) != 0;
+        flashMode &= ~(REACTIVE_AMD_STEP_GRAPH | REACTIVE_AMD_STEP_AUTOTUNE);
         require(flashMode == REACTIVE_AMD_FLASH_FP64 || flashMode == REACTIVE_AMD_FLASH_MIXED,
                 "Unknown flash arithmetic mode");
         if (flashMode == REACTIVE_AMD_FLASH_MIXED && !s.sets) {
@@ -937,6 +1028,9 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
             s.queueCount = s.alloc<unsigned>(1);
         }
         s.mode = flashMode;
+        s.stateSide = s.stateSideOf[s.mode];
+        s.flashBlock = s.flashBlockOf[s.mode];
+        if (autotune && !s.tunedOf[s.mode]) s.tune(cfl, maximum);
         hipLaunchKernelGGL(clearCountersKernel, dim3(1), dim3(64), 0, s.stream, s.counters, 8);
         hipLaunchKernelGGL(clearCountersKernel, dim3(blocksFor(size_t(s.count), 64)), dim3(64), 0, s.stream,
                            s.violations, s.count);
@@ -976,6 +1070,9 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         r.fluxFailures = counters[5];
         r.singlePhaseEvaluations = counters[6];
         r.queuedCells = counters[7];
+        r.tileDepth = uint32_t(s.tileZ);
+        r.flashBlock = s.flashBlock;
+        r.stateBins = uint32_t(s.stateSide);
         // a batch reports its smallest dt and recheck step and the sums over its
         // boxes (box order); reactive_amd_step_report_box gives each box
         r.dt = scalars[0];

@@ -148,7 +148,8 @@ class StepReport(C.Structure):
                 ("boundaryIntegral", C.c_double * 32), ("seconds", C.c_double), ("stageSeconds", C.c_double * 2),
                 ("flashSeconds", C.c_double * 2), ("cflSeconds", C.c_double), ("fluxFailures", C.c_uint64),
                 ("singlePhaseEvaluations", C.c_uint64), ("searchSeconds", C.c_double),
-                ("queuedCells", C.c_uint64)]
+                ("queuedCells", C.c_uint64), ("tileDepth", C.c_uint32), ("flashBlock", C.c_uint32),
+                ("stateBins", C.c_uint32), ("reserved", C.c_uint32)]
 
     @classmethod
     def fresh(cls):

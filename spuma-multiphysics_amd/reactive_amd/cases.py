@@ -70,9 +70,9 @@ class PureFluid:
     """Pure-species Peng-Robinson + NASA9 in numpy (textbook PR; same
     constants as the flash model image)."""
 
-    def __init__(self, name="N2O"):
+    def __init__(self, name="N2O", a_scale=1.0, b_scale=1.0):
         d = mechanism.SPECIES[name]
-        self.W = d["W"]; self.a = d["a"]; self.b = d["b"]
+        self.W = d["W"]; self.a = d["a"] * a_scale; self.b = d["b"] * b_scale
         self.kappa = mechanism.kappa(d["omega"])
         self.Tc = self.a * OMEGA_B / (self.b * OMEGA_A * R)
         self.ranges = np.array(d["ranges"]); self.coef = np.array(d["nasa9"])

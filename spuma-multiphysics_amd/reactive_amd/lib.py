@@ -56,6 +56,9 @@ _SIGNATURES = {
     "reactive_amd_step_read": (C.c_int, [P, P, P]),
     "reactive_amd_step_create_batch": (P, [C.POINTER(C.c_void_p), C.c_int, P, C.c_char_p, C.c_size_t]),
     "reactive_amd_step_report_box": (C.c_int, [P, C.c_int, C.POINTER(StepReport)]),
+    "reactive_amd_flash_adjoint": (C.c_int, [P, C.POINTER(C.c_void_p), C.c_double, P, C.c_int, P, P, C.c_size_t, P,
+                                             P, P, P, C.POINTER(C.c_uint64), C.c_char_p, C.c_size_t]),
+    "reactive_amd_transport_stage_adjoint": (C.c_int, [P, P, P, P, P, P, P, P]),
     "reactive_amd_step_error": (C.c_char_p, [P]),
     "reactive_amd_step_destroy": (None, [P]),
 }

@@ -40,15 +40,15 @@ assert SOLID.itemsize == 1104 and MODEL.itemsize == MODEL_BYTES
 assert MODEL.fields["stableGasPrune"][1] == 45004 and MODEL.fields["weights"][1] == 45008
 
 
-def _fill_table(table, names):
+def _fill_table(table, names, a_scale=1.0, b_scale=1.0):
     table["ns"] = len(names)
     table["refP"] = mechanism.REFERENCE_PRESSURE
     for k, name in enumerate(names):
         data = mechanism.SPECIES[name]
         sp = table["species"][k]
         sp["W"] = data["W"]
-        sp["a"] = data["a"]
-        sp["b"] = data["b"]
+        sp["a"] = data["a"] * a_scale
+        sp["b"] = data["b"] * b_scale
         sp["kappa"] = mechanism.kappa(data["omega"])
         ranges = data["ranges"]
         sp["nreg"] = len(data["nasa9"])
@@ -59,18 +59,19 @@ def _fill_table(table, names):
             sp["regions"][r]["coef"][:] = coef
     for i, ni in enumerate(names):
         for j, nj in enumerate(names):
-            table["aBinary"][i, j] = math.sqrt(mechanism.SPECIES[ni]["a"] * mechanism.SPECIES[nj]["a"])
+            table["aBinary"][i, j] = math.sqrt(mechanism.SPECIES[ni]["a"] * mechanism.SPECIES[nj]["a"]) * a_scale
 
 
 def build_model(gas_species=("N2O",), liquid="N2O", analytic_jacobian=0, stable_gas_prune=0,
-                scalar_recovery=1, limits=None):
+                scalar_recovery=1, limits=None, a_scale=1.0, b_scale=1.0):
     """Return the model image (numpy structured scalar) for a PR gas of
-    `gas_species` with one pure liquid of `liquid` (None for no liquid)."""
+    `gas_species` with one pure liquid of `liquid` (None for no liquid).
+    a_scale, b_scale multiply every species' PR a (and a_ij) and b."""
     lim = dict(mechanism.LIMITS)
     if limits:
         lim.update(limits)
     m = np.zeros((), dtype=MODEL)
-    _fill_table(m["phase"][0], list(gas_species))
+    _fill_table(m["phase"][0], list(gas_species), a_scale, b_scale)
     m["ns"] = len(gas_species)
     m["scalarRecovery"] = scalar_recovery
     m["analyticJacobian"] = analytic_jacobian
@@ -78,7 +79,7 @@ def build_model(gas_species=("N2O",), liquid="N2O", analytic_jacobian=0, stable_
     for k, name in enumerate(gas_species):
         m["weights"][k] = mechanism.SPECIES[name]["W"]
     if liquid is not None:
-        _fill_table(m["phase"][1], [liquid])
+        _fill_table(m["phase"][1], [liquid], a_scale, b_scale)
         m["nl"] = 1
         m["condensable"][0] = list(gas_species).index(liquid)
         m["liquidTmin"][0] = lim["liquidTmin"]

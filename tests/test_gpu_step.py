@@ -151,3 +151,41 @@ def test_graph_replays_match_stream_steps(mode):
         assert np.array_equal(qe, qg)
         assert se.tobytes() == sg.tobytes()
         assert dte == dtg and te == tg
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["fp64", "mixed"])
+def test_autotuned_shapes_leave_results_unchanged(mode):
+    """REACTIVE_AMD_STEP_AUTOTUNE times the candidate launch shapes on the
+    loaded state and keeps the fastest: the steps that follow equal untuned
+    steps bit for bit, the trials leave the loaded state as it was, and the
+    chosen shapes are candidates."""
+    lib = rlib.load()
+    case = cube_case(12, seed=83)
+    runs = {}
+    for tune in (False, True):
+        err = C.create_string_buffer(1024)
+        mb = case["model"]
+        hem = lib.reactive_gpu_hem_create_v1(mb, len(mb), len(case["q"]), err, 1024)
+        cfg = config(len(case["q"]), len(case["faces"]))
+        tr = lib.reactive_transport_create(1, C.byref(cfg), rlib.ptr(case["volume"]), rlib.ptr(case["faces"]), None,
+                                           None, None, None, err, 1024)
+        step = lib.reactive_amd_step_create(tr, hem, err, 1024)
+        assert step, err.value
+        assert lib.reactive_amd_step_load(step, rlib.ptr(case["q"]), rlib.ptr(case["states"])) == 0
+        rep = abi.StepReport.fresh()
+        flags = mode | (0x200 if tune else 0)
+        assert lib.reactive_amd_step_run(step, 0.4, 1.0, 2, flags, C.byref(rep)) == 0, lib.reactive_amd_step_error(step)
+        q = np.zeros_like(case["q"])
+        s = np.zeros(len(q), dtype=abi.STATE_DTYPE)
+        assert lib.reactive_amd_step_read(step, rlib.ptr(q), rlib.ptr(s)) == 0
+        runs[tune] = (q, s, rep.dt, (rep.tileDepth, rep.flashBlock, rep.stateBins))
+        lib.reactive_amd_step_destroy(step)
+        lib.reactive_transport_destroy(tr)
+        lib.reactive_gpu_hem_destroy_v1(hem)
+    print("\nuntuned shapes", runs[False][3], "tuned", runs[True][3])
+    assert runs[False][3] == (4, 64, 64)
+    depth, block, bins = runs[True][3]
+    assert depth in (2, 4, 8) and block in (64, 128, 256) and bins in (32, 64, 128)
+    assert runs[True][2] == runs[False][2]
+    assert np.array_equal(runs[True][0], runs[False][0])
+    assert runs[True][1].tobytes() == runs[False][1].tobytes()
