@@ -355,8 +355,9 @@ def main():
                              else "fp64 (plain, reference arithmetic)"),
                    "seed": a.seed, "two_phase_fraction": round(info["two_phase_fraction"], 4)},
         "launch_shapes": {"stage_tile_depth": int(rep.tileDepth), "flash_block": int(rep.flashBlock),
-                          "state_bins_per_axis": int(rep.stateBins), "fp64_flash_block": int(rep64.flashBlock),
-                          "fp64_state_bins_per_axis": int(rep64.stateBins), "source": "autotuned on the first step"},
+                          "state_bins_per_axis": int(rep.stateBins), "state_cost_levels": int(rep.reserved),
+                          "fp64_flash_block": int(rep64.flashBlock), "fp64_state_bins_per_axis": int(rep64.stateBins),
+                          "fp64_state_cost_levels": int(rep64.reserved), "source": "autotuned in the untimed steps"},
         "roofline": stage_roofline(rep, cells, a.steps, a.n),
         "roofline_flash": flash_roofline(rep, rep64, cells, a.steps, mixed),
         "fp64": fp64,

@@ -49,7 +49,7 @@ typedef struct ReactiveAmdStepReport {
     uint32_t tileDepth;                 /* launch shapes in use: cells per column of a stage workgroup, */
     uint32_t flashBlock;                /* lanes per flash workgroup, */
     uint32_t stateBins;                 /* state-order bins per axis (see REACTIVE_AMD_STEP_AUTOTUNE) */
-    uint32_t reserved;
+    uint32_t reserved;                  /* state-order cost levels per bin */
 } ReactiveAmdStepReport;
 
 /* Flash arithmetic of a step.  FP64: the reference's arithmetic.  MIXED: the
@@ -66,12 +66,14 @@ typedef struct ReactiveAmdStepReport {
  * report then holds totals and the call's device time only, no per-kernel
  * times. */
 #define REACTIVE_AMD_STEP_GRAPH 0x100
-/* OR-ed into flashMode: before its steps, the first run with this flag (per
- * flash arithmetic) times the candidate launch shapes on the loaded state and
- * keeps the fastest for the handle's later runs: stage chunk depth 2 / 4 / 8 cells, flash
- * workgroups of 64 / 128 / 256 lanes, 32 / 64 / 128 state-order bins per
- * axis (the choices are reported in ReactiveAmdStepReport; nothing the
- * trials compute is kept). */
+/* OR-ed into flashMode: the first run with this flag (per flash arithmetic)
+ * times the candidate launch shapes on the current state, before its steps
+ * when a previous flash's costs are known and otherwise after its first step,
+ * and keeps the fastest for the handle's later steps: stage chunk depth 2 / 4
+ * / 8 cells, flash workgroups of 64 / 128 / 256 lanes, state-order bins of
+ * 32, 64 or 128 per axis, or 64 per axis split into 4 levels of the previous
+ * flash's cost (the choices are reported in ReactiveAmdStepReport, the cost
+ * levels in its `reserved` field; nothing the trials compute is kept). */
 #define REACTIVE_AMD_STEP_AUTOTUNE 0x200
 
 /* Face flux of a transport handle (applies to reactive_transport_rhs/stage,

@@ -24,6 +24,10 @@
 #define REACTIVE_AMD_FLASH_DEVICE_H
 #include "pr_device.h"
 
+// The FP32 search's phase-state functions inlined into their callers (knob).
+#ifndef RAMD_SEARCH_INLINE
+#define RAMD_SEARCH_INLINE 0
+#endif
 #ifndef RAMD_NOINLINE
 #define RAMD_NOINLINE __device__ __noinline__
 #endif
@@ -259,7 +263,8 @@ struct Flash {
     }
 
     // Gas PR state at (T, p) on the gas root (evaluateTPTrusted, reactiveDevicePR.h:574-609).
-    RAMD_NOINLINE int gasTP(R T, R p, const R* Y, S& s, R* mu) {
+    RAMD_NOINLINE int gasTP(R T, R p, const R* Y, S& s, R* mu) { return gasTPBody(T, p, Y, s, mu); }
+    __device__ __attribute__((always_inline)) inline int gasTPBody(R T, R p, const R* Y, S& s, R* mu) {
         if (!fin(T) || !fin(p) || T <= 0 || p <= 0) return kBadInput;
         R x[NS], W = 0;
         int st = massToMole(m.gas, Y, x, W);
@@ -281,7 +286,8 @@ struct Flash {
         return kOk;
     }
     // Pure-liquid PR state at (T, p) on the liquid root.
-    RAMD_NOINLINE int liquidTP(R T, R p, S& s, R* mu) {
+    RAMD_NOINLINE int liquidTP(R T, R p, S& s, R* mu) { return liquidTPBody(T, p, s, mu); }
+    __device__ __attribute__((always_inline)) inline int liquidTPBody(R T, R p, S& s, R* mu) {
         if (!fin(T) || !fin(p) || T <= 0 || p <= 0) return kBadInput;
         const R Y[1] = {R(1)};
         R x[1], W = 0;
@@ -313,6 +319,10 @@ struct Flash {
         if (index < 0 && m.enforceBounds)
             for (int k = 0; k < NS; ++k)
                 if (Y[k] > 0 && (T < R(m.gas.tlo[k]) || T > R(m.gas.thi[k]))) return kBadInput;
+        if constexpr (P::single && RAMD_SEARCH_INLINE) {  // the FP32 search: phase states inlined
+            if (index >= 0) return liquidTPBody(T, p, out, mu);
+            return gasTPBody(T, p, Y, out, mu);
+        }
         if (index >= 0) return liquidTP(T, p, out, mu);
         return gasTP(T, p, Y, out, mu);
     }
@@ -631,7 +641,14 @@ struct Flash {
     RAMD_NOINLINE bool activeFinish(const R* x, R scale, const R* f, E& value) {
         const R rho = density();
         R jac[3][3];
-        if (!jacobian(x, scale, f, jac)) return false;
+        if constexpr (A && NS == 1 && !P::single) {
+            // mixed refinement: the exact (analytic) Jacobian at the converged
+            // point, where the reference differences the residual (its c_eq
+            // carries the differencing error, a few 1e-7)
+            analyticJacobian(value, scale, jac);
+        } else {
+            if (!jacobian(x, scale, f, jac)) return false;
+        }
         R rhs[3] = {-R(1) / rho, value.p / (rho * scale), R(0)}, response[3];
         if (!solveN<3>(jac, rhs, response)) return false;
         const R squared = value.p * response[0];
@@ -715,11 +732,15 @@ struct Flash {
         return activeNewton(x, scale, value, 0);
     }
     // Stability (reactiveDeviceFlash.h:449-485) for at most one liquid.
-    RAMD_NOINLINE bool stable(E& value) {
+    // evaluated: value itself came from the evaluation the test repeats (an
+    // active-set candidate: same liquid mass, p, T, virtual liquids), so its
+    // chemical potentials are the comparison's, bit for bit
+    RAMD_NOINLINE bool stable(E& value, bool evaluated = false) {
         const R p = value.p, T = value.T;
         if (value.gasMass > 0) {
             E comparison;
-            if (!evaluate(value.liqMass, p, T, comparison, true)) return false;
+            if (evaluated) comparison = value;
+            else if (!evaluate(value.liqMass, p, T, comparison, true)) return false;
             if (!NL || q[m.condensable] == 0) return true;
             const R affinity = comparison.muLiquid - comparison.muGas;
             if (value.liqMass > 0) {
@@ -750,8 +771,8 @@ struct Flash {
         }
         return tangent >= -mutol;
     }
-    RD void consider(E& candidate, E& best, bool& have) {  // :486-489
-        if (stable(candidate)) {
+    RD void consider(E& candidate, E& best, bool& have, bool evaluated = false) {  // :486-489
+        if (stable(candidate, evaluated)) {
             ++n.stable;
             if (!have || candidate.entropyDensity > best.entropyDensity) { best = candidate; have = true; }
         } else {
@@ -996,7 +1017,7 @@ struct Flash {
             ++i;
         }
         for (; i < set.count && set.c[i].kind == kSeedCandidate; ++i) {
-            if (finishActive(set.c[i], candidate)) consider(candidate, best, have);
+            if (finishActive(set.c[i], candidate)) consider(candidate, best, have, true);
             else ++n.failures;
         }
         if (!have && q[c] > 0 && restartRange()) {
@@ -1013,7 +1034,7 @@ struct Flash {
                     }
             } else {
                 for (; i < set.count; ++i) {
-                    if (finishActive(set.c[i], candidate)) consider(candidate, best, have);
+                    if (finishActive(set.c[i], candidate)) consider(candidate, best, have, true);
                     else ++n.failures;
                 }
             }

@@ -291,12 +291,16 @@ __device__ inline unsigned stateBin(double x, double lo, double hi, int side) {
     return t <= 0 ? 0u : (t >= side - 1 ? unsigned(side - 1) : unsigned(t));
 }
 // bin of every cell (key) and the bin counts; grid-stride over kStateBlocks blocks
+// With a cost array (levels > 1), each state bin is split by the cell's
+// phase-evaluation count in its previous flash: level = min(levels - 1,
+// floor(log2(cost / 32))), so cells of one bin with very different work do
+// not share waves.
 __global__ __launch_bounds__(256) void stateKeyKernel(const double* __restrict__ q, size_t stride, int ns,
                                                       size_t cells, const unsigned long long* __restrict__ range,
-                                                      int side, uint16_t* __restrict__ key,
-                                                      unsigned* __restrict__ count) {
+                                                      int side, const uint16_t* __restrict__ cost, int levels,
+                                                      uint16_t* __restrict__ key, unsigned* __restrict__ count) {
     __shared__ unsigned hist[kStateMaxBins];
-    const int bins = side * side;
+    const int bins = side * side * levels;
     for (int b = threadIdx.x; b < bins; b += blockDim.x) hist[b] = 0;
     __syncthreads();
     const double lo0 = fromOrdered(range[0]), hi0 = fromOrdered(range[1]);
@@ -304,7 +308,12 @@ __global__ __launch_bounds__(256) void stateKeyKernel(const double* __restrict__
     for (size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x; c < cells; c += size_t(gridDim.x) * blockDim.x) {
         double lr, e;
         cellState(q, stride, ns, c, lr, e);
-        const unsigned k = stateBin(lr, lo0, hi0, side) * side + stateBin(e, lo1, hi1, side);
+        unsigned k = stateBin(lr, lo0, hi0, side) * side + stateBin(e, lo1, hi1, side);
+        if (levels > 1) {
+            unsigned level = 0;
+            for (unsigned w = cost[c] >> 5; w > 1 && level + 1 < unsigned(levels); w >>= 1) ++level;
+            k = k * unsigned(levels) + level;
+        }
         key[c] = uint16_t(k);
         atomicAdd(&hist[k], 1u);
     }
@@ -497,7 +506,7 @@ inline unsigned blocksFor(size_t n, unsigned b) { return unsigned((n + b - 1) / 
 #ifndef RAMD_REFINE_QUEUE
 #define RAMD_REFINE_QUEUE 1
 #endif
-constexpr unsigned kQueueBlocks = 2048;  // persistent grid of the queue kernel: 8 waves per CU
+constexpr unsigned kQueueBlocks = 256;  // persistent grid of the queue kernel: one wave per CU
 constexpr int kMarks = 11;  // events per step
 
 // ---------------------------------------------------------------- handle
@@ -518,6 +527,8 @@ struct Step {
     // workgroup size, state-order bins per axis; one stage table per depth
     int tileZ = kChunkZ, stateSide = 64;
     unsigned flashBlock = 64;
+    int stateLevels = 1;                  // cost levels per state bin (1: state only)
+    int stateLevelsOf[2] = {1, 1};
     int stateSideOf[2] = {64, 64};        // per flash arithmetic (FP64, mixed)
     unsigned flashBlockOf[2] = {64, 64};
     bool tunedOf[2] = {false, false};
@@ -662,14 +673,18 @@ struct Step {
 
     // Order of the cells of q by thermodynamic state (into out).
     void stateOrder(const double* q, uint32_t* out) {
+        // cost levels: the previous flash's search (mixed) or FP64 phase counts
+        const uint16_t* levelCost = mode == REACTIVE_AMD_FLASH_MIXED ? (haveCostSearch ? costSearch : nullptr)
+                                                                       : (haveCost ? cost : nullptr);
+        const int levels = levelCost && stateLevels > 1 ? stateLevels : 1;
         hipLaunchKernelGGL(stateRangeInitKernel, dim3(1), dim3(64), 0, stream, stateRange);
         hipLaunchKernelGGL(stateRangeKernel, dim3(kStateBlocks), dim3(256), 0, stream, q, all, ns, cells, stateRange);
+        const int bins = stateSide * stateSide * levels;
         hipLaunchKernelGGL(stateKeyKernel, dim3(kStateBlocks), dim3(256), 0, stream, q, all, ns, cells, stateRange,
-                           stateSide, stateKey, stateCount);
-        hipLaunchKernelGGL(stateScanKernel, dim3(1), dim3(256), 0, stream, stateCount, stateOffset,
-                           stateSide * stateSide);
-        hipLaunchKernelGGL(stateScatterKernel, dim3(kStateBlocks), dim3(256), 0, stream, stateKey, cells,
-                           stateSide * stateSide, stateOffset, out);
+                           stateSide, levelCost, levels, stateKey, stateCount);
+        hipLaunchKernelGGL(stateScanKernel, dim3(1), dim3(256), 0, stream, stateCount, stateOffset, bins);
+        hipLaunchKernelGGL(stateScatterKernel, dim3(kStateBlocks), dim3(256), 0, stream, stateKey, cells, bins,
+                           stateOffset, out);
     }
 
     // Autotuner: times each candidate launch shape on the loaded state (the
@@ -704,14 +719,20 @@ struct Step {
         if (mode == REACTIVE_AMD_FLASH_MIXED) sortCells(haveCostSearch ? costSearch : nullptr, orderSearch);
 #endif
         auto trial = [&] { flash(qA, pB, cB, rB, sB, nullptr, flashCounters, nullptr); };
-        int bestSide = stateSide;
+        int bestSide = stateSide, bestLevels = 1;
         flashBlock = 64;
-        for (int side : kStateSides) {
-            stateSide = side;
+        // (bins per axis, cost levels): cost levels need a previous flash's costs
+        const int candidates[4][2] = {{32, 1}, {64, 1}, {128, 1}, {64, 4}};
+        const bool costKnown = mode == REACTIVE_AMD_FLASH_MIXED ? haveCostSearch : haveCost;
+        for (int k = 0; k < 4; ++k) {
+            if (candidates[k][1] > 1 && !costKnown) continue;
+            stateSide = candidates[k][0];
+            stateLevels = candidates[k][1];
             const float ms = timeIt(trial, 1);
-            if (side == kStateSides[0] || ms < best) { best = ms; bestSide = side; }
+            if (k == 0 || ms < best) { best = ms; bestSide = stateSide; bestLevels = stateLevels; }
         }
         stateSide = bestSide;
+        stateLevels = bestLevels;
         unsigned bestBlock = 64;
         for (unsigned b : {64u, 128u, 256u}) {
             flashBlock = b;
@@ -723,6 +744,7 @@ struct Step {
         for (auto& g : graph)  // captured with the old shapes
             if (g) { RAMD_HIP(hipGraphExecDestroy(g)); g = nullptr; }
         stateSideOf[mode] = stateSide;
+        stateLevelsOf[mode] = stateLevels;
         flashBlockOf[mode] = flashBlock;
         tunedOf[mode] = true;
     }
@@ -1029,8 +1051,16 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         }
         s.mode = flashMode;
         s.stateSide = s.stateSideOf[s.mode];
+        s.stateLevels = s.stateLevelsOf[s.mode];
         s.flashBlock = s.flashBlockOf[s.mode];
-        if (autotune && !s.tunedOf[s.mode]) s.tune(cfl, maximum);
+        // tuning wants the costs of a previous flash (for the cost-level
+        // candidates): without them it runs after this call's first step
+        bool pendingTune = autotune && !s.tunedOf[s.mode];
+        const bool costKnown = s.mode == REACTIVE_AMD_FLASH_MIXED ? s.haveCostSearch : s.haveCost;
+        if (pendingTune && costKnown) {
+            s.tune(cfl, maximum);
+            pendingTune = false;
+        }
         hipLaunchKernelGGL(clearCountersKernel, dim3(1), dim3(64), 0, s.stream, s.counters, 8);
         hipLaunchKernelGGL(clearCountersKernel, dim3(blocksFor(size_t(s.count), 64)), dim3(64), 0, s.stream,
                            s.violations, s.count);
@@ -1043,6 +1073,10 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         for (int i = 0; i < steps; ++i) {
             if (useGraph) graphStep(s, cfl, maximum);
             else s.one(cfl, maximum, s.marks.data() + size_t(i) * kMarks);
+            if (pendingTune) {
+                s.tune(cfl, maximum);
+                pendingTune = false;
+            }
         }
         RAMD_HIP(hipEventRecord(s.events[1], s.stream));
         unsigned long long counters[8];
@@ -1073,6 +1107,7 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         r.tileDepth = uint32_t(s.tileZ);
         r.flashBlock = s.flashBlock;
         r.stateBins = uint32_t(s.stateSide);
+        r.reserved = uint32_t(s.stateLevels);
         // a batch reports its smallest dt and recheck step and the sums over its
         // boxes (box order); reactive_amd_step_report_box gives each box
         r.dt = scalars[0];
