@@ -461,7 +461,10 @@ struct Flash {
     RD R energyScale(const E& v) const { return hi(R(1e5) * v.rho, v.cpDensity * v.T); }
 
     // Single-phase gas energy residual at fixed rho (reactiveDeviceFlash.h:264-275).
-    RAMD_NOINLINE bool gasProbe(R T, R& F, R& cv, R& p, R* cp) {
+    // check = false skips the gas-root consistency test (the refinement's
+    // polish, whose final evaluation at (p, T) repeats it through the volume
+    // residual)
+    RAMD_NOINLINE bool gasProbe(R T, R& F, R& cv, R& p, R* cp, bool check = true) {
         const R rho = density();
         R Y[NS];
         for (int k = 0; k < NS; ++k) Y[k] = q[k] / rho;
@@ -479,9 +482,11 @@ struct Flash {
         const R gasP = prop.p;
         p = gasP + R(0) * R(0);
         if (!bounds(p, T)) return false;
-        S gas;
-        if (phaseStatus(-1, gasP, T, Y, gas, nullptr) != kOk || ab(gas.rho - rho) > R(P::densityCheck) * rho)
-            return false;
+        if (check) {
+            S gas;
+            if (phaseStatus(-1, gasP, T, Y, gas, nullptr) != kOk || ab(gas.rho - rho) > R(P::densityCheck) * rho)
+                return false;
+        }
         F = rho * prop.e - energy;
         cv = rho * prop.cv;
         if (cp) *cp = prop.cp;
@@ -951,19 +956,21 @@ struct Flash {
     // FP64, or whose set overflowed, run the plain FP64 search.
     // Gas candidate of a scalar-recovery model: Newton in T at fixed density
     // from the FP32 temperature, to 1e-3 of the energy tolerance, with the
-    // scalar recovery's acceptance test (reactiveDeviceFlash.h:276-299).
+    // scalar recovery's acceptance test (reactiveDeviceFlash.h:276-299).  The
+    // probes skip the gas-root test: the final evaluation at (p, T) takes the
+    // gas root there, and its volume residual rejects a root of another density.
     RAMD_NOINLINE bool gasPolish(R T, E& result) {
         const R rho = density();
         T = clamp(T, Tmin, Tmax);
         R F, cv, p, cp;
-        if (!gasProbe(T, F, cv, p, &cp)) return false;
+        if (!gasProbe(T, F, cv, p, &cp, false)) return false;
         const R scale = hi(R(1e5) * rho, rho * cp * T);
         int it = 0;
         for (; it < 8 && ab(F) > R(1e-3) * etol * scale; ++it) {
             const R next = T - F / cv;
             if (!fin(next) || next < Tmin || next > Tmax) return false;
             T = next;
-            if (!gasProbe(T, F, cv, p, nullptr)) return false;
+            if (!gasProbe(T, F, cv, p, nullptr, false)) return false;
         }
         if (ab(F) > etol * scale) return false;
         if (!evaluate(R(0), p, T, result, false)) return false;
@@ -972,8 +979,18 @@ struct Flash {
         result.iterations = it;
         return result.volRes <= vtol && result.enRes <= etol;
     }
+    // A frozen candidate from its single-precision point: the gas candidate
+    // of a scalar-recovery model by the polish above, any other by the
+    // frozen solve started there.  When that fails, the candidate is solved
+    // from the cell's own guess, as the reference starts.  (A failed polish
+    // goes straight to that solve: on a 160^3 box, every cell whose polish
+    // failed also failed the frozen solve from the single-precision point,
+    // which cost about 700 phase evaluations each.)
     RAMD_NOINLINE bool finishFrozen(R mass0, const Candidate& c, E& value, int stallLimit) {
-        if (mass0 == 0 && m.scalarRecovery && gasPolish(m_exp(R(c.x[1])), value)) return true;
+        if (mass0 == 0 && m.scalarRecovery) {
+            if (gasPolish(m_exp(R(c.x[1])), value)) return true;
+            return frozen(mass0, value, stallLimit);
+        }
         const R p0 = guessP, T0 = guessT;
         guessP = m_exp(R(c.x[0]));
         guessT = m_exp(R(c.x[1]));

@@ -223,7 +223,6 @@ __global__ __launch_bounds__(256) void bucketScatterKernel(const uint8_t* __rest
 #define RAMD_STATE_ORDER 1
 #endif
 // side: bins per axis, 32, 64 (default) or 128, chosen by the autotuner
-constexpr int kStateSides[3] = {32, 64, 128};
 constexpr int kStateMaxBins = 128 * 128, kStateBlocks = 1024;
 
 __device__ inline unsigned long long orderedBits(double x) {
