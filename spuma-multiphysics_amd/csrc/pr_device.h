@@ -197,6 +197,15 @@ RHD bool isolatedRoot(R center, R delta, R delta2, R q, R bn, R cn, R dn, R b, R
     return true;
 }
 
+// Single precision: a discriminant within this fraction of q^2 + h^2 (a
+// near-repeated root pair) is classified in double precision.  The FP32
+// rounding of the discriminant is a few 1e-7 of q^2 + h^2.  On the seeded
+// 160^3 box 17 % of the search's cubic solves fell within 1e-4 (a third of
+// the wave-level solves had such a lane); 1e-5 makes the search 3 % faster
+// with the same results (no flips, same distance to the converged roots).
+#ifndef RAMD_CUBIC_FP64_BAND
+#define RAMD_CUBIC_FP64_BAND 1e-5
+#endif
 template <class R>
 struct RootsT { R v0, vLast; int count, signedCount, gas, liquid; };
 using Roots = RootsT<double>;
@@ -223,7 +232,7 @@ RHD int roots(R T, R p, R a, R b, R aa, RootsT<R>& out) {
     const R delta = delta2 > 0 ? m_sqrt(delta2) : R(0);
     const R h = R(2) * delta * delta2;
     R disc = q * q - h * h;
-    if (sizeof(R) < sizeof(double) && ab(disc) <= R(1e-4) * (q * q + h * h)) {
+    if (sizeof(R) < sizeof(double) && ab(disc) <= R(RAMD_CUBIC_FP64_BAND) * (q * q + h * h)) {
         RootsT<double> d;
         const int st = roots<double>(double(T), double(p), double(a), double(b), double(aa), d);
         out.v0 = R(d.v0); out.vLast = R(d.vLast);

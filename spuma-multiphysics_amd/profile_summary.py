@@ -5,6 +5,8 @@
       per-kernel calls, total and average duration (the --stats view)
   python3 spuma-multiphysics_amd/profile_summary.py pmc DB [KERNEL_SUBSTRING]
       per-kernel mean of every collected counter per dispatch
+  python3 spuma-multiphysics_amd/profile_summary.py dispatches DB KERNEL_SUBSTRING
+      every dispatch of the matching kernels in launch order: grid, workgroup, duration
 """
 import sqlite3
 import sys
@@ -42,8 +44,21 @@ def pmc(db, pattern=None):
             print(f"  {c:28s} mean {sum(vals) / len(vals):16.1f}   per dispatch {[round(v, 1) for v in vals[:6]]}")
 
 
+def dispatches(db, pattern):
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, dispatch_id, grid_x, workgroup_x, duration from kernels order by start").fetchall()
+    print(f"{'#':>4s} {'dispatch':>8s} {'grid':>10s} {'wg':>5s} {'us':>12s}  kernel")
+    k = 0
+    for name, did, grid, wg, d in rows:
+        if pattern in name:
+            print(f"{k:4d} {did:8d} {grid:10d} {wg:5d} {d / 1e3:12.3f}  {name.split('(')[0]}")
+            k += 1
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "trace":
         trace(sys.argv[2])
+    elif sys.argv[1] == "dispatches":
+        dispatches(sys.argv[2], sys.argv[3])
     else:
         pmc(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else None)
