@@ -13,9 +13,14 @@ the plain FP64 path is timed on the same case for the comparison, and the
 two are compared cell by cell after one step from the same states.
 
 Prints one JSON line (the bench contract) with:
-  roofline       fused flux sweep (both SSPRK2 stages): algorithmic bytes per
+  roofline       the dominant kernel of the step (the mixed flash's FP32
+                 candidate search, about two thirds of the step): VALU FLOPs
+                 (phase evaluations x FLOPs per evaluation) / event-timed
+                 launch duration vs the 157.3 TFLOP/s FP32 vector peak, and
+                 its measured HBM traffic per launch
+  roofline_stage fused flux sweep (both SSPRK2 stages): algorithmic bytes per
                  cell per stage / event-timed kernel duration vs 8 TB/s HBM
-  roofline_flash the flash kernels: time per launch, phase evaluations
+  roofline_flash all flash kernels: time per launch, phase evaluations
   fp64           the plain FP64 path on the same case
   mixed_vs_fp64  flips and relative T, p, c_f errors after one step
   cpu_baseline   the reference's own CPU code (oracle/_ref) on a bounded sample
@@ -193,10 +198,14 @@ def stage_roofline(rep, cells, steps, n, nv=5):
             "avg_launch_ms": [1e3 * rep.stageSeconds[0] / steps, 1e3 * rep.stageSeconds[1] / steps]}
 
 
+# HBM bytes per launch of the FP32 search at 160^3 pure N2O (the bench's
+# seeded box, mixed step), from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled as
+# below: profiles/stepSearch160_traffic.txt
+SEARCH_TRAFFIC_160 = 363.7e9
 # HBM bytes per launch of the fused stage kernel at 160^3 pure N2O, stage 0 and
 # stage 1, from rocprofv3 FETCH_SIZE and WRITE_SIZE (each in its own run,
 # scaled by a plain copy kernel's counts): profiles/cartesianStage160_traffic.txt
-STAGE_TRAFFIC_160 = [469.8e6, 706.5e6]
+STAGE_TRAFFIC_160 = [473.1e6, 708.0e6]
 TIGHT = {"vtol": 1e-12, "etol": 1e-12, "mutol": 1e-10}
 AUTOTUNE = 0x200  # REACTIVE_AMD_STEP_AUTOTUNE
 # VALU FLOPs per phase evaluation (ADD, MUL, TRANS = 1, FMA = 2), measured with
@@ -268,6 +277,17 @@ def flash_roofline(rep, rep64, cells, steps, mixed):
         out["avg_launch_ms"] = 1e3 * t64
         out["share_of_step"] = (rep64.flashSeconds[0] + rep64.flashSeconds[1]) / rep64.seconds
     return out
+
+
+def search_roofline(flash, rep, steps, n):
+    """The dominant kernel: the FP32 candidate search of the mixed flash."""
+    s = flash["fp32_search"]
+    return {"bound": "valu", "kernel": "stepSearchKernel (FP32 candidate search of the mixed flash)",
+            "achieved": s["achieved"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": s["frac"],
+            "traffic": SEARCH_TRAFFIC_160 if n == 160 else None, "avg_launch_ms": s["avg_launch_ms"],
+            "flop_per_phase_evaluation": FLOP_PER_PHASE["fp32_search"],
+            "phase_evaluations_per_cell": s["phase_evaluations_per_cell"],
+            "share_of_step": rep.searchSeconds / rep.seconds}
 
 
 def census(states0, s64, smx):
@@ -345,6 +365,7 @@ def main():
     value = cells * a.steps / wall
     t_flash = (rep.flashSeconds[0] + rep.flashSeconds[1]) / (2 * a.steps)
     mixed = a.flash == "mixed"
+    flash = flash_roofline(rep, rep64, cells, a.steps, mixed)
     result = {
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
@@ -358,8 +379,10 @@ def main():
                           "state_bins_per_axis": int(rep.stateBins), "state_cost_levels": int(rep.reserved),
                           "fp64_flash_block": int(rep64.flashBlock), "fp64_state_bins_per_axis": int(rep64.stateBins),
                           "fp64_state_cost_levels": int(rep64.reserved), "source": "autotuned in the untimed steps"},
-        "roofline": stage_roofline(rep, cells, a.steps, a.n),
-        "roofline_flash": flash_roofline(rep, rep64, cells, a.steps, mixed),
+        "roofline": (search_roofline(flash, rep, a.steps, a.n) if mixed
+                     else dict(flash["fp64_flash"], bound="valu", unit="TFLOP/s")),
+        "roofline_stage": stage_roofline(rep, cells, a.steps, a.n),
+        "roofline_flash": flash,
         "fp64": fp64,
         "speedup_vs_fp64": value / value64,
         "mixed_vs_fp64": accuracy,

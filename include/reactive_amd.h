@@ -50,6 +50,7 @@ typedef struct ReactiveAmdStepReport {
     uint32_t flashBlock;                /* lanes per flash workgroup, */
     uint32_t stateBins;                 /* state-order bins per axis (see REACTIVE_AMD_STEP_AUTOTUNE) */
     uint32_t reserved;                  /* state-order cost levels per bin */
+    uint64_t retries;                   /* REACTIVE_AMD_STEP_RETRY: attempts rolled back and repeated */
 } ReactiveAmdStepReport;
 
 /* Flash arithmetic of a step.  FP64: the reference's arithmetic.  MIXED: the
@@ -75,6 +76,17 @@ typedef struct ReactiveAmdStepReport {
  * flash's cost (the choices are reported in ReactiveAmdStepReport, the cost
  * levels in its `reserved` field; nothing the trials compute is kept). */
 #define REACTIVE_AMD_STEP_AUTOTUNE 0x200
+/* OR-ed into flashMode: the driver's attempt loop around each step
+ * (ReactiveFoam.C:1443-1494).  The step-start q, p, c_frozen and states are
+ * kept; an attempt in which any cell's flash fails or the post-stage-0 CFL
+ * recheck fails is rolled back and repeated with half its dt, at most 12
+ * times and while dt / 2 > 1e-15 (then the call fails with the step-start
+ * state restored).  In a batch each case retries with its own halved dt (the
+ * cases that passed repeat their step unchanged, bit for bit).  Steps run on
+ * the stream (not with REACTIVE_AMD_STEP_GRAPH); the boundary-corrected
+ * conservation check of the driver stays with the caller, from the report's
+ * totals and boundary integral. */
+#define REACTIVE_AMD_STEP_RETRY 0x400
 
 /* Face flux of a transport handle (applies to reactive_transport_rhs/stage,
  * the resident stages and the resident step): REACTIVE_AMD_FLUX_HLL (the

@@ -18,6 +18,7 @@
 // Cells are flashed in heavy-first bucket order built on the device.
 #include <hip/hip_runtime.h>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -44,7 +45,8 @@ __global__ __launch_bounds__(256, RAMD_FLASH_MIN_WAVES) void stepFlashKernel(con
                                                       const double* __restrict__ dtDev, ThermoOut* __restrict__ out,
                                                       double* __restrict__ pOut, double* __restrict__ cOut,
                                                       double* __restrict__ rOut, uint16_t* __restrict__ cost,
-                                                      unsigned long long* __restrict__ counters) {
+                                                      unsigned long long* __restrict__ counters,
+                                                      uint8_t* __restrict__ failMark) {
     const HemModel<NS>& model = workgroupModel(globalModel);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Counts n = {0, 0, 0, 0, 0, 0};
@@ -58,6 +60,7 @@ __global__ __launch_bounds__(256, RAMD_FLASH_MIN_WAVES) void stepFlashKernel(con
         zeroEval(result);
         const bool ok = f.run(result);
         storeCell(ok, result, g, c, out, pOut, cOut, rOut);
+        if (!ok && failMark) failMark[c] = 1;
         failed = ok ? 0u : 1u;
         if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
     }
@@ -76,7 +79,8 @@ __global__ __launch_bounds__(256, RAMD_FLASH_MIN_WAVES) void stepRefineKernel(co
                                                        double* __restrict__ cOut, double* __restrict__ rOut,
                                                        uint16_t* __restrict__ cost,
                                                        unsigned long long* __restrict__ counters,
-                                                       uint32_t* __restrict__ queue, unsigned* __restrict__ queueCount) {
+                                                       uint32_t* __restrict__ queue, unsigned* __restrict__ queueCount,
+                                                       uint8_t* __restrict__ failMark) {
     const HemModel<NS>& model = workgroupModel(globalModel);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Counts n = {0, 0, 0, 0, 0, 0};
@@ -94,6 +98,7 @@ __global__ __launch_bounds__(256, RAMD_FLASH_MIN_WAVES) void stepRefineKernel(co
             queue[atomicAdd(queueCount, 1u)] = c;
         } else {
             storeCell(ok, result, g, c, out, pOut, cOut, rOut);
+            if (!ok && failMark) failMark[c] = 1;
             failed = ok ? 0u : 1u;
             if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
         }
@@ -112,7 +117,7 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void stepRefineQueueKerne
     const ThermoOut* __restrict__ seeds, const uint32_t* __restrict__ queue, const unsigned* __restrict__ queueCount,
     const double* __restrict__ dtDev, const CandidateSet* __restrict__ sets, ThermoOut* __restrict__ out,
     double* __restrict__ pOut, double* __restrict__ cOut, double* __restrict__ rOut, uint16_t* __restrict__ cost,
-    unsigned long long* __restrict__ counters) {
+    unsigned long long* __restrict__ counters, uint8_t* __restrict__ failMark) {
     const unsigned count = *queueCount;
     if (blockIdx.x * 64u >= count) return;  // uniform per workgroup: usually every workgroup leaves here
     const HemModel<NS>& model = workgroupModel(globalModel);
@@ -129,6 +134,7 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void stepRefineQueueKerne
             zeroEval(result);
             const bool ok = f.refine(sets[c], result);
             storeCell(ok, result, g, c, out, pOut, cOut, rOut);
+            if (!ok && failMark) failMark[c] = 1;
             failed = ok ? 0u : 1u;
             if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
         }
@@ -424,9 +430,10 @@ __global__ void totalsFinishKernel(const double2* __restrict__ partial, int bloc
 }
 
 // The CFL minimum of box blockIdx.x over its blocks [start[i], start[i + 1]):
-// scalars[4 i + slot] (-1 if invalid).
+// scalars[4 i + slot] (-1 if invalid), at most limit[i] when limit is given
+// (a repeated attempt's halved dt).
 __global__ void batchFinishCflKernel(const double* __restrict__ blockMin, const long long* __restrict__ start,
-                                     double* __restrict__ scalars, int slot) {
+                                     double* __restrict__ scalars, int slot, const double* __restrict__ limit) {
     const int i = blockIdx.x;
     double v = __builtin_huge_val();
     for (long long b = start[i] + threadIdx.x; b < start[i + 1]; b += blockDim.x) v = blockMin[b] < v ? blockMin[b] : v;
@@ -437,6 +444,7 @@ __global__ void batchFinishCflKernel(const double* __restrict__ blockMin, const 
     if (threadIdx.x == 0) {
         double m = lane[0];
         for (int w = 1; w < int(blockDim.x >> 6); ++w) m = lane[w] < m ? lane[w] : m;
+        if (limit && m > 0 && limit[i] < m) m = limit[i];
         scalars[4 * i + slot] = isfinite(m) && m > 0 ? m : -1;
     }
 }
@@ -461,6 +469,17 @@ __global__ void recheckKernel(double* __restrict__ scalars, int count, unsigned 
         atomicAdd(&counters[4], 1ull);
         if (violations) atomicAdd(&violations[i], 1ull);
     }
+}
+// Cells of box blockIdx.y whose flash failed in the attempt (failMark) into boxFail[box].
+__global__ __launch_bounds__(256) void boxFailKernel(const BoxDev* __restrict__ boxes,
+                                                     const uint8_t* __restrict__ failMark,
+                                                     unsigned long long* __restrict__ boxFail) {
+    const BoxDev& b = boxes[blockIdx.y];
+    uint32_t n = 0;
+    for (size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x; c < b.cells; c += size_t(gridDim.x) * blockDim.x)
+        n += failMark[b.offset + c];
+    n = waveSum(n);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(&boxFail[blockIdx.y], (unsigned long long)n);
 }
 // thread (i, k) = (blockIdx.x, threadIdx.x)
 __global__ void boundaryIntegralKernel(const double* __restrict__ bA, const double* __restrict__ bB,
@@ -535,6 +554,15 @@ struct Step {
     long stageBlocksZ[3] = {0, 0, 0};
     hipEvent_t tuneEvents[2] = {};
     unsigned long long* violations = nullptr;  // per box: failed CFL rechecks
+    // attempts (REACTIVE_AMD_STEP_RETRY): the step-start q, p, c_frozen, r and
+    // states, per-cell flash failure marks, per-box failed cells and dt limits
+    double *qKeep = nullptr, *pKeep = nullptr, *cKeep = nullptr, *rKeep = nullptr;
+    ThermoOut* sKeep = nullptr;
+    uint8_t* failMark = nullptr;  // non-null while an attempt records its failed cells
+    uint8_t* failMarks = nullptr;
+    unsigned long long* boxFail = nullptr;
+    double* limit = nullptr;      // per box, applied to the step's dt while non-null
+    double* limits = nullptr;
     std::vector<void*> allocations;
     double *qA = nullptr, *qB = nullptr, *pA = nullptr, *cA = nullptr, *pB = nullptr, *cB = nullptr;
     double *rA = nullptr, *rB = nullptr;  // recovered densities (read by the HLLC flux)
@@ -617,11 +645,11 @@ struct Step {
     hipLaunchKernelGGL((stepRefineKernel<NS_, NL_>), dim3(blocks), dim3(flashBlock), 0, stream,                     \
                        static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), gate(),    \
                        sets, out, pOut, cOut, rOut, costOut, counter, RAMD_REFINE_QUEUE ? queue : nullptr,         \
-                       queueCount);                                                                                \
+                       queueCount, failMark);                                                                      \
     if (RAMD_REFINE_QUEUE) {                                                                                       \
         hipLaunchKernelGGL((stepRefineQueueKernel<NS_, NL_>), dim3(kQueueBlocks), dim3(64), 0, stream,              \
                            static_cast<const HemModel<NS_>*>(h->model), q, all, sA, queue, queueCount, gate(),      \
-                           sets, out, pOut, cOut, rOut, costOut, counter);                                          \
+                           sets, out, pOut, cOut, rOut, costOut, counter, failMark);                                \
         hipLaunchKernelGGL(queueTotalKernel, dim3(1), dim3(64), 0, stream, queueCount, counters + 7);             \
     }
             if (h->ns == 1 && h->nl == 0) { RAMD_STEP_MIXED(1, 0); }
@@ -633,7 +661,7 @@ struct Step {
 #define RAMD_STEP_FLASH(NS_, NL_)                                                                                 \
     hipLaunchKernelGGL((stepFlashKernel<NS_, NL_>), dim3(blocks), dim3(flashBlock), 0, stream,                     \
                        static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), gate(), out, \
-                       pOut, cOut, rOut, costOut, counter)
+                       pOut, cOut, rOut, costOut, counter, failMark)
             if (h->ns == 1 && h->nl == 0) { RAMD_STEP_FLASH(1, 0); }
             else if (h->ns == 1) { RAMD_STEP_FLASH(1, 1); }
             else if (h->nl == 0) { RAMD_STEP_FLASH(2, 0); }
@@ -654,7 +682,7 @@ struct Step {
         launchBatchCfl(ns, boxes, cflStart, count, cflBlocks, q, all, c, cfl, maxHost, maxDev ? scalars : nullptr,
                        blockMin, stream);
         hipLaunchKernelGGL(batchFinishCflKernel, dim3(unsigned(count)), dim3(256), 0, stream, blockMin, cflStart,
-                           scalars, slot);
+                           scalars, slot, slot == 0 ? limit : nullptr);
         if (count > 1 && slot == 0)
             hipLaunchKernelGGL(batchGateKernel, dim3(1), dim3(64), 0, stream, scalars, count, scalars + 4 * count);
     }
@@ -811,6 +839,88 @@ struct Step {
         RAMD_HIP(hipGetLastError());
     }
 };
+
+// One step under the driver's attempt loop (ReactiveFoam.C:1443-1494): the
+// step-start q, p, c_frozen, r and states are kept; an attempt in which a
+// case has a failed flash or a failed CFL recheck is rolled back and every
+// case repeats it, the failed cases with half their dt (the others recompute
+// the same dt and repeat their step bit for bit), at most 12 times and while
+// dt / 2 > 1e-15.  Returns the attempts rolled back; throws with the
+// step-start state restored when the step cannot be completed.
+uint64_t attemptStep(Step& s, double cfl, double maximum, hipEvent_t* marks) {
+    const int n = s.count;
+    if (!s.qKeep) {
+        s.qKeep = s.alloc<double>(s.all * s.nv);
+        s.pKeep = s.alloc<double>(s.all);
+        s.cKeep = s.alloc<double>(s.all);
+        s.rKeep = s.alloc<double>(s.all);
+        s.sKeep = s.alloc<ThermoOut>(s.cells);
+        s.failMarks = s.alloc<uint8_t>(s.cells);
+        s.boxFail = s.alloc<unsigned long long>(size_t(n));
+        s.limits = s.alloc<double>(size_t(n));
+    }
+    auto copy = [&](void* to, const void* from, size_t bytes) {
+        RAMD_HIP(hipMemcpyAsync(to, from, bytes, hipMemcpyDeviceToDevice, s.stream));
+    };
+    const size_t qBytes = s.all * s.nv * sizeof(double), aBytes = s.all * sizeof(double);
+    copy(s.qKeep, s.qA, qBytes);
+    copy(s.pKeep, s.pA, aBytes);
+    copy(s.cKeep, s.cA, aBytes);
+    copy(s.rKeep, s.rA, aBytes);
+    copy(s.sKeep, s.sA, s.cells * sizeof(ThermoOut));
+    const size_t boxes = size_t(n);
+    std::vector<double> limit(boxes, __builtin_huge_val()), scalars(4 * boxes + 1);
+    std::vector<unsigned long long> before(boxes), after(boxes), failed(boxes);
+    for (uint64_t rolledBack = 0;; ++rolledBack) {
+        RAMD_HIP(hipMemcpyAsync(s.limits, limit.data(), n * sizeof(double), hipMemcpyHostToDevice, s.stream));
+        RAMD_HIP(hipMemsetAsync(s.failMarks, 0, s.cells, s.stream));
+        RAMD_HIP(hipMemsetAsync(s.boxFail, 0, n * sizeof(unsigned long long), s.stream));
+        RAMD_HIP(hipMemcpyAsync(before.data(), s.violations, n * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                s.stream));
+        s.limit = s.limits;
+        s.failMark = s.failMarks;
+        s.one(cfl, maximum, marks);
+        s.failMark = nullptr;
+        s.limit = nullptr;
+        hipLaunchKernelGGL(boxFailKernel, dim3(256, unsigned(n)), dim3(256), 0, s.stream, s.boxes, s.failMarks,
+                           s.boxFail);
+        RAMD_HIP(hipMemcpyAsync(failed.data(), s.boxFail, n * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                s.stream));
+        RAMD_HIP(hipMemcpyAsync(after.data(), s.violations, n * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                s.stream));
+        RAMD_HIP(hipMemcpyAsync(scalars.data(), s.scalars, scalars.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                s.stream));
+        RAMD_HIP(hipStreamSynchronize(s.stream));
+        std::string why;
+        bool any = false, invalid = false, exhausted = false;
+        for (int i = 0; i < n; ++i) {
+            const double dt = scalars[4 * size_t(i)];
+            if (!(dt > 0)) invalid = true;
+            if (!failed[i] && after[i] == before[i]) continue;
+            any = true;
+            if (why.empty()) {
+                char text[160];
+                std::snprintf(text, sizeof text, "case %d: %llu failed flash(es)%s at dt %.6e", i, failed[i],
+                              after[i] > before[i] ? ", CFL recheck failed" : "", dt);
+                why = text;
+            }
+            limit[i] = .5 * dt;
+            if (!(limit[i] > 1e-15)) exhausted = true;
+        }
+        if (!any && !invalid) return rolledBack;
+        // roll back (the attempt's states are in sA after its buffer swap)
+        copy(s.qA, s.qKeep, qBytes);
+        copy(s.pA, s.pKeep, aBytes);
+        copy(s.cA, s.cKeep, aBytes);
+        copy(s.rA, s.rKeep, aBytes);
+        copy(s.sA, s.sKeep, s.cells * sizeof(ThermoOut));
+        RAMD_HIP(hipStreamSynchronize(s.stream));
+        require(!invalid, "Invalid CFL time step (non-finite or non-positive wave speeds)");
+        if (rolledBack + 1 > 12 || exhausted)
+            throw std::runtime_error("Step failed after " + std::to_string(rolledBack + 1) +
+                                     " attempts; the step-start state is restored (" + why + ")");
+    }
+}
 
 // One step from a captured graph.  The first step after a load runs on the
 // stream (it records the costs that order the flashes); later steps replay the
@@ -1040,7 +1150,9 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         require(steps >= 1, "At least one step");
         const bool useGraph = (flashMode & REACTIVE_AMD_STEP_GRAPH) != 0;
         const bool autotune = (flashMode & REACTIVE_AMD_STEP_AUTOTUNE) != 0;
-        flashMode &= ~(REACTIVE_AMD_STEP_GRAPH | REACTIVE_AMD_STEP_AUTOTUNE);
+        const bool retry = (flashMode & REACTIVE_AMD_STEP_RETRY) != 0;
+        flashMode &= ~(REACTIVE_AMD_STEP_GRAPH | REACTIVE_AMD_STEP_AUTOTUNE | REACTIVE_AMD_STEP_RETRY);
+        require(!(retry && useGraph), "Retried steps run on the stream, not from a captured graph");
         require(flashMode == REACTIVE_AMD_FLASH_FP64 || flashMode == REACTIVE_AMD_FLASH_MIXED,
                 "Unknown flash arithmetic mode");
         if (flashMode == REACTIVE_AMD_FLASH_MIXED && !s.sets) {
@@ -1069,8 +1181,10 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
             s.marks.push_back(e);
         }
         RAMD_HIP(hipEventRecord(s.events[0], s.stream));
+        uint64_t retries = 0;
         for (int i = 0; i < steps; ++i) {
             if (useGraph) graphStep(s, cfl, maximum);
+            else if (retry) retries += attemptStep(s, cfl, maximum, s.marks.data() + size_t(i) * kMarks);
             else s.one(cfl, maximum, s.marks.data() + size_t(i) * kMarks);
             if (pendingTune) {
                 s.tune(cfl, maximum);
@@ -1107,6 +1221,7 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         r.flashBlock = s.flashBlock;
         r.stateBins = uint32_t(s.stateSide);
         r.reserved = uint32_t(s.stateLevels);
+        r.retries = retries;
         // a batch reports its smallest dt and recheck step and the sums over its
         // boxes (box order); reactive_amd_step_report_box gives each box
         r.dt = scalars[0];

@@ -261,6 +261,27 @@ def pure_n2o_cube(n, seed=0, edge=0.08, tlo=180.0, thi=300.0, rlo=1.0, rhi=1000.
     return faces, volume, q, seeds, {"two_phase_fraction": float(dome.mean()), "seed": seed, "n": n}
 
 
+def riemann_cube(n, edge=0.016, rho_left=60.0, rho_right=5.0, T=300.0):
+    """Pure-N2O gas at rest in an n^3 box, dense on the x < edge / 2 side:
+    (faces, volume, q, seeds).  Its first stage raises the wave speeds at the
+    interface, so a step at CFL 0.5 fails the post-stage-0 recheck."""
+    nc = n ** 3
+    i = np.arange(nc) % n
+    rho = np.where(i < n // 2, rho_left, rho_right).astype(float)
+    Tc = np.full(nc, T)
+    e, p, liquid, dome = equilibrium_states(Tc, rho)
+    q = np.zeros((nc, 5))
+    q[:, 0] = rho
+    q[:, 4] = rho * e
+    seeds = np.zeros(nc, dtype=abi.STATE_DTYPE)
+    seeds["p"] = p; seeds["T"] = Tc; seeds["rho"] = rho; seeds["e"] = e
+    seeds["liquidMass"][:, 0] = liquid
+    seeds["gasMass"] = rho - liquid
+    seeds["activeLiquids"] = (liquid > 0).astype(np.int32)
+    faces, volume = cartesian_faces(n, edge / n)
+    return faces, volume, q, seeds
+
+
 def jet_cube(n, seed=0, edge=None, spacing=0.001, radius=0.0025, rho_jet=880.0, T_jet=293.15, speed=30.0,
              slip_sides=()):
     """pure_n2o_cube with two liquid-N2O jets entering through fixed-state
