@@ -48,21 +48,28 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def prepare(n, seed):
+# the workloads: pure N2O (the metric's) and the binary N2O/N2 mixture
+# (N2 mass fraction U(0, 0.3), T U(150, 320) K, rho log-uniform 0.5-1000)
+CASES = {"n2o": {"species": ("N2O",), "cube": cases.pure_n2o_cube, "name": "n2o-pure"},
+         "n2o-n2": {"species": ("N2O", "N2"), "cube": cases.binary_cube, "name": "n2o-n2-binary"}}
+
+
+def prepare(n, seed, case="n2o"):
     """Seeded case + flash-consistent step-start states (one device flash)."""
-    faces, volume, q, seeds, info = cases.pure_n2o_cube(n, seed=seed)
-    m = model.model_bytes(model.build_model())
+    faces, volume, q, seeds, info = CASES[case]["cube"](n, seed=seed)
+    ns = len(CASES[case]["species"])
+    m = model.model_bytes(model.build_model(gas_species=CASES[case]["species"]))
     lib = rlib.load()
     err = C.create_string_buffer(1024)
     cells = len(q)
     hem = lib.reactive_gpu_hem_create_v1(m, len(m), cells, err, 1024)
     if not hem:
         raise RuntimeError(err.value.decode())
-    e = cases.internal_energy(q, 1)
+    e = cases.internal_energy(q, ns)
     states = seeds.copy()
     success = np.zeros(cells, np.int32)
     prof = abi.HemProfile.fresh()
-    rc = lib.reactive_gpu_hem_run_v1(hem, rlib.ptr(np.ascontiguousarray(q[:, :1])), rlib.ptr(e), cells,
+    rc = lib.reactive_gpu_hem_run_v1(hem, rlib.ptr(np.ascontiguousarray(q[:, :ns])), rlib.ptr(e), cells,
                                      rlib.ptr(states), rlib.ptr(success), C.byref(prof), err, 1024)
     if rc:
         raise RuntimeError(err.value.decode())
@@ -216,10 +223,11 @@ FLOP_PER_PHASE = {"fp64_flash": (1453 + 1860) / 2, "fp32_search": (414 + 502) / 
 FP32_PEAK_TFLOPS = 157.3
 
 
-def converged_roots(lib, q, seeds):
+def converged_roots(lib, q, seeds, case="n2o"):
     """The FP64 flash of (q, seeds) rerun on the device with tolerances 1e-3
     as tight: the converged root each flash's result is compared with."""
-    tight = model.model_bytes(model.build_model(limits=TIGHT))
+    ns = len(CASES[case]["species"])
+    tight = model.model_bytes(model.build_model(limits=TIGHT, gas_species=CASES[case]["species"]))
     err = C.create_string_buffer(1024)
     cells = len(q)
     hem = lib.reactive_gpu_hem_create_v1(tight, len(tight), cells, err, 1024)
@@ -228,8 +236,8 @@ def converged_roots(lib, q, seeds):
     out = seeds.copy()
     ok = np.zeros(cells, np.int32)
     prof = abi.HemProfile.fresh()
-    rc = lib.reactive_gpu_hem_run_v1(hem, rlib.ptr(np.ascontiguousarray(q[:, :1])),
-                                     rlib.ptr(cases.internal_energy(q, 1)), cells, rlib.ptr(out), rlib.ptr(ok),
+    rc = lib.reactive_gpu_hem_run_v1(hem, rlib.ptr(np.ascontiguousarray(q[:, :ns])),
+                                     rlib.ptr(cases.internal_energy(q, ns)), cells, rlib.ptr(out), rlib.ptr(ok),
                                      C.byref(prof), err, 1024)
     lib.reactive_gpu_hem_destroy_v1(hem)
     if rc:
@@ -279,12 +287,13 @@ def flash_roofline(rep, rep64, cells, steps, mixed):
     return out
 
 
-def search_roofline(flash, rep, steps, n):
+def search_roofline(flash, rep, steps, n, case="n2o"):
     """The dominant kernel: the FP32 candidate search of the mixed flash."""
     s = flash["fp32_search"]
     return {"bound": "valu", "kernel": "stepSearchKernel (FP32 candidate search of the mixed flash)",
             "achieved": s["achieved"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": s["frac"],
-            "traffic": SEARCH_TRAFFIC_160 if n == 160 else None, "avg_launch_ms": s["avg_launch_ms"],
+            "traffic": SEARCH_TRAFFIC_160 if n == 160 and case == "n2o" else None,
+            "avg_launch_ms": s["avg_launch_ms"],
             "flop_per_phase_evaluation": FLOP_PER_PHASE["fp32_search"],
             "phase_evaluations_per_cell": s["phase_evaluations_per_cell"],
             "share_of_step": rep.searchSeconds / rep.seconds}
@@ -322,16 +331,21 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--case", choices=sorted(CASES), default="n2o",
+                    help="n2o: the metric's pure-N2O box; n2o-n2: the binary mixture box (no CPU baseline)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (the multi-GPU design is "
               "documented, not built)", file=sys.stderr)
         sys.exit(2)
     lib = rlib.load()
-    log(f"bench: building {a.n}^3 pure-N2O case (seed {a.seed})")
-    faces, volume, q, states, m, hem, info = prepare(a.n, a.seed)
+    species = CASES[a.case]["species"]
+    ns = len(species)
+    log(f"bench: building {a.n}^3 {CASES[a.case]['name']} case (seed {a.seed})")
+    faces, volume, q, states, m, hem, info = prepare(a.n, a.seed, a.case)
     cells = len(q)
-    cfg = abi.Config(cells=cells, species=1, variables=5, faces=len(faces), fixed=0, viscosity=0, conductivity=0,
+    cfg = abi.Config(cells=cells, species=ns, variables=ns + 4, faces=len(faces), fixed=0, viscosity=0,
+                     conductivity=0,
                      diffusivity=0, waveFactor=1.1, maxBytes=200e9, mechanical=0)
     err = C.create_string_buffer(1024)
     tr = lib.reactive_transport_create(1, C.byref(cfg), rlib.ptr(volume), rlib.ptr(faces), None, None, None, None,
@@ -356,9 +370,9 @@ def main():
         log(f"bench: mixed {wall:.3f} s for {a.steps} steps (device {rep.seconds:.3f} s)")
         accuracy = census(states, s64, smx)
         log("bench: converged roots of both steps' second flash")
-        root, ok = converged_roots(lib, qmx_one, states)
+        root, ok = converged_roots(lib, qmx_one, states, a.case)
         accuracy["mixed_vs_converged_root"] = distance(smx, states, root, ok)
-        root, ok = converged_roots(lib, q64_one, states)
+        root, ok = converged_roots(lib, q64_one, states, a.case)
         accuracy["fp64_vs_converged_root"] = distance(s64, states, root, ok)
     else:
         rep, wall, accuracy = rep64, wall64, None
@@ -370,18 +384,19 @@ def main():
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32+fp64" if mixed else "f64", "data": "synthetic",
-        "config": {"workload": f"cube{a.n}-n2o-pure", "cells": cells, "faces": len(faces), "cfl": a.cfl,
-                   "species": ["N2O"], "liquids": ["N2O"],
+        "config": {"workload": f"cube{a.n}-{CASES[a.case]['name']}", "cells": cells, "faces": len(faces),
+                   "cfl": a.cfl, "species": list(species), "liquids": ["N2O"],
                    "flash": ("mixed: FP32 candidate search, FP64 refinement, stability and selection" if mixed
                              else "fp64 (plain, reference arithmetic)"),
-                   "seed": a.seed, "two_phase_fraction": round(info["two_phase_fraction"], 4)},
+                   "seed": a.seed, "two_phase_fraction": (round(info["two_phase_fraction"], 4)
+                                                          if "two_phase_fraction" in info else None)},
         "launch_shapes": {"stage_tile_depth": int(rep.tileDepth), "flash_block": int(rep.flashBlock),
                           "state_bins_per_axis": int(rep.stateBins), "state_cost_levels": int(rep.reserved),
                           "fp64_flash_block": int(rep64.flashBlock), "fp64_state_bins_per_axis": int(rep64.stateBins),
                           "fp64_state_cost_levels": int(rep64.reserved), "source": "autotuned in the untimed steps"},
-        "roofline": (search_roofline(flash, rep, a.steps, a.n) if mixed
+        "roofline": (search_roofline(flash, rep, a.steps, a.n, a.case) if mixed
                      else dict(flash["fp64_flash"], bound="valu", unit="TFLOP/s")),
-        "roofline_stage": stage_roofline(rep, cells, a.steps, a.n),
+        "roofline_stage": stage_roofline(rep, cells, a.steps, a.n, ns + 4),
         "roofline_flash": flash,
         "fp64": fp64,
         "speedup_vs_fp64": value / value64,
@@ -389,7 +404,7 @@ def main():
         "flash_failures": [int(rep.flashFailures[0]), int(rep.flashFailures[1])],
         "cfl_violations": int(rep.cflViolations), "dt": rep.dt,
     }
-    if not a.no_cpu:
+    if not a.no_cpu and a.case == "n2o":
         log(f"bench: cpu baseline ({a.cpu_sample} cells, {a.cpu_threads} threads)")
         result["cpu_baseline"] = cpu_baseline(m, q_one, states, a.cpu_threads, a.cpu_sample, a.seed)
     lib.reactive_amd_step_destroy(step)

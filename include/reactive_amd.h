@@ -51,6 +51,8 @@ typedef struct ReactiveAmdStepReport {
     uint32_t stateBins;                 /* state-order bins per axis (see REACTIVE_AMD_STEP_AUTOTUNE) */
     uint32_t reserved;                  /* state-order cost levels per bin */
     uint64_t retries;                   /* REACTIVE_AMD_STEP_RETRY: attempts rolled back and repeated */
+    uint64_t reusedCells;               /* REACTIVE_AMD_STEP_REUSE: cells given another cell's flash result
+                                           (the flash counters count the flashed cells only) */
 } ReactiveAmdStepReport;
 
 /* Flash arithmetic of a step.  FP64: the reference's arithmetic.  MIXED: the
@@ -87,6 +89,13 @@ typedef struct ReactiveAmdStepReport {
  * conservation check of the driver stays with the caller, from the report's
  * totals and boundary integral. */
 #define REACTIVE_AMD_STEP_RETRY 0x400
+/* OR-ed into flashMode: exact reuse within each flash (the reference pool's
+ * exactBatchReuse, reactiveClosurePool.inc:3-54).  Cells whose flash inputs
+ * are bitwise equal (species masses, internal energy, seed state) take the
+ * result of the lowest such cell, which alone is flashed; results are those
+ * of flashing every cell, bit for bit.  Uniform regions (an ambient at rest,
+ * a reservoir's plume far from the front) collapse to one flash each. */
+#define REACTIVE_AMD_STEP_REUSE 0x800
 
 /* Face flux of a transport handle (applies to reactive_transport_rhs/stage,
  * the resident stages and the resident step): REACTIVE_AMD_FLUX_HLL (the

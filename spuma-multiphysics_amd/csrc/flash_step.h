@@ -86,10 +86,11 @@ __device__ inline void countFlash(uint32_t failed, uint32_t phases, unsigned lon
 
 // Single-precision candidate search of the mixed flash (search.hip): one
 // launch over the cells in `order` in workgroups of `block` lanes (64, 128 or
-// 256), candidate sets and FP32 costs by cell.
+// 256), candidate sets and FP32 costs by cell; with rep (may be null), only
+// the cells that represent themselves (rep[c] == c) are searched.
 void launchSearch(int ns, int nl, const void* model, const double* q, size_t stride, const ThermoOut* seeds,
                   const uint32_t* order, uint32_t count, const double* dtDev, CandidateSet* sets, uint16_t* cost,
-                  unsigned long long* phases, hipStream_t stream, unsigned block);
+                  unsigned long long* phases, hipStream_t stream, unsigned block, const uint32_t* rep);
 
 }  // namespace ramd
 #endif

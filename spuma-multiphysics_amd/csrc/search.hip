@@ -24,11 +24,12 @@ __global__ __launch_bounds__(256, RAMD_SEARCH_MIN_WAVES) void stepSearchKernel(c
                                                        const uint32_t* __restrict__ order, uint32_t count,
                                                        const double* __restrict__ dtDev,
                                                        CandidateSet* __restrict__ sets, uint16_t* __restrict__ cost,
-                                                       unsigned long long* __restrict__ phases) {
+                                                       unsigned long long* __restrict__ phases,
+                                                       const uint32_t* __restrict__ rep) {
     const HemModel<NS>& model = workgroupModel(globalModel);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Counts n = {0, 0, 0, 0, 0, 0};
-    if (i < count && *dtDev > 0) {
+    if (i < count && *dtDev > 0 && !(rep && rep[order[i]] != order[i])) {
         const uint32_t c = order[i];
         Flash<NS, NL, float> f(model, n);
         loadCell(f, q, stride, seeds[c], c, NS);
@@ -43,12 +44,12 @@ __global__ __launch_bounds__(256, RAMD_SEARCH_MIN_WAVES) void stepSearchKernel(c
 
 void launchSearch(int ns, int nl, const void* model, const double* q, size_t stride, const ThermoOut* seeds,
                   const uint32_t* order, uint32_t count, const double* dtDev, CandidateSet* sets, uint16_t* cost,
-                  unsigned long long* phases, hipStream_t stream, unsigned block) {
+                  unsigned long long* phases, hipStream_t stream, unsigned block, const uint32_t* rep) {
     const unsigned blocks = unsigned((size_t(count) + block - 1) / block);
 #define RAMD_SEARCH(NS_, NL_)                                                                                     \
     hipLaunchKernelGGL((stepSearchKernel<NS_, NL_>), dim3(blocks), dim3(block), 0, stream,                         \
                        static_cast<const HemModel<NS_>*>(model), q, stride, seeds, order, count, dtDev, sets, cost, \
-                       phases)
+                       phases, rep)
     if (ns == 1 && nl == 0) { RAMD_SEARCH(1, 0); }
     else if (ns == 1) { RAMD_SEARCH(1, 1); }
     else if (nl == 0) { RAMD_SEARCH(2, 0); }

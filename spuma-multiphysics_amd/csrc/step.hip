@@ -46,12 +46,13 @@ __global__ __launch_bounds__(256, RAMD_FLASH_MIN_WAVES) void stepFlashKernel(con
                                                       double* __restrict__ pOut, double* __restrict__ cOut,
                                                       double* __restrict__ rOut, uint16_t* __restrict__ cost,
                                                       unsigned long long* __restrict__ counters,
-                                                      uint8_t* __restrict__ failMark) {
+                                                      uint8_t* __restrict__ failMark,
+                                                      const uint32_t* __restrict__ rep) {
     const HemModel<NS>& model = workgroupModel(globalModel);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Counts n = {0, 0, 0, 0, 0, 0};
     uint32_t failed = 0;
-    if (i < count && *dtDev > 0) {
+    if (i < count && *dtDev > 0 && !(rep && rep[order[i]] != order[i])) {
         const uint32_t c = order[i];
         const ThermoOut g = seeds[c];
         Flash<NS, NL> f(model, n);
@@ -80,12 +81,13 @@ __global__ __launch_bounds__(256, RAMD_FLASH_MIN_WAVES) void stepRefineKernel(co
                                                        uint16_t* __restrict__ cost,
                                                        unsigned long long* __restrict__ counters,
                                                        uint32_t* __restrict__ queue, unsigned* __restrict__ queueCount,
-                                                       uint8_t* __restrict__ failMark) {
+                                                       uint8_t* __restrict__ failMark,
+                                                       const uint32_t* __restrict__ rep) {
     const HemModel<NS>& model = workgroupModel(globalModel);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Counts n = {0, 0, 0, 0, 0, 0};
     uint32_t failed = 0;
-    if (i < count && *dtDev > 0) {
+    if (i < count && *dtDev > 0 && !(rep && rep[order[i]] != order[i])) {
         const uint32_t c = order[i];
         const ThermoOut g = seeds[c];
         Flash<NS, NL, double, true> f(model, n);
@@ -491,6 +493,93 @@ __global__ void boundaryIntegralKernel(const double* __restrict__ bA, const doub
 __global__ void setScalarKernel(double* __restrict__ dst, double value) {
     if (threadIdx.x == 0) *dst = value;
 }
+// ---- exact reuse (REACTIVE_AMD_STEP_REUSE): cells whose flash inputs are
+// bitwise equal (species masses, e = rhoE - |rho u|^2 / (2 rho), the 176-byte
+// seed state) share one flash, the one of the lowest such cell index, as the
+// reference's pool does within a batch (reactiveClosurePool.inc:3-54).  An
+// open-addressing table of cell indices (twice the cells, a power of two):
+// a slot goes from empty to a cell and then only to a lower cell with the
+// same key, so every key ends at its lowest cell whatever the insert order.
+constexpr uint32_t kEmptySlot = 0xffffffffu;
+__global__ void reuseClearKernel(uint32_t* __restrict__ table, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        table[i] = kEmptySlot;
+}
+// word w of cell c's flash input key (ns masses, e, the seed's 22 words)
+__device__ inline uint64_t reuseWord(const double* __restrict__ q, size_t stride, int ns,
+                                     const ThermoOut* __restrict__ seeds, size_t c, int w) {
+    if (w < ns) return uint64_t(__double_as_longlong(q[size_t(w) * stride + c]));
+    if (w == ns) {
+        double rho = 0;
+        for (int k = 0; k < ns; ++k) rho += q[size_t(k) * stride + c];
+        const double m0 = q[size_t(ns) * stride + c], m1 = q[size_t(ns + 1) * stride + c];
+        const double m2 = q[size_t(ns + 2) * stride + c], E = q[size_t(ns + 3) * stride + c];
+        return uint64_t(__double_as_longlong(E - .5 * (m0 * m0 + m1 * m1 + m2 * m2) / rho));
+    }
+    return reinterpret_cast<const uint64_t*>(seeds + c)[w - ns - 1];
+}
+constexpr int kSeedWords = int(sizeof(ThermoOut) / 8);
+__global__ __launch_bounds__(256) void reuseInsertKernel(const double* __restrict__ q, size_t stride, int ns,
+                                                         const ThermoOut* __restrict__ seeds, size_t cells,
+                                                         uint32_t* __restrict__ table, uint32_t mask,
+                                                         uint32_t* __restrict__ slotOf) {
+    const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const int words = ns + 1 + kSeedWords;
+    uint64_t h = 0xcbf29ce484222325ull;  // FNV-1a offset basis over 64-bit words, then a 64-bit finaliser
+    for (int w = 0; w < words; ++w) {
+        h ^= reuseWord(q, stride, ns, seeds, c, w);
+        h *= 0x100000001b3ull;
+    }
+    h ^= h >> 33; h *= 0xff51afd7ed558ccdull; h ^= h >> 33; h *= 0xc4ceb9fe1a85ec53ull; h ^= h >> 33;
+    uint32_t slot = uint32_t(h) & mask;
+    for (;;) {  // the table holds at least twice the cells: an empty slot is always reached
+        uint32_t other = table[slot];
+        if (other == kEmptySlot) {
+            other = atomicCAS(&table[slot], kEmptySlot, uint32_t(c));
+            if (other == kEmptySlot) break;
+        }
+        bool same = true;
+        for (int w = 0; w < words && same; ++w)
+            same = reuseWord(q, stride, ns, seeds, c, w) == reuseWord(q, stride, ns, seeds, other, w);
+        if (same) {
+            atomicMin(&table[slot], uint32_t(c));
+            break;
+        }
+        slot = (slot + 1) & mask;
+    }
+    slotOf[c] = slot;
+}
+__global__ void reuseResolveKernel(const uint32_t* __restrict__ table, const uint32_t* __restrict__ slotOf,
+                                   size_t cells, uint32_t* __restrict__ rep) {
+    const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c < cells) rep[c] = table[slotOf[c]];
+}
+// after the flash: every other cell takes its representative's results
+__global__ __launch_bounds__(256) void reuseCopyKernel(const uint32_t* __restrict__ rep, size_t cells,
+                                                       const double* __restrict__ dtDev, ThermoOut* __restrict__ out,
+                                                       double* __restrict__ pOut, double* __restrict__ cOut,
+                                                       double* __restrict__ rOut, uint16_t* __restrict__ cost,
+                                                       uint8_t* __restrict__ failMark,
+                                                       unsigned long long* __restrict__ reused) {
+    const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint32_t copied = 0;
+    if (c < cells && *dtDev > 0) {
+        const uint32_t r = rep[c];
+        if (r != uint32_t(c)) {
+            out[c] = out[r];
+            pOut[c] = pOut[r];
+            cOut[c] = cOut[r];
+            rOut[c] = rOut[r];
+            if (cost) cost[c] = cost[r];
+            if (failMark) failMark[c] = failMark[r];
+            copied = 1;
+        }
+    }
+    const uint32_t sum = waveSum(copied);
+    if ((threadIdx.x & 63) == 0 && sum) atomicAdd(reused, (unsigned long long)sum);
+}
+
 __global__ void clearCountersKernel(unsigned long long* __restrict__ c, int n) {
     const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
     if (i < n) c[i] = 0;
@@ -563,6 +652,10 @@ struct Step {
     unsigned long long* boxFail = nullptr;
     double* limit = nullptr;      // per box, applied to the step's dt while non-null
     double* limits = nullptr;
+    // exact reuse (REACTIVE_AMD_STEP_REUSE): hash table of cell indices, each
+    // cell's slot and representative (rep non-null while reuse is on)
+    uint32_t *reuseTable = nullptr, *reuseSlot = nullptr, *reuseRep = nullptr, *rep = nullptr;
+    uint32_t reuseMask = 0;
     std::vector<void*> allocations;
     double *qA = nullptr, *qB = nullptr, *pA = nullptr, *cA = nullptr, *pB = nullptr, *cB = nullptr;
     double *rA = nullptr, *rB = nullptr;  // recovered densities (read by the HLLC flux)
@@ -589,7 +682,7 @@ struct Step {
     double *bA = nullptr, *bB = nullptr, *boundary = nullptr, *totals = nullptr;  // nv per box
     double2* totalPartial = nullptr;
     // fail1, phases1, fail2, phases2, violations, HLLC rejections, FP32 phases, queued cells (mixed);
-    // [8, 16): the autotuner's trial flashes
+    // [8, 16): the autotuner's trial flashes; [16]: cells given a reused flash result ([24]: tuner's)
     unsigned long long* counters = nullptr;
     unsigned long long* flashCounters = nullptr;  // where flash() counts: counters, or counters + 8 while tuning
     hipStream_t stream = nullptr;
@@ -602,6 +695,7 @@ struct Step {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int graphMode[2] = {-1, -1}, graphScheme[2] = {-1, -1};
     double graphCfl[2] = {0, 0}, graphMax[2] = {0, 0};
+    const uint32_t* graphRep[2] = {nullptr, nullptr};
     int parity = 0;  // flips with every step (the seed and result buffers swap)
     uint64_t lastSteps = 0;
     bool loaded = false;
@@ -636,16 +730,23 @@ struct Step {
         stateOrder(q, order);
         uint32_t* orderSearch = order;
 #endif
+        if (rep) {
+            hipLaunchKernelGGL(reuseClearKernel, dim3(1024), dim3(256), 0, stream, reuseTable, size_t(reuseMask) + 1);
+            hipLaunchKernelGGL(reuseInsertKernel, dim3(blocksFor(cells, 256)), dim3(256), 0, stream, q, all, ns, sA,
+                               cells, reuseTable, reuseMask, reuseSlot);
+            hipLaunchKernelGGL(reuseResolveKernel, dim3(blocksFor(cells, 256)), dim3(256), 0, stream, reuseTable,
+                               reuseSlot, cells, rep);
+        }
         if (mode == REACTIVE_AMD_FLASH_MIXED) {
 #define RAMD_STEP_MIXED(NS_, NL_)                                                                                  \
     launchSearch(NS_, NL_, h->model, q, all, sA, orderSearch, uint32_t(cells), gate(), sets,                      \
-                 costOut ? costSearch : nullptr, counters + 6, stream, flashBlock);                                \
+                 costOut ? costSearch : nullptr, counters + 6, stream, flashBlock, rep);                           \
     if (split) RAMD_HIP(hipEventRecord(split, stream));                                                            \
     if (RAMD_REFINE_QUEUE) hipLaunchKernelGGL(clearQueueKernel, dim3(1), dim3(64), 0, stream, queueCount);          \
     hipLaunchKernelGGL((stepRefineKernel<NS_, NL_>), dim3(blocks), dim3(flashBlock), 0, stream,                     \
                        static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), gate(),    \
                        sets, out, pOut, cOut, rOut, costOut, counter, RAMD_REFINE_QUEUE ? queue : nullptr,         \
-                       queueCount, failMark);                                                                      \
+                       queueCount, failMark, rep);                                                                 \
     if (RAMD_REFINE_QUEUE) {                                                                                       \
         hipLaunchKernelGGL((stepRefineQueueKernel<NS_, NL_>), dim3(kQueueBlocks), dim3(64), 0, stream,              \
                            static_cast<const HemModel<NS_>*>(h->model), q, all, sA, queue, queueCount, gate(),      \
@@ -661,13 +762,16 @@ struct Step {
 #define RAMD_STEP_FLASH(NS_, NL_)                                                                                 \
     hipLaunchKernelGGL((stepFlashKernel<NS_, NL_>), dim3(blocks), dim3(flashBlock), 0, stream,                     \
                        static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), gate(), out, \
-                       pOut, cOut, rOut, costOut, counter, failMark)
+                       pOut, cOut, rOut, costOut, counter, failMark, rep)
             if (h->ns == 1 && h->nl == 0) { RAMD_STEP_FLASH(1, 0); }
             else if (h->ns == 1) { RAMD_STEP_FLASH(1, 1); }
             else if (h->nl == 0) { RAMD_STEP_FLASH(2, 0); }
             else { RAMD_STEP_FLASH(2, 1); }
 #undef RAMD_STEP_FLASH
         }
+        if (rep)
+            hipLaunchKernelGGL(reuseCopyKernel, dim3(blocksFor(cells, 256)), dim3(256), 0, stream, rep, cells, gate(),
+                               out, pOut, cOut, rOut, costOut, failMark, counters + 16);
         RAMD_HIP(hipGetLastError());
     }
 
@@ -934,7 +1038,7 @@ void graphStep(Step& s, double cfl, double maximum) {
     }
     const int p = s.parity;
     if (!s.graph[p] || s.graphMode[p] != s.mode || s.graphScheme[p] != s.t->scheme || s.graphCfl[p] != cfl ||
-        s.graphMax[p] != maximum) {
+        s.graphMax[p] != maximum || s.graphRep[p] != s.rep) {
         if (s.graph[p]) {
             RAMD_HIP(hipGraphExecDestroy(s.graph[p]));
             s.graph[p] = nullptr;
@@ -952,6 +1056,7 @@ void graphStep(Step& s, double cfl, double maximum) {
         s.graphScheme[p] = s.t->scheme;
         s.graphCfl[p] = cfl;
         s.graphMax[p] = maximum;
+        s.graphRep[p] = s.rep;
     }
     RAMD_HIP(hipGraphLaunch(s.graph[p], s.stream));
     std::swap(s.sA, s.sB);
@@ -1061,7 +1166,7 @@ static Step* createStep(const std::vector<Transport*>& list, HemHandle* hem) {
     s->boundary = s->alloc<double>(size_t(s->nv) * n);
     s->totals = s->alloc<double>(size_t(s->nv) * n);
     s->totalPartial = s->alloc<double2>(size_t(s->nv) * kTotalBlocks * n);
-    s->counters = s->alloc<unsigned long long>(16);
+    s->counters = s->alloc<unsigned long long>(32);
     s->flashCounters = s->counters;
     for (int z = 0; z < 3; ++z) {
         s->stageStarts[z] = s->alloc<long long>(size_t(n) + 1);
@@ -1151,7 +1256,9 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         const bool useGraph = (flashMode & REACTIVE_AMD_STEP_GRAPH) != 0;
         const bool autotune = (flashMode & REACTIVE_AMD_STEP_AUTOTUNE) != 0;
         const bool retry = (flashMode & REACTIVE_AMD_STEP_RETRY) != 0;
-        flashMode &= ~(REACTIVE_AMD_STEP_GRAPH | REACTIVE_AMD_STEP_AUTOTUNE | REACTIVE_AMD_STEP_RETRY);
+        const bool reuse = (flashMode & REACTIVE_AMD_STEP_REUSE) != 0;
+        flashMode &= ~(REACTIVE_AMD_STEP_GRAPH | REACTIVE_AMD_STEP_AUTOTUNE | REACTIVE_AMD_STEP_RETRY |
+                       REACTIVE_AMD_STEP_REUSE);
         require(!(retry && useGraph), "Retried steps run on the stream, not from a captured graph");
         require(flashMode == REACTIVE_AMD_FLASH_FP64 || flashMode == REACTIVE_AMD_FLASH_MIXED,
                 "Unknown flash arithmetic mode");
@@ -1161,6 +1268,15 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
             s.queueCount = s.alloc<unsigned>(1);
         }
         s.mode = flashMode;
+        if (reuse && !s.reuseTable) {
+            uint32_t size = 1;
+            while (size < 2 * s.cells) size <<= 1;
+            s.reuseTable = s.alloc<uint32_t>(size);
+            s.reuseMask = size - 1;
+            s.reuseSlot = s.alloc<uint32_t>(s.cells);
+            s.reuseRep = s.alloc<uint32_t>(s.cells);
+        }
+        s.rep = reuse ? s.reuseRep : nullptr;
         s.stateSide = s.stateSideOf[s.mode];
         s.stateLevels = s.stateLevelsOf[s.mode];
         s.flashBlock = s.flashBlockOf[s.mode];
@@ -1172,7 +1288,7 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
             s.tune(cfl, maximum);
             pendingTune = false;
         }
-        hipLaunchKernelGGL(clearCountersKernel, dim3(1), dim3(64), 0, s.stream, s.counters, 8);
+        hipLaunchKernelGGL(clearCountersKernel, dim3(1), dim3(64), 0, s.stream, s.counters, 32);
         hipLaunchKernelGGL(clearCountersKernel, dim3(blocksFor(size_t(s.count), 64)), dim3(64), 0, s.stream,
                            s.violations, s.count);
         while (s.marks.size() < size_t(steps) * kMarks) {
@@ -1192,7 +1308,7 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
             }
         }
         RAMD_HIP(hipEventRecord(s.events[1], s.stream));
-        unsigned long long counters[8];
+        unsigned long long counters[17];
         const size_t n = size_t(s.count);
         std::vector<double> scalars(4 * n + 1), totals(s.nv * n), boundary(s.nv * n);
         RAMD_HIP(hipMemcpyAsync(counters, s.counters, sizeof(counters), hipMemcpyDeviceToHost, s.stream));
@@ -1222,6 +1338,7 @@ int reactive_amd_step_run(void* step, double cfl, double maximum, int steps, int
         r.stateBins = uint32_t(s.stateSide);
         r.reserved = uint32_t(s.stateLevels);
         r.retries = retries;
+        r.reusedCells = counters[16];
         // a batch reports its smallest dt and recheck step and the sums over its
         // boxes (box order); reactive_amd_step_report_box gives each box
         r.dt = scalars[0];

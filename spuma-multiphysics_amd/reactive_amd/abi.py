@@ -149,7 +149,8 @@ class StepReport(C.Structure):
                 ("flashSeconds", C.c_double * 2), ("cflSeconds", C.c_double), ("fluxFailures", C.c_uint64),
                 ("singlePhaseEvaluations", C.c_uint64), ("searchSeconds", C.c_double),
                 ("queuedCells", C.c_uint64), ("tileDepth", C.c_uint32), ("flashBlock", C.c_uint32),
-                ("stateBins", C.c_uint32), ("reserved", C.c_uint32), ("retries", C.c_uint64)]
+                ("stateBins", C.c_uint32), ("reserved", C.c_uint32), ("retries", C.c_uint64),
+                ("reusedCells", C.c_uint64)]
 
     @classmethod
     def fresh(cls):

@@ -44,3 +44,19 @@ def test_abi_layouts():
 
 def test_policy_string_is_the_legacy_fp64_policy():
     assert rlib.load().reactive_gpu_hem_numerical_policy_v1() == b""
+
+
+def test_step_report_layout_matches_the_header(tmp_path):
+    """ReactiveAmdStepReport as gcc lays it out from include/reactive_amd.h
+    against the ctypes mirror: size and every field offset."""
+    fields = [f[0] for f in abi.StepReport._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "reactive_amd.h"\nint main(void) {\n'
+                   '  printf("%zu\\n", sizeof(ReactiveAmdStepReport));\n' +
+                   "".join(f'  printf("%zu\\n", offsetof(ReactiveAmdStepReport, {f}));\n' for f in fields) +
+                   "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    values = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert values[0] == C.sizeof(abi.StepReport)
+    assert values[1:] == [getattr(abi.StepReport, f).offset for f in fields]
