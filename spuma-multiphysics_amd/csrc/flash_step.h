@@ -8,12 +8,17 @@
 #include "flash_device.h"
 
 // Tuning knobs (compile-time): the model image in LDS, and the minimum waves
-// per SIMD the FP64 flash kernels' register allocation targets.
+// per SIMD the register allocation targets for the FP64 flash kernels (3:
+// 168 VGPRs, at 160^3 17 % faster than 2) and for the mixed flash's FP64
+// refinement kernels (2: at 3 the refinement ran 8 % slower).
 #ifndef RAMD_MODEL_LDS
 #define RAMD_MODEL_LDS 1
 #endif
 #ifndef RAMD_FLASH_MIN_WAVES
-#define RAMD_FLASH_MIN_WAVES 2
+#define RAMD_FLASH_MIN_WAVES 3
+#endif
+#ifndef RAMD_REFINE_MIN_WAVES
+#define RAMD_REFINE_MIN_WAVES 2
 #endif
 
 namespace ramd {

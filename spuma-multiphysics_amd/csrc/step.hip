@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256, RAMD_FLASH_MIN_WAVES) void stepFlashKernel(con
 
 // Mixed precision, second half: FP64 refinement, stability and selection.
 template <int NS, int NL>
-__global__ __launch_bounds__(256, RAMD_FLASH_MIN_WAVES) void stepRefineKernel(const HemModel<NS>* __restrict__ globalModel,
+__global__ __launch_bounds__(256, RAMD_REFINE_MIN_WAVES) void stepRefineKernel(const HemModel<NS>* __restrict__ globalModel,
                                                        const double* __restrict__ q, size_t stride,
                                                        const ThermoOut* __restrict__ seeds,
                                                        const uint32_t* __restrict__ order, uint32_t count,
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256, RAMD_FLASH_MIN_WAVES) void stepRefineKernel(co
 // waves of the bulk.  queueCount is read on the device (the launch does not
 // depend on it).
 template <int NS, int NL>
-__global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void stepRefineQueueKernel(
+__global__ __launch_bounds__(64, RAMD_REFINE_MIN_WAVES) void stepRefineQueueKernel(
     const HemModel<NS>* __restrict__ globalModel, const double* __restrict__ q, size_t stride,
     const ThermoOut* __restrict__ seeds, const uint32_t* __restrict__ queue, const unsigned* __restrict__ queueCount,
     const double* __restrict__ dtDev, const CandidateSet* __restrict__ sets, ThermoOut* __restrict__ out,
