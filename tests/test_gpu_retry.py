@@ -60,7 +60,7 @@ def setup(lib, boxes):
     return hem, trs, prepared_boxes
 
 
-@pytest.mark.parametrize("mode", [0, 1], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("mode", [0, 1, 0x800], ids=["fp64", "mixed", "fp64-reuse"])
 def test_retried_step_equals_step_at_half_dt(mode):
     lib = rlib.load()
     box = cases.riemann_cube(12)
