@@ -601,6 +601,12 @@ struct Flash {
             for (R fraction = 1; fraction > R(1e-9); fraction *= R(.5)) {
                 E trial;
                 const R pp = p * m_exp(fraction * step[0]), tt = T * m_exp(fraction * step[1]);
+                // FP32 search: a fraction too small to move the point (nor
+                // any smaller one) would repeat the rejected evaluation, so
+                // the halving stops there (13 % of the search at 160^3); the
+                // FP64 flashes keep every halving, as the reference counts them
+                if constexpr (P::single)
+                    if (pp == p && tt == T) break;
                 if (evaluate(mass0, pp, tt, trial, false)) {
                     const R next = hi(ab(trial.volume - R(1)), ab(trial.energy - energy) / scale);
                     if (next < previous * (R(1) - R(1e-4) * fraction) || next < lo(vtol, etol)) {
@@ -748,6 +754,8 @@ struct Flash {
                     fraction = R(::ldexp(1.0, -upper));
                     for (int j = 0; j < 3; ++j) tx[j] = x[j] + fraction * step[j];
                 }
+                if constexpr (P::single)  // FP32 search: no movement now or at any smaller fraction
+                    if (tx[0] == x[0] && tx[1] == x[1] && tx[2] == x[2]) break;
                 E trial;
                 if (residual(tx, scale, tf, &trial) &&
                     hi(hi(hi(R(0), ab(tf[0])), ab(tf[1])), ab(tf[2])) < previous * (R(1) - R(1e-4) * fraction)) {

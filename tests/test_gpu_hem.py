@@ -137,3 +137,41 @@ def test_resident_frozen_batch_matches_host_batch_and_oracle():
     assert not dev.get(dout2, blank).view(np.uint8).any()
     dev.close()
     lib.reactive_gpu_hem_destroy_v1(r)
+
+
+def test_error_contracts():
+    """The reference's argument checks (reactiveGpuHem.cuh:190-233): create
+    rejects a wrong model size and a zero capacity with a NULL handle and a
+    message; run rejects an empty batch, one above the capacity and a profile
+    of another ABI with rc 1, and writes nothing; a cell no flash recovers is
+    a per-cell failure (rc 0, success 0, its state untouched, counted in
+    deviceFailures)."""
+    lib = rlib.load()
+    case = cube_case(3, seed=11)
+    mb, q, e, seeds = case["model"], case["q"][:, :1].copy(), case["energy"].copy(), case["seeds"]
+    err = C.create_string_buffer(1024)
+    assert not lib.reactive_gpu_hem_create_v1(mb, len(mb) - 8, len(e), err, 1024)
+    assert b"Invalid" in err.value
+    assert not lib.reactive_gpu_hem_create_v1(mb, len(mb), 0, err, 1024)
+    h = lib.reactive_gpu_hem_create_v1(mb, len(mb), len(e), err, 1024)
+    assert h, err.value
+    states = seeds.copy()
+    success = np.full(len(e), -7, np.int32)
+    prof = abi.HemProfile.fresh()
+    for count in (0, len(e) + 1):
+        rc = lib.reactive_gpu_hem_run_v1(h, rlib.ptr(q), rlib.ptr(e), count, rlib.ptr(states), rlib.ptr(success),
+                                         C.byref(prof), err, 1024)
+        assert rc == 1 and b"capacity" in err.value
+    bad = abi.HemProfile.fresh()
+    bad.abiVersion = 2
+    assert lib.reactive_gpu_hem_run_v1(h, rlib.ptr(q), rlib.ptr(e), len(e), rlib.ptr(states), rlib.ptr(success),
+                                       C.byref(bad), err, 1024) == 1
+    assert np.array_equal(states.view(np.uint8), seeds.view(np.uint8)) and (success == -7).all()
+    e2 = e.copy()
+    e2[5] = np.nan
+    prof = abi.HemProfile.fresh()
+    assert lib.reactive_gpu_hem_run_v1(h, rlib.ptr(q), rlib.ptr(e2), len(e), rlib.ptr(states), rlib.ptr(success),
+                                       C.byref(prof), err, 1024) == 0, err.value
+    assert success[5] == 0 and prof.deviceFailures >= 1
+    assert np.array_equal(states[5:6].view(np.uint8), seeds[5:6].view(np.uint8))
+    lib.reactive_gpu_hem_destroy_v1(h)
