@@ -520,6 +520,9 @@ struct Flash {
                     R candidate = hi(Tmin, T - radius), value, dcv, dp;
                     bool found = false;
                     for (int retry = 0; retry < 8 && !found; ++retry) {
+                        // FP32 search: the bracket end itself (clamped at Tmin, or reached
+                        // by the halving) was probed already, with this result
+                        if (P::single && candidate == lower) { found = true; break; }
                         if (gasProbe(candidate, value, dcv, dp, nullptr)) { fl = value; lower = candidate; found = true; }
                         else candidate = R(.5) * (lower + candidate);
                     }
@@ -529,6 +532,7 @@ struct Flash {
                     R candidate = lo(Tmax, T + radius), value, dcv, dp;
                     bool found = false;
                     for (int retry = 0; retry < 8 && !found; ++retry) {
+                        if (P::single && candidate == upper) { found = true; break; }
                         if (gasProbe(candidate, value, dcv, dp, nullptr)) { fh = value; upper = candidate; found = true; }
                         else candidate = R(.5) * (upper + candidate);
                     }
