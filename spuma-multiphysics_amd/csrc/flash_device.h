@@ -79,17 +79,20 @@ struct Counts {
     uint32_t phases, residuals, candidates, stable, failures, fd;
 };
 
-// Per-phase properties the analytic Jacobian reads (specific volume and
-// energy, molar enthalpy / RT, compressibility); empty when the flash uses
-// only the reference's finite-difference Jacobian.
+// Per-phase properties the analytic Jacobian reads: the condensable's
+// partial specific volume and energy, partial molar enthalpy / RT and
+// p v / RT in each phase (for a pure phase the phase's own specific and
+// molar values), and muN, the change of the gas's mu_c / RT per unit mass of
+// the condensable added to the gas at fixed (p, T) (zero for a pure gas).
+// Empty when the flash uses only the reference's finite-difference Jacobian.
 template <class R, bool A>
 struct PhaseParts {
     RD void clear() {}
 };
 template <class R>
 struct PhaseParts<R, true> {
-    R vG, vL, eG, eL, hG, hL, zG, zL;
-    RD void clear() { vG = vL = eG = eL = hG = hL = zG = zL = 0; }
+    R vG, vL, eG, eL, hG, hL, zG, zL, muN;
+    RD void clear() { vG = vL = eG = eL = hG = hL = zG = zL = muN = 0; }
 };
 
 // Evaluation record (reactiveDeviceFlash.h:64-73) for at most one liquid;
@@ -221,10 +224,12 @@ template <> struct FlashPrecision<float> {
     static constexpr float heuristicReuse = RAMD_HEURISTIC_REUSE;
 };
 
-// A (analytic): the active-set Newton steps of a single-species gas use the
-// analytic Jacobian; the equilibrium sound speed keeps the reference's
-// finite-difference Jacobian.  On for the FP32 search and the FP64
-// refinement of the mixed flash, off for the reference-arithmetic flash.
+// A (analytic): the active-set Newton steps and the refinement's equilibrium
+// sound speed use the analytic Jacobian (a gas mixture through the
+// condensable's partial quantities, falling back to the reference's
+// finite differences where those are not finite).  On for the FP32 search
+// and the FP64 refinement of the mixed flash, off for the reference-arithmetic
+// flash.
 template <int NS, int NL, class R = double, bool A = FlashPrecision<R>::single>
 struct Flash {
     static_assert(NL == 0 || NL == 1, "one pure liquid at most");
@@ -370,13 +375,72 @@ struct Flash {
         z = p * s.V / RT;
     }
 
-    // Analytic Jacobian of the one-liquid residual for a single gas species
-    // (mixed flash): with the gas of the condensable's own
-    // species, the volume and energy rows are the (p, T) derivatives the
-    // evaluation accumulates plus the change of one unit of mass from gas to
-    // liquid, and the chemical-potential row follows from
-    // d(mu/RT)/d ln p = Z and d(mu/RT)/d ln T = -h/RT (Gibbs-Helmholtz); mu
-    // does not depend on the phase amounts.
+    // Condensable's partial quantities in a PR gas mixture at (p, T) (gas
+    // mass mg, mass fractions Y, state s from gasTP), by differentiating the
+    // residual Helmholtz energy A_r = -n R T ln(1 - B/V) - D ln((V + (1+√2)B) /
+    // (V + (1-√2)B)) / (2√2 B), with B = Σ n_i b_i and D = Σ n_i n_j a_ij(T),
+    // in the moles n_c at fixed (T, V) and moving to fixed (T, p) with
+    // v_c = -(∂p/∂n_c) / (∂p/∂V):
+    //   partial volume v_c, energy e_c = e_c^ig + ∂U_r/∂n_c + (T ∂p/∂T - p) v_c,
+    //   enthalpy h_c = e_c + p v_c, and
+    //   ∂mu_c/∂n_c|_{T,p} = R T / n_c + ∂²A_r/∂n_c² + (∂p/∂n_c)² / (∂p/∂V).
+    // mu_c is the reference's gas chemical potential (reactiveDevicePR.h:900-916),
+    // whose (T, p, n) derivatives these are.
+    RD void gasPartials(R p, R T, R mg, const R* Y, const S& s, PhaseParts<R, true>& out) const {
+        const int c = m.condensable;
+        const auto& t = m.gas;
+        R x[NS], W = 0;
+        if (massToMole(t, Y, x, W) != kOk) { out.muN = R(Real<R>::big) * R(2); return; }
+        R qv[NS], dq[NS];
+        R b = 0, aa = 0, da = 0;
+        for (int i = 0; i < NS; ++i) {
+            const R Tc = R(t.Tc[i]), kappa = R(t.kappa[i]), sqrtTr = m_sqrt(T / Tc);
+            const R f = R(1) + kappa * (R(1) - sqrtTr);
+            qv[i] = ab(f);
+            dq[i] = -(f > 0 ? R(1) : R(-1)) * kappa / (R(2) * Tc * sqrtTr);
+            b += x[i] * R(t.b[i]);
+        }
+        R pp = 0, dpp = 0;
+        for (int i = 0; i < NS; ++i) {
+            for (int j = 0; j < NS; ++j) {
+                const R w = x[i] * x[j] * R(t.aij[i][j]);
+                aa += w * qv[i] * qv[j];
+                da += w * (dq[i] * qv[j] + qv[i] * dq[j]);
+            }
+            const R aci = R(t.aij[c][i]);
+            pp += x[i] * aci * qv[i] * qv[c];
+            dpp += x[i] * aci * (dq[c] * qv[i] + qv[c] * dq[i]);
+        }
+        const R Rg = R(kR), sq2 = R(kSqrt2), RT = Rg * T, V = s.V, bc = R(t.b[c]);
+        const R vmb = V - b, den = V * V + R(2) * V * b - b * b;
+        const R L = m_log((V + (R(1) + sq2) * b) / (V + (R(1) - sq2) * b)) / (R(2) * sq2 * b);
+        const R pn = RT / vmb + RT * bc / (vmb * vmb) - R(2) * pp / den + R(2) * aa * vmb * bc / (den * den);
+        const R vbar = -pn / s.dpdV;
+        R cpR, hRT, sR;
+        nasa(region(t, c, T), T, cpR, hRT, sR);
+        const R dU = R(2) * (T * dpp - pp) * L + (T * da - aa) * (bc / b) * (V / den - L);
+        const R ebar = RT * hRT - RT + dU + (T * s.dpdT - p) * vbar;
+        const R acc = R(t.aij[c][c]) * qv[c] * qv[c];
+        const R d2A = R(2) * RT * bc / vmb + RT * bc * bc / (vmb * vmb) - R(2) * acc * L -
+                      R(4) * pp * bc * (V / (b * den) - L / b) -
+                      aa * bc * bc * (-R(2) * V / (b * b * den) - R(2) * V * vmb / (b * den * den) + R(2) * L / (b * b));
+        const R nmu = RT / x[c] + d2A + pn * pn / s.dpdV;  // n_total * ∂mu_c/∂n_c|_{T,p}
+        const R Wc = R(t.W[c]);
+        out.vG = vbar / Wc;
+        out.eG = ebar / Wc;
+        out.hG = (ebar + p * vbar) / RT;
+        out.zG = p * vbar / RT;
+        out.muN = nmu * W / (RT * mg * Wc);  // n_total = mg / W, dn_c = dm / W_c
+    }
+
+    // Analytic Jacobian of the one-liquid residual (mixed flash): the volume
+    // and energy rows are the (p, T) derivatives the evaluation accumulates
+    // plus the change of one unit of the condensable's mass from gas to
+    // liquid (the gas loses its partial volume and energy), and the
+    // chemical-potential row follows from d(mu/RT)/d ln p = p v / RT and
+    // d(mu/RT)/d ln T = -h/RT (Gibbs-Helmholtz) with partial molar v and h,
+    // and, in the liquid amount, from the gas composition's change (muN; a
+    // pure gas's mu does not depend on the phase amounts).
     RD void analyticJacobian(const E& v, R scale, R (&jac)[3][3]) const {
         const R qc = q[m.condensable];
         jac[0][0] = v.Vp * v.p;
@@ -387,7 +451,14 @@ struct Flash {
         jac[1][2] = qc * (v.parts.eL - v.parts.eG) / scale;
         jac[2][0] = v.parts.zL - v.parts.zG;
         jac[2][1] = -(v.parts.hL - v.parts.hG);
-        jac[2][2] = 0;
+        jac[2][2] = NS == 1 ? R(0) : qc * v.parts.muN;
+    }
+    RD bool analyticJacobianFinite(const E& v, R scale, R (&jac)[3][3]) const {
+        analyticJacobian(v, scale, jac);
+        bool ok = true;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) ok = ok && fin(jac[i][j]);
+        return ok;
     }
 
     // Gas + liquid mixture at common (p, T) (reactiveDeviceFlash.h:208-262).
@@ -429,7 +500,10 @@ struct Flash {
             accumulate(v, mg, prop, p);
             v.alphaGas = mg / prop.rho;
             v.rhoGas = prop.rho;
-            if constexpr (A) recordPhase(prop, p, T, v.parts.vG, v.parts.eG, v.parts.hG, v.parts.zG);
+            if constexpr (A) {
+                if constexpr (NS == 1) recordPhase(prop, p, T, v.parts.vG, v.parts.eG, v.parts.hG, v.parts.zG);
+                else gasPartials(p, T, mg, gas, prop, v.parts);
+            }
             if (NL && virtualLiquids) v.muGas = mu[c] / (R(8314.46261815324) * T);
         }
         if (NL) {
@@ -695,6 +769,8 @@ struct Flash {
             // point, where the reference differences the residual (its c_eq
             // carries the differencing error, a few 1e-7)
             analyticJacobian(value, scale, jac);
+        } else if constexpr (A && !P::single) {
+            if (!analyticJacobianFinite(value, scale, jac) && !jacobian(x, scale, f, jac)) return false;
         } else {
             if (!jacobian(x, scale, f, jac)) return false;
         }
@@ -730,6 +806,9 @@ struct Flash {
             }
             if constexpr (A && NS == 1) {
                 analyticJacobian(value, scale, jac);
+            } else if constexpr (A) {
+                if (!analyticJacobianFinite(value, scale, jac) && !jacobian(x, scale, f, jac))
+                    return polishing && activeFinish(x, scale, f, value);
             } else {
                 if (!jacobian(x, scale, f, jac)) return polishing && activeFinish(x, scale, f, value);
             }
