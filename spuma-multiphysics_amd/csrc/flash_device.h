@@ -283,9 +283,7 @@ struct Flash {
         for (int k = 0; k < NS; ++k)
             hit = hit && (P::single ? ab(memoKey[k] - x[k]) <= R(P::heuristicReuse) : memoKey[k] == x[k]);
         if (!hit) {
-            double xd[NS];
-            for (int k = 0; k < NS; ++k) xd[k] = double(x[k]);
-            phaseHeuristic(m.gas, xd, double(W), double(mx.a), double(mx.b), memo);
+            phaseHeuristic<NS, R>(m.gas, x, W, mx.a, mx.b, memo);
             for (int k = 0; k < NS; ++k) memoKey[k] = x[k];
             memoValid = true;
         }

@@ -327,35 +327,38 @@ RHD int massToMole(const Table<N>& t, const R* Y, R* x, R& W) {  // :310-327
 }
 
 // Composition part of Cantera's phaseState heuristic (reactiveDevicePR.h:
-// 428-471): everything but the final rho < rhoMid(T) comparison.  Always FP64.
+// 428-471): everything but the final rho < rhoMid(T) comparison.  FP64 for
+// every flash that reports states; the FP32 candidate search instantiates it
+// in FP32 (its labels only steer which candidates the search finds, and the
+// FP64 refinement evaluates every candidate again with the FP64 heuristic).
 struct PhaseHeuristic {
     double tc, critDensity, densityDelta, tempDelta;
     int status;  // status of the tmid root loop (reported only when T < tc)
 };
 
-template <int N>
-RHD void phaseHeuristic(const Table<N>& t, const double* x, double W, double mixA, double mixB, PhaseHeuristic& h) {
-    const double tc = mixA * kOmegaB / (mixB * kOmegaA * kR);
-    const double pc = kOmegaB * kR * tc / mixB;
-    const double vc = kOmegaVc * kR * tc / pc;
+template <int N, class R = double>
+RHD void phaseHeuristic(const Table<N>& t, const R* x, R W, R mixA, R mixB, PhaseHeuristic& h) {
+    const R tc = mixA * R(kOmegaB) / (mixB * R(kOmegaA) * R(kR));
+    const R pc = R(kOmegaB) * R(kR) * tc / mixB;
+    const R vc = R(kOmegaVc) * R(kR) * tc / pc;
     h.tc = tc;
-    double tmid = tc - 100.0;
-    if (tmid < 0) tmid = .5 * tc;
-    const double ratio = tc / tmid;
-    double pp = pc * m_exp(-.8734 * ratio * ratio - 3.4522 * ratio + 4.2918);
-    Mix mm;
+    R tmid = tc - R(100.0);
+    if (tmid < 0) tmid = R(.5) * tc;
+    const R ratio = tc / tmid;
+    R pp = pc * m_exp(R(-.8734) * ratio * ratio - R(3.4522) * ratio + R(4.2918));
+    MixT<R> mm;
     int st = mixing(t, tmid, x, mm);
     if (st != kOk) { h.status = st; return; }
-    double liquidVolume = -1;
+    R liquidVolume = -1;
     bool found = false;
     for (int it = 0; it < 100; ++it) {
-        Roots r;
+        RootsT<R> r;
         st = roots(tmid, pp, mm.a, mm.b, mm.aa, r);
         if (st != kOk) { h.status = st; return; }
         if (r.signedCount == 1 || r.signedCount == 2) {
             found = pp > pc;
             liquidVolume = r.v0;
-            pp *= 1.04;
+            pp *= R(1.04);
             if (found) break;
         } else {
             liquidVolume = r.v0;
@@ -364,10 +367,10 @@ RHD void phaseHeuristic(const Table<N>& t, const double* x, double W, double mix
         }
     }
     if (!found || !(liquidVolume > 0) || !fin(liquidVolume)) { h.status = kNonFinite; return; }
-    const double critDensity = W / vc;
-    const double liquidMid = W / liquidVolume;
-    const double gasMid = W / (kR * tmid / pp);
-    const double midDensity = .5 * (liquidMid + gasMid);
+    const R critDensity = W / vc;
+    const R liquidMid = W / liquidVolume;
+    const R gasMid = W / (R(kR) * tmid / pp);
+    const R midDensity = R(.5) * (liquidMid + gasMid);
     h.critDensity = critDensity;
     h.densityDelta = critDensity - midDensity;
     h.tempDelta = tc - tmid;
