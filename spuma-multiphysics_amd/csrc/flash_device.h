@@ -1145,17 +1145,22 @@ struct Flash {
         }
         if (!NL) return have;
         const bool prune = prunable && gasStable;
-        if (prune != ((set.flags & kSeedsPruned) != 0)) {
+        if (!prune && (set.flags & kSeedsPruned)) {
+            // the single-precision search pruned the seeds FP64 needs
             have = false;
             if (deferred) { *deferred = true; return false; }
             return search(best);
         }
+        // FP64 prunes where the single-precision search did not: the
+        // reference's search stops at the gas (and pure-liquid) candidate,
+        // so the recorded seeds are skipped (have holds: the gas is stable)
         const int c = m.condensable;
         if (i < set.count && set.c[i].kind == kLiquidCandidate) {
             if (finishFrozen(q[c], set.c[i], candidate, 0)) consider(candidate, best, have);
             else ++n.failures;
             ++i;
         }
+        if (prune) return have;
         for (; i < set.count && set.c[i].kind == kSeedCandidate; ++i) {
             if (finishActive(set.c[i], candidate)) consider(candidate, best, have, true);
             else ++n.failures;
