@@ -613,7 +613,11 @@ inline unsigned blocksFor(size_t n, unsigned b) { return unsigned((n + b - 1) / 
 #ifndef RAMD_REFINE_QUEUE
 #define RAMD_REFINE_QUEUE 1
 #endif
-constexpr unsigned kQueueBlocks = 256;  // persistent grid of the queue kernel: one wave per CU
+// Persistent grid of the queue kernel: 1024 single-wave workgroups (four waves
+// per CU) take up to 65,536 deferred cells in one round, so no lane runs two
+// full FP64 searches one after the other; workgroups beyond the queue leave at
+// once.
+constexpr unsigned kQueueBlocks = 1024;
 constexpr int kMarks = 11;  // events per step
 
 // ---------------------------------------------------------------- handle
