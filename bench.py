@@ -84,114 +84,214 @@ def prepare(n, seed, case="n2o"):
     return faces, volume, q, states, m, hem, info
 
 
-def cpu_baseline(m, q_end, seeds_start, threads, sample, seed):
-    """Reference CPU path on a bounded sample: the reference's own flash
-    (oracle/_ref, compiled from its sources) on `sample` cells of a real step's
-    second flash, in `threads` workers; the reference transport (serial,
-    backend 0) on a 48^3 box; per-cell costs scaled to cell-steps/s."""
-    ref_path = os.path.join(ROOT, "oracle", "_ref", "libref.so")
-    kind = "reference" if os.path.exists(ref_path) else "port"
-    rng = np.random.default_rng(seed)
-    idx = np.sort(rng.choice(len(q_end), size=min(sample, len(q_end)), replace=False))
-    qs = np.ascontiguousarray(q_end[idx, :1])
-    e = cases.internal_energy(q_end[idx], 1)
-    st = seeds_start[idx].copy()
-    mb = C.create_string_buffer(m, len(m))
-    chunks = np.array_split(np.arange(len(idx)), threads * 4)
-    if kind == "reference":
-        ref = C.CDLL(ref_path)
-        ref.ref_flash_batch.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
-                                        C.c_void_p, C.c_void_p, C.c_void_p]
-        out = st.copy()
-        succ = np.zeros(len(idx), np.int32)
+def host_cores():
+    """The host's CPU description: nproc, the cores this process may use, the
+    model name from lscpu."""
+    out = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    try:
+        import subprocess
+        text = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in text.splitlines():
+            key, _, val = line.partition(":")
+            if key.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core"):
+                out[key.strip()] = val.strip()
+    except (OSError, ValueError):
+        pass
+    return out
+
+
+class CpuPath:
+    """The reference's own CPU code (oracle/_ref/libref.so, compiled from its
+    sources) or, where that is absent, the oracle restatement: the flash over
+    cells in `threads` workers (the reference's ThermoPool, one worker per
+    core, reactivePool.inc:48-52) and the serial transport operators
+    (backend 0)."""
+
+    def __init__(self, threads):
+        ref_path = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+        self.kind = "reference" if os.path.exists(ref_path) else "port"
+        self.threads = threads
+        V = C.c_void_p
+        if self.kind == "reference":
+            self.lib = C.CDLL(ref_path)
+            self.lib.ref_flash_batch.argtypes = [V, C.c_size_t, V, V, V, C.c_int, V, V, V]
+            self.lib.reactive_transport_create.restype = V
+            self.lib.reactive_transport_create.argtypes = [C.c_int, V] + [V] * 6 + [C.c_char_p, C.c_size_t]
+            self.lib.reactive_transport_stage.argtypes = [V] * 5 + [C.c_double, C.c_int, V]
+            self.lib.reactive_transport_stable_step.argtypes = [V, V, V, C.c_double, C.c_double,
+                                                                C.POINTER(C.c_double)]
+            self.lib.reactive_transport_destroy.argtypes = [V]
+        else:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            from helpers import oracle
+            self.lib = oracle()
+
+    def flash(self, m, q, e, seeds):
+        """Flash every cell (q: cells x ns); returns (states, success, wall seconds)."""
+        mb = C.create_string_buffer(m, len(m))
+        q = np.ascontiguousarray(q)
+        out = seeds.copy()
+        succ = np.zeros(len(e), np.int32)
+        chunks = np.array_split(np.arange(len(e)), self.threads * 8)
 
         def work(ch):
             if ch.size:
                 a, b = ch[0], ch[-1] + 1
-                ref.ref_flash_batch(mb, b - a, rlib.ptr(qs[a:b]), rlib.ptr(e[a:b]), rlib.ptr(st[a:b]), 0,
-                                    rlib.ptr(out[a:b]), rlib.ptr(succ[a:b]), None)
-    else:
+                if self.kind == "reference":
+                    self.lib.ref_flash_batch(mb, b - a, rlib.ptr(q[a:b]), rlib.ptr(e[a:b]), rlib.ptr(seeds[a:b]), 0,
+                                             rlib.ptr(out[a:b]), rlib.ptr(succ[a:b]), None)
+                else:
+                    self.lib.oracle_flash_batch(mb, b - a, rlib.ptr(q[a:b]), rlib.ptr(e[a:b]), rlib.ptr(out[a:b]),
+                                                rlib.ptr(succ[a:b]), None, 1)
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(self.threads) as pool:
+            list(pool.map(work, chunks))
+        return out, succ, time.perf_counter() - t0
+
+    def transport(self, faces, volume, ns):
+        cfg = abi.Config(cells=len(volume), species=ns, variables=ns + 4, faces=len(faces), fixed=0, viscosity=0,
+                         conductivity=0, diffusivity=0, waveFactor=1.1, maxBytes=1e12, mechanical=0)
+        err = C.create_string_buffer(1024)
+        if self.kind == "reference":
+            h = self.lib.reactive_transport_create(0, C.byref(cfg), rlib.ptr(volume), rlib.ptr(faces), None, None,
+                                                   None, None, err, 1024)
+            if not h:
+                raise RuntimeError(err.value.decode())
+            lib = self.lib
+
+            class T:
+                def stage(self, q, st, dt, stage):
+                    b = np.zeros(ns + 4)
+                    assert lib.reactive_transport_stage(h, rlib.ptr(q), rlib.ptr(st), None, None, dt, stage,
+                                                        rlib.ptr(b)) == 0
+
+                def stable_step(self, q, st, cfl):
+                    dt = C.c_double()
+                    assert lib.reactive_transport_stable_step(h, rlib.ptr(q), rlib.ptr(st), cfl, 1.0,
+                                                              C.byref(dt)) == 0
+                    return dt.value
+
+                def close(self):
+                    lib.reactive_transport_destroy(h)
+            return T()
         sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from helpers import oracle
-        ora = oracle()
-        out = st.copy()
-        succ = np.zeros(len(idx), np.int32)
+        from helpers import OracleTransport
+        ot = OracleTransport(cfg, volume, faces)
 
-        def work(ch):
-            if ch.size:
-                a, b = ch[0], ch[-1] + 1
-                ora.oracle_flash_batch(mb, b - a, rlib.ptr(qs[a:b]), rlib.ptr(e[a:b]), rlib.ptr(out[a:b]),
-                                       rlib.ptr(succ[a:b]), None, 1)
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(threads) as pool:
-        list(pool.map(work, chunks))
-    flash_wall = time.perf_counter() - t0
-    # transport: reference serial operators (backend 0) on a 48^3 box, two stages
+        class O:
+            def stage(self, q, st, dt, stage):
+                ot.stage(q, st, dt, stage)
+
+            def stable_step(self, q, st, cfl):
+                rc, dt = ot.stable_step(q, st, cfl, 1.0)
+                assert rc == 0
+                return dt
+
+            def close(self):
+                ot.close()
+        return O()
+
+
+def stratified_sample(states, size, rng):
+    """Cells drawn per phase class (gas, two-phase, liquid by the step-start
+    gas volume fraction) in proportion to the class's share of the box."""
+    ag = states["alphaGas"]
+    classes = {"gas": np.where(ag >= 1.0)[0], "two-phase": np.where((ag > 0) & (ag < 1))[0],
+               "liquid": np.where(ag <= 0)[0]}
+    n = len(states)
+    picks, counts = [], {}
+    for name, idx in classes.items():
+        k = int(round(size * idx.size / n))
+        k = min(k, idx.size)
+        if k:
+            picks.append(rng.choice(idx, size=k, replace=False))
+        counts[name] = k
+    return np.sort(np.concatenate(picks)), counts
+
+
+def cpu_baseline(m, q_end, seeds_start, threads, sample, seed):
+    """Reference CPU path on a bounded sample of the same workload: its flash
+    on `sample` cells of a real step's second flash, stratified by phase class,
+    in `threads` workers; its serial transport on a 48^3 box; per-cell costs
+    scaled to cell-steps/s.  Also one whole step of the 40^3 pure-N2O box
+    (CFL, stage 0, flash, stage 1, flash), timed end to end."""
+    cpu = CpuPath(threads)
+    rng = np.random.default_rng(seed)
+    idx, counts = stratified_sample(seeds_start, min(sample, len(q_end)), rng)
+    e = cases.internal_energy(q_end[idx], 1)
+    _, succ, flash_wall = cpu.flash(m, q_end[idx, :1], e, seeds_start[idx].copy())
+    # transport: serial operators on a 48^3 box, two stages
     tn = 48
     faces, volume, q0, seeds0, _ = cases.pure_n2o_cube(tn, seed=seed)
     tst = np.zeros(len(q0), dtype=abi.TSTATE_DTYPE)
     tst["p"] = 1e5; tst["T"] = 250.0; tst["rho"] = q0[:, 0]; tst["cv"] = 800.0; tst["sound"] = 200.0
     tst["gasMass"] = q0[:, 0]
-    cfg = abi.Config(cells=len(q0), species=1, variables=5, faces=len(faces), fixed=0, viscosity=0,
-                     conductivity=0, diffusivity=0, waveFactor=1.1, maxBytes=1e12, mechanical=0)
-    err = C.create_string_buffer(1024)
-    if kind == "reference":
-        lib = C.CDLL(ref_path)
-        lib.reactive_transport_create.restype = C.c_void_p
-        lib.reactive_transport_create.argtypes = [C.c_int, C.c_void_p] + [C.c_void_p] * 6 + [C.c_char_p, C.c_size_t]
-        lib.reactive_transport_stage.argtypes = [C.c_void_p] * 5 + [C.c_double, C.c_int, C.c_void_p]
-        lib.reactive_transport_destroy.argtypes = [C.c_void_p]
-        h = lib.reactive_transport_create(0, C.byref(cfg), rlib.ptr(volume), rlib.ptr(faces), None, None, None,
-                                          None, err, 1024)
-        qq = q0.copy()
-        b = np.zeros(5)
-        t0 = time.perf_counter()
-        for stage in (0, 1):
-            lib.reactive_transport_stage(h, rlib.ptr(qq), rlib.ptr(tst), None, None, 1e-8, stage, rlib.ptr(b))
-        transport_wall = time.perf_counter() - t0
-        lib.reactive_transport_destroy(h)
-    else:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from helpers import OracleTransport
-        ot = OracleTransport(cfg, volume, faces)
-        qq = q0.copy()
-        t0 = time.perf_counter()
-        for stage in (0, 1):
-            ot.stage(qq, tst, 1e-8, stage)
-        transport_wall = time.perf_counter() - t0
-        ot.close()
+    tr = cpu.transport(faces, volume, 1)
+    qq = q0.copy()
+    t0 = time.perf_counter()
+    for stage in (0, 1):
+        tr.stage(qq, tst, 1e-8, stage)
+    transport_wall = time.perf_counter() - t0
+    tr.close()
     flash_per_cell = flash_wall / len(idx)            # wall seconds per cell-flash with `threads` workers
     transport_per_cell = transport_wall / len(q0)     # serial seconds per cell for both stages
     per_cell_step = 2 * flash_per_cell + transport_per_cell
-    return {"value": 1.0 / per_cell_step, "unit": "cell-steps/s", "cores": threads, "kind": kind,
-            "sample": (f"{len(idx)} seeded-random cells of a real step's second flash (flash x2 per step, "
-                       f"{threads} threads, {flash_wall:.2f} s wall) + reference serial transport, both SSPRK2 "
-                       f"stages on a {tn}^3 box ({transport_wall:.2f} s); per-cell costs scaled; "
-                       f"{int(succ.sum())}/{len(idx)} sample flashes converged")}
+    # one full step of the 40^3 box
+    n0 = 40
+    faces, volume, q, seeds, _ = cases.pure_n2o_cube(n0, seed=seed)
+    states, ok, _ = cpu.flash(m, q[:, :1], cases.internal_energy(q, 1), seeds)
+    states[ok == 0] = states[np.where(ok == 1)[0][0]]
+    tr = cpu.transport(faces, volume, 1)
+    t0 = time.perf_counter()
+    ts = cases.transport_states(states)
+    dt = tr.stable_step(q, ts, 0.4)
+    q0 = q.copy()
+    tr.stage(q, ts, dt, 0)
+    s1, ok1, _ = cpu.flash(m, q[:, :1], cases.internal_energy(q, 1), states)
+    s1 = np.where(ok1 == 1, s1, states)
+    tr.stage(q, cases.transport_states(s1), dt, 1)
+    s2, ok2, _ = cpu.flash(m, q[:, :1], cases.internal_energy(q, 1), states)
+    step_wall = time.perf_counter() - t0
+    tr.close()
+    del q0
+    return {"value": 1.0 / per_cell_step, "unit": "cell-steps/s", "cores": threads, "kind": cpu.kind,
+            "host": host_cores(),
+            "sample": (f"{len(idx)} cells of a real 160^3 step's second flash, stratified by phase class "
+                       f"({', '.join(f'{k} {v}' for k, v in counts.items())}; flash x2 per step, {threads} "
+                       f"worker threads, {flash_wall:.2f} s wall) + serial transport, both SSPRK2 stages on a "
+                       f"{tn}^3 box ({transport_wall:.2f} s); per-cell costs scaled; {int(succ.sum())}/{len(idx)} "
+                       f"sample flashes converged"),
+            "config0_step": {"workload": "cube40-n2o-pure, one whole step (CFL, stage 0, flash, stage 1, flash)",
+                             "cells": len(q), "seconds": step_wall, "value": len(q) / step_wall,
+                             "unit": "cell-steps/s", "flash_failures": [int((ok1 == 0).sum()), int((ok2 == 0).sum())]}}
 
 
-def run_steps(lib, step, q, states, cfl, steps, mode, warmup):
-    """Loads (q, states), runs `warmup` untimed steps (at least one: it also
-    records the per-cell costs that order the timed flashes), then times
-    `steps` steps.  Returns (report, wall seconds, q and states after the
-    first warmup step)."""
+def run_steps(lib, step, q, states, cfl, steps, mode, warmup, profile_steps):
+    """Loads (q, states), runs `warmup` untimed steps (at least one, on the
+    stream: it autotunes the launch shapes and records the per-cell costs that
+    order the timed flashes; the others replay the captured graph), then times
+    `steps` steps replayed from the captured HIP graph (REACTIVE_AMD_STEP_GRAPH:
+    the metric's wall clock), then runs `profile_steps` steps on the stream for
+    the per-kernel event times the rooflines use.  Returns (timed report, wall
+    seconds, kernel-time report, q and states after the first warmup step)."""
     def check(rc):
         if rc:
             raise RuntimeError(lib.reactive_amd_step_error(step).decode())
     check(lib.reactive_amd_step_load(step, rlib.ptr(q), rlib.ptr(states)))
     rep = abi.StepReport.fresh()
-    # the first (untimed) step also autotunes the launch shapes on this state
     check(lib.reactive_amd_step_run(step, cfl, 1.0, 1, mode | AUTOTUNE, C.byref(rep)))
     q_one = np.zeros_like(q)
     s_one = np.zeros(len(q), dtype=abi.STATE_DTYPE)
     check(lib.reactive_amd_step_read(step, rlib.ptr(q_one), rlib.ptr(s_one)))
-    if warmup > 1:
-        check(lib.reactive_amd_step_run(step, cfl, 1.0, warmup - 1, mode, C.byref(rep)))
+    # the first graph call captures (a graph per buffer parity): keep both captures out of the timing
+    check(lib.reactive_amd_step_run(step, cfl, 1.0, max(2, warmup - 1), mode | GRAPH, C.byref(rep)))
     rep = abi.StepReport.fresh()
     t0 = time.perf_counter()
-    check(lib.reactive_amd_step_run(step, cfl, 1.0, steps, mode, C.byref(rep)))
+    check(lib.reactive_amd_step_run(step, cfl, 1.0, steps, mode | GRAPH, C.byref(rep)))
     wall = time.perf_counter() - t0
-    return rep, wall, q_one, s_one
+    kernels = abi.StepReport.fresh()
+    check(lib.reactive_amd_step_run(step, cfl, 1.0, profile_steps, mode, C.byref(kernels)))
+    return rep, wall, kernels, q_one, s_one
 
 
 def stage_roofline(rep, cells, steps, n, nv=5):
@@ -215,6 +315,7 @@ SEARCH_TRAFFIC_160 = 363.7e9
 STAGE_TRAFFIC_160 = [473.1e6, 708.0e6]
 TIGHT = {"vtol": 1e-12, "etol": 1e-12, "mutol": 1e-10}
 AUTOTUNE = 0x200  # REACTIVE_AMD_STEP_AUTOTUNE
+GRAPH = 0x100  # REACTIVE_AMD_STEP_GRAPH
 # VALU FLOPs per phase evaluation (ADD, MUL, TRANS = 1, FMA = 2), measured with
 # rocprofv3 on divergence-free uniform cells, mean of a two-phase cell and a
 # gas cell: profiles/flash_valu_uniform32.txt.  The flash roofline's achieved
@@ -328,8 +429,12 @@ def main():
     ap.add_argument("--cfl", type=float, default=0.4)
     ap.add_argument("--flash", choices=["mixed", "fp64"], default="mixed",
                     help="timed flash arithmetic; mixed also times the FP64 path for the comparison")
+    ap.add_argument("--profile-steps", type=int, default=3,
+                    help="stream steps after the timed ones, for the per-kernel event times")
     ap.add_argument("--cpu-sample", type=int, default=65536)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    # the reference pool runs one worker per core up to 64 (reactivePool.inc:50); a one-GPU box's
+    # CPU share is 16 cores
+    ap.add_argument("--cpu-threads", type=int, default=min(16, 64, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--case", choices=sorted(CASES), default="n2o",
                     help="n2o: the metric's pure-N2O box; n2o-n2: the binary mixture box (no CPU baseline)")
@@ -355,18 +460,20 @@ def main():
     step = lib.reactive_amd_step_create(tr, hem, err, 1024)
     if not step:
         raise RuntimeError(err.value.decode())
-    log(f"bench: FP64 flash, {a.warmup} warmup + {a.steps} timed step(s)")
-    rep64, wall64, q_one, s64 = run_steps(lib, step, q, states, a.cfl, a.steps, 0, a.warmup)
+    prof_steps = max(1, min(a.steps, a.profile_steps))
+    log(f"bench: FP64 flash, {a.warmup} warmup + {a.steps} timed (graph) + {prof_steps} kernel-timed step(s)")
+    rep64, wall64, kern64, q_one, s64 = run_steps(lib, step, q, states, a.cfl, a.steps, 0, a.warmup, prof_steps)
     q64_one = q_one
     log(f"bench: FP64 {wall64:.3f} s for {a.steps} steps (device {rep64.seconds:.3f} s)")
     value64 = cells * a.steps / wall64
-    t_flash64 = (rep64.flashSeconds[0] + rep64.flashSeconds[1]) / (2 * a.steps)
+    t_flash64 = (kern64.flashSeconds[0] + kern64.flashSeconds[1]) / (2 * prof_steps)
     fp64 = {"value": value64, "ms_per_step": 1e3 * wall64 / a.steps, "flash_ms": 1e3 * t_flash64,
             "phase_evaluations_per_cell_step": rep64.phaseEvaluations / (cells * a.steps),
-            "flash_failures": [int(rep64.flashFailures[0]), int(rep64.flashFailures[1])]}
+            "flash_failures": [int(rep64.flashFailures[0]), int(rep64.flashFailures[1])],
+            "flash_failure_share": [int(rep64.flashFailures[i]) / (cells * a.steps) for i in (0, 1)]}
     if a.flash == "mixed":
-        log(f"bench: mixed flash, {a.warmup} warmup + {a.steps} timed step(s)")
-        rep, wall, qmx_one, smx = run_steps(lib, step, q, states, a.cfl, a.steps, 1, a.warmup)
+        log(f"bench: mixed flash, {a.warmup} warmup + {a.steps} timed (graph) + {prof_steps} kernel-timed step(s)")
+        rep, wall, kern, qmx_one, smx = run_steps(lib, step, q, states, a.cfl, a.steps, 1, a.warmup, prof_steps)
         log(f"bench: mixed {wall:.3f} s for {a.steps} steps (device {rep.seconds:.3f} s)")
         accuracy = census(states, s64, smx)
         log("bench: converged roots of both steps' second flash")
@@ -375,11 +482,10 @@ def main():
         root, ok = converged_roots(lib, q64_one, states, a.case)
         accuracy["fp64_vs_converged_root"] = distance(s64, states, root, ok)
     else:
-        rep, wall, accuracy = rep64, wall64, None
+        rep, wall, kern, accuracy = rep64, wall64, kern64, None
     value = cells * a.steps / wall
-    t_flash = (rep.flashSeconds[0] + rep.flashSeconds[1]) / (2 * a.steps)
     mixed = a.flash == "mixed"
-    flash = flash_roofline(rep, rep64, cells, a.steps, mixed)
+    flash = flash_roofline(kern, kern64, cells, prof_steps, mixed)
     result = {
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
@@ -389,19 +495,25 @@ def main():
                    "flash": ("mixed: FP32 candidate search, FP64 refinement, stability and selection" if mixed
                              else "fp64 (plain, reference arithmetic)"),
                    "seed": a.seed, "two_phase_fraction": (round(info["two_phase_fraction"], 4)
-                                                          if "two_phase_fraction" in info else None)},
-        "launch_shapes": {"stage_tile_depth": int(rep.tileDepth), "flash_block": int(rep.flashBlock),
-                          "state_bins_per_axis": int(rep.stateBins), "state_cost_levels": int(rep.reserved),
-                          "fp64_flash_block": int(rep64.flashBlock), "fp64_state_bins_per_axis": int(rep64.stateBins),
-                          "fp64_state_cost_levels": int(rep64.reserved), "source": "autotuned in the untimed steps"},
-        "roofline": (search_roofline(flash, rep, a.steps, a.n, a.case) if mixed
+                                                          if "two_phase_fraction" in info else None),
+                   "timed_path": "steps replayed from captured HIP graphs (REACTIVE_AMD_STEP_GRAPH)",
+                   "kernel_times": f"HIP events on the step stream over {prof_steps} further step(s)"},
+        "launch_shapes": {"stage_tile_depth": int(kern.tileDepth), "flash_block": int(kern.flashBlock),
+                          "state_bins_per_axis": int(kern.stateBins), "state_cost_levels": int(kern.reserved),
+                          "fp64_flash_block": int(kern64.flashBlock), "fp64_state_bins_per_axis": int(kern64.stateBins),
+                          "fp64_state_cost_levels": int(kern64.reserved), "source": "autotuned in the untimed steps"},
+        "roofline": (search_roofline(flash, kern, prof_steps, a.n, a.case) if mixed
                      else dict(flash["fp64_flash"], bound="valu", unit="TFLOP/s")),
-        "roofline_stage": stage_roofline(rep, cells, a.steps, a.n, ns + 4),
+        "roofline_stage": stage_roofline(kern, cells, prof_steps, a.n, ns + 4),
         "roofline_flash": flash,
         "fp64": fp64,
         "speedup_vs_fp64": value / value64,
         "mixed_vs_fp64": accuracy,
         "flash_failures": [int(rep.flashFailures[0]), int(rep.flashFailures[1])],
+        # cells whose flash failed (they keep their step-start state), per flash and step; the
+        # reference driver's attempt loop (ReactiveFoam.C:1487-1494) would reject such a step and
+        # retry it at half the dt: these steps are counted as the resident step runs them (no retry)
+        "flash_failure_share": [int(rep.flashFailures[i]) / (cells * a.steps) for i in (0, 1)],
         "cfl_violations": int(rep.cflViolations), "dt": rep.dt,
     }
     if not a.no_cpu and a.case == "n2o":

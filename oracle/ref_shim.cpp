@@ -47,6 +47,45 @@ int ref_flash_recover(const void* model, const double* q, double energy, const v
     return o.success;
 }
 
+// One capillary-interface recovery, exactly ReactiveDeviceFlash::recoverCapillary
+// (reactiveDeviceFlash.h:572-590): equilibrium=0 is the frozen condensed
+// inventory solve from guess.liquidMass, equilibrium=1 the UV equilibrium
+// search with the boundary recovery.  Same counters layout as ref_flash_recover.
+int ref_flash_capillary(const void* model, const double* q, double energy, const void* guess, double color,
+                        double pressureJump, int equilibrium, void* out, uint64_t* counters) {
+    static thread_local ReactiveDeviceFlash::Model m;
+    std::memcpy(static_cast<void*>(&m), model, sizeof(m));
+    ReactiveDeviceFlash::Input in{};
+    for (int k = 0; k < m.ns; ++k) in.q[k] = q[k];
+    in.energy = energy;
+    std::memcpy(&in.guess, guess, sizeof(ReactiveThermoState));
+    const auto o = ReactiveDeviceFlash::recoverCapillary(m, in, color, pressureJump, equilibrium != 0, nullptr);
+    if (o.success) std::memcpy(out, &o.state, sizeof(ReactiveThermoState));
+    if (counters) {
+        counters[0] = o.counters.phases; counters[1] = o.counters.residuals;
+        counters[2] = o.counters.candidates; counters[3] = o.counters.stable;
+        counters[4] = o.counters.failures; counters[5] = o.counters.analyticJacobians;
+        counters[6] = o.counters.finiteDifferenceJacobians;
+        counters[7] = o.counters.mixedLinearAttempts; counters[8] = o.counters.mixedLinearAccepted;
+    }
+    return o.success;
+}
+
+int ref_flash_capillary_batch(const void* model, size_t count, const double* q, const double* energy,
+                              const void* guess, int equilibrium, void* out, int* success, uint64_t* counters) {
+    const auto& m = *static_cast<const ReactiveDeviceFlash::Model*>(model);
+    int ok = 0;
+    for (size_t c = 0; c < count; ++c) {
+        const char* g = static_cast<const char*>(guess) + c * sizeof(ReactiveThermoState);
+        char* o = static_cast<char*>(out) + c * sizeof(ReactiveThermoState);
+        std::memcpy(o, g, sizeof(ReactiveThermoState));
+        success[c] = ref_flash_capillary(model, q + c * m.ns, energy[c], g, 0.0, 0.0, equilibrium, o,
+                                         counters ? counters + 9 * c : nullptr);
+        ok += success[c];
+    }
+    return ok;
+}
+
 // Batch form for fixture generation.
 int ref_flash_batch(const void* model, size_t count, const double* q, const double* energy,
                     const void* guess, int useCache, void* out, int* success, uint64_t* counters) {

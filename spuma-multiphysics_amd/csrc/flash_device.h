@@ -1131,15 +1131,20 @@ struct Flash {
         bool gasStable = false;
         const bool prunable = m.stableGasPrune != 0;
         uint32_t i = 0;
-        if (i < set.count && set.c[i].kind == kGasCandidate) {
-            if (finishFrozen(R(0), set.c[i], candidate, prunable ? 4 : 0)) {
-                const uint32_t before = n.stable;
-                consider(candidate, best, have);
-                gasStable = n.stable > before && candidate.gasMass > 0;
-            } else {
-                ++n.failures;
-            }
-            ++i;
+        const bool recorded = i < set.count && set.c[i].kind == kGasCandidate;
+        // With seed pruning the reference decides on its own all-gas Newton,
+        // started from the cell's guess with stall limit 4
+        // (reactiveDeviceFlash.h:515-523); a polish from the single-precision
+        // point can converge where that solve crawls and fails, so the gas
+        // candidate is that very solve here.
+        bool gasOk = false;
+        if (prunable) gasOk = frozen(R(0), candidate, 4);
+        else if (recorded) gasOk = finishFrozen(R(0), set.c[i], candidate, 0);
+        if (recorded) ++i;
+        if (gasOk) {
+            const uint32_t before = n.stable;
+            consider(candidate, best, have);
+            gasStable = n.stable > before && candidate.gasMass > 0;
         } else {
             ++n.failures;
         }

@@ -7,7 +7,8 @@ host-compilable device flash (reactiveDeviceFlash.h), compiled from the
 sources under /root/reference by `make -C oracle ref`.  Each fixture holds
 inputs and the reference's outputs only.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py frozen     # only the named sets
 """
 import ctypes as C
 import os
@@ -33,6 +34,7 @@ def ref_lib():
     lib = C.CDLL(REF)
     V = C.c_void_p
     lib.ref_flash_batch.argtypes = [V, C.c_size_t, V, V, V, C.c_int, V, V, V]
+    lib.ref_flash_capillary_batch.argtypes = [V, C.c_size_t, V, V, V, C.c_int, V, V, V]
     lib.reactive_transport_create.restype = V
     lib.reactive_transport_create.argtypes = [C.c_int, V, V, V, V, V, V, V, C.c_char_p, C.c_size_t]
     lib.reactive_transport_rhs.argtypes = [V] * 7
@@ -50,6 +52,19 @@ def flash_set(lib, mb, q, e, seeds):
     buf = C.create_string_buffer(mb, len(mb))
     lib.ref_flash_batch(buf, n, p(np.ascontiguousarray(q)), p(np.ascontiguousarray(e)), p(seeds), 0, p(out),
                         p(success), p(counters))
+    return out, success, counters
+
+
+def frozen_set(lib, mb, q, e, guesses):
+    """recoverCapillary(..., equilibrium = 0) per cell: the frozen condensed
+    inventory solve at each guess's liquid masses."""
+    n = len(e)
+    out = guesses.copy()
+    success = np.zeros(n, np.int32)
+    counters = np.zeros((n, 9), np.uint64)
+    buf = C.create_string_buffer(mb, len(mb))
+    lib.ref_flash_capillary_batch(buf, n, p(np.ascontiguousarray(q)), p(np.ascontiguousarray(e)), p(guesses), 0,
+                                  p(out), p(success), p(counters))
     return out, success, counters
 
 
@@ -87,8 +102,50 @@ def edge_states(seed):
     return q, rho * e, seeds
 
 
-def main():
-    lib = ref_lib()
+def frozen_fixture(lib):
+    """Frozen-inventory recovery (the run_v2 / run_resident_v1 path with
+    equilibrium = 0): the 8^3 cube's inputs with two kinds of guesses (the
+    synthetic seed states and the reference's converged equilibrium states),
+    the edge classes, and the binary cube."""
+    g = np.load(os.path.join(HERE, "flash_n2o.npz"))
+    gb = np.load(os.path.join(HERE, "flash_binary.npz"))
+    mb, mb2 = g["model"].tobytes(), gb["model"].tobytes()
+    out = {}
+    for name, (m, q, e, guess) in {
+            "seeded": (mb, g["cube_q"], g["cube_energy"], g["cube_seeds"]),
+            "converged": (mb, g["cube_q"], g["cube_energy"],
+                          np.where(g["cube_success"] == 1, g["cube_out"], g["cube_seeds"])),
+            "edge": (mb, g["edge_q"], g["edge_energy"], g["edge_seeds"]),
+            "binary": (mb2, gb["q"], gb["energy"], gb["seeds"])}.items():
+        o, s, c = frozen_set(lib, m, q, e, guess)
+        out.update({f"{name}_q": q, f"{name}_energy": e, f"{name}_guess": guess, f"{name}_out": o,
+                    f"{name}_success": s, f"{name}_counters": c})
+        print("frozen", name, "success", int(s.sum()), "/", len(e))
+    np.savez_compressed(os.path.join(HERE, "flash_frozen.npz"), model=np.frombuffer(mb, np.uint8),
+                        model_binary=np.frombuffer(mb2, np.uint8), **out)
+
+
+def prune_fixture(lib):
+    """The opt-in stable-gas pruning (Model::stableGasPrune = 1,
+    reactiveDeviceFlash.h:509-523): the equilibrium flash of the golden cube,
+    edge and binary inputs with the pruning model."""
+    g = np.load(os.path.join(HERE, "flash_n2o.npz"))
+    gb = np.load(os.path.join(HERE, "flash_binary.npz"))
+    mb = model.model_bytes(model.build_model(stable_gas_prune=1))
+    mb2 = model.model_bytes(model.build_model(gas_species=("N2O", "N2"), stable_gas_prune=1))
+    out = {}
+    for name, (m, q, e, seeds) in {
+            "cube": (mb, g["cube_q"], g["cube_energy"], g["cube_seeds"]),
+            "edge": (mb, g["edge_q"], g["edge_energy"], g["edge_seeds"]),
+            "binary": (mb2, gb["q"], gb["energy"], gb["seeds"])}.items():
+        o, s, c = flash_set(lib, m, q, e, seeds)
+        out.update({f"{name}_out": o, f"{name}_success": s, f"{name}_counters": c})
+        print("prune", name, "success", int(s.sum()), "/", len(e))
+    np.savez_compressed(os.path.join(HERE, "flash_prune.npz"), model=np.frombuffer(mb, np.uint8),
+                        model_binary=np.frombuffer(mb2, np.uint8), **out)
+
+
+def base_fixtures(lib):
     mb = model.model_bytes(model.build_model())
     # flash: a seeded 8^3 cube (cube_case inputs) and the edge classes
     faces, volume, qc, seeds, _ = cases.pure_n2o_cube(8, seed=1)
@@ -142,5 +199,14 @@ def main():
     print("binary success", int(success2.sum()), "/", len(e2))
 
 
+SETS = {"base": base_fixtures, "frozen": frozen_fixture, "prune": prune_fixture}
+
+
+def main(names):
+    lib = ref_lib()
+    for name in names or list(SETS):
+        SETS[name](lib)
+
+
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -175,3 +175,15 @@ def test_error_contracts():
     assert success[5] == 0 and prof.deviceFailures >= 1
     assert np.array_equal(states[5:6].view(np.uint8), seeds[5:6].view(np.uint8))
     lib.reactive_gpu_hem_destroy_v1(h)
+
+
+@pytest.mark.parametrize("part", ["cube", "edge", "binary"])
+def test_flash_stable_gas_prune_bitwise(part):
+    """The opt-in stable-gas pruning (stableGasPrune = 1, reactiveDeviceFlash.h:509-523)
+    on the golden inputs: the device equals the twin oracle bit for bit, and the
+    twin's glibc build equals the reference (test_oracle_golden.py)."""
+    g = np.load(os.path.join(GOLDEN, "flash_prune.npz"))
+    base = np.load(os.path.join(GOLDEN, "flash_binary.npz" if part == "binary" else "flash_n2o.npz"))
+    key = "" if part == "binary" else f"{part}_"
+    mb = (g["model_binary"] if part == "binary" else g["model"]).tobytes()
+    check_bitwise(mb, base[f"{key}q"], base[f"{key}energy"], base[f"{key}seeds"])

@@ -69,3 +69,53 @@ def test_flash_fresh_inputs_vs_reference():
                         seeds.ctypes.data, 1, out.ctypes.data, success.ctypes.data, None)
     mine, msuccess, _ = oracle_flash(mb, q, e, seeds)
     assert np.array_equal(success, msuccess) and out.tobytes() == mine.tobytes()
+
+
+def oracle_frozen(mb, q, e, guesses, twin=False):
+    """The oracle's frozen-inventory recovery, cell by cell: (states, success, counters[9])."""
+    from helpers import p
+    lib = oracle(twin)
+    img = C.create_string_buffer(mb, len(mb))
+    n = len(e)
+    out = guesses.copy()
+    success = np.zeros(n, np.int32)
+    counters = np.zeros((n, 9), np.uint64)
+    q = np.ascontiguousarray(q)
+    for c in range(n):
+        o = guesses[c:c + 1].copy()
+        success[c] = lib.oracle_flash_frozen(img, p(q[c]), C.c_double(e[c]), p(guesses[c:c + 1]), p(o),
+                                             p(counters[c]))
+        if success[c]:
+            out[c] = o[0]
+    return out, success, counters
+
+
+@pytest.mark.parametrize("part", ["seeded", "converged", "edge", "binary"])
+def test_frozen_recovery_bitwise_vs_reference(part):
+    """oracle_flash_frozen against the reference's recoverCapillary(...,
+    equilibrium = 0) (reactiveDeviceFlash.h:572-590) on seeded, converged,
+    edge-class and binary guesses: states, success flags and counters."""
+    g = load("flash_frozen.npz")
+    mb = (g["model_binary"] if part == "binary" else g["model"]).tobytes()
+    out, success, counters = oracle_frozen(mb, g[f"{part}_q"], g[f"{part}_energy"], g[f"{part}_guess"])
+    assert np.array_equal(success, g[f"{part}_success"])
+    assert 0 < success.sum() < len(success) or part != "edge"
+    assert out.tobytes() == g[f"{part}_out"].tobytes()
+    assert np.array_equal(counters, g[f"{part}_counters"])
+
+
+@pytest.mark.parametrize("part", ["cube", "edge", "binary"])
+def test_stable_gas_prune_bitwise_vs_reference(part):
+    """The opt-in stable-gas pruning (stableGasPrune = 1,
+    reactiveDeviceFlash.h:509-523): the oracle's flash with the pruning model
+    equals the reference's, and pruning changes the work (fewer candidates)."""
+    g = load("flash_prune.npz")
+    base = load("flash_binary.npz" if part == "binary" else "flash_n2o.npz")
+    key = "" if part == "binary" else f"{part}_"
+    mb = (g["model_binary"] if part == "binary" else g["model"]).tobytes()
+    assert oracle().oracle_model_supported(C.create_string_buffer(mb, len(mb))) == 1
+    out, success, counters = oracle_flash(mb, base[f"{key}q"], base[f"{key}energy"], base[f"{key}seeds"])
+    assert np.array_equal(success, g[f"{part}_success"])
+    assert out.tobytes() == g[f"{part}_out"].tobytes()
+    assert np.array_equal(counters, g[f"{part}_counters"])
+    assert counters[:, 2].sum() < base[f"{key}counters"][:, 2].sum()
