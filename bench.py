@@ -350,11 +350,12 @@ def distance(s, states0, root, ok):
     cells = len(s)
     fail = (s.view(np.uint8).reshape(cells, -1) == states0.view(np.uint8).reshape(cells, -1)).all(1)
     use = ~fail & ok & (s["activeLiquids"] == root["activeLiquids"])
-    out = {"compared_cells": int(use.sum())}
+    out = {"compared_cells": int(use.sum()),
+           "other_phase_set_than_root": int((~fail & ok & (s["activeLiquids"] != root["activeLiquids"])).sum())}
     for f in ("T", "p"):
         r = np.abs(s[f][use] - root[f][use]) / np.abs(root[f][use])
         out[f] = {"max": float(r.max()), "p99": float(np.percentile(r, 99)),
-                  "share_le_1e-10": float((r <= 1e-10).mean())}
+                  "share_le_1e-10": float((r <= 1e-10).mean()), "cells_over_1e-10": int((r > 1e-10).sum())}
     return out
 
 

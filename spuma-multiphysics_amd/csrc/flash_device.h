@@ -1114,6 +1114,40 @@ struct Flash {
         x[2] = R(c.x[2]);
         return activeNewton(x, scale, value, 2);
     }
+    // The winner of an FP64 search run inside the refinement (a deferred
+    // cell, or the FP64 restarts) stops, like the reference's, at the first
+    // iterate inside the tolerances; it is refined further from its own
+    // point as every recorded candidate is (up to two more Newton steps
+    // toward 1e-3 of the tolerances).  A polish that fails or lands on
+    // another phase set keeps the searched result.
+    RAMD_NOINLINE void polishBest(E& best) {
+        const int c = m.condensable;
+        const R p0 = guessP, T0 = guessT;
+        guessP = best.p;
+        guessT = best.T;
+        E value;
+        bool ok;
+        if (!(best.liqMass > 0)) {
+            ok = frozen(R(0), value, 0, R(0), 2);
+        } else if (!NL || best.liqMass >= q[c]) {
+            ok = frozen(best.liqMass, value, 0, R(0), 2);
+        } else {
+            R x[3], scale;
+            activeStart(R(.5), false, R(0), x, scale);
+            x[0] = m_log(best.p);
+            x[1] = m_log(best.T);
+            x[2] = best.liqMass / q[c];
+            ok = activeNewton(x, scale, value, 2);
+        }
+        guessP = p0;
+        guessT = T0;
+        if (ok && value.active == best.active) best = value;
+    }
+    RAMD_NOINLINE bool searchPolished(E& best) {
+        if (!search(best)) return false;
+        polishBest(best);
+        return true;
+    }
     // deferred (when given): instead of running the plain FP64 search or the
     // FP64 restart search inline, the refinement stops with *deferred = true;
     // the caller reruns such cells with deferred == nullptr in a launch of
@@ -1124,7 +1158,7 @@ struct Flash {
             if (!fin(q[k]) || q[k] < 0) return false;
         if (set.flags & kOverflow) {
             if (deferred) { *deferred = true; return false; }
-            return search(best);
+            return searchPolished(best);
         }
         bool have = false;
         E candidate;
@@ -1137,8 +1171,10 @@ struct Flash {
         // (reactiveDeviceFlash.h:515-523); a polish from the single-precision
         // point can converge where that solve crawls and fails, so the gas
         // candidate is that very solve here.
+        // (Its polish, two more Newton steps once it has converged, does not
+        // change whether it converges.)
         bool gasOk = false;
-        if (prunable) gasOk = frozen(R(0), candidate, 4);
+        if (prunable) gasOk = frozen(R(0), candidate, 4, R(0), 2);
         else if (recorded) gasOk = finishFrozen(R(0), set.c[i], candidate, 0);
         if (recorded) ++i;
         if (gasOk) {
@@ -1154,7 +1190,7 @@ struct Flash {
             // the single-precision search pruned the seeds FP64 needs
             have = false;
             if (deferred) { *deferred = true; return false; }
-            return search(best);
+            return searchPolished(best);
         }
         // FP64 prunes where the single-precision search did not: the
         // reference's search stops at the gas (and pure-liquid) candidate,
@@ -1182,6 +1218,7 @@ struct Flash {
                             consider(candidate, best, have);
                         else ++n.failures;
                     }
+                if (have) polishBest(best);
             } else {
                 for (; i < set.count; ++i) {
                     if (finishActive(set.c[i], candidate)) consider(candidate, best, have, true);

@@ -7,10 +7,14 @@ mutol = 1e-7; in compressed-liquid and two-phase cells that iterate lies up to
 refines each candidate to 1e-3 of those tolerances.  Each step's second flash
 is therefore compared with the converged root of its own inputs: the oracle
 rerun with tolerances 1e-3 as tight (vtol = etol = 1e-12, mutol = 1e-10),
-seeded as the step seeds it.  Gates: no flips (cells where the two flashes
-fail differently or choose another phase set) beyond 0.2 %, mixed within
-1e-10 (T, p) of the converged root for 99 % of cells and within 1e-8 for all,
-and mixed within the FP64 flash's own distance to its root plus 1e-10."""
+seeded as the step seeds it.
+
+Gates: no flips (cells where the two flashes fail differently or choose
+another phase set) beyond 0.2 %; the mixed flash within 1e-10 (T, p) of the
+converged root on every cell whose phase set the root shares (the metric's
+accuracy condition), with at most 0.2 % of the converged cells choosing
+another phase set than the root.  On the binary box some cells must take the
+deferred FP64 queue, so that path is covered."""
 import ctypes as C
 
 import numpy as np
@@ -95,15 +99,19 @@ def test_mixed_step_against_fp64_and_converged_root(kind):
     for name, q in (("fp64", q64), ("mixed", qmx)):
         r, ok, _ = oracle_flash(tight, q[:, :ns], cases.internal_energy(q, ns), case["states"])
         roots[name] = (r, ok == 1)
-    report = {}
+    report, dropped, uses = {}, {}, {}
     for name, s, fl in (("fp64", s64, f64), ("mixed", smx, fmx)):
         r, ok = roots[name]
         use = ~fl & ok & (s["activeLiquids"] == r["activeLiquids"])
+        dropped[name] = int((~fl & ok & (s["activeLiquids"] != r["activeLiquids"])).sum())
+        uses[name] = use
         report[name] = {f: rel(s[f][use], r[f][use]) for f in ("T", "p")}
-    print(f"\ncells {cells}, flips {flips}, FP64 failures {int(f64.sum())}, mixed failures {int(fmx.sum())}")
+    print(f"\ncells {cells}, flips {flips}, FP64 failures {int(f64.sum())}, mixed failures {int(fmx.sum())}, "
+          f"deferred {int(rmx.queuedCells)}, other phase set than the root {dropped}")
     for name, d in report.items():
         print(f"{name} vs converged root:", {f: (float(v.max()), float(np.percentile(v, 99)),
-                                                 float((v <= 1e-10).mean())) for f, v in d.items()})
+                                                 float((v <= 1e-10).mean()), int((v > 1e-10).sum()))
+                                             for f, v in d.items()})
     same = both & (smx["activeLiquids"] == s64["activeLiquids"])
     errors = {}
     for f in FIELDS:  # absolute for fractions and liquid mass (zero in gas cells), relative otherwise
@@ -115,7 +123,9 @@ def test_mixed_step_against_fp64_and_converged_root(kind):
     print("phase evaluations FP64 / mixed FP64 / mixed FP32:", r64.phaseEvaluations, rmx.phaseEvaluations,
           rmx.singlePhaseEvaluations)
     assert flips <= max(2, cells // 500)
+    assert dropped["mixed"] <= max(2, cells // 500)
     for f in ("T", "p"):
         v = report["mixed"][f]
-        assert (v <= 1e-10).mean() >= 0.99, (f, float((v <= 1e-10).mean()))
-        assert v.max() <= 1e-8, (f, float(v.max()))
+        assert v.max() <= 1e-10, (f, float(v.max()), int((v > 1e-10).sum()))
+    if kind == "n2o-n2":
+        assert rmx.queuedCells > 0
