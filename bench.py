@@ -76,6 +76,8 @@ def prepare(n, seed, case="n2o"):
     bad = np.where(success == 0)[0]
     if bad.size:  # a failed synthetic cell takes the state and conserved values of a converged neighbour
         good = np.where(success == 1)[0]
+        if not good.size:
+            raise RuntimeError("the device flash recovered none of the synthetic cells")
         src = good[np.searchsorted(good, bad).clip(0, good.size - 1)]
         q[bad] = q[src]
         states[bad] = states[src]

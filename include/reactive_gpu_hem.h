@@ -77,9 +77,12 @@ void reactive_gpu_hem_destroy_v1(void* hem);
 /* reactiveGpuHem.h:51-52: host arrays, count <= capacity */
 int reactive_gpu_hem_run_v1(void* hem, const double* q, const double* energy, size_t count,
     ReactiveThermoState* state, int* success, ReactiveGpuHemProfileV1* profile, char* error, size_t errorSize);
-/* reactiveGpuHem.h:63-65: capillary input per cell; only color=0,
- * pressureJump=0 (no interface) is accepted by this build, with equilibrium
- * 1 (UV equilibrium flash) or 0 (frozen condensed inventory). */
+/* reactiveGpuHem.h:63-65: capillary input per cell.  This build accepts
+ * color = 0, pressureJump = 0 (no interface) with equilibrium = 0, the frozen
+ * condensed-inventory recovery (reactiveDeviceFlash.h:572-590).  A batch
+ * holding a cell with an interface or with equilibrium = 1 (the capillary
+ * UV equilibrium search with boundary recovery) is rejected with rc 1 before
+ * any flash runs; nothing is written. */
 int reactive_gpu_hem_run_v2(void* hem, const double* q, const double* energy,
     const ReactiveGpuHemCapillaryInputV2* capillary, size_t count,
     ReactiveThermoState* state, int* success, ReactiveGpuHemProfileV1* profile, char* error, size_t errorSize);
@@ -88,7 +91,8 @@ int reactive_gpu_hem_initialize_tp_v1(void* hem, double* q, double* energy,
     const ReactiveGpuHemCapillaryInputV2* capillary, size_t count,
     ReactiveThermoState* state, int* success, ReactiveGpuHemProfileV1* profile, char* error, size_t errorSize);
 /* reactiveGpuHem.h:81-84: every pointer is DEVICE memory; order[0..count)
- * names the cells, output[cell] receives each result. */
+ * names the cells, output[cell] receives each result.  Capillary input as
+ * for run_v2 (frozen-inventory cells only). */
 int reactive_gpu_hem_run_resident_v1(void* hem, const double* q, const double* energy,
     const ReactiveThermoState* seed, const ReactiveGpuHemCapillaryInputV2* capillary,
     const uint32_t* order, size_t count, ReactiveGpuHemOutputV1* output,

@@ -6,9 +6,10 @@
 // against glibc, results equal the compiled reference bit for bit.
 //
 // Scope: pure-liquid condensables (condensedKind 0), no capillary interface
-// (color = pressureJump = 0), boundaryRecovery = 0, finite-difference or
-// analytic-free Jacobian (analyticJacobian = 0).  Other model images are
-// rejected by oracle_model_supported().
+// (color = pressureJump = 0), boundaryRecovery = 0; the finite-difference
+// Jacobian or, with analyticJacobian = 1, the analytic one for one active
+// liquid (reactiveDeviceFlashJacobian.h).  Other model images are rejected by
+// oracle_model_supported().
 #include "oracle.h"
 #include <cfloat>
 #include <cmath>
@@ -445,6 +446,186 @@ Status validateTable(const Table& t) {  // :157-195
 }
 
 // ---------------------------------------------------------------------------
+// Analytic active-set Jacobian (reactiveDeviceFlashJacobian.h), for one
+// active liquid in the flat (no-capillary), non-adaptive flash.
+
+// Partial molar h, u, v of every gas species (phasePartials, :68-132).
+bool phasePartials(const Table& t, double T, const double* Y, const PRState& state, double* hbar, double* ubar,
+                   double* vbar, double* x, double& W) {
+    if (massToMole(t, Y, x, W) != OK) return false;
+    double a = 0, b = 0, aa = 0, da = 0, d2a = 0;
+    if (mixing(t, T, x, a, b, aa, da, d2a) != OK) return false;
+    const double V = state.V, p = state.p, RT = R * T;
+    const double vmb = V - b;
+    const double den = V * V + 2.0 * V * b - b * b;
+    if (!(vmb > 0) || !(den > 0) || !(state.dpdV < 0)) return false;
+    const double logRatio = olog((V + (1.0 + kSqrt2) * b) / (V + (1.0 - kSqrt2) * b));
+    const double L = logRatio / (2.0 * kSqrt2 * b);
+    const double fac = T * da - aa;
+    const double fac2 = V + T * state.dpdT / state.dpdV;
+    double q[kMaxSpecies]{}, dq[kMaxSpecies]{}, pp[kMaxSpecies]{}, tmp[kMaxSpecies]{};
+    for (int k = 0; k < t.ns; ++k) {
+        const Species& sp = t.sp[k];
+        const double Tc = sp.a * kOmegaB / (sp.b * kOmegaA * R);
+        const double sqrtTr = std::sqrt(T / Tc);
+        const double f = 1.0 + sp.kappa * (1.0 - sqrtTr);
+        if (ab(f) <= 1e-14) return false;
+        const double sign = f > 0 ? 1.0 : -1.0;
+        q[k] = ab(f);
+        dq[k] = -sign * sp.kappa / (2.0 * Tc * sqrtTr);
+    }
+    for (int k = 0; k < t.ns; ++k)
+        for (int i = 0; i < t.ns; ++i) {
+            const double pair = t.aij[k][i] * q[k] * q[i];
+            pp[k] += x[i] * pair;
+            tmp[k] += x[i] * pair * (2.0 * dq[i] / q[i] + 2.0 * dq[k] / q[k]);
+        }
+    for (int k = 0; k < t.ns; ++k) {
+        double cpR = 0, hRT = 0, sR = 0;
+        nasa(region(t, k, T), T, cpR, hRT, sR);
+        const double bk = t.sp[k].b;
+        const double dpdni = RT / vmb + RT * bk / (vmb * vmb) - 2.0 * pp[k] / den + 2.0 * vmb * aa * bk / (den * den);
+        const double hEv = V * dpdni - RT - bk / (2.0 * kSqrt2 * b * b) * logRatio * fac + V * bk / (b * den) * fac +
+                           L * (T * tmp[k] - 2.0 * pp[k]);
+        hbar[k] = RT * hRT + hEv - fac2 * dpdni;
+        const double volumeNumerator = RT + RT * b / vmb + RT * bk / vmb + RT * b * bk / (vmb * vmb) -
+                                       2.0 * V * pp[k] / den + 2.0 * V * vmb * aa * bk / (den * den);
+        const double volumeDenominator = p + RT * b / (vmb * vmb) + aa / den - 2.0 * V * (V + b) * aa / (den * den);
+        vbar[k] = volumeNumerator / volumeDenominator;
+        ubar[k] = hbar[k] - p * vbar[k];
+        if (!fin(hbar[k]) || !fin(vbar[k]) || !fin(ubar[k])) return false;
+    }
+    return true;
+}
+
+// d(mu_k/RT)/dn_j at fixed (T, p) for k = j = the condensable
+// (gasCompositionPartials, :136-198, one row and column).
+bool gasCompositionPartial(const Table& t, double T, const double* x, double totalMoles, const PRState& state, int k,
+                           double& dpsi) {
+    if (!(totalMoles > 0)) return false;
+    double a = 0, b = 0, aa = 0, da = 0, d2a = 0;
+    if (mixing(t, T, x, a, b, aa, da, d2a) != OK) return false;
+    const double V = state.V, RT = R * T;
+    const double vmb = V - b, den = V * V + 2.0 * V * b - b * b;
+    if (!(vmb > 0) || !(den > 0) || !(state.dpdV < 0)) return false;
+    const double c1 = 1.0 + kSqrt2, c2 = 1.0 - kSqrt2;
+    const double A = V + c1 * b, B = V + c2 * b;
+    const double logRatio = olog(A / B);
+    const double C = 2.0 * kSqrt2;
+    double q[kMaxSpecies]{}, pp[kMaxSpecies]{};
+    for (int i = 0; i < t.ns; ++i) {
+        const Species& sp = t.sp[i];
+        const double Tc = sp.a * kOmegaB / (sp.b * kOmegaA * R);
+        q[i] = ab(1.0 + sp.kappa * (1.0 - std::sqrt(T / Tc)));
+        if (!(q[i] > 1e-14)) return false;
+    }
+    for (int i = 0; i < t.ns; ++i)
+        for (int j = 0; j < t.ns; ++j) pp[i] += x[j] * t.aij[i][j] * q[i] * q[j];
+    const double Pb = RT / (vmb * vmb) + 2.0 * aa * vmb / (den * den);
+    const int j = k;
+    const double db = t.sp[j].b - b;
+    const double daa = 2.0 * (pp[j] - aa);
+    const double dV = -(Pb * db - daa / den) / state.dpdV;
+    const double dDen = 2.0 * (V + b) * dV + 2.0 * (V - b) * db;
+    const double dLogRatio = (dV + c1 * db) / A - (dV + c2 * db) / B;
+    if (!(x[k] > 0)) return false;
+    const double dxk = 1.0 - x[k];
+    const double dpp = t.aij[k][j] * q[k] * q[j] - pp[k];
+    const double bk = t.sp[k].b;
+    const double num = 2.0 * b * pp[k] - aa * bk;
+    const double dnum = 2.0 * db * pp[k] + 2.0 * b * dpp - daa * bk;
+    const double K = num / (C * b * b);
+    const double dK = (dnum - 2.0 * num * db / b) / (C * b * b);
+    const double dResidual = -RT * dV / V + RT * (dV / V - (dV - db) / vmb) - RT * bk * (dV - db) / (vmb * vmb) -
+                             dK * logRatio - K * dLogRatio +
+                             (-aa * bk * V / (b * den)) * (daa / aa + dV / V - db / b - dDen / den);
+    const double dPsiDirection = dxk / x[k] + dResidual / RT;
+    dpsi = dPsiDirection / totalMoles;
+    return fin(dpsi);
+}
+
+// The condensable's partial h, u, v and d(mu/RT)/dn in the gas, from the
+// gas root's molar volume at (T, p) (activeGasDerivatives, :203-308, n = 1).
+bool activeGasDerivatives(const Table& t, double T, double p, double molarVolume, const double* Y, double gasMass,
+                          int k, double& hbar, double& ubar, double& vbar, double& dpsi) {
+    if (!(T > 0) || !(p > 0) || !(molarVolume > 0) || !(gasMass > 0)) return false;
+    double x[kMaxSpecies]{}, W = 0;
+    if (massToMole(t, Y, x, W) != OK) return false;
+    const double totalMoles = gasMass / W;
+    const double V = molarVolume, RT = R * T;
+    double q[kMaxSpecies]{}, dq[kMaxSpecies]{};
+    double b = 0, aa = 0, da = 0;
+    for (int i = 0; i < t.ns; ++i) {
+        const Species& sp = t.sp[i];
+        const double Tc = sp.a * kOmegaB / (sp.b * kOmegaA * R);
+        const double sqrtTr = std::sqrt(T / Tc);
+        const double f = 1.0 + sp.kappa * (1.0 - sqrtTr);
+        if (!(Tc > 0) || !(sqrtTr > 0) || ab(f) <= 1e-14) return false;
+        const double sign = f > 0 ? 1.0 : -1.0;
+        q[i] = ab(f);
+        dq[i] = -sign * sp.kappa / (2.0 * Tc * sqrtTr);
+        b += x[i] * sp.b;
+    }
+    for (int i = 0; i < t.ns; ++i)
+        for (int j = 0; j < t.ns; ++j) {
+            const double w = x[i] * x[j] * t.aij[i][j];
+            aa += w * q[i] * q[j];
+            da += w * (dq[i] * q[j] + q[i] * dq[j]);
+        }
+    const double vmb = V - b, den = V * V + 2.0 * V * b - b * b;
+    if (!(b > 0) || !(aa > 0) || !(vmb > 0) || !(den > 0)) return false;
+    const double calculatedP = RT / vmb - aa / den;
+    const double dpdV = -RT / (vmb * vmb) + 2.0 * aa * (V + b) / (den * den);
+    const double dpdT = R / vmb - da / den;
+    if (!(dpdV < 0) || !fin(calculatedP) || !fin(dpdT) || ab(calculatedP - p) > 1e-7 * hi(1.0, p)) return false;
+    const double c1 = 1.0 + kSqrt2, c2 = 1.0 - kSqrt2;
+    const double A = V + c1 * b, B = V + c2 * b;
+    const double logRatio = olog(A / B);
+    const double L = logRatio / (2.0 * kSqrt2 * b);
+    const double fac = T * da - aa;
+    const double fac2 = V + T * dpdT / dpdV;
+    if (!(x[k] > 0)) return false;
+    double pp = 0, tmp = 0;
+    for (int i = 0; i < t.ns; ++i) {
+        const double pair = t.aij[k][i] * q[k] * q[i];
+        pp += x[i] * pair;
+        tmp += x[i] * pair * (2.0 * dq[i] / q[i] + 2.0 * dq[k] / q[k]);
+    }
+    double cpR = 0, hRT = 0, sR = 0;
+    nasa(region(t, k, T), T, cpR, hRT, sR);
+    const double bk = t.sp[k].b;
+    const double dpdni = RT / vmb + RT * bk / (vmb * vmb) - 2.0 * pp / den + 2.0 * vmb * aa * bk / (den * den);
+    const double hEv = V * dpdni - RT - bk / (2.0 * kSqrt2 * b * b) * logRatio * fac + V * bk / (b * den) * fac +
+                       L * (T * tmp - 2.0 * pp);
+    hbar = RT * hRT + hEv - fac2 * dpdni;
+    const double volumeNumerator = RT + RT * b / vmb + RT * bk / vmb + RT * b * bk / (vmb * vmb) -
+                                   2.0 * V * pp / den + 2.0 * V * vmb * aa * bk / (den * den);
+    const double volumeDenominator = p + RT * b / (vmb * vmb) + aa / den - 2.0 * V * (V + b) * aa / (den * den);
+    vbar = volumeNumerator / volumeDenominator;
+    ubar = hbar - p * vbar;
+    if (!fin(hbar) || !fin(ubar) || !fin(vbar)) return false;
+    const double Pb = RT / (vmb * vmb) + 2.0 * aa * vmb / (den * den);
+    const double C = 2.0 * kSqrt2;
+    const int j = k;
+    const double db = t.sp[j].b - b;
+    const double daa = 2.0 * (pp - aa);
+    const double dV = -(Pb * db - daa / den) / dpdV;
+    const double dDen = 2.0 * (V + b) * dV + 2.0 * (V - b) * db;
+    const double dLogRatio = (dV + c1 * db) / A - (dV + c2 * db) / B;
+    const double dxk = 1.0 - x[k];
+    const double dpp = t.aij[k][j] * q[k] * q[j] - pp;
+    const double num = 2.0 * b * pp - aa * bk;
+    const double dnum = 2.0 * db * pp + 2.0 * b * dpp - daa * bk;
+    const double K = num / (C * b * b);
+    const double dK = (dnum - 2.0 * num * db / b) / (C * b * b);
+    const double dResidual = -RT * dV / V + RT * (dV / V - (dV - db) / vmb) - RT * bk * (dV - db) / (vmb * vmb) -
+                             dK * logRatio - K * dLogRatio +
+                             (-aa * bk * V / (b * den)) * (daa / aa + dV / V - db / b - dDen / den);
+    dpsi = (dxk / x[k] + dResidual / RT) / totalMoles;
+    return fin(dpsi);
+}
+
+// ---------------------------------------------------------------------------
 // HEM UV flash (reactiveDeviceFlash.h)
 struct Evaluation {  // :64-73
     ThermoState state;
@@ -792,6 +973,126 @@ struct Flash {
         }
         return true;
     }
+    // Analytic Jacobian from the residual point's own properties
+    // (buildCached, reactiveDeviceFlashJacobian.h:535-640; one active liquid,
+    // flat interface, normal coordinates).
+    bool buildCached(const int* active, int n, const double* x, double scale, const Evaluation& v, double jac[4][4]) {
+        const double p = v.state.p, T = v.state.T;
+        if (n != 1 || !(p > 0) || !(T > 0) || !(scale > 0) || !(v.state.gasMass > 0) || !(v.gasMolarVolume > 0) ||
+            !fin(p) || !fin(T) || !fin(scale) || !fin(v.Vp) || !fin(v.VT) || !fin(v.Ep) || !fin(v.ET) ||
+            !fin(v.state.gasMass) || !fin(v.gasMolarVolume))
+            return false;
+        double Y[kMaxSpecies]{};
+        for (int k = 0; k < m.ns; ++k) {
+            if (!fin(q[k]) || q[k] < 0) return false;
+            Y[k] = q[k];
+        }
+        for (int i = 0; i < m.nl; ++i) {
+            const int k = m.condensable[i];
+            if (k < 0 || k >= m.ns || !fin(v.gasCondensable[i]) || v.gasCondensable[i] < 0 ||
+                v.gasCondensable[i] > q[k])
+                return false;
+            Y[k] = v.gasCondensable[i];
+        }
+        const int i = active[0];
+        if (i < 0 || i >= m.nl) return false;
+        const int k = m.condensable[i];
+        if (!(q[k] > 0)) return false;
+        if (!(x[2] >= 0 && x[2] < 1)) return false;
+        const double product = x[2] * q[k];  // the residual's own two roundings (:35-47)
+        const double expectedGas = q[k] - product;
+        const double dLiquid = q[k];
+        if (!(expectedGas > 0) || v.gasCondensable[i] != expectedGas) return false;
+        if (!(v.liquidMask & (1u << i)) || !fin(v.liquidHbar[i]) || !fin(v.liquidVbar[i]) || !(v.liquidVbar[i] > 0))
+            return false;
+        for (int l = 0; l < m.nl; ++l)
+            if (l != i && v.gasCondensable[l] != q[m.condensable[l]]) return false;
+        double reconstructed = 0;
+        for (int s = 0; s < m.ns; ++s) reconstructed += Y[s];
+        if (reconstructed != v.state.gasMass) return false;
+        for (int s = 0; s < m.ns; ++s) Y[s] /= v.state.gasMass;
+        double hbar, ubar, vbar, dpsi;
+        if (!activeGasDerivatives(m.phase[0], T, p, v.gasMolarVolume, Y, v.state.gasMass, k, hbar, ubar, vbar, dpsi))
+            return false;
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) jac[r][c] = 0;
+        jac[0][0] = v.Vp * p;
+        jac[0][1] = v.VT * T;
+        jac[1][0] = v.Ep * p / scale;
+        jac[1][1] = v.ET * T / scale;
+        const double Wk = m.phase[0].sp[k].W, Wl = m.phase[i + 1].sp[0].W;
+        const double liquidUbar = v.liquidHbar[i] - p * v.liquidVbar[i];
+        jac[0][2] = dLiquid * (v.liquidVbar[i] / Wl - vbar / Wk);
+        jac[1][2] = dLiquid * (liquidUbar / Wl - ubar / Wk) / scale;
+        jac[2][0] = p * (v.liquidVbar[i] - vbar) / (R * T);
+        jac[2][1] = -(v.liquidHbar[i] - hbar) / (R * T);
+        jac[2][2] = dpsi * dLiquid / Wk;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+                if (!fin(jac[r][c])) return false;
+        return true;
+    }
+    // Analytic Jacobian from fresh gas and liquid states at (p, T)
+    // (build, reactiveDeviceFlashJacobian.h:436-530; one active liquid).
+    bool buildFresh(const int* active, int n, const double* x, double p, double T, double scale, const Evaluation& v,
+                    double jac[4][4]) {
+        if (n != 1 || !(p > 0) || !(T > 0) || !(scale > 0)) return false;
+        double gasMass[kMaxSpecies]{}, Y[kMaxSpecies]{}, total = 0;
+        for (int s = 0; s < m.ns; ++s) gasMass[s] = q[s];
+        const int i = active[0];
+        if (i < 0 || i >= m.nl) return false;
+        const int k = m.condensable[i];
+        if (k < 0 || k >= m.ns || !(q[k] > 0)) return false;
+        if (!(x[2] >= 0 && x[2] < 1)) return false;
+        gasMass[k] -= x[2] * q[k];
+        const double dLiquid = q[k];
+        for (int s = 0; s < m.ns; ++s) total += gasMass[s];
+        if (!(total > 0)) return false;
+        for (int s = 0; s < m.ns; ++s) Y[s] = gasMass[s] / total;
+        PRState gas{};
+        ++count.phases;
+        if (evaluateTP(m.phase[0], T, p, Y, GasRoot, gas, false) != OK) return false;
+        double hbar[kMaxSpecies]{}, ubar[kMaxSpecies]{}, vbar[kMaxSpecies]{}, xm[kMaxSpecies]{}, W = 0;
+        if (!phasePartials(m.phase[0], T, Y, gas, hbar, ubar, vbar, xm, W)) return false;
+        const double totalMoles = total / W;
+        PRState liquid{};
+        double pure[kMaxSpecies]{};
+        pure[0] = 1;
+        ++count.phases;
+        if (evaluateTP(m.phase[i + 1], T, p, pure, LiquidRoot, liquid, false) != OK) return false;
+        double dpsi;
+        if (!gasCompositionPartial(m.phase[0], T, xm, totalMoles, gas, k, dpsi)) return false;
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) jac[r][c] = 0;
+        jac[0][0] = v.Vp * p;
+        jac[0][1] = v.VT * T;
+        jac[1][0] = v.Ep * p / scale;
+        jac[1][1] = v.ET * T / scale;
+        const double Wk = m.phase[0].sp[k].W;
+        jac[0][2] = dLiquid * (1.0 / liquid.rho - vbar[k] / Wk);
+        jac[1][2] = dLiquid * (liquid.e - ubar[k] / Wk) / scale;
+        const double Wl = m.phase[i + 1].sp[0].W;
+        const double liquidHbar = liquid.h * Wl;
+        jac[2][0] = p * (liquid.V - vbar[k]) / (R * T);
+        jac[2][1] = -(liquidHbar - hbar[k]) / (R * T);
+        jac[2][2] = dpsi * dLiquid / Wk;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+                if (!fin(jac[r][c])) return false;
+        return true;
+    }
+    // The Jacobian at a residual point (:338-352): analytic when the model asks
+    // for it and it can be built, else central differences.
+    bool jacobianAt(const int* active, int n, const double* x, double scale, const double* f, double jac[4][4],
+                    const Evaluation& point) {
+        if (m.analyticJacobian &&
+            (buildCached(active, n, x, scale, point, jac) ||
+             buildFresh(active, n, x, point.state.p, point.state.T, scale, point, jac))) {
+            ++count.analytic;
+            return true;
+        }
+        return jacobian(active, n, x, scale, f, jac);
+    }
     // Damped Newton on the active set, seeded by liquid fraction (:374-448).
     bool activeFlash(const int* active, int n, double seed, Evaluation& value, const ThermoState* restart = nullptr) {
         ++count.candidates;
@@ -815,7 +1116,7 @@ struct Flash {
                 value.state.energyResidual = std::fabs(f[1]);
                 value.state.chemicalResidual = chemical;
                 value.state.iterations = it;
-                if (!jacobian(active, n, x, scale, f, jac)) return false;
+                if (!jacobianAt(active, n, x, scale, f, jac, value)) return false;
                 double rhs[4]{-1 / rho, value.state.p / (rho * scale)}, response[4]{};
                 if (!solve(jac, rhs, size, response)) return false;
                 const double squared = value.state.p * response[0];
@@ -823,7 +1124,7 @@ struct Flash {
                 value.state.soundEquilibrium = std::sqrt(squared);
                 return value.state.soundEquilibrium <= value.state.soundFrozen * (1 + 2e-3);
             }
-            if (!jacobian(active, n, x, scale, f, jac)) return false;
+            if (!jacobianAt(active, n, x, scale, f, jac, value)) return false;
             double rhs[4]{}, step[4]{};
             for (int j = 0; j < size; ++j) rhs[j] = -f[j];
             if (!solve(jac, rhs, size, step)) return false;
@@ -1068,7 +1369,7 @@ extern "C" {
 int oracle_model_supported(const void* image) {
     const Model& m = *static_cast<const Model*>(image);
     if (!validModel(m)) return 0;
-    if (m.boundaryRecovery || m.analyticJacobian) return 0;
+    if (m.boundaryRecovery) return 0;
     for (int i = 0; i < m.nl; ++i)
         if (m.condensedKind[i]) return 0;
     return 1;

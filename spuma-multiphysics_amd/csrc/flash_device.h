@@ -23,6 +23,7 @@
 #ifndef REACTIVE_AMD_FLASH_DEVICE_H
 #define REACTIVE_AMD_FLASH_DEVICE_H
 #include "pr_device.h"
+#include "flash_jacobian.h"
 
 // Inlining of the phase evaluation (knobs, per arithmetic: the FP32 search
 // and the FP64 flash kernels): 1 = the phase-state functions (gasTP,
@@ -60,7 +61,7 @@ struct HemModel {
     Table<1> liquid;
     PhaseHeuristic gasFixed;     // composition-independent heuristic when NS == 1
     PhaseHeuristic liquidFixed;  // pure liquid table
-    int nl, condensable, scalarRecovery, stableGasPrune, enforceBounds;
+    int nl, condensable, scalarRecovery, stableGasPrune, enforceBounds, analyticJacobian;
     double weights[NS];
     double liquidTmin, liquidTc, Tmin, Tmax, pmin, pmax, vtol, etol, mutol;
 };
@@ -76,7 +77,7 @@ struct ThermoOut {
 static_assert(sizeof(ThermoOut) == 176, "ReactiveThermoState layout");
 
 struct Counts {
-    uint32_t phases, residuals, candidates, stable, failures, fd;
+    uint32_t phases, residuals, candidates, stable, failures, fd, analytic;
 };
 
 // Per-phase properties the analytic Jacobian reads: the condensable's
@@ -95,29 +96,50 @@ struct PhaseParts<R, true> {
     RD void clear() { vG = vL = eG = eL = hG = hL = zG = zL = muN = 0; }
 };
 
+// The residual point's own properties the reference's analytic Jacobian
+// reads (Evaluation::gasCondensable, liquidHbar, liquidVbar,
+// liquidThermoMask, gasMolarVolume; reactiveDeviceFlash.h:64-73): kept only
+// by the flash that runs the reference's analytic-Jacobian mode.
+template <bool RJ>
+struct RefParts {
+    RD void clear() {}
+};
+template <>
+struct RefParts<true> {
+    double gasMolarVolume, gasCondensable, liquidHbar, liquidVbar;
+    int liquidMask;
+    RD void clear() {
+        gasMolarVolume = gasCondensable = liquidHbar = liquidVbar = 0;
+        liquidMask = 0;
+    }
+};
+
 // Evaluation record (reactiveDeviceFlash.h:64-73) for at most one liquid;
-// A adds the per-phase parts of the analytic Jacobian.
-template <class R, bool A = false>
+// A adds the per-phase parts of this build's analytic Jacobian (mixed
+// flash), RJ the inputs of the reference's analytic Jacobian.
+template <class R, bool A = false, bool RJ = false>
 struct EvalT {
     R p, T, rho, e, entropy, cf, ceq, volRes, enRes, chemRes;
     R alphaGas, alphaLiq, rhoGas, rhoLiq, liqMass, gasMass, cp, cv;
     int active, iterations;
     R volume, energy, entropyDensity, Vp, VT, Ep, ET, cpDensity, muGas, muLiquid;
     PhaseParts<R, A> parts;
+    RefParts<RJ> ref;
 };
 using Eval = EvalT<double>;
 
-template <class R, bool A>
-RD void zeroEval(EvalT<R, A>& v) {
+template <class R, bool A, bool RJ>
+RD void zeroEval(EvalT<R, A, RJ>& v) {
     v.p = v.T = v.rho = v.e = v.entropy = v.cf = v.ceq = v.volRes = v.enRes = v.chemRes = 0;
     v.alphaGas = v.alphaLiq = v.rhoGas = v.rhoLiq = v.liqMass = v.gasMass = v.cp = v.cv = 0;
     v.active = v.iterations = 0;
     v.volume = v.energy = v.entropyDensity = v.Vp = v.VT = v.Ep = v.ET = v.cpDensity = v.muGas = v.muLiquid = 0;
     v.parts.clear();
+    v.ref.clear();
 }
 
-template <bool A>
-RD void writeOut(const EvalT<double, A>& v, ThermoOut& o) {
+template <bool A, bool RJ>
+RD void writeOut(const EvalT<double, A, RJ>& v, ThermoOut& o) {
     o.p = v.p; o.T = v.T; o.rho = v.rho; o.e = v.e; o.entropy = v.entropy;
     o.soundFrozen = v.cf; o.soundEquilibrium = v.ceq;
     o.volumeResidual = v.volRes; o.energyResidual = v.enRes; o.chemicalResidual = v.chemRes;
@@ -230,11 +252,14 @@ template <> struct FlashPrecision<float> {
 // finite differences where those are not finite).  On for the FP32 search
 // and the FP64 refinement of the mixed flash, off for the reference-arithmetic
 // flash.
-template <int NS, int NL, class R = double, bool A = FlashPrecision<R>::single>
+// RJ: the reference's analytic-Jacobian mode (Model::analyticJacobian = 1;
+// FP64 flash only).
+template <int NS, int NL, class R = double, bool A = FlashPrecision<R>::single, bool RJ = false>
 struct Flash {
     static_assert(NL == 0 || NL == 1, "one pure liquid at most");
+    static_assert(!RJ || (!A && !FlashPrecision<R>::single), "the reference's analytic mode is an FP64 flash");
     using P = FlashPrecision<R>;
-    using E = EvalT<R, A>;
+    using E = EvalT<R, A, RJ>;
     using S = PRStateT<R>;
     using M = MixT<R>;
     const HemModel<NS>& m;
@@ -486,6 +511,7 @@ struct Flash {
             if (!fin(mass0) || mass0 < 0 || mass0 > q[c]) return false;
             gas[c] = gas[c] - mass0;
             if (!fin(gas[c]) || gas[c] < 0 || gas[c] > q[c]) return false;
+            if constexpr (RJ) v.ref.gasCondensable = gas[c];
         }
         R mg = 0;
         for (int k = 0; k < NS; ++k) mg += gas[k];
@@ -495,6 +521,7 @@ struct Flash {
             S prop;
             R mu[NS];
             if (phaseStatus(-1, p, T, gas, prop, virtualLiquids ? mu : nullptr) != kOk) return false;
+            if constexpr (RJ) v.ref.gasMolarVolume = prop.V;
             accumulate(v, mg, prop, p);
             v.alphaGas = mg / prop.rho;
             v.rhoGas = prop.rho;
@@ -525,6 +552,11 @@ struct Flash {
                 } else {
                     if (virtualLiquids) v.muLiquid = mu[0] / (R(8314.46261815324) * T);
                     if constexpr (A) recordPhase(prop, p, T, v.parts.vL, v.parts.eL, v.parts.hL, v.parts.zL);
+                    if constexpr (RJ) {
+                        v.ref.liquidHbar = prop.h * m.liquid.W[0];
+                        v.ref.liquidVbar = prop.V;
+                        v.ref.liquidMask |= 1;
+                    }
                     if (mass0 > 0) {
                         accumulate(v, mass0, prop, p);
                         v.rhoLiq = prop.rho;
@@ -741,6 +773,107 @@ struct Flash {
         }
         return true;
     }
+    // The reference's analytic Jacobian at a residual point
+    // (reactiveDeviceFlash.h:338-352): from the point's own properties
+    // (buildCached, reactiveDeviceFlashJacobian.h:535-640), else from fresh
+    // gas and liquid states at its (p, T) (build, :436-530); false (the
+    // caller then differences) when neither can be built or the mode is off.
+    RD bool referenceJacobian(const R* x, R scale, const E& v, R (&jac)[3][3]) {
+        if constexpr (RJ) {
+            if (jacobianCached(x, scale, v, jac) || jacobianFresh(x, v.p, v.T, scale, v, jac)) {
+                ++n.analytic;
+                return true;
+            }
+        }
+        return false;
+    }
+    RAMD_NOINLINE bool jacobianCached(const R* x, R scale, const E& v, R (&jac)[3][3]) {
+        if constexpr (RJ) {
+            const double p = v.p, T = v.T, gm = v.gasMass, V = v.ref.gasMolarVolume;
+            if (!(p > 0) || !(T > 0) || !(scale > 0) || !(gm > 0) || !(V > 0) || !fin(p) || !fin(T) || !fin(scale) ||
+                !fin(v.Vp) || !fin(v.VT) || !fin(v.Ep) || !fin(v.ET) || !fin(gm) || !fin(V))
+                return false;
+            const int c = m.condensable;
+            double Y[NS];
+            for (int k = 0; k < NS; ++k) {
+                if (!fin(q[k]) || q[k] < 0) return false;
+                Y[k] = q[k];
+            }
+            const double gc = v.ref.gasCondensable;
+            if (!fin(gc) || gc < 0 || gc > q[c]) return false;
+            Y[c] = gc;
+            if (!(q[c] > 0) || !(x[2] >= 0 && x[2] < 1)) return false;
+            const double product = x[2] * q[c];  // the residual's own two roundings (:35-47)
+            const double expectedGas = q[c] - product;
+            const double dLiquid = q[c];
+            if (!(expectedGas > 0) || gc != expectedGas) return false;
+            const double Hl = v.ref.liquidHbar, Vl = v.ref.liquidVbar;
+            if (!(v.ref.liquidMask & 1) || !fin(Hl) || !fin(Vl) || !(Vl > 0)) return false;
+            double reconstructed = 0;
+            for (int k = 0; k < NS; ++k) reconstructed += Y[k];
+            if (reconstructed != gm) return false;
+            for (int k = 0; k < NS; ++k) Y[k] /= gm;
+            GasPartials g;
+            if (!refActiveGasDerivatives(m.gas, T, p, V, Y, gm, c, g)) return false;
+            const double Wk = m.gas.W[c], Wl = m.liquid.W[0];
+            const double liquidUbar = Hl - p * Vl;
+            jac[0][0] = v.Vp * p;
+            jac[0][1] = v.VT * T;
+            jac[1][0] = v.Ep * p / scale;
+            jac[1][1] = v.ET * T / scale;
+            jac[0][2] = dLiquid * (Vl / Wl - g.vbar / Wk);
+            jac[1][2] = dLiquid * (liquidUbar / Wl - g.ubar / Wk) / scale;
+            jac[2][0] = p * (Vl - g.vbar) / (kR * T);
+            jac[2][1] = -(Hl - g.hbar) / (kR * T);
+            jac[2][2] = g.dpsi * dLiquid / Wk;
+            bool ok = true;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) ok = ok && fin(jac[i][j]);
+            return ok;
+        }
+        return false;
+    }
+    RAMD_NOINLINE bool jacobianFresh(const R* x, double p, double T, R scale, const E& v, R (&jac)[3][3]) {
+        if constexpr (RJ) {
+            if (!(p > 0) || !(T > 0) || !(scale > 0)) return false;
+            const int c = m.condensable;
+            double gm[NS], Y[NS], total = 0;
+            for (int k = 0; k < NS; ++k) gm[k] = q[k];
+            if (!(q[c] > 0) || !(x[2] >= 0 && x[2] < 1)) return false;
+            gm[c] -= x[2] * q[c];
+            const double dLiquid = q[c];
+            for (int k = 0; k < NS; ++k) total += gm[k];
+            if (!(total > 0)) return false;
+            for (int k = 0; k < NS; ++k) Y[k] = gm[k] / total;
+            S gas;
+            ++n.phases;
+            if (gasTP(T, p, Y, gas, nullptr) != kOk) return false;
+            double hbar[NS], ubar[NS], vbar[NS], xm[NS], W = 0;
+            if (!refPhasePartials(m.gas, T, Y, gas, hbar, ubar, vbar, xm, W)) return false;
+            const double totalMoles = total / W;
+            S liquid;
+            ++n.phases;
+            if (liquidTP(T, p, liquid, nullptr) != kOk) return false;
+            double dpsi;
+            if (!refGasCompositionPartial(m.gas, T, xm, totalMoles, gas, c, dpsi)) return false;
+            const double Wk = m.gas.W[c], Wl = m.liquid.W[0];
+            jac[0][0] = v.Vp * p;
+            jac[0][1] = v.VT * T;
+            jac[1][0] = v.Ep * p / scale;
+            jac[1][1] = v.ET * T / scale;
+            jac[0][2] = dLiquid * (1.0 / liquid.rho - vbar[c] / Wk);
+            jac[1][2] = dLiquid * (liquid.e - ubar[c] / Wk) / scale;
+            const double liquidHbar = liquid.h * Wl;
+            jac[2][0] = p * (liquid.V - vbar[c]) / (kR * T);
+            jac[2][1] = -(liquidHbar - hbar[c]) / (kR * T);
+            jac[2][2] = dpsi * dLiquid / Wk;
+            bool ok = true;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) ok = ok && fin(jac[i][j]);
+            return ok;
+        }
+        return false;
+    }
     // Start of an active-set solve: x = [ln p, ln T, seed fraction] and the
     // energy scale (reactiveDeviceFlash.h:374-390).
     RD void activeStart(R seed, bool restart, R restartT, R* x, R& scale) const {
@@ -770,7 +903,7 @@ struct Flash {
         } else if constexpr (A && !P::single) {
             if (!analyticJacobianFinite(value, scale, jac) && !jacobian(x, scale, f, jac)) return false;
         } else {
-            if (!jacobian(x, scale, f, jac)) return false;
+            if (!referenceJacobian(x, scale, value, jac) && !jacobian(x, scale, f, jac)) return false;
         }
         R rhs[3] = {-R(1) / rho, value.p / (rho * scale), R(0)}, response[3];
         if (!solveN<3>(jac, rhs, response)) return false;
@@ -808,7 +941,8 @@ struct Flash {
                 if (!analyticJacobianFinite(value, scale, jac) && !jacobian(x, scale, f, jac))
                     return polishing && activeFinish(x, scale, f, value);
             } else {
-                if (!jacobian(x, scale, f, jac)) return polishing && activeFinish(x, scale, f, value);
+                if (!referenceJacobian(x, scale, value, jac) && !jacobian(x, scale, f, jac))
+                    return polishing && activeFinish(x, scale, f, value);
             }
             R rhs[3] = {-f[0], -f[1], -f[2]}, step[3];
             if (!solveN<3>(jac, rhs, step)) return polishing && activeFinish(x, scale, f, value);

@@ -64,8 +64,8 @@ __device__ inline void loadCell(F& f, const double* __restrict__ q, size_t strid
 // Commits a flash result, or keeps the step-start state of a failed cell
 // (the reference commits nothing for it); p, c_frozen and rho go to the SoA
 // arrays the next transport stage reads.
-template <bool A>
-__device__ inline void storeCell(bool ok, const EvalT<double, A>& result, const ThermoOut& g, uint32_t c,
+template <bool A, bool RJ>
+__device__ inline void storeCell(bool ok, const EvalT<double, A, RJ>& result, const ThermoOut& g, uint32_t c,
                                  ThermoOut* __restrict__ out, double* __restrict__ pOut, double* __restrict__ cOut,
                                  double* __restrict__ rOut) {
     if (ok) {

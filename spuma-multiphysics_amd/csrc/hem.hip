@@ -37,8 +37,9 @@ static_assert(sizeof(HemOutputDev) == sizeof(ReactiveGpuHemOutputV1) &&
 // Without capillary input: equilibrium UV flash (recover,
 // reactiveDeviceFlash.h:564-567).  With it (validated to color = jump = 0,
 // equilibrium = 0): frozen inventory at the seed's liquid masses
-// (recoverCapillary, reactiveDeviceFlash.h:572-590).
-template <int NS, int NL>
+// (recoverCapillary, reactiveDeviceFlash.h:572-590).  RJ: the model's
+// analytic-Jacobian mode (analyticJacobian = 1).
+template <int NS, int NL, bool RJ>
 __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void hemFlashKernel(const HemModel<NS>* __restrict__ globalModel,
                                                      const double* __restrict__ q, const double* __restrict__ energy,
                                                      const ThermoOut* __restrict__ guess,
@@ -49,18 +50,19 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void hemFlashKernel(const
                                                      uint32_t* __restrict__ cellPhases) {
     const HemModel<NS>& model = workgroupModel(globalModel);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    Counts n = {0, 0, 0, 0, 0, 0};
+    Counts n = {0, 0, 0, 0, 0, 0, 0};
     uint32_t succeeded = 0;
     if (i < count) {
         const uint32_t c = order ? order[i] : i;
-        Flash<NS, NL> f(model, n);
+        using F = Flash<NS, NL, double, false, RJ>;
+        F f(model, n);
         for (int k = 0; k < NS; ++k) f.q[k] = q[size_t(c) * NS + k];
         f.energy = energy[c];
         const ThermoOut& g = guess[c];
         f.guessP = g.p;
         f.guessT = g.T;
         f.guessLiquid = g.liquidMass[0];
-        Eval result;
+        typename F::E result;
         zeroEval(result);
         bool ok;
         if (capillary) ok = f.frozen(NL ? g.liquidMass[0] : 0.0, result, 0, g.liquidMass[1]);
@@ -76,7 +78,7 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void hemFlashKernel(const
     if (totals) {
         const uint32_t s0 = waveSum(n.phases), s1 = waveSum(n.residuals), s2 = waveSum(n.candidates);
         const uint32_t s3 = waveSum(n.stable), s4 = waveSum(n.failures), s5 = waveSum(n.fd);
-        const uint32_t s6 = waveSum(succeeded);
+        const uint32_t s6 = waveSum(succeeded), s7 = waveSum(n.analytic);
         if ((threadIdx.x & 63) == 0) {
             atomicAdd(&totals[0], (unsigned long long)s0);
             atomicAdd(&totals[1], (unsigned long long)s1);
@@ -85,6 +87,7 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void hemFlashKernel(const
             atomicAdd(&totals[4], (unsigned long long)s4);
             atomicAdd(&totals[5], (unsigned long long)s5);
             atomicAdd(&totals[6], (unsigned long long)s6);
+            atomicAdd(&totals[7], (unsigned long long)s7);
         }
     }
 }
@@ -152,15 +155,17 @@ void HemHandle::launch(const double* q, const double* e, const ThermoOut* g, con
                        const HemCapillaryDev* capillary, HemOutputDev* out, uint32_t* cellPhases, hipStream_t s) {
     if (!count) return;
     const unsigned blocks = unsigned((count + 63) / 64);
-    hipLaunchKernelGGL(zeroTotalsKernel, dim3(1), dim3(64), 0, s, totals, 7);
-#define RAMD_HEM_LAUNCH(NS_, NL_)                                                                                  \
-    hipLaunchKernelGGL((hemFlashKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, s,                                   \
+    hipLaunchKernelGGL(zeroTotalsKernel, dim3(1), dim3(64), 0, s, totals, 8);
+#define RAMD_HEM_LAUNCH(NS_, NL_, RJ_)                                                                             \
+    hipLaunchKernelGGL((hemFlashKernel<NS_, NL_, RJ_>), dim3(blocks), dim3(64), 0, s,                              \
                        static_cast<const HemModel<NS_>*>(model), q, e, g, order, uint32_t(count), capillary, out, totals, \
                        cellPhases)
-    if (ns == 1 && nl == 0) RAMD_HEM_LAUNCH(1, 0);
-    else if (ns == 1 && nl == 1) RAMD_HEM_LAUNCH(1, 1);
-    else if (ns == 2 && nl == 0) RAMD_HEM_LAUNCH(2, 0);
-    else RAMD_HEM_LAUNCH(2, 1);
+    if (ns == 1 && nl == 0) RAMD_HEM_LAUNCH(1, 0, false);
+    else if (ns == 1 && !analytic) RAMD_HEM_LAUNCH(1, 1, false);
+    else if (ns == 1) RAMD_HEM_LAUNCH(1, 1, true);
+    else if (nl == 0) RAMD_HEM_LAUNCH(2, 0, false);
+    else if (!analytic) RAMD_HEM_LAUNCH(2, 1, false);
+    else RAMD_HEM_LAUNCH(2, 1, true);
 #undef RAMD_HEM_LAUNCH
     RAMD_HIP(hipGetLastError());
 }
@@ -175,13 +180,13 @@ HemHandle* createHem(const void* raw, size_t bytes, size_t capacity, bool hostPa
     require(m.nl <= 1, "This HEM build supports at most one pure liquid");
     require(!(m.nl && m.condensedKind[0]), "Solid condensed phases are not provided by this HEM build");
     require(!m.boundaryRecovery, "Boundary-recovery (adaptive vapour) search is not provided by this HEM build");
-    require(!m.analyticJacobian, "Analytic flash Jacobian is not provided by this HEM build; use the FD model");
     int devices = 0;
     RAMD_HIP(hipGetDeviceCount(&devices));
     require(devices > 0, "No HIP device");
     std::unique_ptr<HemHandle> h(new HemHandle);
     h->ns = m.ns;
     h->nl = m.nl;
+    h->analytic = m.analyticJacobian != 0;
     h->capacity = capacity;
     h->device = currentDevice();
     RAMD_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -250,7 +255,7 @@ int runHost(HemHandle* h, const double* q, const double* energy, const ReactiveG
     h->launch(h->dq, h->de, h->dguess, h->dorder, count, capillary ? h->dcap : nullptr, h->dout, nullptr, s);
     RAMD_HIP(hipEventRecord(h->events[2], s));
     RAMD_HIP(hipMemcpyAsync(h->hostOut.data(), h->dout, count * sizeof(HemOutputDev), hipMemcpyDeviceToHost, s));
-    unsigned long long totals[7] = {};
+    unsigned long long totals[8] = {};
     RAMD_HIP(hipMemcpyAsync(totals, h->totals, sizeof(totals), hipMemcpyDeviceToHost, s));
     RAMD_HIP(hipEventRecord(h->events[3], s));
     RAMD_HIP(hipEventSynchronize(h->events[3]));
@@ -277,6 +282,7 @@ int runHost(HemHandle* h, const double* q, const double* energy, const ReactiveG
     p.flashCandidates = totals[2];
     p.stableCandidates = totals[3];
     p.finiteDifferenceJacobians = totals[5];
+    p.analyticJacobians = totals[7];
     p.transferBytes = count * (h->ns * sizeof(double) + sizeof(double) + sizeof(ThermoOut) + sizeof(uint32_t) +
                                sizeof(HemOutputDev) + (capillary ? sizeof(int) : 0)) +
                       sizeof(totals);
@@ -381,7 +387,7 @@ int reactive_gpu_hem_run_resident_v1(void* raw, const double* q, const double* e
         h->launch(q, energy, reinterpret_cast<const ThermoOut*>(seed), order, count, cap,
                   reinterpret_cast<HemOutputDev*>(output), nullptr, s);
         RAMD_HIP(hipEventRecord(h->events[1], s));
-        unsigned long long totals[7] = {};
+        unsigned long long totals[8] = {};
         RAMD_HIP(hipMemcpyAsync(totals, h->totals, sizeof(totals), hipMemcpyDeviceToHost, s));
         RAMD_HIP(hipEventRecord(h->events[2], s));
         RAMD_HIP(hipEventSynchronize(h->events[2]));
@@ -400,6 +406,7 @@ int reactive_gpu_hem_run_resident_v1(void* raw, const double* q, const double* e
         p.flashCandidates = totals[2];
         p.stableCandidates = totals[3];
         p.finiteDifferenceJacobians = totals[5];
+        p.analyticJacobians = totals[7];
         p.transferBytes = sizeof(totals) + sizeof(rejected);
         p.deviceBytes = h->deviceBytes;
         p.wallSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - begin).count();

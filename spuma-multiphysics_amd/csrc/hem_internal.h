@@ -26,9 +26,10 @@ struct HemCapillaryDev {
 
 struct HemHandle {
     int ns = 0, nl = 0, device = 0;
+    bool analytic = false;  // the model's analytic-Jacobian mode (FP64 flash)
     size_t capacity = 0, deviceBytes = 0, modelBytes = 0;
     void* model = nullptr;  // HemModel<ns> in device memory
-    unsigned long long* totals = nullptr;  // phases residuals candidates stable failures fd successes
+    unsigned long long* totals = nullptr;  // phases residuals candidates stable failures fd successes analytic
     // host-path staging (allocated on the first host batch)
     bool hostReady = false;
     double *dq = nullptr, *de = nullptr;
@@ -49,7 +50,8 @@ struct HemHandle {
     void ensureHostPath();
     // Flash count cells named by order (or 0..count-1) on stream s; q, e,
     // guess, capillary and out are indexed by cell.  Totals: phases,
-    // residuals, candidates, stable, failures, fd Jacobians, successes.
+    // residuals, candidates, stable, failures, fd Jacobians, successes,
+    // analytic Jacobians.
     void launch(const double* q, const double* e, const ThermoOut* guess, const uint32_t* order, size_t count,
                 const HemCapillaryDev* capillary, HemOutputDev* out, uint32_t* cellPhases, hipStream_t s);
 };

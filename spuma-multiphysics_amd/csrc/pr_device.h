@@ -210,14 +210,28 @@ template <class R>
 struct RootsT { R v0, vLast; int count, signedCount, gas, liquid; };
 using Roots = RootsT<double>;
 
-template <class R> RHD int roots(R T, R p, R a, R b, R aa, RootsT<R>& out);
+// The trigonometric branch's acos and cos: the flash's own (correctly
+// rounded in FP64), or, for the FP64 classification inside the FP32 search
+// (Exact = false: it only seeds candidates), the shorter ~1-ulp versions.
+template <class R, bool Exact>
+struct CubicTrig {
+    static RHD R acos(R x) { return m_acos(x); }
+    static RHD R cos(R x) { return m_cos(x); }
+};
+template <>
+struct CubicTrig<double, false> {
+    static RHD double acos(double x) { return elem::acosFast(x); }
+    static RHD double cos(double x) { return elem::cosFast(x); }
+};
+
+template <class R, bool Exact = true> RHD int roots(R T, R p, R a, R b, R aa, RootsT<R>& out);
 
 // Cubic root set at (T, p) from the mixture coefficients
 // (rootsFromMixingCached, reactiveDevicePR.h:701-790).  Only the smallest and
 // largest physical roots are kept: the flash selects nothing else.
 // In single precision the classification of a near-repeated root pair (the
 // discriminant within rounding of zero) is decided in double precision.
-template <class R>
+template <class R, bool Exact>
 RHD int roots(R T, R p, R a, R b, R aa, RootsT<R>& out) {
     if (!fin(T) || !fin(p) || !fin(a) || !fin(b) || !fin(aa) || T <= 0 || p <= 0 || a <= 0 || b <= 0 || aa <= 0)
         return kBadInput;
@@ -234,7 +248,7 @@ RHD int roots(R T, R p, R a, R b, R aa, RootsT<R>& out) {
     R disc = q * q - h * h;
     if (sizeof(R) < sizeof(double) && ab(disc) <= R(RAMD_CUBIC_FP64_BAND) * (q * q + h * h)) {
         RootsT<double> d;
-        const int st = roots<double>(double(T), double(p), double(a), double(b), double(aa), d);
+        const int st = roots<double, false>(double(T), double(p), double(a), double(b), double(aa), d);
         out.v0 = R(d.v0); out.vLast = R(d.vLast);
         out.count = d.count; out.signedCount = d.signedCount; out.gas = d.gas; out.liquid = d.liquid;
         return st;
@@ -259,10 +273,11 @@ RHD int roots(R T, R p, R a, R b, R aa, RootsT<R>& out) {
     } else if (disc < R(-1e-14)) {
         R arg = -R(.5) * q / (delta * delta * delta);
         arg = hi(R(-1), lo(R(1), arg));
-        const R theta = m_acos(arg) / R(3);
-        raw[0] = center + R(2) * delta * m_cos(theta);
-        raw[1] = center + R(2) * delta * m_cos(theta + R(2) * R(kPi) / R(3));
-        raw[2] = center + R(2) * delta * m_cos(theta + R(4) * R(kPi) / R(3));
+        using Trig = CubicTrig<R, Exact>;
+        const R theta = Trig::acos(arg) / R(3);
+        raw[0] = center + R(2) * delta * Trig::cos(theta);
+        raw[1] = center + R(2) * delta * Trig::cos(theta + R(2) * R(kPi) / R(3));
+        raw[2] = center + R(2) * delta * Trig::cos(theta + R(4) * R(kPi) / R(3));
         n = 3;
     } else {
         return kNoRoot;

@@ -36,8 +36,9 @@
 namespace ramd {
 
 // ---------------------------------------------------------------- kernels
-// Plain FP64 flash of the cells in `order`.
-template <int NS, int NL>
+// Plain FP64 flash of the cells in `order`; RJ: the model's
+// analytic-Jacobian mode.
+template <int NS, int NL, bool RJ>
 __global__ __launch_bounds__(256, RAMD_FLASH_MIN_WAVES) void stepFlashKernel(const HemModel<NS>* __restrict__ globalModel,
                                                       const double* __restrict__ q, size_t stride,
                                                       const ThermoOut* __restrict__ seeds,
@@ -55,9 +56,10 @@ __global__ __launch_bounds__(256, RAMD_FLASH_MIN_WAVES) void stepFlashKernel(con
     if (i < count && *dtDev > 0 && !(rep && rep[order[i]] != order[i])) {
         const uint32_t c = order[i];
         const ThermoOut g = seeds[c];
-        Flash<NS, NL> f(model, n);
+        using F = Flash<NS, NL, double, false, RJ>;
+        F f(model, n);
         loadCell(f, q, stride, g, c, NS);
-        Eval result;
+        typename F::E result;
         zeroEval(result);
         const bool ok = f.run(result);
         storeCell(ok, result, g, c, out, pOut, cOut, rOut);
@@ -763,14 +765,16 @@ struct Step {
             else { RAMD_STEP_MIXED(2, 1); }
 #undef RAMD_STEP_MIXED
         } else {
-#define RAMD_STEP_FLASH(NS_, NL_)                                                                                 \
-    hipLaunchKernelGGL((stepFlashKernel<NS_, NL_>), dim3(blocks), dim3(flashBlock), 0, stream,                     \
+#define RAMD_STEP_FLASH(NS_, NL_, RJ_)                                                                            \
+    hipLaunchKernelGGL((stepFlashKernel<NS_, NL_, RJ_>), dim3(blocks), dim3(flashBlock), 0, stream,                \
                        static_cast<const HemModel<NS_>*>(h->model), q, all, sA, order, uint32_t(cells), gate(), out, \
                        pOut, cOut, rOut, costOut, counter, failMark, rep)
-            if (h->ns == 1 && h->nl == 0) { RAMD_STEP_FLASH(1, 0); }
-            else if (h->ns == 1) { RAMD_STEP_FLASH(1, 1); }
-            else if (h->nl == 0) { RAMD_STEP_FLASH(2, 0); }
-            else { RAMD_STEP_FLASH(2, 1); }
+            if (h->ns == 1 && h->nl == 0) { RAMD_STEP_FLASH(1, 0, false); }
+            else if (h->ns == 1 && !h->analytic) { RAMD_STEP_FLASH(1, 1, false); }
+            else if (h->ns == 1) { RAMD_STEP_FLASH(1, 1, true); }
+            else if (h->nl == 0) { RAMD_STEP_FLASH(2, 0, false); }
+            else if (!h->analytic) { RAMD_STEP_FLASH(2, 1, false); }
+            else { RAMD_STEP_FLASH(2, 1, true); }
 #undef RAMD_STEP_FLASH
         }
         if (rep)

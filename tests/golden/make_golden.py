@@ -145,6 +145,30 @@ def prune_fixture(lib):
                         model_binary=np.frombuffer(mb2, np.uint8), **out)
 
 
+def analytic_fixture(lib):
+    """The reference's analytic active-set Jacobian (Model::analyticJacobian
+    = 1, reactiveDeviceFlash.h:338-352, reactiveDeviceFlashJacobian.h): the
+    equilibrium flash of the golden cube, edge and binary inputs with the
+    analytic model, and of a fresh seeded 12^3 cube."""
+    g = np.load(os.path.join(HERE, "flash_n2o.npz"))
+    gb = np.load(os.path.join(HERE, "flash_binary.npz"))
+    mb = model.model_bytes(model.build_model(analytic_jacobian=1))
+    mb2 = model.model_bytes(model.build_model(gas_species=("N2O", "N2"), analytic_jacobian=1))
+    _, _, q12, seeds12, _ = cases.pure_n2o_cube(12, seed=41)
+    out = {"cube12_q": q12[:, :1], "cube12_energy": cases.internal_energy(q12, 1), "cube12_seeds": seeds12}
+    for name, (m, q, e, seeds) in {
+            "cube": (mb, g["cube_q"], g["cube_energy"], g["cube_seeds"]),
+            "edge": (mb, g["edge_q"], g["edge_energy"], g["edge_seeds"]),
+            "binary": (mb2, gb["q"], gb["energy"], gb["seeds"]),
+            "cube12": (mb, out["cube12_q"], out["cube12_energy"], seeds12)}.items():
+        o, s, c = flash_set(lib, m, q, e, seeds)
+        out.update({f"{name}_out": o, f"{name}_success": s, f"{name}_counters": c})
+        print("analytic", name, "success", int(s.sum()), "/", len(e), "analytic Jacobians", int(c[:, 5].sum()),
+              "finite-difference", int(c[:, 6].sum()))
+    np.savez_compressed(os.path.join(HERE, "flash_analytic.npz"), model=np.frombuffer(mb, np.uint8),
+                        model_binary=np.frombuffer(mb2, np.uint8), **out)
+
+
 def base_fixtures(lib):
     mb = model.model_bytes(model.build_model())
     # flash: a seeded 8^3 cube (cube_case inputs) and the edge classes
@@ -199,7 +223,7 @@ def base_fixtures(lib):
     print("binary success", int(success2.sum()), "/", len(e2))
 
 
-SETS = {"base": base_fixtures, "frozen": frozen_fixture, "prune": prune_fixture}
+SETS = {"base": base_fixtures, "frozen": frozen_fixture, "prune": prune_fixture, "analytic": analytic_fixture}
 
 
 def main(names):

@@ -250,3 +250,69 @@ class Device:
         for b in self.buffers:
             self.hip.hipFree(b)
         self.buffers = []
+
+
+# Parity with the reference (the glibc oracle, equal to the compiled reference
+# bit for bit) at 1e-12: relative for p, T, rho, e and the frozen sound speed,
+# absolute for the gas volume fraction and the vapour mass fraction.  The
+# equilibrium sound speed of the finite-difference Jacobian
+# (reactiveDeviceFlash.h:338-373, 400-409) carries the differencing error of
+# the iterate it is taken at: a cell whose elementary functions differ in a
+# last bit anywhere (glibc misrounds a few arguments in 10^4; the device's
+# functions are correctly rounded) converges to another iterate, and its c_eq
+# moves by up to a few 1e-9 although T and p move by less than 1e-12.
+PARITY = 1e-12
+CEQ_FD_NOISE = 1e-8
+
+
+def reference_distance(s, ok, r, rok):
+    """Distance of flash states s to the reference's r (both ThermoOut
+    arrays with success flags): per field, the maximum and the cells over
+    1e-12, plus the cells that are not bit-for-bit equal."""
+    n = len(s)
+    both = (ok == 1) & (rok == 1)
+    same = (s.view(np.uint8).reshape(n, -1) == r.view(np.uint8).reshape(n, -1)).all(1)
+    out = {"success_equal": bool(np.array_equal(ok == 1, rok == 1)), "cells": int(both.sum()),
+           "not_bitwise": np.where(both & ~same)[0]}
+    fields = {"p": "rel", "T": "rel", "rho": "rel", "e": "rel", "soundFrozen": "rel", "soundEquilibrium": "rel",
+              "alphaGas": "abs", "vapourFraction": "abs"}
+    for f, kind in fields.items():
+        with np.errstate(invalid="ignore", divide="ignore"):  # failed cells hold zero density
+            a = s["gasMass"] / s["rho"] if f == "vapourFraction" else s[f]
+            b = r["gasMass"] / r["rho"] if f == "vapourFraction" else r[f]
+        d = np.zeros(n)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            d[both] = np.abs(a[both] - b[both])
+        if kind == "rel":
+            d[both] /= np.maximum(np.abs(b[both]), 1e-300)
+        out[f] = (float(d.max()), np.where(d > PARITY)[0])
+    return out
+
+
+def assert_reference_parity(s, ok, r, rok, max_not_bitwise=0.03, exceptions=None, analytic=False):
+    """The FP64 parity gate: equal success flags; every cell within 1e-12 of
+    the reference in p, T, rho, e, c_f and the vapour fractions; c_eq within
+    1e-12 on every bit-for-bit equal cell and within the differencing noise
+    (1e-8) on the others, which are at most max_not_bitwise of the cells.
+    analytic: the model's analytic-Jacobian mode, whose c_eq carries no
+    differencing error: c_eq within 1e-12 on every cell as well.
+    exceptions: {field: {cell: bound}}, the listed cells that exceed 1e-12 in
+    that field, each within its own bound; every other cell must meet 1e-12."""
+    d = reference_distance(s, ok, r, rok)
+    assert d["success_equal"]
+    exceptions = exceptions or {}
+    fields = ("p", "T", "rho", "e", "soundFrozen", "alphaGas", "vapourFraction")
+    for f in fields + (("soundEquilibrium",) if analytic else ()):
+        listed = exceptions.get(f, {})
+        over = [int(c) for c in d[f][1]]
+        assert sorted(over) == sorted(listed), (f, d[f][0], over[:8], sorted(listed))
+        for c, bound in listed.items():
+            a = s["gasMass"][c] / s["rho"][c] if f == "vapourFraction" else s[f][c]
+            b = r["gasMass"][c] / r["rho"][c] if f == "vapourFraction" else r[f][c]
+            dist = abs(a - b) / (abs(b) if f not in ("alphaGas", "vapourFraction") else 1.0)
+            assert dist <= bound, (f, c, dist, bound)
+    over = d["soundEquilibrium"][1]
+    assert np.isin(over, d["not_bitwise"]).all(), over[~np.isin(over, d["not_bitwise"])][:8]
+    assert d["soundEquilibrium"][0] <= CEQ_FD_NOISE, d["soundEquilibrium"][0]
+    assert d["not_bitwise"].size <= max(2, max_not_bitwise * d["cells"]), d["not_bitwise"].size
+    return d

@@ -1,103 +1,134 @@
 """The shared elementary functions (spuma-multiphysics_amd/csrc/elementary.h),
-exercised through the oracle's twin build, against 80-digit references
-computed with Python's decimal module; the glibc build is the comparison."""
-import decimal
-import math
-from decimal import Decimal as D
+exercised through the oracle's twin build, against 200-bit references
+(mpmath) rounded to nearest; the glibc build is the comparison.
 
+log, exp, acos and cos are evaluated to about 2^-68 before their final
+rounding: every sampled argument must give the correctly rounded double.
+glibc's versions are correctly rounded on all but a few in 10^4 arguments,
+and the twin equals glibc wherever glibc is correctly rounded."""
+import math
+
+import mpmath
 import numpy as np
 import pytest
+from mpmath.libmp import mpf_pos, to_float
 
 from helpers import oracle, p
 
-decimal.getcontext().prec = 80
+mpmath.mp.prec = 200
 
 
-def lib(twin):
-    return oracle(twin)
-
-
-def _pi():
-    def arctan_inv(x):
-        x = D(x)
-        s, t, n, sign = D(0), 1 / x, 1, 1
-        while t / n > D(10) ** -78:
-            s += sign * t / n
-            t /= x * x
-            n += 2
-            sign = -sign
-        return s
-    return 16 * arctan_inv(5) - 4 * arctan_inv(239)
-
-
-PI = _pi()
-
-
-def dcos(x):
-    x = D(x)
-    twopi = 2 * PI
-    x = x - twopi * (x / twopi).to_integral_value(rounding=decimal.ROUND_FLOOR)
-    s, t, n = D(0), D(1), 0
-    while abs(t) > D(10) ** -79:
-        s += t
-        t = -t * x * x / ((n + 1) * (n + 2))
-        n += 2
-    return s
-
-
-def dacos(x):
-    # Newton on cos(y) = x from the double estimate
-    y = D(math.acos(x))
-    for _ in range(4):
-        c = dcos(y)
-        s = (1 - c * c).sqrt()
-        if s == 0:
-            break
-        y = y + (c - D(x)) / s
-    return y
+def nearest(v):
+    return to_float(mpf_pos(mpmath.mpf(v)._mpf_, 53, "n"))
 
 
 def ulps(got, exact):
-    """|got - exact| in units of the last place of the double nearest exact."""
-    e = float(exact)
+    e = nearest(exact)
     ulp = math.ulp(e) if e != 0 else math.ulp(0.0)
-    return float(abs(D(got) - exact) / D(ulp))
+    return float(abs(mpmath.mpf(got) - exact) / ulp)
 
 
+THIRD = mpmath.mpf(1.0 / 3.0)
+
+
+def args_log(r):
+    return np.concatenate([np.exp(r.uniform(-40, 40, 1500)), 1 + r.uniform(-0.01, 0.01, 400),
+                           1 + r.uniform(-1e-9, 1e-9, 100), np.exp(r.uniform(-744, 709, 200))])
+
+
+def args_exp(r):
+    return np.concatenate([r.uniform(-40, 40, 1500), r.uniform(-1e-3, 1e-3, 300), r.uniform(-745, 709, 200)])
+
+
+def args_acos(r):
+    near = 1 - np.exp(r.uniform(-36, -1, 400))
+    return np.concatenate([r.uniform(-1, 1, 1200), near, -near, [-1.0, -0.5, 0.0, 0.5, 1.0]])
+
+
+def args_cos(r):
+    # the flash's arguments: theta in [0, pi/3] and theta + 2 pi/3, theta + 4 pi/3
+    th = r.uniform(0, math.pi / 3, 700)
+    return np.concatenate([th, th + 2.0 * math.pi / 3.0, th + 4.0 * math.pi / 3.0, r.uniform(-100, 100, 200),
+                           [3 * math.pi / 2, math.pi / 2, 1e-300, 0.0]])
+
+
+# which: (name, arguments, exact value, correctly rounded required, ulp bound otherwise)
 CASES = {
-    0: ("log", lambda r: np.exp(r.uniform(-40, 40, 600)), lambda x: D(x).ln(), 1.0),
-    1: ("exp", lambda r: r.uniform(-40, 40, 600), lambda x: D(x).exp(), 1.0),
-    2: ("pow(x, 1/3)", lambda r: np.concatenate([np.exp(r.uniform(-30, 30, 400)), np.exp(r.uniform(-744, 709, 400))]),
-        lambda x: (D(1.0 / 3.0) * D(x).ln()).exp(), 1.0),
-    3: ("acos", lambda r: r.uniform(-1, 1, 400), dacos, 2.0),
-    4: ("cos", lambda r: r.uniform(0, 5.3, 400), dcos, 2.0),
+    0: ("log", args_log, mpmath.log, True, 0.5),
+    1: ("exp", args_exp, mpmath.exp, True, 0.5),
+    2: ("pow(x, 1/3)", lambda r: np.concatenate([np.exp(r.uniform(-30, 30, 800)), np.exp(r.uniform(-744, 709, 400))]),
+        lambda x: mpmath.power(x, THIRD), False, 1.0),
+    3: ("acos", args_acos, mpmath.acos, True, 0.5),
+    4: ("cos", args_cos, mpmath.cos, True, 0.5),
 }
 
 
 @pytest.mark.parametrize("which", sorted(CASES))
 def test_elementary_accuracy(which):
-    name, gen, exact, bound = CASES[which]
+    name, gen, exact, rounded, bound = CASES[which]
     x = gen(np.random.default_rng(100 + which))
-    if which == 3:
-        x = np.concatenate([x, [-1.0, -0.5, 0.0, 0.5, 1.0, 0.9999999, -0.9999999]])
     twin, glibc = np.zeros_like(x), np.zeros_like(x)
-    lib(True).oracle_elementary(which, len(x), p(x), p(twin))
-    lib(False).oracle_elementary(which, len(x), p(x), p(glibc))
-    err = np.array([ulps(t, exact(v)) for t, v in zip(twin, x)])
-    assert err.max() <= bound, (name, err.max(), x[err.argmax()])
-    same = float((twin == glibc).mean())
-    # informational: the share of arguments on which the twin equals glibc
-    print(f"{name}: max {err.max():.3f} ulp, equal to glibc on {100 * same:.1f} %")
-    assert same > 0.5
+    oracle(True).oracle_elementary(which, len(x), p(x), p(twin))
+    oracle(False).oracle_elementary(which, len(x), p(x), p(glibc))
+    ref = [exact(mpmath.mpf(float(v))) for v in x]
+    cr = np.array([nearest(v) for v in ref])
+    err = np.array([ulps(t, v) for t, v in zip(twin, ref)])
+    glibc_cr = float((glibc == cr).mean())
+    print(f"{name}: max {err.max():.3f} ulp, correctly rounded on {100 * (twin == cr).mean():.3f} %, "
+          f"glibc correctly rounded on {100 * glibc_cr:.3f} %, equal to glibc on {100 * (twin == glibc).mean():.3f} %")
+    if rounded:
+        bad = np.where(twin != cr)[0]
+        assert bad.size == 0, (name, x[bad[:5]], err[bad[:5]])
+        # wherever glibc rounds correctly, the two agree
+        assert np.array_equal(twin[glibc == cr], glibc[glibc == cr])
+    else:
+        assert err.max() <= bound, (name, err.max(), x[err.argmax()])
 
 
 def test_elementary_special_values():
-    L = lib(True)
+    L = oracle(True)
     x = np.array([0.0, math.inf, math.nan, -1.0, 1.0, 1e-310])
     y = np.zeros_like(x)
     L.oracle_elementary(0, len(x), p(x), p(y))
     assert y[0] == -math.inf and y[1] == math.inf and math.isnan(y[2]) and math.isnan(y[3]) and y[4] == 0.0
-    assert abs(y[5] - math.log(1e-310)) <= 2 * math.ulp(math.log(1e-310))
+    assert y[5] == nearest(mpmath.log(mpmath.mpf(1e-310)))
     x = np.array([0.0, 800.0, -800.0, math.nan, 1.0])
     L.oracle_elementary(1, len(x), p(x), p(y[:5]))
     assert y[0] == 1.0 and y[1] == math.inf and y[2] == 0.0 and math.isnan(y[3]) and y[4] == math.e
+    x = np.array([1.0000000000000002, -1.0000000000000002, math.nan])
+    L.oracle_elementary(3, len(x), p(x), p(y[:3]))
+    assert math.isnan(y[0]) and math.isnan(y[1]) and math.isnan(y[2])
+    x = np.array([math.inf, -math.inf, math.nan])
+    L.oracle_elementary(4, len(x), p(x), p(y[:3]))
+    assert all(math.isnan(v) for v in y[:3])
+
+
+def parse_tables():
+    """The constant tables of elementary_tables.h, as written."""
+    import os
+    import re
+    path = os.path.join(os.path.dirname(__file__), "..", "spuma-multiphysics_amd", "csrc", "elementary_tables.h")
+    text = open(path).read()
+    out = {}
+    for name, body in re.findall(r"double (\w+)\[\d+\] = \{(.*?)\};", text, re.S):
+        out[name] = [float.fromhex(v) if "0x" in v else float(v) for v in re.findall(r"[-0-9a-fx.p+]+", body)]
+    return out
+
+
+def test_tables_are_200_bit_values_rounded_to_nearest():
+    """Each table entry against a fresh 200-bit evaluation: double-doubles
+    hi + lo with hi the nearest double and lo the nearest double to the rest."""
+    t = parse_tables()
+
+    def pair(v):
+        hi = nearest(v)
+        return [hi, nearest(mpmath.mpf(v) - hi)]
+    exp2 = [x for j in range(64) for x in pair(mpmath.power(2, mpmath.mpf(j) / 64))]
+    assert t["kExp2Table"] == exp2
+    logt = []
+    for i in range(97):
+        c = nearest(1 / (mpmath.mpf(3) / 4 + mpmath.mpf(i) / 128))
+        logt += [c] + pair(-mpmath.log(mpmath.mpf(c)))
+    assert t["kLogTable"] == logt
+    sc = [x for j in range(52) for x in pair(mpmath.sin(mpmath.mpf(j) / 64)) + pair(mpmath.cos(mpmath.mpf(j) / 64))]
+    assert t["kSinCosTable"] == sc

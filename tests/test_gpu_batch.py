@@ -2,7 +2,10 @@
 pure-N2O cubes stepped together, one flux launch per SSPRK2 stage and one
 flash over every cell, each with its own CFL dt.  Every case must equal the
 same case stepped alone on the device, bit for bit (q, every state field,
-dt, recheck step, totals, boundary integral)."""
+dt, recheck step, totals, boundary integral).  At full size (96 cubes with
+edges drawn from {16, 24, 32, 40, 48} cells, about 4.3 million cells) three
+of the cubes are also stepped on the oracle: bit for bit with its twin build
+and at the parity gate with the reference's arithmetic."""
 import ctypes as C
 
 import numpy as np
@@ -121,6 +124,84 @@ def test_batch_rejects_mixed_inputs():
     assert not lib.reactive_amd_step_create_batch(handles, 2, hem, err, 1024)
     assert b"flux" in err.value
     assert not lib.reactive_amd_step_create_batch(handles, 0, hem, err, 1024)
+    for t in trs:
+        lib.reactive_transport_destroy(t)
+    lib.reactive_gpu_hem_destroy_v1(hem)
+
+
+def test_batch_96_uneven_cubes_full_size():
+    """The uneven multi-case workload at its size: 96 pure-N2O cubes, edges
+    uniform over {16, 24, 32, 40, 48} (2 mm cells, each cube seeded as the
+    40^3 box), one FP64 step of the whole batch.  Every cube equals its own
+    single-case step bit for bit; three cubes (the first of each of three
+    sizes) equal the oracle's step."""
+    from test_gpu_step import check_reference_step, oracle_step
+    lib = rlib.load()
+    batch = cases.multi_case(96, seed=2026)
+    sizes = [len(c[2]) for c in batch]
+    edges = [round(n ** (1 / 3)) for n in sizes]
+    print(f"96 cubes, {sum(sizes)} cells, edges {sorted(set(edges))}", flush=True)
+    assert sum(sizes) > 4_000_000
+    mb = model.model_bytes(model.build_model())
+    err = C.create_string_buffer(1024)
+    hem = lib.reactive_gpu_hem_create_v1(mb, len(mb), sum(sizes), err, 1024)
+    assert hem, err.value
+    # flash-consistent step-start states, one device flash over every cube
+    q_all = np.concatenate([c[2] for c in batch])
+    seeds_all = np.concatenate([c[3] for c in batch])
+    q_all, s_all = prepared(lib, hem, (None, None, q_all, seeds_all, None))
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    qs = [q_all[offsets[i]:offsets[i + 1]] for i in range(len(batch))]
+    ss = [s_all[offsets[i]:offsets[i + 1]] for i in range(len(batch))]
+    trs = [transport(lib, c, 0) for c in batch]
+    handles = (C.c_void_p * len(trs))(*trs)
+    step = lib.reactive_amd_step_create_batch(handles, len(trs), hem, err, 1024)
+    assert step, err.value
+    assert lib.reactive_amd_step_load(step, rlib.ptr(q_all), rlib.ptr(s_all)) == 0
+    rep = abi.StepReport.fresh()
+    assert lib.reactive_amd_step_run(step, 0.4, 1.0, 1, 0, C.byref(rep)) == 0, lib.reactive_amd_step_error(step)
+    qb = np.zeros_like(q_all)
+    sb = np.zeros(len(q_all), dtype=abi.STATE_DTYPE)
+    assert lib.reactive_amd_step_read(step, rlib.ptr(qb), rlib.ptr(sb)) == 0
+    boxes = []
+    for i in range(len(trs)):
+        r = abi.StepReport.fresh()
+        assert lib.reactive_amd_step_report_box(step, i, C.byref(r)) == 0
+        boxes.append(r)
+    lib.reactive_amd_step_destroy(step)
+    print(f"batch step {rep.seconds:.3f} s on the device, {rep.flashFailures[1]} failed second flashes", flush=True)
+    singles = {}
+    for i, tr in enumerate(trs):
+        single = lib.reactive_amd_step_create(tr, hem, err, 1024)
+        assert single, err.value
+        assert lib.reactive_amd_step_load(single, rlib.ptr(qs[i]), rlib.ptr(ss[i])) == 0
+        r = abi.StepReport.fresh()
+        assert lib.reactive_amd_step_run(single, 0.4, 1.0, 1, 0, C.byref(r)) == 0, lib.reactive_amd_step_error(single)
+        q = np.zeros_like(qs[i])
+        s = np.zeros(len(q), dtype=abi.STATE_DTYPE)
+        assert lib.reactive_amd_step_read(single, rlib.ptr(q), rlib.ptr(s)) == 0
+        lib.reactive_amd_step_destroy(single)
+        a, b = offsets[i], offsets[i + 1]
+        assert np.array_equal(qb[a:b], q), f"cube {i}: q differs"
+        assert sb[a:b].tobytes() == s.tobytes(), f"cube {i}: states differ"
+        assert boxes[i].dt == r.dt and boxes[i].recheckDt == r.recheckDt
+        assert list(boxes[i].totals)[:5] == list(r.totals)[:5]
+        singles[i] = (q, s, r)
+    assert len({b.dt for b in boxes}) > 1
+    # three cubes of three sizes against the oracle's step
+    picked = []
+    for edge in sorted(set(edges))[:3]:
+        picked.append(edges.index(edge))
+    for i in picked:
+        faces, volume = batch[i][0], batch[i][1]
+        case = dict(model=mb, faces=faces, volume=volume, q=qs[i], states=ss[i], ns=1)
+        ref = oracle_step(case, 0.4)
+        q, s, r = singles[i]
+        assert r.dt == ref["dt"]
+        assert np.array_equal(q, ref["q"]), f"cube {i} ({edges[i]}^3): q differs from the oracle"
+        assert s.tobytes() == ref["s2"].tobytes(), f"cube {i} ({edges[i]}^3): states differ from the oracle"
+        check_reference_step(case, 0.4, 0, r, q, s)
+        print(f"cube {i} ({edges[i]}^3) equals the oracle step", flush=True)
     for t in trs:
         lib.reactive_transport_destroy(t)
     lib.reactive_gpu_hem_destroy_v1(hem)

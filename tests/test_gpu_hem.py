@@ -1,18 +1,20 @@
 """HEM UV flash on the GPU (reactive_gpu_hem_run_v1) against the oracle.
 
-The device flash uses the elementary functions of csrc/elementary.h; the
-oracle's twin build uses the same ones, so the device result must equal it
-bit for bit (every field of every state, the success flags and the phase
-evaluation count).  The distance to the glibc build (which equals the
-compiled reference) is the twin-vs-glibc envelope, bounded on the CPU in
-test_twin_envelope.py; it is printed here for the record."""
+Two gates.  The device flash uses the elementary functions of
+csrc/elementary.h; the oracle's twin build uses the same ones, so the device
+result must equal it bit for bit (every field of every state, the success
+flags and the phase evaluation count).  And the device flash must meet the
+reference itself (the glibc oracle, equal to the compiled reference bit for
+bit) at 1e-12: p, T, rho, e, c_f and the vapour fractions on every cell, c_eq
+on every cell except those where a last-bit difference of glibc's elementary
+functions moved the finite-difference Jacobian (helpers.assert_reference_parity)."""
 import ctypes as C
 import os
 
 import numpy as np
 import pytest
 
-from helpers import GOLDEN, cube_case, oracle_flash
+from helpers import GOLDEN, assert_reference_parity, cube_case, oracle_flash
 from reactive_amd import abi, lib as rlib
 
 pytestmark = pytest.mark.gpu
@@ -39,7 +41,7 @@ def rel(a, b):
     return np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
 
 
-def check_bitwise(mb, q, e, seeds):
+def check_bitwise(mb, q, e, seeds, analytic=False):
     gs, gsucc, prof = run_gpu(mb, q, e, seeds)
     ts, tsucc, tcnt = oracle_flash(mb, q, e, seeds, twin=True)
     assert np.array_equal(gsucc, tsucc)
@@ -49,10 +51,12 @@ def check_bitwise(mb, q, e, seeds):
     assert prof.phaseEvaluations == int(tcnt[:, 0].sum())
     assert prof.residualEvaluations == int(tcnt[:, 1].sum())
     assert prof.deviceFailures == int((tsucc == 0).sum())
+    assert prof.analyticJacobians == int(tcnt[:, 5].sum())
+    assert prof.finiteDifferenceJacobians == int(tcnt[:, 6].sum())
     gl, glsucc, _ = oracle_flash(mb, q, e, seeds)
-    ok = (glsucc == 1) & (gsucc == 1)
-    print("GPU vs glibc oracle, max relative difference:",
-          {f: float(rel(gs[f][ok], gl[f][ok]).max()) for f in FIELDS})
+    d = assert_reference_parity(gs, gsucc, gl, glsucc, analytic=analytic)
+    print(f"GPU vs the reference (glibc oracle), {d['cells']} cells, {d['not_bitwise'].size} not bit for bit:",
+          {f: d[f][0] for f in ("p", "T", "soundFrozen", "soundEquilibrium", "alphaGas", "vapourFraction")})
 
 
 def test_flash_cube_bitwise():
@@ -118,15 +122,21 @@ def test_resident_frozen_batch_matches_host_batch_and_oracle():
     assert np.array_equal(out["success"][order], hsucc[order])
     good = order[hsucc[order] == 1]
     assert out["state"][good].tobytes() == hs[good].tobytes()
-    # oracle (twin) frozen recovery, cell by cell
-    L = oracle(True)
+    # oracle frozen recovery, every submitted cell: the twin bit for bit, the
+    # glibc build (the reference) at 1e-12
     img = C.create_string_buffer(mb, len(mb))
-    for c in order[:40]:
-        o = np.zeros(1, abi.STATE_DTYPE)
-        ok = L.oracle_flash_frozen(img, p(q[c]), C.c_double(e[c]), p(seeds[c:c + 1]), p(o), None)
-        assert ok == out["success"][c]
-        if ok:
-            assert o.tobytes() == out["state"][c:c + 1].tobytes()
+    ref = {}
+    for twin in (True, False):
+        L = oracle(twin)
+        o = np.zeros(n, abi.STATE_DTYPE)
+        okc = np.zeros(n, np.int32)
+        for c in order:
+            okc[c] = L.oracle_flash_frozen(img, p(q[c]), C.c_double(e[c]), p(seeds[c:c + 1]), p(o[c:c + 1]), None)
+        ref[twin] = (o[order], okc[order])
+    assert np.array_equal(out["success"][order], ref[True][1])
+    good = out["success"][order] == 1
+    assert out["state"][order][good].tobytes() == ref[True][0][good].tobytes()
+    assert_reference_parity(out["state"][order], out["success"][order], ref[False][0], ref[False][1])
     # a cell asking for the equilibrium (boundary-recovery) search: rejected, nothing written
     cap2 = cap.copy()
     cap2["equilibrium"][order[3]] = 1
@@ -187,3 +197,25 @@ def test_flash_stable_gas_prune_bitwise(part):
     key = "" if part == "binary" else f"{part}_"
     mb = (g["model_binary"] if part == "binary" else g["model"]).tobytes()
     check_bitwise(mb, base[f"{key}q"], base[f"{key}energy"], base[f"{key}seeds"])
+
+
+@pytest.mark.parametrize("part", ["cube", "edge", "binary", "cube12"])
+def test_flash_analytic_jacobian(part):
+    """The reference's analytic-Jacobian mode (analyticJacobian = 1,
+    reactiveDeviceFlashJacobian.h, its fastest published configuration): the
+    device equals the twin oracle bit for bit (analytic-Jacobian count
+    included, no finite-difference Jacobian), and every field of every cell,
+    the equilibrium sound speed included, is within 1e-12 of the reference."""
+    g = np.load(os.path.join(GOLDEN, "flash_analytic.npz"))
+    if part == "cube12":
+        q, e, seeds = g["cube12_q"], g["cube12_energy"], g["cube12_seeds"]
+    else:
+        base = np.load(os.path.join(GOLDEN, "flash_binary.npz" if part == "binary" else "flash_n2o.npz"))
+        key = "" if part == "binary" else f"{part}_"
+        q, e, seeds = base[f"{key}q"], base[f"{key}energy"], base[f"{key}seeds"]
+    mb = (g["model_binary"] if part == "binary" else g["model"]).tobytes()
+    check_bitwise(mb, q, e, seeds, analytic=True)
+    # and the reference's own outputs (the fixture): within the same gate
+    gs, gsucc, prof = run_gpu(mb, q, e, seeds)
+    assert prof.analyticJacobians > 0 and prof.finiteDifferenceJacobians == 0
+    assert_reference_parity(gs, gsucc, g[f"{part}_out"], g[f"{part}_success"], analytic=True)

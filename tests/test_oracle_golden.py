@@ -119,3 +119,29 @@ def test_stable_gas_prune_bitwise_vs_reference(part):
     assert out.tobytes() == g[f"{part}_out"].tobytes()
     assert np.array_equal(counters, g[f"{part}_counters"])
     assert counters[:, 2].sum() < base[f"{key}counters"][:, 2].sum()
+
+
+@pytest.mark.parametrize("part", ["cube", "edge", "binary", "cube12"])
+def test_analytic_jacobian_bitwise_vs_reference(part):
+    """The reference's analytic active-set Jacobian (analyticJacobian = 1,
+    reactiveDeviceFlashJacobian.h build/buildCached, dispatched at
+    reactiveDeviceFlash.h:338-352): the oracle's flash with the analytic
+    model equals the reference's (states, success flags and every counter,
+    the analytic-Jacobian count included), and the mode changes the work
+    (fewer phase evaluations than the finite-difference model)."""
+    g = load("flash_analytic.npz")
+    if part == "cube12":
+        q, e, seeds, base = g["cube12_q"], g["cube12_energy"], g["cube12_seeds"], None
+    else:
+        base = load("flash_binary.npz" if part == "binary" else "flash_n2o.npz")
+        key = "" if part == "binary" else f"{part}_"
+        q, e, seeds = base[f"{key}q"], base[f"{key}energy"], base[f"{key}seeds"]
+    mb = (g["model_binary"] if part == "binary" else g["model"]).tobytes()
+    assert oracle().oracle_model_supported(C.create_string_buffer(mb, len(mb))) == 1
+    out, success, counters = oracle_flash(mb, q, e, seeds)
+    assert np.array_equal(success, g[f"{part}_success"])
+    assert out.tobytes() == g[f"{part}_out"].tobytes()
+    assert np.array_equal(counters, g[f"{part}_counters"])
+    assert counters[:, 5].sum() > 0 and counters[:, 6].sum() == 0
+    if base is not None:
+        assert counters[:, 0].sum() < base[f"{key}counters"][:, 0].sum()
