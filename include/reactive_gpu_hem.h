@@ -97,6 +97,17 @@ int reactive_gpu_hem_run_resident_v1(void* hem, const double* q, const double* e
     const ReactiveThermoState* seed, const ReactiveGpuHemCapillaryInputV2* capillary,
     const uint32_t* order, size_t count, ReactiveGpuHemOutputV1* output,
     ReactiveGpuHemProfileV1* profile, char* error, size_t errorSize);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  Same arguments,
+ * semantics and results as reactive_gpu_hem_run_resident_v1; no device memory
+ * is allocated or freed.  The work counters go to a static device scratch of
+ * 1 MiB + 64 B per device (held until the process ends), which limits a batch
+ * to 1,048,576 cells (rc 1 with a message beyond that).  Calls that overlap
+ * on one device share that scratch and race: serialise them. */
+int reactive_gpu_hem_run_resident_noalloc_v1(void* hem, const double* q, const double* energy,
+    const ReactiveThermoState* seed, const ReactiveGpuHemCapillaryInputV2* capillary,
+    const uint32_t* order, size_t count, ReactiveGpuHemOutputV1* output,
+    ReactiveGpuHemProfileV1* profile, char* error, size_t errorSize);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "hip_common.h"
@@ -47,7 +48,8 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void hemFlashKernel(const
                                                      const HemCapillaryDev* __restrict__ capillary,
                                                      HemOutputDev* __restrict__ out,
                                                      unsigned long long* __restrict__ totals,
-                                                     uint32_t* __restrict__ cellPhases) {
+                                                     uint32_t* __restrict__ cellPhases,
+                                                     unsigned long long* __restrict__ partials) {
     const HemModel<NS>& model = workgroupModel(globalModel);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Counts n = {0, 0, 0, 0, 0, 0, 0};
@@ -75,7 +77,16 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void hemFlashKernel(const
         succeeded = ok ? 1u : 0u;
         if (cellPhases) cellPhases[c] = n.phases;
     }
-    if (totals) {
+    if (partials) {  // one 64-lane workgroup: its eight sums go to its own slot
+        const uint32_t s0 = waveSum(n.phases), s1 = waveSum(n.residuals), s2 = waveSum(n.candidates);
+        const uint32_t s3 = waveSum(n.stable), s4 = waveSum(n.failures), s5 = waveSum(n.fd);
+        const uint32_t s6 = waveSum(succeeded), s7 = waveSum(n.analytic);
+        if (threadIdx.x == 0) {
+            unsigned long long* slot = partials + size_t(blockIdx.x) * 8;
+            slot[0] = s0; slot[1] = s1; slot[2] = s2; slot[3] = s3;
+            slot[4] = s4; slot[5] = s5; slot[6] = s6; slot[7] = s7;
+        }
+    } else if (totals) {
         const uint32_t s0 = waveSum(n.phases), s1 = waveSum(n.residuals), s2 = waveSum(n.candidates);
         const uint32_t s3 = waveSum(n.stable), s4 = waveSum(n.failures), s5 = waveSum(n.fd);
         const uint32_t s6 = waveSum(succeeded), s7 = waveSum(n.analytic);
@@ -94,9 +105,12 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void hemFlashKernel(const
 
 // Counts the ordered cells whose capillary input this build cannot run
 // (non-finite or out-of-range fields, an interface, or equilibrium = 1).
+// With partials (64-lane workgroups), each workgroup writes its count to its
+// own slot instead of adding it to *rejected.
 __global__ __launch_bounds__(256) void capillaryCheckKernel(const HemCapillaryDev* __restrict__ capillary,
                                                             const uint32_t* __restrict__ order, uint32_t count,
-                                                            unsigned long long* __restrict__ rejected) {
+                                                            unsigned long long* __restrict__ rejected,
+                                                            unsigned long long* __restrict__ partials) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t bad = 0;
     if (i < count) {
@@ -104,7 +118,24 @@ __global__ __launch_bounds__(256) void capillaryCheckKernel(const HemCapillaryDe
         bad = !(fin(x.color) && x.color == 0 && fin(x.pressureJump) && x.pressureJump == 0 && x.equilibrium == 0);
     }
     const uint32_t sum = waveSum(bad);
-    if ((threadIdx.x & 63) == 0 && sum) atomicAdd(rejected, (unsigned long long)sum);
+    if (partials) {
+        if (threadIdx.x == 0) partials[blockIdx.x] = sum;
+    } else if ((threadIdx.x & 63) == 0 && sum) {
+        atomicAdd(rejected, (unsigned long long)sum);
+    }
+}
+
+// totals[k] = sum over b < blocks of partials[b * width + k], k < width: one
+// 64-lane workgroup, a fixed summation order (no atomics, nothing cleared).
+__global__ __launch_bounds__(64) void reducePartialsKernel(const unsigned long long* __restrict__ partials,
+                                                          unsigned blocks, int width,
+                                                          unsigned long long* __restrict__ totals) {
+    for (int k = 0; k < width; ++k) {
+        unsigned long long sum = 0;
+        for (unsigned b = threadIdx.x; b < blocks; b += 64) sum += partials[size_t(b) * width + k];
+        for (int offset = 32; offset > 0; offset >>= 1) sum += __shfl_xor(sum, offset, 64);
+        if (threadIdx.x == 0) totals[k] = sum;
+    }
 }
 
 __global__ void zeroTotalsKernel(unsigned long long* totals, int n) {
@@ -152,14 +183,15 @@ void HemHandle::ensureHostPath() {
 }
 
 void HemHandle::launch(const double* q, const double* e, const ThermoOut* g, const uint32_t* order, size_t count,
-                       const HemCapillaryDev* capillary, HemOutputDev* out, uint32_t* cellPhases, hipStream_t s) {
+                       const HemCapillaryDev* capillary, HemOutputDev* out, uint32_t* cellPhases, hipStream_t s,
+                       unsigned long long* partials) {
     if (!count) return;
     const unsigned blocks = unsigned((count + 63) / 64);
-    hipLaunchKernelGGL(zeroTotalsKernel, dim3(1), dim3(64), 0, s, totals, 8);
+    if (!partials) hipLaunchKernelGGL(zeroTotalsKernel, dim3(1), dim3(64), 0, s, totals, 8);
 #define RAMD_HEM_LAUNCH(NS_, NL_, RJ_)                                                                             \
     hipLaunchKernelGGL((hemFlashKernel<NS_, NL_, RJ_>), dim3(blocks), dim3(64), 0, s,                              \
                        static_cast<const HemModel<NS_>*>(model), q, e, g, order, uint32_t(count), capillary, out, totals, \
-                       cellPhases)
+                       cellPhases, partials)
     if (ns == 1 && nl == 0) RAMD_HEM_LAUNCH(1, 0, false);
     else if (ns == 1 && !analytic) RAMD_HEM_LAUNCH(1, 1, false);
     else if (ns == 1) RAMD_HEM_LAUNCH(1, 1, true);
@@ -295,6 +327,32 @@ int runHost(HemHandle* h, const double* q, const double* energy, const ReactiveG
     return 0;
 }
 
+// ---- device scratch of the allocation-free resident batch
+// (reactive_gpu_hem_run_resident_noalloc_v1).  One slot of 8 counters per
+// 64-cell workgroup of a launch, then the 8 batch totals: 1 MiB + 64 B on
+// every device the library runs on, held until the process ends.  Every slot
+// a call reads, the same call has written first, so nothing is cleared
+// between calls; calls that overlap on one device share it and race.
+constexpr unsigned kNoAllocBlocks = 16384;
+constexpr size_t kNoAllocMaxCells = size_t(kNoAllocBlocks) * 64;  // 1,048,576 cells per call
+static __device__ __align__(256) unsigned long long scratch[size_t(kNoAllocBlocks) * 8 + 8];
+
+// scratch's address on `device` (the host-side name of the array is only a
+// stand-in): looked up once per device with that device current.
+unsigned long long* scratchAddress(int device) {
+    constexpr int kDevices = 64;
+    static std::mutex lock;
+    static unsigned long long* address[kDevices] = {};
+    require(device >= 0 && device < kDevices, "HIP device index beyond the allocation-free batch's table");
+    std::lock_guard<std::mutex> guard(lock);
+    if (!address[device]) {
+        void* ptr = nullptr;
+        RAMD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        address[device] = static_cast<unsigned long long*>(ptr);
+    }
+    return address[device];
+}
+
 }  // namespace ramd
 
 using namespace ramd;
@@ -376,7 +434,7 @@ int reactive_gpu_hem_run_resident_v1(void* raw, const double* q, const double* e
         const auto* cap = reinterpret_cast<const HemCapillaryDev*>(capillary);
         hipLaunchKernelGGL(zeroTotalsKernel, dim3(1), dim3(64), 0, s, h->rejected, 1);
         hipLaunchKernelGGL(capillaryCheckKernel, dim3(unsigned((count + 255) / 256)), dim3(256), 0, s, cap, order,
-                           uint32_t(count), h->rejected);
+                           uint32_t(count), h->rejected, nullptr);
         RAMD_HIP(hipGetLastError());
         unsigned long long rejected = 0;
         RAMD_HIP(hipMemcpyAsync(&rejected, h->rejected, sizeof(rejected), hipMemcpyDeviceToHost, s));
@@ -389,6 +447,77 @@ int reactive_gpu_hem_run_resident_v1(void* raw, const double* q, const double* e
         RAMD_HIP(hipEventRecord(h->events[1], s));
         unsigned long long totals[8] = {};
         RAMD_HIP(hipMemcpyAsync(totals, h->totals, sizeof(totals), hipMemcpyDeviceToHost, s));
+        RAMD_HIP(hipEventRecord(h->events[2], s));
+        RAMD_HIP(hipEventSynchronize(h->events[2]));
+        float kernel = 0, download = 0;
+        RAMD_HIP(hipEventElapsedTime(&kernel, h->events[0], h->events[1]));
+        RAMD_HIP(hipEventElapsedTime(&download, h->events[1], h->events[2]));
+        ReactiveGpuHemProfileV1 p{};
+        p.abiVersion = 1;
+        p.structBytes = sizeof(p);
+        p.batches = 1;
+        p.submitted = count;
+        p.succeeded = totals[6];
+        p.deviceFailures = count - totals[6];
+        p.phaseEvaluations = totals[0];
+        p.residualEvaluations = totals[1];
+        p.flashCandidates = totals[2];
+        p.stableCandidates = totals[3];
+        p.finiteDifferenceJacobians = totals[5];
+        p.analyticJacobians = totals[7];
+        p.transferBytes = sizeof(totals) + sizeof(rejected);
+        p.deviceBytes = h->deviceBytes;
+        p.wallSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - begin).count();
+        p.kernelSeconds = kernel / 1000.;
+        p.copySeconds = download / 1000.;
+        *profile = p;
+        return 0;
+    } catch (const std::exception& e) {
+        writeError(error, size, e.what());
+        return 1;
+    }
+}
+
+// Same contract and results as reactive_gpu_hem_run_resident_v1, without any
+// device allocation: the work counters go to the library's static scratch.
+int reactive_gpu_hem_run_resident_noalloc_v1(void* raw, const double* q, const double* energy,
+                                             const ReactiveThermoState* seed,
+                                             const ReactiveGpuHemCapillaryInputV2* capillary, const uint32_t* order,
+                                             size_t count, ReactiveGpuHemOutputV1* output,
+                                             ReactiveGpuHemProfileV1* profile, char* error, size_t size) {
+    try {
+        require(raw && q && energy && seed && capillary && order && output && profile && profile->abiVersion == 1 &&
+                    profile->structBytes == sizeof(*profile),
+                "Invalid resident HIP HEM batch arguments");
+        HemHandle* h = static_cast<HemHandle*>(raw);
+        require(count && count <= h->capacity, "Resident HIP HEM batch exceeds capacity");
+        require(count <= kNoAllocMaxCells, "reactive_gpu_hem_run_resident_noalloc_v1: a batch holds at most "
+                                           "1048576 cells (the size of its static device scratch)");
+        const auto begin = std::chrono::steady_clock::now();
+        RAMD_HIP(hipSetDevice(h->device));
+        unsigned long long* partials = scratchAddress(h->device);
+        unsigned long long* totalsDev = partials + size_t(kNoAllocBlocks) * 8;
+        hipStream_t s = h->stream;
+        const unsigned blocks = unsigned((count + 63) / 64);
+        const auto* cap = reinterpret_cast<const HemCapillaryDev*>(capillary);
+        // the capillary check: one count per 64-cell workgroup, then their sum
+        hipLaunchKernelGGL(capillaryCheckKernel, dim3(blocks), dim3(64), 0, s, cap, order, uint32_t(count), nullptr,
+                           partials);
+        hipLaunchKernelGGL(reducePartialsKernel, dim3(1), dim3(64), 0, s, partials, blocks, 1, totalsDev);
+        RAMD_HIP(hipGetLastError());
+        unsigned long long rejected = 0;
+        RAMD_HIP(hipMemcpyAsync(&rejected, totalsDev, sizeof(rejected), hipMemcpyDeviceToHost, s));
+        RAMD_HIP(hipStreamSynchronize(s));
+        require(rejected == 0, "Resident HIP HEM batch holds cells with an interface (color or pressureJump != 0), "
+                               "equilibrium = 1 or invalid capillary input; this build runs frozen-inventory cells only");
+        RAMD_HIP(hipEventRecord(h->events[0], s));
+        h->launch(q, energy, reinterpret_cast<const ThermoOut*>(seed), order, count, cap,
+                  reinterpret_cast<HemOutputDev*>(output), nullptr, s, partials);
+        hipLaunchKernelGGL(reducePartialsKernel, dim3(1), dim3(64), 0, s, partials, blocks, 8, totalsDev);
+        RAMD_HIP(hipGetLastError());
+        RAMD_HIP(hipEventRecord(h->events[1], s));
+        unsigned long long totals[8] = {};
+        RAMD_HIP(hipMemcpyAsync(totals, totalsDev, sizeof(totals), hipMemcpyDeviceToHost, s));
         RAMD_HIP(hipEventRecord(h->events[2], s));
         RAMD_HIP(hipEventSynchronize(h->events[2]));
         float kernel = 0, download = 0;

@@ -52,8 +52,11 @@ struct HemHandle {
     // guess, capillary and out are indexed by cell.  Totals: phases,
     // residuals, candidates, stable, failures, fd Jacobians, successes,
     // analytic Jacobians.
+    // With partials (one slot of 8 counters per 64-cell workgroup), the
+    // workgroups write their sums there instead of adding them to totals.
     void launch(const double* q, const double* e, const ThermoOut* guess, const uint32_t* order, size_t count,
-                const HemCapillaryDev* capillary, HemOutputDev* out, uint32_t* cellPhases, hipStream_t s);
+                const HemCapillaryDev* capillary, HemOutputDev* out, uint32_t* cellPhases, hipStream_t s,
+                unsigned long long* partials = nullptr);
 };
 
 HemHandle* createHem(const void* raw, size_t bytes, size_t capacity, bool hostPath);

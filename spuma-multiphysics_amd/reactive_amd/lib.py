@@ -49,6 +49,8 @@ _SIGNATURES = {
                                           C.c_size_t]),
     "reactive_gpu_hem_run_resident_v1": (C.c_int, [P, P, P, P, P, P, C.c_size_t, P, C.POINTER(HemProfile),
                                                    C.c_char_p, C.c_size_t]),
+    "reactive_gpu_hem_run_resident_noalloc_v1": (C.c_int, [P, P, P, P, P, P, C.c_size_t, P, C.POINTER(HemProfile),
+                                                           C.c_char_p, C.c_size_t]),
     "reactive_amd_transport_set_flux": (C.c_int, [P, C.c_int]),
     "reactive_amd_step_create": (P, [P, P, C.c_char_p, C.c_size_t]),
     "reactive_amd_step_load": (C.c_int, [P, P, P]),
