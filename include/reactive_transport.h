@@ -108,7 +108,12 @@ typedef struct ReactiveTransportProfileV21 {
 } ReactiveTransportProfileV21;
 
 /* ---- Core entry points, built on the device ---------------------------- */
-/* reactiveTransport.h:118-121 */
+/* reactiveTransport.h:118-121.  Physics: HLL (or the HLLC option) face flux,
+ * constant viscosity (stress and its work) and conductivity (heat flux) with
+ * the CFL diffusion number, and the WALE closure below.  A configuration with
+ * diffusivity > 0 or the mechanical closure is rejected.  Viscous or
+ * conductive transports run the generic face-graph kernels even on a
+ * structured box. */
 void* reactive_transport_create(int backend, const ReactiveTransportConfig* config,
     const double* volume, const ReactiveTransportFace* faces, const double* fixedQ,
     const ReactiveTransportState* fixedStates, const double* fixedGasY,
@@ -206,11 +211,29 @@ int reactive_transport_geometric_diagnostic_v1(void* transport, const void* opti
     const double* q, const ReactiveTransportState* state, const double* cellColor,
     const void* cellGeometry, const void* faceGeometry, double* rhs, double* boundaryRate,
     void* residual);
-/* reactiveTurbulence.h:20-81 (WALE LES) */
-int reactive_transport_set_wale_v1(void* transport, const void* options);
-int reactive_transport_wale_profile_v1(void* transport, void* profile);
+/* reactiveTurbulence.h:12-26 (WALE LES stress closure, Delta = cbrt(cell
+ * volume)).  set_wale installs the closure once, before any upload or step;
+ * the eddy viscosity is rebuilt from each call's state (Gradients +
+ * WaleViscosities, reactiveTransportKernels.h:290-313) and adds rho nu_t to
+ * the face viscosity of the stress, its work and the CFL diffusion number.
+ * wale_primitives returns nu_t (m^2/s) of the given primitives and states.
+ * Generic face graphs only: the resident step and the stage adjoint reject a
+ * transport with the closure installed. */
+typedef struct ReactiveWaleOptionsV1 {
+    uint32_t abiVersion, structBytes;
+    double Cw;
+} ReactiveWaleOptionsV1;
+typedef struct ReactiveWaleProfileV1 {
+    uint32_t abiVersion, structBytes;
+    uint64_t gradientBuilds, viscosityBuilds, cellsEvaluated, workspaceBytes;
+} ReactiveWaleProfileV1;
+int reactive_transport_set_wale_v1(void* transport, const ReactiveWaleOptionsV1* options);
+int reactive_transport_wale_profile_v1(void* transport, ReactiveWaleProfileV1* profile);
 int reactive_transport_wale_primitives_v1(void* transport, const ReactiveTransportPrimitive* primitive,
     const ReactiveTransportState* state, double* nut);
+/* reactiveTurbulence.h:27-81: the scalar closure (turbulent Prandtl and
+ * Schmidt fluxes) and the PR property epochs are not provided (stubs that
+ * return 1 with an error). */
 int reactive_transport_set_wale_scalars_v1(void* transport, const void* options);
 int reactive_transport_wale_scalar_profile_v1(void* transport, void* profile);
 int reactive_transport_wale_scalar_fields_v1(void* transport, const double* cellCp,

@@ -1,16 +1,18 @@
 // SPDX-License-Identifier: GPL-3.0-or-later
 // CPU ORACLE — test infrastructure only (see oracle.h).  Serial restatement
-// of the reference's inviscid transport operators for backend 0:
-// Cells / Faces (HLL part) / Rhs / Advance / FaceSpeeds / Step and the
-// Minimum and BoundaryPartial/BoundaryFinish reductions
+// of the reference's transport operators for backend 0:
+// Cells / Gradients / WaleViscosities / Faces / Rhs / Advance / FaceSpeeds /
+// Step and the Minimum and BoundaryPartial/BoundaryFinish reductions
 // (src/reactiveTransport/reactiveTransportKernels.h, reactiveTransport.cpp).
-// Scope: viscosity = conductivity = diffusivity = 0, no mechanical closure,
-// no WALE or capillary; those configurations are rejected at create.
+// Scope: constant viscosity and conductivity, the WALE stress closure
+// (reactiveWale.h; no scalar closure), diffusivity = 0, no mechanical
+// closure, no capillary model; other configurations are rejected at create.
 // Face flux: HLL (reactiveTransportKernels.h:343-356), or the HLLC option
 // (scheme 1): the capillary branch's contact-preserving flux
 // (reactiveBalancedCapillary.h:55-142) with sigma = curvature = 0 and no
 // surface stress, whose coefficients replace the HLL ones
-// (reactiveTransportKernels.h:358-417).
+// (reactiveTransportKernels.h:358-417); viscous traction and work and the
+// conductive heat flux follow (:418-439).
 #include "oracle.h"
 #include <cmath>
 #include <cstdio>
@@ -28,8 +30,73 @@ void require(bool ok, const char* message) { if (!ok) throw std::runtime_error(m
 
 struct Prim { double rho, u[3]; };
 struct Work { double advectL, advectR, pressureMomentum, pressureEnergy; };
+// Viscous traction and the viscous + conductive energy flux of a face
+// (FaceWork::traction/energy, reactiveTransportKernels.h:418-439).
+struct Visc { double traction[3], energy; };
+struct Primitive { double rho, u[3]; };
+static_assert(sizeof(Primitive) == 32, "ReactiveTransportPrimitive");
 
 inline bool finite(double x) { return x == x && x <= 1.7976931348623157e308 && x >= -1.7976931348623157e308; }
+
+// WALE kinematic eddy viscosity of one cell (ReactiveWale::eddyViscosity,
+// reactiveWale.h:93-167): the invariants of the velocity gradient g
+// (row-major, g[3i+j] = d u_i / d x_j) scaled by max |g_ij|, S = symm(g),
+// Sd = dev(symm(g g)), then nu_t = (Cw delta)^2 (Sd:Sd)^(3/2) /
+// ((S:S)^(5/2) + (Sd:Sd)^(5/4)) with the product Cw^2 delta^2 scale ratio
+// assembled from mantissas and exponents (:47-80).  False where the
+// reference fails.
+bool waleEddyViscosity(const double* g, double delta, double cw, double& nut) {
+    nut = std::nan("");
+    if (!std::isfinite(delta) || delta < 0 || !std::isfinite(cw) || cw < 0) return false;
+    double scale = 0;
+    for (int i = 0; i < 9; ++i) {
+        if (!std::isfinite(g[i])) return false;
+        const double m = g[i] < 0 ? -g[i] : g[i];
+        if (m > scale) scale = m;
+    }
+    double strain2 = 0, dev2 = 0;
+    if (scale != 0) {
+        double s[9], sq[9] = {};
+        for (int i = 0; i < 9; ++i) s[i] = g[i] / scale;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) sq[3 * i + j] += s[3 * i + k] * s[3 * k + j];
+        const double trace = sq[0] + sq[4] + sq[8];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const double strain = 0.5 * (s[3 * i + j] + s[3 * j + i]);
+                double dev = 0.5 * (sq[3 * i + j] + sq[3 * j + i]);
+                if (i == j) dev -= trace / 3.0;
+                strain2 += strain * strain;
+                dev2 += dev * dev;
+            }
+        if (!std::isfinite(strain2) || !std::isfinite(dev2)) return false;
+    }
+    if (scale == 0 || cw == 0 || delta == 0 || dev2 == 0) {
+        nut = 0;
+        return true;
+    }
+    const double strainPower = strain2 * strain2 * std::sqrt(strain2);
+    const double root = std::sqrt(dev2);
+    const double devPower = dev2 * root;
+    const double denominator = strainPower + dev2 * std::sqrt(root);
+    if (!(denominator > 0) || !std::isfinite(denominator) || !std::isfinite(devPower)) return false;
+    const double ratio = devPower / denominator;
+    if (ratio == 0) {
+        nut = 0;
+        return true;
+    }
+    int ec = 0, ed = 0, eg = 0, er = 0, em = 0;
+    const double mc = std::frexp(cw, &ec), md = std::frexp(delta, &ed);
+    const double mg = std::frexp(scale, &eg), mr = std::frexp(ratio, &er);
+    const double m = std::frexp(mc * mc * md * md * mg * mr, &em);
+    nut = std::ldexp(m, 2 * ec + 2 * ed + eg + er + em);
+    if (!std::isfinite(nut)) {
+        nut = std::nan("");
+        return false;
+    }
+    return true;
+}
 
 // HLLC coefficients for one face (reactiveBalancedCapillary.h:55-142, sigma =
 // curvature = colorFace = 0, zero surface stress, so pi = p).  Inputs: side
@@ -116,9 +183,13 @@ struct Transport {
     std::vector<TState> state;
     std::vector<Prim> prim;
     std::vector<Work> work;
+    std::vector<Visc> visc;
+    std::vector<double> gradient, nut;  // gradient[c*9 + 3i + j] = d u_i / d x_j
     std::vector<double> faceSpeed, step;
     size_t partitions = 1;
     int scheme = 0;  // 0 HLL, 1 HLLC
+    bool wale = false, used = false;
+    double waleCw = 0;
     bool haveInitial = false;
     double stageDt = 0;
     std::string error;
@@ -132,8 +203,11 @@ struct Transport {
         require(cfg.cells && cfg.species && cfg.faces, "Empty transport mesh/model");
         require(cfg.variables >= cfg.species + 4 && cfg.variables <= cfg.species + 6, "Wrong transport conserved-variable count");
         require(std::isfinite(cfg.waveFactor) && cfg.waveFactor >= 1, "Invalid transport wave factor");
-        require(cfg.viscosity == 0 && cfg.conductivity == 0 && cfg.diffusivity == 0 && !cfg.mechanical,
-                "Oracle covers the inviscid, non-diffusive, non-mechanical transport only");
+        require(std::isfinite(cfg.viscosity) && std::isfinite(cfg.conductivity) && std::isfinite(cfg.diffusivity) &&
+                    cfg.viscosity >= 0 && cfg.conductivity >= 0 && cfg.diffusivity >= 0,
+                "Invalid transport coefficients");
+        require(cfg.diffusivity == 0 && !cfg.mechanical,
+                "Oracle covers the non-diffusive, non-mechanical transport only");
         require(volume && f, "Null transport geometry");
         row.assign(cfg.cells + 1, 0);
         for (size_t i = 0; i < cfg.cells; ++i) require(std::isfinite(volume[i]) && volume[i] > 0, "Invalid transport volume");
@@ -167,6 +241,8 @@ struct Transport {
         state.assign(all(), TState{});
         prim.assign(all(), Prim{});
         work.assign(cfg.faces, Work{});
+        visc.assign(cfg.faces, Visc{});
+        if (cfg.viscosity > 0) gradient.assign(cfg.cells * 9, 0);
         faceSpeed.assign(cfg.faces, 0);
         step.assign(cfg.cells, 0);
         partitions = std::min(size_t(256), std::max(size_t(1), (boundary.size() + 255) / 256));
@@ -176,6 +252,15 @@ struct Transport {
         require(!cfg.fixed || fixedStates, "Null transport state");
         validateStates(fixedStates, cfg.fixed);
         for (size_t i = 0; i < cfg.fixed; ++i) state[cfg.cells + i] = fixedStates[i];
+        for (size_t i = 0; i < cfg.fixed; ++i) {  // fixed primitives (reactiveTransport.cpp:492-499)
+            Prim& p = prim[cfg.cells + i];
+            for (size_t k = 0; k < cfg.species; ++k) p.rho += fixedQ[i * cfg.variables + k];
+            require(std::isfinite(p.rho) && p.rho > 0, "Invalid fixed density");
+            for (int d = 0; d < 3; ++d) {
+                p.u[d] = fixedQ[i * cfg.variables + cfg.species + d] / p.rho;
+                require(std::isfinite(p.u[d]), "Invalid fixed velocity");
+            }
+        }
     }
     static void validateStates(const TState* s, size_t n) {  // reactiveTransport.cpp:537-545
         for (size_t c = 0; c < n; ++c)
@@ -205,17 +290,91 @@ struct Transport {
         if (f.kind == 1)
             for (int d = 0; d < 3; ++d) ur[d] -= 2 * dot * f.normal[d];
     }
-    double faceFlux(size_t fi, size_t k) const {  // :180-191 (inviscid)
+    double faceFlux(size_t fi, size_t k) const {  // :180-191
         const Face& f = faces[fi];
         const Work& w = work[fi];
+        const Visc& v = visc[fi];
         const double ql = q[qi(f.owner, k)], qr = rightValue(f, k);
         double flux = w.advectL * ql + w.advectR * qr;
         if (k >= cfg.species && k < cfg.species + 3) flux += (w.pressureMomentum - 0.0) * f.normal[k - cfg.species];
         if (k == cfg.species + 3) flux += w.pressureEnergy;
         if (k < cfg.species) flux += 0.0;                        // species diffusion, D = 0
-        else if (k < cfg.species + 3) flux -= 0.0;               // viscous traction, mu = 0
-        else if (k == cfg.species + 3) flux += 0.0;              // viscous/conductive energy
+        else if (k < cfg.species + 3) flux -= v.traction[k - cfg.species];
+        else if (k == cfg.species + 3) flux += v.energy;
         return flux * f.area;
+    }
+    // faceNut (reactiveTransportKernels.h:80-83) weighted by density: the
+    // face's rho nu_t, owner side by ownerWeight.
+    double faceRhoNut(const Face& f, const TState& sl, const TState& sr) const {
+        const double right = nut[f.neighbour >= 0 ? size_t(f.neighbour) : size_t(f.owner)];
+        return f.ownerWeight * sl.rho * nut[f.owner] + (1 - f.ownerWeight) * sr.rho * right;
+    }
+    void gradients() {  // Gradients, :290-301
+        for (size_t c = 0; c < cfg.cells; ++c) {
+            double g[9] = {};
+            for (size_t j = row[c]; j < row[c + 1]; ++j) {
+                const int64_t entry = incidence[j];
+                const Face& f = faces[size_t(entry < 0 ? -entry - 1 : entry - 1)];
+                double ul[3], ur[3];
+                velocities(f, ul, ur);
+                const double scale = (entry < 0 ? 1.0 : -1.0) * f.area * inverseVolume[c];
+                for (int i = 0; i < 3; ++i)
+                    for (int d = 0; d < 3; ++d)
+                        g[3 * i + d] += (f.ownerWeight * ul[i] + (1 - f.ownerWeight) * ur[i]) * f.normal[d] * scale;
+            }
+            for (int j = 0; j < 9; ++j) gradient[c * 9 + j] = g[j];
+        }
+    }
+    void buildWale() {  // buildWale, reactiveTransport.cpp:841-849: Gradients + WaleViscosities (:303-313)
+        gradients();
+        bool failed = false;
+        for (size_t c = 0; c < cfg.cells; ++c) {
+            const double volume = 1 / inverseVolume[c];
+            double delta = std::nan(""), value = 0;
+            bool ok = std::isfinite(volume) && volume > 0;  // filterWidthFromCellVolume (reactiveWale.h:171-182)
+            if (ok) {
+                delta = std::cbrt(volume);
+                ok = std::isfinite(delta) && delta > 0;
+            }
+            ok = ok && waleEddyViscosity(&gradient[c * 9], delta, waleCw, value);
+            if (!ok || !std::isfinite(value) || value < 0 || !std::isfinite(state[c].rho * value)) {
+                failed = true;
+                continue;
+            }
+            nut[c] = value;
+        }
+        require(!failed, "Invalid WALE gradient/filter width/eddy viscosity");
+    }
+    // The viscous and conductive part of Faces (:418-439) for one face.
+    Visc viscousWork(const Face& f, const double* ul, const double* ur, const TState& sl, const TState& sr) const {
+        Visc w{{0, 0, 0}, 0};
+        if (cfg.viscosity > 0 || wale) {
+            const size_t l = size_t(f.owner), r = rightCell(f);
+            double g[9];
+            for (int j = 0; j < 9; ++j) {
+                g[j] = gradient[l * 9 + j];
+                if (f.neighbour >= 0) g[j] = f.ownerWeight * g[j] + (1 - f.ownerWeight) * gradient[r * 9 + j];
+            }
+            for (int i = 0; i < 3; ++i) {  // replace the normal derivative by the two-point one
+                double normal = 0;
+                for (int d = 0; d < 3; ++d) normal += g[i * 3 + d] * f.normal[d];
+                const double target = (ur[i] - ul[i]) / ((f.kind == 1 ? 2 : 1) * f.distance);
+                for (int d = 0; d < 3; ++d) g[i * 3 + d] += f.normal[d] * (target - normal);
+            }
+            const double div = g[0] + g[4] + g[8];
+            const double mu = cfg.viscosity + (wale ? faceRhoNut(f, sl, sr) : 0);
+            for (int i = 0; i < 3; ++i)
+                for (int d = 0; d < 3; ++d)
+                    w.traction[i] += mu * (g[3 * i + d] + g[3 * d + i] - (i == d ? 2 * div / 3 : 0)) * f.normal[d];
+            if (f.kind == 1) {  // a slip wall carries the normal stress only
+                double dot = 0;
+                for (int d = 0; d < 3; ++d) dot += w.traction[d] * f.normal[d];
+                for (int d = 0; d < 3; ++d) w.traction[d] = dot * f.normal[d];
+            }
+            for (int d = 0; d < 3; ++d) w.energy -= w.traction[d] * (f.ownerWeight * ul[d] + (1 - f.ownerWeight) * ur[d]);
+        }
+        if (cfg.conductivity > 0 && f.kind != 1) w.energy -= cfg.conductivity * (sr.T - sl.T) / f.distance;
+        return w;
     }
     void cells() {  // Cells, :193-199
         for (size_t c = 0; c < all(); ++c) {
@@ -225,7 +384,8 @@ struct Transport {
             prim[c] = p;
         }
     }
-    void facesHll() {  // Faces HLL coefficients, :343-356
+    void facesHll() {  // Faces: HLL coefficients (:343-356), viscous and conductive terms (:418-439)
+        bool waleFailed = false;
         for (size_t fi = 0; fi < cfg.faces; ++fi) {
             const Face& f = faces[fi];
             const TState& sl = state[f.owner];
@@ -248,9 +408,15 @@ struct Transport {
                         "HLLC face flux failed");
             }
             work[fi] = w;
+            const Visc v = viscousWork(f, ul, ur, sl, sr);
+            if (wale && (!std::isfinite(v.energy) || !std::isfinite(v.traction[0]) || !std::isfinite(v.traction[1]) ||
+                         !std::isfinite(v.traction[2])))
+                waleFailed = true;
+            visc[fi] = v;
         }
+        require(!waleFailed, "Nonfinite WALE traction/energy flux; stage remains uncommitted");
     }
-    void faceSpeeds() {  // FaceSpeeds, :315-342 (D = 0)
+    void faceSpeeds() {  // FaceSpeeds, :315-342 (no scalar closure)
         for (size_t fi = 0; fi < cfg.faces; ++fi) {
             const Face& f = faces[fi];
             const TState& sl = state[f.owner];
@@ -261,7 +427,8 @@ struct Transport {
             const double aL = cfg.waveFactor * sl.sound, aR = cfg.waveFactor * sr.sound;
             const double left = mn(0, mn(unL - aL, unR - aR)), right = mx(0, mx(unL + aL, unR + aR));
             const double invWave = 1 / (right - left);
-            const double muFace = cfg.viscosity + 0;
+            const double rhoNut = wale ? faceRhoNut(f, sl, sr) : 0;
+            const double muFace = cfg.viscosity + rhoNut;
             const double D = cfg.diffusivity + mx(mx(4 * muFace / (3 * sl.rho), 4 * muFace / (3 * sr.rho)),
                                                   mx(cfg.conductivity / (sl.rho * sl.cv), cfg.conductivity / (sr.rho * sr.cv)));
             double speed = mx(std::fabs(unL) + aL, std::fabs(unR) + aR) + 2 * (D + 0) / f.distance;
@@ -302,7 +469,14 @@ struct Transport {
             out[k] = tree256(v);
         }
     }
+    // prepareResident's order (reactiveTransport.cpp:1379-1393): Cells, then the
+    // WALE viscosities, or for the flux the velocity gradients of a viscous model.
+    void prepare(bool transport) {
+        if (wale) buildWale();
+        else if (transport && cfg.viscosity > 0) gradients();
+    }
     void upload(const double* hostQ, const TState* states) {
+        used = true;
         require(hostQ, "Null packed source");
         for (size_t c = 0; c < cfg.cells; ++c)
             for (size_t k = 0; k < cfg.variables; ++k) q[qi(c, k)] = hostQ[c * cfg.variables + k];
@@ -387,6 +561,7 @@ int oracle_transport_rhs(void* h, const double* q, const TState* states, double*
     return guard(h, [&](Transport& t) {
         t.haveInitial = false;
         t.upload(q, states);
+        t.prepare(true);
         t.facesHll();
         for (size_t c = 0; c < t.cfg.cells; ++c)
             for (size_t k = 0; k < t.cfg.variables; ++k) rhs[c * t.cfg.variables + k] = t.rate(c, k);
@@ -398,6 +573,7 @@ int oracle_transport_stage(void* h, double* q, const TState* states, double dt, 
         require(std::isfinite(dt) && dt > 0 && (stage == 0 || stage == 1), "Invalid RK stage");
         require(stage == 0 || (t.haveInitial && t.stageDt == dt), "RK stage 1 requires matching stage 0");
         t.upload(q, states);
+        t.prepare(true);
         t.facesHll();
         t.boundaryRates(boundary);
         t.advance(dt, stage);
@@ -410,8 +586,54 @@ int oracle_transport_stable_step(void* h, const double* q, const TState* states,
         require(dt && std::isfinite(cfl) && cfl > 0 && cfl <= .5 && std::isfinite(maximum) && maximum > 0,
                 "Invalid transport CFL controls");
         t.upload(q, states);
+        t.prepare(false);
         t.faceSpeeds();
         *dt = t.minimumStep(cfl, maximum);
+    });
+}
+// uploadPrimitives (reactiveTransport.cpp:850-858): cell primitives and states.
+static void uploadPrimitives(Transport& t, const Primitive* primitive, const TState* states) {
+    require(primitive, "Null transport primitives");
+    for (size_t c = 0; c < t.cfg.cells; ++c) {
+        require(std::isfinite(primitive[c].rho) && primitive[c].rho > 0, "Invalid primitive density");
+        for (double u : primitive[c].u) require(std::isfinite(u), "Invalid primitive velocity");
+    }
+    require(states, "Null transport state");
+    Transport::validateStates(states, t.cfg.cells);
+    t.used = true;
+    for (size_t c = 0; c < t.cfg.cells; ++c) {
+        t.state[c] = states[c];
+        t.prim[c].rho = primitive[c].rho;
+        for (int d = 0; d < 3; ++d) t.prim[c].u[d] = primitive[c].u[d];
+    }
+}
+int oracle_transport_stable_step_primitives(void* h, const Primitive* primitive, const TState* states, double cfl,
+                                            double maximum, double* dt) {
+    return guard(h, [&](Transport& t) {  // reactiveTransport.cpp:1647-1666
+        uploadPrimitives(t, primitive, states);
+        if (t.wale) t.buildWale();
+        t.faceSpeeds();
+        require(dt, "Null transport output buffer");
+        *dt = t.minimumStep(cfl, maximum);
+    });
+}
+// installWale (reactiveTransport.cpp:636-652): once, before any upload.
+int oracle_transport_set_wale(void* h, double cw) {
+    return guard(h, [&](Transport& t) {
+        require(!t.wale && !t.used, "WALE is immutable and must be installed before stepping");
+        require(std::isfinite(cw) && cw >= 0, "Invalid WALE coefficient");
+        t.gradient.assign(t.cfg.cells * 9, 0);
+        t.nut.assign(t.cfg.cells, 0);
+        t.waleCw = cw;
+        t.wale = true;
+    });
+}
+int oracle_transport_wale_primitives(void* h, const Primitive* primitive, const TState* states, double* nut) {
+    return guard(h, [&](Transport& t) {  // reactiveTransport.cpp:1703-1708
+        require(t.wale && nut, "WALE not selected or missing output");
+        uploadPrimitives(t, primitive, states);
+        t.buildWale();
+        for (size_t c = 0; c < t.cfg.cells; ++c) nut[c] = t.nut[c];
     });
 }
 int oracle_transport_set_scheme(void* h, int scheme) {
@@ -429,6 +651,17 @@ int oracle_hllc_face(const double* left, const double* right, const double* norm
         return 0;
     out[0] = w.advectL; out[1] = w.advectR; out[2] = w.pressureMomentum; out[3] = w.pressureEnergy;
     return 1;
+}
+// The velocity gradients (cells x 9) of the last WALE build or viscous flux.
+int oracle_transport_gradient(void* h, double* out) {
+    return guard(h, [&](Transport& t) {
+        require(!t.gradient.empty() && out, "No velocity gradient on this transport");
+        for (size_t j = 0; j < t.gradient.size(); ++j) out[j] = t.gradient[j];
+    });
+}
+// One WALE eddy viscosity: 1 and nut, or 0 where the reference fails.
+int oracle_wale_eddy_viscosity(const double* gradient, double delta, double cw, double* nut) {
+    return waleEddyViscosity(gradient, delta, cw, *nut) ? 1 : 0;
 }
 // Conservation totals sum_c q_ck V_c in serial cell order (ReactiveFoam.C:953-958).
 void oracle_totals(size_t cells, size_t variables, const double* q, const double* volume, double* out) {

@@ -1109,6 +1109,7 @@ static Step* createStep(const std::vector<Transport*>& list, HemHandle* hem) {
     size_t total = 0;
     for (Transport* t : list) {
         require(t, "Null transport handle");
+        require(!t->viscous(), "The resident step carries the inviscid flux (viscosity, conductivity and WALE off)");
         require(t->cartesian.valid, "The resident step needs a structured-box transport mesh");
         require(t->device == hem->device, "Transport and HEM handles are on different devices");
         require(size_t(hem->ns) == t->cfg.species && t->cfg.variables == t->cfg.species + 4,
@@ -1196,6 +1197,7 @@ static Step* createStep(const std::vector<Transport*>& list, HemHandle* hem) {
     RAMD_HIP(hipMemsetAsync(s->stateCount, 0, kStateMaxBins * sizeof(unsigned), s->stream));
     RAMD_HIP(hipMemsetAsync(s->violations, 0, n * sizeof(unsigned long long), s->stream));
     RAMD_HIP(hipStreamSynchronize(s->stream));
+    for (Transport* t : list) t->stepAttached = true;  // the WALE closure can no longer be installed
     return s.release();
 }
 

@@ -6,14 +6,16 @@
 // Data layout in HBM: conserved q is structure-of-arrays, q[k*(cells+fixed)+c]
 // (the reference's qi(), reactiveTransportKernels.h:85), so every kernel
 // streams each variable with unit stride.  Face geometry is compacted to
-// 48-byte records (owner, resolved right cell, kind, normal, area), the cell
+// 64-byte records (owner, resolved right cell, kind, normal, area, distance,
+// ownerWeight), the cell
 // face lists are a CSR of signed face ids in the reference's order (faces
 // sorted by id within a cell, owner side negative), which fixes the FP64
 // summation order of every cell rate.
 //
 // Kernels (one launch each, no host round trip between them):
 //   primitives   rho = sum q_k, u = (rho u)/rho               (Cells)
-//   hll          per-face HLL coefficients                    (Faces)
+//   gradient     per-cell Gauss velocity gradient [+ WALE nu_t] (Gradients, WaleViscosities)
+//   hll          per-face HLL coefficients [+ viscous record]  (Faces)
 //   advance      per (cell, variable) CSR gather + SSPRK2     (Rhs + Advance)
 //   boundary     per-variable boundary flux, fixed 256-lane tree (BoundaryPartial/Finish)
 //   faceSpeed / cellStep / min-reduction                       (FaceSpeeds, Step, Minimum)
@@ -74,13 +76,170 @@ __device__ inline void normalVelocities(const FaceDev& f, const double* __restri
     }
 }
 
+// ---- viscous stress, conduction and the WALE closure ----------------------
+// Parameters of the viscous part of the face flux: the cell velocity
+// gradients, the WALE eddy viscosities (null without the closure) and the
+// constant viscosity and conductivity.
+struct ViscousArgs {
+    const double* gradient;
+    const double* nut;
+    double viscosity, conductivity;
+};
+
+// Gradients (reactiveTransportKernels.h:290-301): Gauss sum over the cell's
+// faces in CSR order of the interpolated face velocity, ownerWeight on the
+// owner side.  With the WALE closure (Wale) the same thread goes on to the
+// cell's eddy viscosity (WaleViscosities, :303-313); a cell the closure
+// rejects sets *error.
+__device__ inline bool waleEddyViscosity(const double* g, double delta, double cw, double& nut);
+
+template <bool Wale>
+__global__ __launch_bounds__(256) void gradientKernel(const FaceDev* __restrict__ faces,
+                                                      const uint32_t* __restrict__ row,
+                                                      const int32_t* __restrict__ incidence,
+                                                      const double* __restrict__ invV, const double* __restrict__ prim,
+                                                      size_t all, size_t cells, double* __restrict__ gradient,
+                                                      const double* __restrict__ filterWidth, double cw,
+                                                      const ReactiveTransportState* __restrict__ state,
+                                                      double* __restrict__ nut, unsigned* __restrict__ error) {
+    const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    double g[9];
+    for (int j = 0; j < 9; ++j) g[j] = 0;
+    for (uint32_t e = row[c]; e < row[c + 1]; ++e) {
+        const int32_t entry = incidence[e];
+        const FaceDev f = faces[entry < 0 ? -entry - 1 : entry - 1];
+        double ul[3], ur[3], unL, unR;
+        normalVelocities(f, prim, all, ul, ur, unL, unR);
+        const double scale = (entry < 0 ? 1.0 : -1.0) * f.area * invV[c];
+        for (int i = 0; i < 3; ++i) {
+            const double face = f.weight * ul[i] + (1 - f.weight) * ur[i];
+            for (int d = 0; d < 3; ++d) g[3 * i + d] += face * f.n[d] * scale;
+        }
+    }
+    for (int j = 0; j < 9; ++j) gradient[c * 9 + j] = g[j];
+    if (Wale) {
+        double value = 0;
+        const bool ok = waleEddyViscosity(g, filterWidth[c], cw, value);
+        if (!ok || !isfinite(value) || value < 0 || !isfinite(state[c].rho * value)) atomicOr(error, 1u);
+        else nut[c] = value;
+    }
+}
+
+// ReactiveWale::eddyViscosity (reactiveWale.h:93-167): the invariants of g
+// scaled by max |g_ij| (S = symm(g), Sd = dev(symm(g g))), nu_t = (Cw
+// delta)^2 (Sd:Sd)^(3/2) / ((S:S)^(5/2) + (Sd:Sd)^(5/4)), the product
+// Cw^2 delta^2 scale ratio assembled from mantissas and exponents
+// (:47-80) so that no partial product over- or underflows.  frexp, ldexp
+// and sqrt are exact or correctly rounded, so every step rounds as on the
+// host.  delta > 0 is the cell's filter width.
+__device__ inline bool waleEddyViscosity(const double* g, double delta, double cw, double& nut) {
+    if (!isfinite(delta) || !(delta > 0) || !isfinite(cw) || cw < 0) return false;
+    double scale = 0;
+    for (int i = 0; i < 9; ++i) {
+        if (!isfinite(g[i])) return false;
+        const double m = g[i] < 0 ? -g[i] : g[i];
+        if (m > scale) scale = m;
+    }
+    if (scale == 0 || cw == 0) {
+        nut = 0;
+        return true;
+    }
+    double s[9], sq[9];
+    for (int i = 0; i < 9; ++i) s[i] = g[i] / scale;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double v = 0;
+            for (int k = 0; k < 3; ++k) v += s[3 * i + k] * s[3 * k + j];
+            sq[3 * i + j] = v;
+        }
+    const double trace = sq[0] + sq[4] + sq[8];
+    double strain2 = 0, dev2 = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const double strain = 0.5 * (s[3 * i + j] + s[3 * j + i]);
+            double dev = 0.5 * (sq[3 * i + j] + sq[3 * j + i]);
+            if (i == j) dev -= trace / 3.0;
+            strain2 += strain * strain;
+            dev2 += dev * dev;
+        }
+    if (!isfinite(strain2) || !isfinite(dev2)) return false;
+    if (dev2 == 0) {
+        nut = 0;
+        return true;
+    }
+    const double strainPower = strain2 * strain2 * sqrt(strain2);
+    const double root = sqrt(dev2);
+    const double devPower = dev2 * root;
+    const double denominator = strainPower + dev2 * sqrt(root);
+    if (!(denominator > 0) || !isfinite(denominator) || !isfinite(devPower)) return false;
+    const double ratio = devPower / denominator;
+    if (ratio == 0) {
+        nut = 0;
+        return true;
+    }
+    int ec, ed, eg, er, em;
+    const double mc = frexp(cw, &ec), md = frexp(delta, &ed), mg = frexp(scale, &eg), mr = frexp(ratio, &er);
+    const double m = frexp(mc * mc * md * md * mg * mr, &em);
+    nut = ldexp(m, 2 * ec + 2 * ed + eg + er + em);
+    return isfinite(nut);
+}
+
+// The face's rho nu_t: owner and right densities times the eddy viscosities
+// of the owner and (on internal faces) the neighbour, weighted by ownerWeight
+// (faceNut, reactiveTransportKernels.h:80-83).
+__device__ inline double faceRhoNut(const FaceDev& f, double rhoL, double rhoR, const double* __restrict__ nut) {
+    return f.weight * rhoL * nut[f.owner] + (1 - f.weight) * rhoR * nut[f.kind == 0 ? f.right : f.owner];
+}
+
+// Viscous traction and the viscous + conductive energy flux of one face
+// (Faces, reactiveTransportKernels.h:418-439): the interpolated gradient with
+// its normal part replaced by the two-point difference, the stress
+// mu (g + g^T - 2/3 div I) n (normal part only on slip walls), its work with
+// the interpolated face velocity, and -k (T_R - T_L) / distance off slip walls.
+__device__ inline double4 viscousWork(const FaceDev& f, const double* ul, const double* ur,
+                                      const ReactiveTransportState& sl, const ReactiveTransportState& sr,
+                                      const ViscousArgs& v) {
+    double t[3] = {0, 0, 0}, energy = 0;
+    if (v.viscosity > 0 || v.nut) {
+        double g[9];
+        for (int j = 0; j < 9; ++j) {
+            g[j] = v.gradient[size_t(f.owner) * 9 + j];
+            if (f.kind == 0) g[j] = f.weight * g[j] + (1 - f.weight) * v.gradient[size_t(f.right) * 9 + j];
+        }
+        for (int i = 0; i < 3; ++i) {
+            double normal = 0;
+            for (int d = 0; d < 3; ++d) normal += g[i * 3 + d] * f.n[d];
+            const double target = (ur[i] - ul[i]) / ((f.kind == 1 ? 2.0 : 1.0) * f.distance);
+            for (int d = 0; d < 3; ++d) g[i * 3 + d] += f.n[d] * (target - normal);
+        }
+        const double div = g[0] + g[4] + g[8];
+        const double mu = v.viscosity + (v.nut ? faceRhoNut(f, sl.rho, sr.rho, v.nut) : 0.0);
+        for (int i = 0; i < 3; ++i)
+            for (int d = 0; d < 3; ++d)
+                t[i] += mu * (g[3 * i + d] + g[3 * d + i] - (i == d ? 2 * div / 3 : 0.0)) * f.n[d];
+        if (f.kind == 1) {
+            double dot = 0;
+            for (int d = 0; d < 3; ++d) dot += t[d] * f.n[d];
+            for (int d = 0; d < 3; ++d) t[d] = dot * f.n[d];
+        }
+        for (int d = 0; d < 3; ++d) energy -= t[d] * (f.weight * ul[d] + (1 - f.weight) * ur[d]);
+    }
+    if (v.conductivity > 0 && f.kind != 1) energy -= v.conductivity * (sr.T - sl.T) / f.distance;
+    return make_double4(t[0], t[1], t[2], energy);
+}
+
 // Face coefficients (Faces, reactiveTransportKernels.h:343-356): HLL, or the
 // HLLC option (flux_device.h), which also reads the total energies from q.
 // A face the HLLC flux rejects keeps its HLL coefficients and is counted.
+// With V the same thread adds the viscous record (visc[fi]); with the WALE
+// closure a non-finite traction or energy sets *waleError.
+template <bool V>
 __global__ void hllKernel(const FaceDev* __restrict__ faces, size_t nf, const double* __restrict__ prim, size_t all,
                           const ReactiveTransportState* __restrict__ state, double waveFactor,
                           double4* __restrict__ work, int scheme, const double* __restrict__ q, int ns,
-                          unsigned long long* __restrict__ failures) {
+                          unsigned long long* __restrict__ failures, ViscousArgs va, double4* __restrict__ visc,
+                          unsigned* __restrict__ waleError) {
     const size_t fi = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (fi >= nf) return;
     const FaceDev f = faces[fi];
@@ -100,11 +259,17 @@ __global__ void hllKernel(const FaceDev* __restrict__ faces, size_t nf, const do
                           pR, aR, q[size_t(ns + 3) * all + f.right], f.n, w))
         atomicAdd(failures, 1ull);
     work[fi] = w;
+    if (V) {
+        const double4 v = viscousWork(f, ul, ur, state[f.owner], state[f.right], va);
+        if (va.nut && !(isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w))) atomicOr(waleError, 2u);
+        visc[fi] = v;
+    }
 }
 
-// faceFlux (reactiveTransportKernels.h:180-191) for the inviscid model.
-__device__ inline double faceFlux(const FaceDev& f, const double4& w, const double* __restrict__ q, size_t all, int ns,
-                                  int k) {
+// faceFlux (reactiveTransportKernels.h:180-191); visc (null for the
+// inviscid model) holds the face's traction and energy flux.
+__device__ inline double faceFlux(const FaceDev& f, const double4& w, const double4* __restrict__ visc, uint32_t fi,
+                                  const double* __restrict__ q, size_t all, int ns, int k) {
     const double ql = q[size_t(k) * all + f.owner];
     double qr = q[size_t(k) * all + f.right];
     if (f.kind == 1 && k >= ns && k < ns + 3) {
@@ -112,12 +277,13 @@ __device__ inline double faceFlux(const FaceDev& f, const double4& w, const doub
         for (int d = 0; d < 3; ++d) normal += q[size_t(ns + d) * all + f.right] * f.n[d];
         qr -= 2 * normal * f.n[k - ns];
     }
+    const double4 v = visc ? visc[fi] : make_double4(0, 0, 0, 0);
     double flux = w.x * ql + w.y * qr;
     if (k >= ns && k < ns + 3) flux += (w.z - 0.0) * f.n[k - ns];
     if (k == ns + 3) flux += w.w;
     if (k < ns) flux += 0.0;
-    else if (k < ns + 3) flux -= 0.0;
-    else if (k == ns + 3) flux += 0.0;
+    else if (k < ns + 3) flux -= k == ns ? v.x : (k == ns + 1 ? v.y : v.z);
+    else if (k == ns + 3) flux += v.w;
     return flux * f.area;
 }
 
@@ -125,7 +291,8 @@ __device__ inline double faceFlux(const FaceDev& f, const double4& w, const doub
 // mode 0: stage 0 -> out = q + dt L; 1: stage 1 -> out = .5 out + .5 (q + dt L);
 // 2: rhs only -> out = L (cell-major index of a cells x nv array).
 __global__ void advanceKernel(const FaceDev* __restrict__ faces, const double4* __restrict__ work,
-                              const uint32_t* __restrict__ row, const int32_t* __restrict__ incidence,
+                              const double4* __restrict__ visc, const uint32_t* __restrict__ row,
+                              const int32_t* __restrict__ incidence,
                               const double* __restrict__ invV, const double* __restrict__ q, size_t cells, size_t all,
                               int ns, int nv, double dt, int mode, double* __restrict__ out) {
     const size_t j = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -138,7 +305,7 @@ __global__ void advanceKernel(const FaceDev* __restrict__ faces, const double4* 
         const int32_t entry = incidence[e];
         const uint32_t fi = uint32_t(entry < 0 ? -entry - 1 : entry - 1);
         const double sign = entry < 0 ? -1 : 1;
-        rate += sign * faceFlux(faces[fi], work[fi], q, all, ns, k) * inv;
+        rate += sign * faceFlux(faces[fi], work[fi], visc, fi, q, all, ns, k) * inv;
     }
     const size_t index = size_t(k) * all + c;
     if (mode == 0) out[index] = q[index] + dt * rate;
@@ -150,6 +317,7 @@ __global__ void advanceKernel(const FaceDev* __restrict__ faces, const double4* 
 // then the 256-lane tree of reduceGroups (reactiveTransport.cpp:70-81, 233-246).
 __global__ __launch_bounds__(256) void boundaryPartialKernel(const FaceDev* __restrict__ faces,
                                                              const double4* __restrict__ work,
+                                                             const double4* __restrict__ visc,
                                                              const uint32_t* __restrict__ boundary, size_t nb,
                                                              size_t partitions, const double* __restrict__ q,
                                                              size_t all, int ns, double* __restrict__ partial) {
@@ -160,7 +328,7 @@ __global__ __launch_bounds__(256) void boundaryPartialKernel(const FaceDev* __re
     double sum = 0;
     for (size_t j = tile * 256 + threadIdx.x; j < nb; j += partitions * 256) {
         const uint32_t fi = boundary[j];
-        sum += faceFlux(faces[fi], work[fi], q, all, ns, k);
+        sum += faceFlux(faces[fi], work[fi], visc, fi, q, all, ns, k);
     }
     values[threadIdx.x] = sum;
     __syncthreads();
@@ -184,19 +352,28 @@ __global__ __launch_bounds__(256) void boundaryFinishKernel(const double* __rest
     if (threadIdx.x == 0) rate[k] = values[0];
 }
 
-// FaceSpeeds (reactiveTransportKernels.h:315-342) with zero diffusivities.
+// FaceSpeeds (reactiveTransportKernels.h:315-342): wave speed plus twice the
+// face diffusion number D / distance, D the largest of 4 mu / (3 rho) and
+// k / (rho cv) over the two sides (mu with the WALE rho nu_t when nut is set;
+// no species diffusion or scalar closure).
 __global__ void faceSpeedKernel(const FaceDev* __restrict__ faces, size_t nf, const double* __restrict__ prim,
                                 size_t all, const ReactiveTransportState* __restrict__ state, double waveFactor,
+                                double viscosity, double conductivity, const double* __restrict__ nut,
                                 double* __restrict__ speed) {
     const size_t fi = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (fi >= nf) return;
     const FaceDev f = faces[fi];
     double ul[3], ur[3], unL, unR;
     normalVelocities(f, prim, all, ul, ur, unL, unR);
-    const double aL = waveFactor * state[f.owner].sound, aR = waveFactor * state[f.right].sound;
+    const ReactiveTransportState& sl = state[f.owner];
+    const ReactiveTransportState& sr = state[f.right];
+    const double aL = waveFactor * sl.sound, aR = waveFactor * sr.sound;
     const double left = dmin(0, dmin(unL - aL, unR - aR)), right = dmax(0, dmax(unL + aL, unR + aR));
     const double invWave = 1 / (right - left);
-    double s = dmax(::fabs(unL) + aL, ::fabs(unR) + aR) + 2 * (0.0 + 0) / f.distance;
+    const double muFace = viscosity + (nut ? faceRhoNut(f, sl.rho, sr.rho, nut) : 0.0);
+    const double D = 0.0 + dmax(dmax(4 * muFace / (3 * sl.rho), 4 * muFace / (3 * sr.rho)),
+                                dmax(conductivity / (sl.rho * sl.cv), conductivity / (sr.rho * sr.cv)));
+    double s = dmax(::fabs(unL) + aL, ::fabs(unR) + aR) + 2 * (D + 0.0) / f.distance;
     if (!isfinite(unL) || !isfinite(unR) || !isfinite(s) || !isfinite(invWave) || right <= left) s = -1;
     speed[fi] = s;
 }
@@ -359,8 +536,8 @@ Transport::Transport(int backend, const ReactiveTransportConfig& c, const double
     require(std::isfinite(cfg.viscosity) && std::isfinite(cfg.conductivity) && std::isfinite(cfg.diffusivity) &&
                 cfg.viscosity >= 0 && cfg.conductivity >= 0 && cfg.diffusivity >= 0,
             "Invalid transport coefficients");
-    require(cfg.viscosity == 0 && cfg.conductivity == 0 && cfg.diffusivity == 0,
-            "Viscous, conductive and diffusive transport are not provided by the MI355X build (inviscid HLL only)");
+    require(cfg.diffusivity == 0,
+            "Diffusive species transport (diffusivity > 0) is not provided by the MI355X build");
     require(!cfg.mechanical, "The mechanical-equilibrium closure is not provided by the MI355X build");
     require(volumes && hostFaces, "Null transport geometry");
     require(cfg.cells <= std::numeric_limits<size_t>::max() - cfg.fixed, "Fixed-state index overflow");
@@ -399,6 +576,7 @@ Transport::Transport(int backend, const ReactiveTransportConfig& c, const double
         for (int k = 0; k < 3; ++k) d.n[k] = f.normal[k];
         d.area = f.area;
         d.distance = f.distance;
+        d.weight = f.ownerWeight;
     }
     for (size_t c2 = 0; c2 < cfg.cells; ++c2) row[c2 + 1] += row[c2];
     std::vector<int32_t> incidence(row.back());
@@ -429,7 +607,9 @@ Transport::Transport(int backend, const ReactiveTransportConfig& c, const double
     staging = allocate<double>(std::max(all, cfg.cells) * nv);
     fluxFailures = allocate<unsigned long long>(1);
     RAMD_HIP(hipMemsetAsync(fluxFailures, 0, sizeof(unsigned long long), stream));
-    deviceProfile.faceWorkspaceBytes = cfg.faces * (sizeof(double4) + sizeof(double));
+    if (cfg.viscosity > 0) gradient = allocate<double>(cfg.cells * 9);
+    if (cfg.viscosity > 0 || cfg.conductivity > 0) visc = allocate<double4>(cfg.faces);
+    deviceProfile.faceWorkspaceBytes = cfg.faces * (sizeof(double4) + sizeof(double)) + (visc ? cfg.faces * 32 : 0);
     std::vector<double> inverse(cfg.cells);
     for (size_t c2 = 0; c2 < cfg.cells; ++c2) {
         inverse[c2] = 1 / volumes[c2];
@@ -440,7 +620,8 @@ Transport::Transport(int backend, const ReactiveTransportConfig& c, const double
     upload(rowDev, row.data(), row.size());
     upload(incidenceDev, incidence.data(), incidence.size());
     if (nBoundary) upload(boundaryDev, boundary.data(), nBoundary);
-    cartesian = detectCartesian(hostFaces, volumes, row, incidence);
+    // the structured-box kernels carry the inviscid flux only
+    if (!viscous()) cartesian = detectCartesian(hostFaces, volumes, row, incidence);
     if (cfg.fixed) {
         require(fixedQ, "Null packed source");
         validateStates(fixedStates, cfg.fixed);
@@ -557,23 +738,100 @@ void Transport::primitives() {
     launched();
 }
 
+// prepareResident's order (reactiveTransport.cpp:1379-1393): states, Cells,
+// then the WALE viscosities or, for the flux of a viscous model, the velocity
+// gradients, then the face coefficients or the CFL face speeds.
 void Transport::prepareResident(const ReactiveTransportState* states, bool transport) {
     uploadState(states);
     primitives();
-    if (transport) {
-        hipLaunchKernelGGL(hllKernel, dim3(gridFor(cfg.faces, 256)), dim3(256), 0, stream, faces, cfg.faces, prim, all,
-                           state, cfg.waveFactor, work, scheme, q, ns, fluxFailures);
-        RAMD_HIP(hipGetLastError());
-        launched();
-        ++deviceProfile.transportFaceLaunches;
-    } else {
-        launchFaceSpeeds();
+    if (wale) buildWale();
+    else if (transport && cfg.viscosity > 0) launchGradients(false);
+    if (transport) launchFaces();
+    else launchFaceSpeeds();
+}
+
+void Transport::launchFaces() {
+    const ViscousArgs va{gradient, wale ? nut : nullptr, cfg.viscosity, cfg.conductivity};
+    if (wale) RAMD_HIP(hipMemsetAsync(waleError, 0, sizeof(unsigned), stream));
+    if (viscous())
+        hipLaunchKernelGGL(hllKernel<true>, dim3(gridFor(cfg.faces, 256)), dim3(256), 0, stream, faces, cfg.faces,
+                           prim, all, state, cfg.waveFactor, work, scheme, q, ns, fluxFailures, va, visc, waleError);
+    else
+        hipLaunchKernelGGL(hllKernel<false>, dim3(gridFor(cfg.faces, 256)), dim3(256), 0, stream, faces, cfg.faces,
+                           prim, all, state, cfg.waveFactor, work, scheme, q, ns, fluxFailures, va, visc, waleError);
+    RAMD_HIP(hipGetLastError());
+    launched();
+    ++deviceProfile.transportFaceLaunches;
+    if (wale) {
+        unsigned status = 0;
+        RAMD_HIP(hipMemcpyAsync(&status, waleError, sizeof(status), hipMemcpyDeviceToHost, stream));
+        finish();
+        require(!status, "Nonfinite WALE traction/energy flux; stage remains uncommitted");
     }
+}
+
+void Transport::launchGradients(bool withWale) {
+    const unsigned blocks = gridFor(cfg.cells, 256);
+    if (withWale)
+        hipLaunchKernelGGL(gradientKernel<true>, dim3(blocks), dim3(256), 0, stream, faces, rowDev, incidenceDev, invV,
+                           prim, all, cfg.cells, gradient, filterWidth, waleCw, state, nut, waleError);
+    else
+        hipLaunchKernelGGL(gradientKernel<false>, dim3(blocks), dim3(256), 0, stream, faces, rowDev, incidenceDev,
+                           invV, prim, all, cfg.cells, gradient, filterWidth, waleCw, state, nut, waleError);
+    RAMD_HIP(hipGetLastError());
+    launched();
+}
+
+// buildWale (reactiveTransport.cpp:841-849): gradients and eddy viscosities
+// in one pass, then the closure's status.
+void Transport::buildWale() {
+    RAMD_HIP(hipMemsetAsync(waleError, 0, sizeof(unsigned), stream));
+    launchGradients(true);
+    ++waleProfile.gradientBuilds;
+    ++waleProfile.viscosityBuilds;
+    waleProfile.cellsEvaluated += cfg.cells;
+    unsigned status = 0;
+    RAMD_HIP(hipMemcpyAsync(&status, waleError, sizeof(status), hipMemcpyDeviceToHost, stream));
+    finish();
+    require(!status, "Invalid WALE gradient/filter width/eddy viscosity");
+}
+
+// installWale (reactiveTransport.cpp:636-652): once, before any upload or
+// step.  The filter width cbrt(1 / (1 / V)) of each cell is fixed by the
+// geometry and taken on the host, as the reference's filter does.
+void Transport::installWale(const ReactiveWaleOptionsV1& options) {
+    require(options.abiVersion == 1 && options.structBytes == sizeof(options), "Invalid WALE options ABI");
+    require(!wale && !highestVersion && !stats.stages && !stats.stepQueries && profile.conservedUploads == 0 &&
+                profile.stateUploadBytes == 0,
+            "WALE is immutable and must be installed before stepping");
+    require(!stepAttached, "WALE must be installed before a resident step is created on this transport");
+    require(!cfg.mechanical, "WALE mechanical-environment closure is not implemented");
+    require(std::isfinite(options.Cw) && options.Cw >= 0, "Invalid WALE coefficient");
+    const uint64_t before = stats.allocatedBytes;
+    std::vector<double> inverse(cfg.cells), width(cfg.cells);
+    downloadBytes(inverse.data(), invV, cfg.cells * sizeof(double));
+    finish();
+    for (size_t c = 0; c < cfg.cells; ++c) {
+        const double volume = 1 / inverse[c];
+        width[c] = std::isfinite(volume) && volume > 0 ? std::cbrt(volume) : std::numeric_limits<double>::quiet_NaN();
+    }
+    if (!gradient) gradient = allocate<double>(cfg.cells * 9);
+    if (!visc) visc = allocate<double4>(cfg.faces);
+    nut = allocate<double>(cfg.cells);
+    filterWidth = allocate<double>(cfg.cells);
+    waleError = allocate<unsigned>(1);
+    upload(filterWidth, width.data(), cfg.cells);
+    RAMD_HIP(hipMemsetAsync(nut, 0, cfg.cells * sizeof(double), stream));
+    finish();
+    waleCw = options.Cw;
+    wale = true;
+    cartesian.valid = false;
+    waleProfile.workspaceBytes = stats.allocatedBytes - before;
 }
 
 void Transport::launchFaceSpeeds() {
     hipLaunchKernelGGL(faceSpeedKernel, dim3(gridFor(cfg.faces, 256)), dim3(256), 0, stream, faces, cfg.faces, prim,
-                       all, state, cfg.waveFactor, faceSpeed);
+                       all, state, cfg.waveFactor, cfg.viscosity, cfg.conductivity, wale ? nut : nullptr, faceSpeed);
     RAMD_HIP(hipGetLastError());
     launched();
     ++deviceProfile.cflFaceLaunches;
@@ -581,7 +839,7 @@ void Transport::launchFaceSpeeds() {
 
 void Transport::gather() {
     hipLaunchKernelGGL(boundaryPartialKernel, dim3(unsigned(nv * partitions)), dim3(256), 0, stream, faces, work,
-                       boundaryDev, nBoundary, partitions, q, all, ns, boundaryPartial);
+                       visc, boundaryDev, nBoundary, partitions, q, all, ns, boundaryPartial);
     RAMD_HIP(hipGetLastError());
     hipLaunchKernelGGL(boundaryFinishKernel, dim3(unsigned(nv)), dim3(256), 0, stream, boundaryPartial, partitions,
                        boundaryRate);
@@ -598,7 +856,8 @@ void Transport::boundaryRatesFrom(const double* qv, const double* p, const doubl
         RAMD_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(boundaryPartialKernel, dim3(unsigned(nv * partitions)), dim3(256), 0, s, faces, work,
-                       boundaryDev, nBoundary, partitions, qv, stride, ns, boundaryPartial);
+                       static_cast<const double4*>(nullptr), boundaryDev, nBoundary, partitions, qv, stride, ns,
+                       boundaryPartial);
     RAMD_HIP(hipGetLastError());
     hipLaunchKernelGGL(boundaryFinishKernel, dim3(unsigned(nv)), dim3(256), 0, s, boundaryPartial, partitions, out);
     RAMD_HIP(hipGetLastError());
@@ -634,7 +893,7 @@ void Transport::advance(double dt, int stage, double* hostQ, double* boundary) {
         cartesianAdvance(dt, stage);
     } else {
         hipLaunchKernelGGL(advanceKernel, dim3(gridFor(cfg.cells * nv, 256)), dim3(256), 0, stream, faces, work,
-                           rowDev, incidenceDev, invV, q, cfg.cells, all, ns, nv, dt, stage, initial);
+                           visc, rowDev, incidenceDev, invV, q, cfg.cells, all, ns, nv, dt, stage, initial);
         RAMD_HIP(hipGetLastError());
         launched();
     }
@@ -653,8 +912,8 @@ void Transport::advance(double dt, int stage, double* hostQ, double* boundary) {
 
 void Transport::rhs(double* hostRhs, double* boundary) {
     double* out = rhsBuffer ? rhsBuffer : (rhsBuffer = allocate<double>(cfg.cells * nv));
-    hipLaunchKernelGGL(advanceKernel, dim3(gridFor(cfg.cells * nv, 256)), dim3(256), 0, stream, faces, work, rowDev,
-                       incidenceDev, invV, q, cfg.cells, all, ns, nv, 0.0, 2, out);
+    hipLaunchKernelGGL(advanceKernel, dim3(gridFor(cfg.cells * nv, 256)), dim3(256), 0, stream, faces, work, visc,
+                       rowDev, incidenceDev, invV, q, cfg.cells, all, ns, nv, 0.0, 2, out);
     RAMD_HIP(hipGetLastError());
     launched();
     if (scheme == kHllc) checkFluxFailures();
@@ -809,8 +1068,9 @@ int reactive_transport_stable_step(void* t, const double* q, const ReactiveTrans
 int reactive_transport_stable_step_primitives(void* t, const ReactiveTransportPrimitive* primitive,
                                               const ReactiveTransportState* state, double cfl, double maximum,
                                               double* dt) {
-    return protect(t, [&](Transport& x) {
+    return protect(t, [&](Transport& x) {  // reactiveTransport.cpp:1647-1666
         x.uploadPrimitives(primitive, state);
+        if (x.wale) x.buildWale();
         x.launchFaceSpeeds();
         x.stableStep(cfl, maximum, dt);
     });
@@ -1031,15 +1291,28 @@ int reactive_transport_geometric_diagnostic_v1(void* t, const void*, const doubl
                                                const double*, const void*, const void*, double*, double*, void*) {
     return stub(t, "reactive_transport_geometric_diagnostic_v1", "geometric capillary diagnostic");
 }
-int reactive_transport_set_wale_v1(void* t, const void*) {
-    return stub(t, "reactive_transport_set_wale_v1", "WALE LES");
+int reactive_transport_set_wale_v1(void* t, const ReactiveWaleOptionsV1* options) {
+    return protect(t, [&](Transport& x) {  // reactiveTransport.cpp:1668-1670
+        require(options, "Null WALE options");
+        x.installWale(*options);
+    });
 }
-int reactive_transport_wale_profile_v1(void* t, void*) {
-    return stub(t, "reactive_transport_wale_profile_v1", "WALE LES");
+int reactive_transport_wale_profile_v1(void* t, ReactiveWaleProfileV1* profile) {
+    return protect(t, [&](Transport& x) {  // reactiveTransport.cpp:1699-1702
+        require(profile && profile->abiVersion == 1 && profile->structBytes == sizeof(*profile),
+                "Invalid WALE profile ABI");
+        *profile = x.waleProfile;
+    });
 }
-int reactive_transport_wale_primitives_v1(void* t, const ReactiveTransportPrimitive*, const ReactiveTransportState*,
-                                          double*) {
-    return stub(t, "reactive_transport_wale_primitives_v1", "WALE LES");
+int reactive_transport_wale_primitives_v1(void* t, const ReactiveTransportPrimitive* primitive,
+                                          const ReactiveTransportState* state, double* nut) {
+    return protect(t, [&](Transport& x) {  // reactiveTransport.cpp:1703-1708
+        require(x.wale && nut, "WALE not selected or missing output");
+        x.uploadPrimitives(primitive, state);
+        x.buildWale();
+        x.downloadBytes(nut, x.nut, x.cfg.cells * sizeof(double));
+        x.finish();
+    });
 }
 int reactive_transport_set_wale_scalars_v1(void* t, const void*) {
     return stub(t, "reactive_transport_set_wale_scalars_v1", "WALE scalar closure");

@@ -12,12 +12,14 @@
 
 namespace ramd {
 
-// Compact face record (56 B): right = neighbour, the owner itself for
-// extrapolated boundaries, or cells + fixed for fixed-state boundaries.
+// Compact face record (64 B): right = neighbour, the owner itself for
+// extrapolated boundaries, or cells + fixed for fixed-state boundaries;
+// weight = ownerWeight, the owner's share of face interpolation.
 struct FaceDev {
     int32_t owner, right, kind, pad;
-    double n[3], area, distance;
+    double n[3], area, distance, weight;
 };
+static_assert(sizeof(FaceDev) == 64, "face record");
 
 // Device-side description of one structured box (see CartesianInfo).
 struct CartesianArgs {
@@ -86,6 +88,16 @@ public:
     double* density = nullptr;                     // SoA recovered density (HLLC on the Cartesian path)
     int scheme = 0;                                // face flux: 0 HLL, 1 HLLC
     unsigned long long* fluxFailures = nullptr;    // faces the HLLC flux rejected since the last check
+    // viscous stress, conduction and the WALE closure (generic face graphs only)
+    double* gradient = nullptr;     // cells x 9: gradient[9 c + 3 i + j] = d u_i / d x_j
+    double* nut = nullptr;          // WALE kinematic eddy viscosity per cell
+    double* filterWidth = nullptr;  // WALE filter width cbrt(cell volume) per cell
+    double4* visc = nullptr;        // per face: viscous traction (x, y, z), viscous + conductive energy flux (w)
+    unsigned* waleError = nullptr;  // set by a WALE viscosity or face that the closure rejects
+    bool wale = false;
+    double waleCw = 0;
+    bool stepAttached = false;      // a resident step was created on this transport
+    ReactiveWaleProfileV1 waleProfile{1, sizeof(ReactiveWaleProfileV1), 0, 0, 0, 0};
     CartesianInfo cartesian;
     // versions, tokens, attempts (reactiveTransport.cpp:384-393)
     std::string physicalHash;
@@ -120,6 +132,12 @@ public:
     void uploadQ(const double* hostQ);
     void uploadState(const ReactiveTransportState* states);
     void primitives();
+    // true where the face flux carries viscous, conductive or WALE terms
+    bool viscous() const { return cfg.viscosity > 0 || cfg.conductivity > 0 || wale; }
+    void installWale(const ReactiveWaleOptionsV1& options);
+    void launchGradients(bool withWale);
+    void buildWale();
+    void launchFaces();
     void prepareResident(const ReactiveTransportState* states, bool transport);
     void launchFaceSpeeds();
     void gather();

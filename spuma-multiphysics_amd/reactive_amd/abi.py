@@ -110,6 +110,28 @@ class Primitive(C.Structure):
     _fields_ = [("rho", C.c_double), ("u", C.c_double * 3)]
 
 
+PRIMITIVE_DTYPE = np.dtype([("rho", "<f8"), ("u", "<f8", (3,))])
+
+
+class WaleOptions(C.Structure):
+    """ReactiveWaleOptionsV1 (reactiveTurbulence.h:12-15)."""
+    _fields_ = [("abiVersion", C.c_uint32), ("structBytes", C.c_uint32), ("Cw", C.c_double)]
+
+    @classmethod
+    def make(cls, cw):
+        return cls(abiVersion=1, structBytes=C.sizeof(cls), Cw=cw)
+
+
+class WaleProfile(C.Structure):
+    """ReactiveWaleProfileV1 (reactiveTurbulence.h:16-19)."""
+    _fields_ = [("abiVersion", C.c_uint32), ("structBytes", C.c_uint32), ("gradientBuilds", C.c_uint64),
+                ("viscosityBuilds", C.c_uint64), ("cellsEvaluated", C.c_uint64), ("workspaceBytes", C.c_uint64)]
+
+    @classmethod
+    def fresh(cls):
+        return cls(abiVersion=1, structBytes=C.sizeof(cls))
+
+
 class Profile(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("conservedUploads", "conservedUploadBytes", "conservedDownloadBytes",
                                           "stateUploadBytes", "primitiveUploadBytes", "gasUploadBytes",
@@ -138,6 +160,7 @@ class ProfileV21(C.Structure):
 
 assert C.sizeof(Config) == 88 and C.sizeof(OptionsV2) == 88 and C.sizeof(OptionsV21) == 120
 assert C.sizeof(Token) == 24 and C.sizeof(TransportState) == 56 and C.sizeof(Primitive) == 32
+assert PRIMITIVE_DTYPE.itemsize == 32 and C.sizeof(WaleOptions) == 16 and C.sizeof(WaleProfile) == 40
 
 
 class StepReport(C.Structure):

@@ -41,6 +41,12 @@ def ref_lib():
     lib.reactive_transport_stage.argtypes = [V] * 5 + [C.c_double, C.c_int, V]
     lib.reactive_transport_stable_step.argtypes = [V, V, V, C.c_double, C.c_double, C.POINTER(C.c_double)]
     lib.reactive_transport_destroy.argtypes = [V]
+    lib.reactive_transport_error.restype = C.c_char_p
+    lib.reactive_transport_error.argtypes = [V]
+    lib.reactive_transport_set_wale_v1.argtypes = [V, V]
+    lib.reactive_transport_wale_primitives_v1.argtypes = [V, V, V, V]
+    lib.reactive_transport_stable_step_primitives.argtypes = [V, V, V, C.c_double, C.c_double,
+                                                             C.POINTER(C.c_double)]
     return lib
 
 
@@ -223,7 +229,69 @@ def base_fixtures(lib):
     print("binary success", int(success2.sum()), "/", len(e2))
 
 
-SETS = {"base": base_fixtures, "frozen": frozen_fixture, "prune": prune_fixture, "analytic": analytic_fixture}
+# Viscous, conductive and WALE transport: (viscosity, conductivity, WALE Cw or None)
+VISCOUS_MODELS = {"mu": (0.05, 0.0, None), "k": (0.0, 40.0, None), "mu_k": (0.05, 40.0, None),
+                  "wale": (0.0, 0.0, 0.325), "wale_mu_k": (2e-3, 15.0, 0.5)}
+
+
+def viscous_fixture(lib):
+    """The reference's serial operators with the viscous stress, the
+    conductive heat flux and the WALE closure (reactiveTransportKernels.h:
+    290-313, 315-342, 418-439): on the oblique graph with every face kind
+    (helpers.random_graph) and on a 6^3 box with flash-consistent states,
+    rhs, CFL step and both RK stages; with WALE also the eddy viscosities
+    and the CFL step from primitives."""
+    from helpers import primitives
+    faces6, volume6, q6, seeds6, _ = cases.pure_n2o_cube(6, seed=5)
+    mb = model.model_bytes(model.build_model())
+    st6, ok6, _ = flash_set(lib, mb, q6[:, :1], cases.internal_energy(q6, 1), seeds6)
+    st6[ok6 == 0] = st6[np.where(ok6 == 1)[0][0]]
+    meshes = {"graph": random_graph(193, 7), "cube6": (faces6, volume6, q6, cases.transport_states(st6), None, None)}
+    sets = {}
+    for mesh, (fc, vol, q, ts, fq, fst) in meshes.items():
+        for name, (mu, k, cw) in VISCOUS_MODELS.items():
+            if mesh == "cube6":  # liquid-like coefficients for the N2O box
+                mu, k = mu * 4e-3, k * 5e-3
+            cfg = config(len(q), len(fc), fixed=0 if fq is None else len(fq), viscosity=mu, conductivity=k)
+            err = C.create_string_buffer(1024)
+            h = lib.reactive_transport_create(0, C.byref(cfg), p(vol), p(fc), p(fq), p(fst), None, None, err, 1024)
+            assert h, err.value
+            out = {}
+            if cw is not None:
+                opt = abi.WaleOptions.make(cw)
+                assert lib.reactive_transport_set_wale_v1(h, C.byref(opt)) == 0, lib.reactive_transport_error(h)
+                prim = primitives(q, 1)
+                nut = np.zeros(len(q))
+                assert lib.reactive_transport_wale_primitives_v1(h, p(prim), p(ts), p(nut)) == 0, \
+                    lib.reactive_transport_error(h)
+                dtp = C.c_double()
+                assert lib.reactive_transport_stable_step_primitives(h, p(prim), p(ts), 0.4, 1.0, C.byref(dtp)) == 0
+                out.update(nut=nut, dt_primitives=dtp.value)
+            rhs = np.zeros_like(q)
+            b = np.zeros(5)
+            assert lib.reactive_transport_rhs(h, p(q), p(ts), None, None, p(rhs), p(b)) == 0, \
+                lib.reactive_transport_error(h)
+            dt = C.c_double()
+            assert lib.reactive_transport_stable_step(h, p(q), p(ts), 0.4, 1.0, C.byref(dt)) == 0
+            q1 = q.copy()
+            b0 = np.zeros(5)
+            assert lib.reactive_transport_stage(h, p(q1), p(ts), None, None, dt.value, 0, p(b0)) == 0
+            q2 = q1.copy()
+            b1 = np.zeros(5)
+            assert lib.reactive_transport_stage(h, p(q2), p(ts), None, None, dt.value, 1, p(b1)) == 0
+            lib.reactive_transport_destroy(h)
+            out.update(viscosity=mu, conductivity=k, cw=-1.0 if cw is None else cw, rhs=rhs, boundary=b,
+                       dt=dt.value, q1=q1, b0=b0, q2=q2, b1=b1)
+            sets[f"{mesh}_{name}"] = out
+        sets[mesh] = dict(faces=fc, volume=vol, q=q, states=ts, fixed_q=fq if fq is not None else np.zeros((0, 5)),
+                          fixed_states=fst if fst is not None else np.zeros(0, dtype=abi.TSTATE_DTYPE))
+    np.savez_compressed(os.path.join(HERE, "transport_viscous.npz"),
+                        **{f"{n}__{k}": v for n, d in sets.items() for k, v in d.items()})
+    print("viscous sets", sorted(sets))
+
+
+SETS = {"base": base_fixtures, "frozen": frozen_fixture, "prune": prune_fixture, "analytic": analytic_fixture,
+        "viscous": viscous_fixture}
 
 
 def main(names):

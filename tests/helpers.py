@@ -54,6 +54,12 @@ def oracle(twin=False):
         lib.oracle_nasa.argtypes = [V, C.c_int, C.c_int, C.c_double, V]
         lib.oracle_elementary.argtypes = [C.c_int, C.c_size_t, V, V]
         lib.oracle_transport_set_scheme.argtypes = [V, C.c_int]
+        lib.oracle_transport_set_wale.argtypes = [V, C.c_double]
+        lib.oracle_transport_gradient.argtypes = [V, V]
+        lib.oracle_wale_eddy_viscosity.argtypes = [V, C.c_double, C.c_double, V]
+        lib.oracle_transport_wale_primitives.argtypes = [V, V, V, V]
+        lib.oracle_transport_stable_step_primitives.argtypes = [V, V, V, C.c_double, C.c_double,
+                                                                C.POINTER(C.c_double)]
         lib.oracle_hllc_face.argtypes = [V, V, V, V]
         _ORACLE[twin] = lib
     return _ORACLE[twin]
@@ -108,13 +114,45 @@ class OracleTransport:
         rc = self.lib.oracle_transport_stable_step(self.h, p(q), p(states), cfl, maximum, C.byref(dt))
         return rc, dt.value
 
+    def set_wale(self, cw):
+        assert self.lib.oracle_transport_set_wale(self.h, cw) == 0, self.lib.oracle_transport_error(self.h)
+
+    def wale_primitives(self, prim, states):
+        nut = np.zeros(self.nc)
+        rc = self.lib.oracle_transport_wale_primitives(self.h, p(prim), p(states), p(nut))
+        assert rc == 0, self.lib.oracle_transport_error(self.h)
+        return nut
+
+    def gradient(self):
+        g = np.zeros((self.nc, 9))
+        assert self.lib.oracle_transport_gradient(self.h, p(g)) == 0, self.lib.oracle_transport_error(self.h)
+        return g
+
+    def stable_step_primitives(self, prim, states, cfl, maximum):
+        dt = C.c_double()
+        rc = self.lib.oracle_transport_stable_step_primitives(self.h, p(prim), p(states), cfl, maximum, C.byref(dt))
+        return rc, dt.value
+
     def close(self):
         self.lib.oracle_transport_destroy(self.h)
 
 
-def config(nc, nf, ns=1, nv=None, fixed=0, wave=1.1, max_bytes=64e9):
-    return abi.Config(cells=nc, species=ns, variables=nv or ns + 4, faces=nf, fixed=fixed, viscosity=0,
-                      conductivity=0, diffusivity=0, waveFactor=wave, maxBytes=max_bytes, mechanical=0)
+def config(nc, nf, ns=1, nv=None, fixed=0, wave=1.1, max_bytes=64e9, viscosity=0.0, conductivity=0.0):
+    return abi.Config(cells=nc, species=ns, variables=nv or ns + 4, faces=nf, fixed=fixed, viscosity=viscosity,
+                      conductivity=conductivity, diffusivity=0, waveFactor=wave, maxBytes=max_bytes, mechanical=0)
+
+
+def primitives(q, ns):
+    """ReactiveTransportPrimitive records of cell-major q (Cells,
+    reactiveTransportKernels.h:193-199): rho = sum of the species masses,
+    u = momentum / rho."""
+    out = np.zeros(len(q), dtype=abi.PRIMITIVE_DTYPE)
+    rho = np.zeros(len(q))
+    for k in range(ns):
+        rho = rho + q[:, k]
+    out["rho"] = rho
+    out["u"] = q[:, ns:ns + 3] / rho[:, None]
+    return out
 
 
 def binary_case(n, seed=1, threads=8):

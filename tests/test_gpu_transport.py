@@ -203,6 +203,8 @@ def test_error_contracts():
         qn[position, 1] = np.nan
         assert lib.reactive_transport_stable_step(h, rlib.ptr(qn), rlib.ptr(st), 0.4, 1.0, C.byref(dt)) == 1
     assert lib.reactive_transport_set_wale_v1(h, None) == 1
+    assert b"Null WALE options" in lib.reactive_transport_error(h)
+    assert lib.reactive_transport_set_wale_scalars_v1(h, None) == 1
     assert b"not provided" in lib.reactive_transport_error(h)
     assert lib.reactive_transport_destroy(h) is None
 
