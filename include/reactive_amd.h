@@ -64,6 +64,17 @@ typedef struct ReactiveAmdStepReport {
  * FP64 contradicts are searched again in FP64. */
 #define REACTIVE_AMD_FLASH_FP64 0
 #define REACTIVE_AMD_FLASH_MIXED 1
+
+/* Flash arithmetic of a HEM handle's host batches (reactive_gpu_hem_run_v1,
+ * equilibrium flash), REACTIVE_AMD_FLASH_FP64 or _MIXED; a new handle takes
+ * the process default (REACTIVE_AMD_HEM_ARITHMETIC, see
+ * reactive_gpu_hem_numerical_policy_v1).  No reference counterpart: the
+ * reference selects its FP32 policies by building another library.  Returns
+ * 0, 1 for an unknown mode, -1 for a null handle. */
+int reactive_amd_hem_set_arithmetic(void* hem, int mode);
+/* The last host batch's FP32 phase evaluations and deferred cells (zero after
+ * an FP64 batch).  The profile's phaseEvaluations counts both precisions. */
+int reactive_amd_hem_mixed_report(void* hem, uint64_t* singlePhaseEvaluations, uint64_t* queuedCells);
 /* OR-ed into flashMode: replay each step from a captured HIP graph (after a
  * first step on the stream that records the flash scheduling costs).  The
  * report then holds totals and the call's device time only, no per-kernel

@@ -64,8 +64,15 @@ typedef struct ReactiveGpuHemOutputV1 {
     int success, reserved;
 } ReactiveGpuHemOutputV1;
 
-/* reactiveGpuHem.h:40.  Returns "" for the plain FP64 build (separate
- * rounding, no contraction), which the reference backend accepts as legacy. */
+/* reactiveGpuHem.h:40.  Returns "" for plain FP64 (separate rounding, no
+ * contraction), which the reference backend accepts as legacy, or, when the
+ * process was started with REACTIVE_AMD_HEM_ARITHMETIC=mixed,
+ * REACTIVE_GPU_HEM_FP32_COMBINED_POLICY_V1 ("fp32-cubic-newton-fp64-accept-v2:
+ * experimental", reactiveGpuHem.h:37): handles created in such a process run
+ * the equilibrium flash of reactive_gpu_hem_run_v1 in the mixed arithmetic of
+ * REACTIVE_AMD_FLASH_MIXED (include/reactive_amd.h).  The variable is read
+ * once per process.  Frozen-inventory batches (run_v2, run_resident_v1) run
+ * in FP64 either way. */
 const char* reactive_gpu_hem_numerical_policy_v1(void);
 /* reactiveGpuHem.h:41 */
 void* reactive_gpu_hem_create_v1(const void* model, size_t modelBytes, size_t capacity, char* error, size_t errorSize);

@@ -61,6 +61,19 @@ __device__ inline void loadCell(F& f, const double* __restrict__ q, size_t strid
     f.guessLiquid = R(g.liquidMass[0]);
 }
 
+// Loads cell c of a HEM batch into a flash: species masses q[c * NS + k], the
+// given internal energy and the seed.
+template <int NS, class F>
+__device__ inline void loadHemCell(F& f, const double* __restrict__ q, const double* __restrict__ energy,
+                                   const ThermoOut& g, uint32_t c) {
+    using R = decltype(f.energy);
+    for (int k = 0; k < NS; ++k) f.q[k] = R(q[size_t(c) * NS + k]);
+    f.energy = R(energy[c]);
+    f.guessP = R(g.p);
+    f.guessT = R(g.T);
+    f.guessLiquid = R(g.liquidMass[0]);
+}
+
 // Commits a flash result, or keeps the step-start state of a failed cell
 // (the reference commits nothing for it); p, c_frozen and rho go to the SoA
 // arrays the next transport stage reads.
@@ -96,6 +109,10 @@ __device__ inline void countFlash(uint32_t failed, uint32_t phases, unsigned lon
 void launchSearch(int ns, int nl, const void* model, const double* q, size_t stride, const ThermoOut* seeds,
                   const uint32_t* order, uint32_t count, const double* dtDev, CandidateSet* sets, uint16_t* cost,
                   unsigned long long* phases, hipStream_t stream, unsigned block, const uint32_t* rep);
+// The same search over a HEM batch (loadHemCell's layout), 64-lane workgroups.
+void launchHemSearch(int ns, int nl, const void* model, const double* q, const double* energy,
+                     const ThermoOut* guess, const uint32_t* order, uint32_t count, CandidateSet* sets,
+                     unsigned long long* phases, hipStream_t stream);
 
 }  // namespace ramd
 #endif

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -24,6 +25,7 @@
 #include "hem_internal.h"
 #include "flash_step.h"
 #include "reactive_gpu_hem.h"
+#include "reactive_amd.h"
 
 namespace ramd {
 
@@ -34,6 +36,34 @@ static_assert(sizeof(HemCapillaryDev) == sizeof(ReactiveGpuHemCapillaryInputV2) 
 static_assert(sizeof(HemOutputDev) == sizeof(ReactiveGpuHemOutputV1) &&
                   offsetof(HemOutputDev, success) == offsetof(ReactiveGpuHemOutputV1, success),
               "output layout");
+
+// Adds one wavefront's work counters (phases, residuals, candidates, stable,
+// failures, fd Jacobians, successes, analytic Jacobians) to totals: one
+// 64-bit atomic per counter per wavefront.
+__device__ inline void addTotals(const Counts& n, uint32_t succeeded, unsigned long long* __restrict__ totals) {
+    const uint32_t s0 = waveSum(n.phases), s1 = waveSum(n.residuals), s2 = waveSum(n.candidates);
+    const uint32_t s3 = waveSum(n.stable), s4 = waveSum(n.failures), s5 = waveSum(n.fd);
+    const uint32_t s6 = waveSum(succeeded), s7 = waveSum(n.analytic);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&totals[0], (unsigned long long)s0);
+        atomicAdd(&totals[1], (unsigned long long)s1);
+        atomicAdd(&totals[2], (unsigned long long)s2);
+        atomicAdd(&totals[3], (unsigned long long)s3);
+        atomicAdd(&totals[4], (unsigned long long)s4);
+        atomicAdd(&totals[5], (unsigned long long)s5);
+        atomicAdd(&totals[6], (unsigned long long)s6);
+        atomicAdd(&totals[7], (unsigned long long)s7);
+    }
+}
+
+// A flash result into the batch's output record (zero state where it failed).
+template <class E>
+__device__ inline void storeHemCell(bool ok, const E& result, HemOutputDev& o) {
+    if (ok) writeOut(result, o.state);
+    else std::memset(&o.state, 0, sizeof(o.state));
+    o.success = ok ? 1 : 0;
+    o.reserved = 0;
+}
 
 // Without capillary input: equilibrium UV flash (recover,
 // reactiveDeviceFlash.h:564-567).  With it (validated to color = jump = 0,
@@ -69,11 +99,7 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void hemFlashKernel(const
         bool ok;
         if (capillary) ok = f.frozen(NL ? g.liquidMass[0] : 0.0, result, 0, g.liquidMass[1]);
         else ok = f.run(result);
-        HemOutputDev& o = out[c];
-        if (ok) writeOut(result, o.state);
-        else std::memset(&o.state, 0, sizeof(o.state));
-        o.success = ok ? 1 : 0;
-        o.reserved = 0;
+        storeHemCell(ok, result, out[c]);
         succeeded = ok ? 1u : 0u;
         if (cellPhases) cellPhases[c] = n.phases;
     }
@@ -87,20 +113,74 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void hemFlashKernel(const
             slot[4] = s4; slot[5] = s5; slot[6] = s6; slot[7] = s7;
         }
     } else if (totals) {
-        const uint32_t s0 = waveSum(n.phases), s1 = waveSum(n.residuals), s2 = waveSum(n.candidates);
-        const uint32_t s3 = waveSum(n.stable), s4 = waveSum(n.failures), s5 = waveSum(n.fd);
-        const uint32_t s6 = waveSum(succeeded), s7 = waveSum(n.analytic);
-        if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&totals[0], (unsigned long long)s0);
-            atomicAdd(&totals[1], (unsigned long long)s1);
-            atomicAdd(&totals[2], (unsigned long long)s2);
-            atomicAdd(&totals[3], (unsigned long long)s3);
-            atomicAdd(&totals[4], (unsigned long long)s4);
-            atomicAdd(&totals[5], (unsigned long long)s5);
-            atomicAdd(&totals[6], (unsigned long long)s6);
-            atomicAdd(&totals[7], (unsigned long long)s7);
+        addTotals(n, succeeded, totals);
+    }
+}
+
+// Mixed arithmetic, second half (after hemSearchKernel): FP64 refinement,
+// stability and selection of each cell's FP32 candidates; cells the
+// refinement defers (a full FP64 search or FP64 restarts) go to the queue.
+template <int NS, int NL>
+__global__ __launch_bounds__(64, RAMD_REFINE_MIN_WAVES) void hemRefineKernel(const HemModel<NS>* __restrict__ globalModel,
+                                                     const double* __restrict__ q, const double* __restrict__ energy,
+                                                     const ThermoOut* __restrict__ guess,
+                                                     const uint32_t* __restrict__ order, uint32_t count,
+                                                     const CandidateSet* __restrict__ sets,
+                                                     HemOutputDev* __restrict__ out,
+                                                     unsigned long long* __restrict__ totals,
+                                                     uint32_t* __restrict__ queue, unsigned* __restrict__ queueCount) {
+    const HemModel<NS>& model = workgroupModel(globalModel);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    Counts n = {0, 0, 0, 0, 0, 0, 0};
+    uint32_t succeeded = 0;
+    if (i < count) {
+        const uint32_t c = order[i];
+        Flash<NS, NL, double, true> f(model, n);
+        loadHemCell<NS>(f, q, energy, guess[c], c);
+        EvalT<double, true> result;
+        zeroEval(result);
+        bool deferred = false;
+        const bool ok = f.refine(sets[c], result, &deferred);
+        if (deferred) {
+            queue[atomicAdd(queueCount, 1u)] = c;
+        } else {
+            storeHemCell(ok, result, out[c]);
+            succeeded = ok ? 1u : 0u;
         }
     }
+    addTotals(n, succeeded, totals);
+}
+
+// The deferred cells, refined without deferral by a persistent grid that
+// sweeps the queue (the count is read on the device).
+template <int NS, int NL>
+__global__ __launch_bounds__(64, RAMD_REFINE_MIN_WAVES) void hemRefineQueueKernel(
+    const HemModel<NS>* __restrict__ globalModel, const double* __restrict__ q, const double* __restrict__ energy,
+    const ThermoOut* __restrict__ guess, const uint32_t* __restrict__ queue, const unsigned* __restrict__ queueCount,
+    const CandidateSet* __restrict__ sets, HemOutputDev* __restrict__ out, unsigned long long* __restrict__ totals) {
+    const unsigned count = *queueCount;
+    if (blockIdx.x * 64u >= count) return;  // uniform per workgroup
+    const HemModel<NS>& model = workgroupModel(globalModel);
+    for (unsigned base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u) {  // uniform per workgroup
+        const unsigned i = base + threadIdx.x;
+        Counts n = {0, 0, 0, 0, 0, 0, 0};
+        uint32_t succeeded = 0;
+        if (i < count) {
+            const uint32_t c = queue[i];
+            Flash<NS, NL, double, true> f(model, n);
+            loadHemCell<NS>(f, q, energy, guess[c], c);
+            EvalT<double, true> result;
+            zeroEval(result);
+            const bool ok = f.refine(sets[c], result);
+            storeHemCell(ok, result, out[c]);
+            succeeded = ok ? 1u : 0u;
+        }
+        addTotals(n, succeeded, totals);
+    }
+}
+
+__global__ void clearCountKernel(unsigned* __restrict__ count) {
+    if (threadIdx.x == 0) *count = 0;
 }
 
 // Counts the ordered cells whose capillary input this build cannot run
@@ -141,6 +221,12 @@ __global__ __launch_bounds__(64) void reducePartialsKernel(const unsigned long l
 __global__ void zeroTotalsKernel(unsigned long long* totals, int n) {
     if (threadIdx.x < n) totals[threadIdx.x] = 0;
 }
+__global__ void queueCountKernel(const unsigned* __restrict__ count, unsigned long long* __restrict__ total) {
+    if (threadIdx.x == 0) *total = *count;
+}
+
+// Workgroups of the persistent deferred-refinement grid (64 lanes each).
+constexpr unsigned kQueueBlocks = 1024;
 
 // Scheduling key (reactiveHemBucket.h:13-18).
 inline unsigned bucketKey(const double* q, int ns, int activeLiquids) {
@@ -202,6 +288,44 @@ void HemHandle::launch(const double* q, const double* e, const ThermoOut* g, con
     RAMD_HIP(hipGetLastError());
 }
 
+void HemHandle::launchMixed(const double* q, const double* e, const ThermoOut* g, const uint32_t* order,
+                            size_t count, HemOutputDev* out, hipStream_t s) {
+    if (!count) return;
+    if (!sets) {
+        sets = static_cast<CandidateSet*>(allocate(capacity * sizeof(CandidateSet)));
+        queue = static_cast<uint32_t*>(allocate(capacity * sizeof(uint32_t)));
+        queueCount = static_cast<unsigned*>(allocate(sizeof(unsigned)));
+    }
+    const unsigned blocks = unsigned((count + 63) / 64);
+    hipLaunchKernelGGL(zeroTotalsKernel, dim3(1), dim3(64), 0, s, totals, 10);
+    launchHemSearch(ns, nl, model, q, e, g, order, uint32_t(count), sets, totals + 8, s);
+    hipLaunchKernelGGL(clearCountKernel, dim3(1), dim3(64), 0, s, queueCount);
+#define RAMD_HEM_MIXED(NS_, NL_)                                                                                  \
+    hipLaunchKernelGGL((hemRefineKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, s,                                   \
+                       static_cast<const HemModel<NS_>*>(model), q, e, g, order, uint32_t(count), sets, out, totals, \
+                       queue, queueCount);                                                                          \
+    hipLaunchKernelGGL((hemRefineQueueKernel<NS_, NL_>), dim3(kQueueBlocks), dim3(64), 0, s,                        \
+                       static_cast<const HemModel<NS_>*>(model), q, e, g, queue, queueCount, sets, out, totals)
+    if (ns == 1 && nl == 0) { RAMD_HEM_MIXED(1, 0); }
+    else if (ns == 1) { RAMD_HEM_MIXED(1, 1); }
+    else if (nl == 0) { RAMD_HEM_MIXED(2, 0); }
+    else { RAMD_HEM_MIXED(2, 1); }
+#undef RAMD_HEM_MIXED
+    hipLaunchKernelGGL(queueCountKernel, dim3(1), dim3(64), 0, s, queueCount, totals + 9);
+    RAMD_HIP(hipGetLastError());
+}
+
+// The process's default flash arithmetic: REACTIVE_AMD_HEM_ARITHMETIC=mixed
+// selects the mixed flash, anything else (or nothing) plain FP64.  Read once,
+// so the numerical policy a driver hashes and the handles it creates agree.
+static int defaultArithmetic() {
+    static const int mode = [] {
+        const char* v = std::getenv("REACTIVE_AMD_HEM_ARITHMETIC");
+        return v && std::strcmp(v, "mixed") == 0 ? REACTIVE_AMD_FLASH_MIXED : REACTIVE_AMD_FLASH_FP64;
+    }();
+    return mode;
+}
+
 HemHandle* createHem(const void* raw, size_t bytes, size_t capacity, bool hostPath) {
     using namespace image;
     require(raw && bytes == sizeof(Model) && capacity && capacity <= UINT32_MAX, "Invalid HIP HEM model/capacity");
@@ -219,6 +343,7 @@ HemHandle* createHem(const void* raw, size_t bytes, size_t capacity, bool hostPa
     h->ns = m.ns;
     h->nl = m.nl;
     h->analytic = m.analyticJacobian != 0;
+    h->arithmetic = defaultArithmetic();
     h->capacity = capacity;
     h->device = currentDevice();
     RAMD_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -234,7 +359,7 @@ HemHandle* createHem(const void* raw, size_t bytes, size_t capacity, bool hostPa
         h->model = h->allocate(sizeof(d));
         RAMD_HIP(hipMemcpy(h->model, &d, sizeof(d), hipMemcpyHostToDevice));
     }
-    h->totals = static_cast<unsigned long long*>(h->allocate(8 * sizeof(unsigned long long)));
+    h->totals = static_cast<unsigned long long*>(h->allocate(10 * sizeof(unsigned long long)));
     h->modelBytes = h->deviceBytes;
     if (hostPath) h->ensureHostPath();
     return h.release();
@@ -284,10 +409,12 @@ int runHost(HemHandle* h, const double* q, const double* energy, const ReactiveG
     if (capillary)
         RAMD_HIP(hipMemcpyAsync(h->dcap, capillary, count * sizeof(HemCapillaryDev), hipMemcpyHostToDevice, s));
     RAMD_HIP(hipEventRecord(h->events[1], s));
-    h->launch(h->dq, h->de, h->dguess, h->dorder, count, capillary ? h->dcap : nullptr, h->dout, nullptr, s);
+    const bool mixed = !capillary && h->arithmetic == REACTIVE_AMD_FLASH_MIXED;
+    if (mixed) h->launchMixed(h->dq, h->de, h->dguess, h->dorder, count, h->dout, s);
+    else h->launch(h->dq, h->de, h->dguess, h->dorder, count, capillary ? h->dcap : nullptr, h->dout, nullptr, s);
     RAMD_HIP(hipEventRecord(h->events[2], s));
     RAMD_HIP(hipMemcpyAsync(h->hostOut.data(), h->dout, count * sizeof(HemOutputDev), hipMemcpyDeviceToHost, s));
-    unsigned long long totals[8] = {};
+    unsigned long long totals[10] = {};
     RAMD_HIP(hipMemcpyAsync(totals, h->totals, sizeof(totals), hipMemcpyDeviceToHost, s));
     RAMD_HIP(hipEventRecord(h->events[3], s));
     RAMD_HIP(hipEventSynchronize(h->events[3]));
@@ -309,7 +436,10 @@ int runHost(HemHandle* h, const double* q, const double* energy, const ReactiveG
         }
     }
     p.deviceFailures = count - p.succeeded;
-    p.phaseEvaluations = totals[0];
+    // the mixed arithmetic counts its FP32 search's phase evaluations too
+    p.phaseEvaluations = totals[0] + (mixed ? totals[8] : 0);
+    h->lastSinglePhases = mixed ? totals[8] : 0;
+    h->lastQueued = mixed ? totals[9] : 0;
     p.residualEvaluations = totals[1];
     p.flashCandidates = totals[2];
     p.stableCandidates = totals[3];
@@ -359,7 +489,28 @@ using namespace ramd;
 
 extern "C" {
 
-const char* reactive_gpu_hem_numerical_policy_v1(void) { return ""; }
+// "" (plain FP64, separate rounding) or, with the mixed arithmetic as the
+// process default, the FP32 combined policy (reactiveGpuHem.h:37); both are
+// strings reactiveThermo.cpp:294-301 accepts.
+const char* reactive_gpu_hem_numerical_policy_v1(void) {
+    return defaultArithmetic() == REACTIVE_AMD_FLASH_MIXED ? "fp32-cubic-newton-fp64-accept-v2:experimental" : "";
+}
+
+int reactive_amd_hem_set_arithmetic(void* raw, int mode) {
+    if (!raw) return -1;
+    if (mode != REACTIVE_AMD_FLASH_FP64 && mode != REACTIVE_AMD_FLASH_MIXED) return 1;
+    static_cast<HemHandle*>(raw)->arithmetic = mode;
+    return 0;
+}
+
+int reactive_amd_hem_mixed_report(void* raw, uint64_t* singlePhaseEvaluations, uint64_t* queuedCells) {
+    if (!raw) return -1;
+    if (!singlePhaseEvaluations || !queuedCells) return 1;
+    const HemHandle* h = static_cast<const HemHandle*>(raw);
+    *singlePhaseEvaluations = h->lastSinglePhases;
+    *queuedCells = h->lastQueued;
+    return 0;
+}
 
 void* reactive_gpu_hem_create_v1(const void* model, size_t bytes, size_t capacity, char* error, size_t size) {
     try {

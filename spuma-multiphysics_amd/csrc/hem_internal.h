@@ -27,9 +27,19 @@ struct HemCapillaryDev {
 struct HemHandle {
     int ns = 0, nl = 0, device = 0;
     bool analytic = false;  // the model's analytic-Jacobian mode (FP64 flash)
+    int arithmetic = 0;     // equilibrium flash of host batches: REACTIVE_AMD_FLASH_FP64 or _MIXED
     size_t capacity = 0, deviceBytes = 0, modelBytes = 0;
     void* model = nullptr;  // HemModel<ns> in device memory
-    unsigned long long* totals = nullptr;  // phases residuals candidates stable failures fd successes analytic
+    // phases residuals candidates stable failures fd successes analytic, then
+    // (mixed arithmetic) FP32 phases and deferred cells
+    unsigned long long* totals = nullptr;
+    // mixed arithmetic: candidate sets and the deferred-refinement queue
+    // (allocated on the first mixed batch), the last batch's FP32 phase
+    // evaluations and deferred cells
+    CandidateSet* sets = nullptr;
+    uint32_t* queue = nullptr;
+    unsigned* queueCount = nullptr;
+    uint64_t lastSinglePhases = 0, lastQueued = 0;
     // host-path staging (allocated on the first host batch)
     bool hostReady = false;
     double *dq = nullptr, *de = nullptr;
@@ -57,6 +67,11 @@ struct HemHandle {
     void launch(const double* q, const double* e, const ThermoOut* guess, const uint32_t* order, size_t count,
                 const HemCapillaryDev* capillary, HemOutputDev* out, uint32_t* cellPhases, hipStream_t s,
                 unsigned long long* partials = nullptr);
+    // The equilibrium flash of the same cells in mixed arithmetic: FP32
+    // candidate search, FP64 refinement and the deferred queue; totals as
+    // launch's plus totals[8] FP32 phases and totals[9] deferred cells.
+    void launchMixed(const double* q, const double* e, const ThermoOut* guess, const uint32_t* order, size_t count,
+                     HemOutputDev* out, hipStream_t s);
 };
 
 HemHandle* createHem(const void* raw, size_t bytes, size_t capacity, bool hostPath);

@@ -43,6 +43,47 @@ __global__ __launch_bounds__(256, RAMD_SEARCH_MIN_WAVES) void stepSearchKernel(c
     if ((threadIdx.x & 63) == 0) atomicAdd(phases, (unsigned long long)sum);
 }
 
+// The same search for a HEM batch (reactive_gpu_hem_run_v1 with the mixed
+// arithmetic): species masses q[c * NS + k], internal energy energy[c] and the
+// seed guess[c] by cell, cells in `order`.
+template <int NS, int NL>
+__global__ __launch_bounds__(256, RAMD_SEARCH_MIN_WAVES) void hemSearchKernel(const HemModel<NS>* __restrict__ globalModel,
+                                                      const double* __restrict__ q,
+                                                      const double* __restrict__ energy,
+                                                      const ThermoOut* __restrict__ guess,
+                                                      const uint32_t* __restrict__ order, uint32_t count,
+                                                      CandidateSet* __restrict__ sets,
+                                                      unsigned long long* __restrict__ phases) {
+    const HemModel<NS>& model = workgroupModel(globalModel);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    Counts n = {0, 0, 0, 0, 0, 0, 0};
+    if (i < count) {
+        const uint32_t c = order[i];
+        Flash<NS, NL, float> f(model, n);
+        loadHemCell<NS>(f, q, energy, guess[c], c);
+        CandidateSet set;
+        f.searchRecord(set);
+        sets[c] = set;
+    }
+    const uint32_t sum = waveSum(n.phases);
+    if ((threadIdx.x & 63) == 0) atomicAdd(phases, (unsigned long long)sum);
+}
+
+void launchHemSearch(int ns, int nl, const void* model, const double* q, const double* energy,
+                     const ThermoOut* guess, const uint32_t* order, uint32_t count, CandidateSet* sets,
+                     unsigned long long* phases, hipStream_t stream) {
+    const unsigned blocks = unsigned((size_t(count) + 63) / 64);
+#define RAMD_HEM_SEARCH(NS_, NL_)                                                                                 \
+    hipLaunchKernelGGL((hemSearchKernel<NS_, NL_>), dim3(blocks), dim3(64), 0, stream,                             \
+                       static_cast<const HemModel<NS_>*>(model), q, energy, guess, order, count, sets, phases)
+    if (ns == 1 && nl == 0) { RAMD_HEM_SEARCH(1, 0); }
+    else if (ns == 1) { RAMD_HEM_SEARCH(1, 1); }
+    else if (nl == 0) { RAMD_HEM_SEARCH(2, 0); }
+    else { RAMD_HEM_SEARCH(2, 1); }
+#undef RAMD_HEM_SEARCH
+    RAMD_HIP(hipGetLastError());
+}
+
 void launchSearch(int ns, int nl, const void* model, const double* q, size_t stride, const ThermoOut* seeds,
                   const uint32_t* order, uint32_t count, const double* dtDev, CandidateSet* sets, uint16_t* cost,
                   unsigned long long* phases, hipStream_t stream, unsigned block, const uint32_t* rep) {

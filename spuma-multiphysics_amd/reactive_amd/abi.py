@@ -54,6 +54,11 @@ class HemOutput(C.Structure):
 
 assert C.sizeof(CapillaryInput) == 24 and C.sizeof(HemOutput) == 184
 HEM_OUTPUT_DTYPE = np.dtype([("state", STATE_DTYPE), ("success", "<i4"), ("reserved", "<i4")])
+CAPILLARY_DTYPE = np.dtype([("color", "<f8"), ("pressureJump", "<f8"), ("equilibrium", "<i4"), ("pad", "<i4")])
+assert CAPILLARY_DTYPE.itemsize == 24
+
+# flash arithmetic (REACTIVE_AMD_FLASH_FP64 / _MIXED, include/reactive_amd.h)
+FP64, MIXED = 0, 1
 
 
 class Face(C.Structure):

@@ -52,6 +52,8 @@ _SIGNATURES = {
     "reactive_gpu_hem_run_resident_noalloc_v1": (C.c_int, [P, P, P, P, P, P, C.c_size_t, P, C.POINTER(HemProfile),
                                                            C.c_char_p, C.c_size_t]),
     "reactive_amd_transport_set_flux": (C.c_int, [P, C.c_int]),
+    "reactive_amd_hem_set_arithmetic": (C.c_int, [P, C.c_int]),
+    "reactive_amd_hem_mixed_report": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "reactive_amd_step_create": (P, [P, P, C.c_char_p, C.c_size_t]),
     "reactive_amd_step_load": (C.c_int, [P, P, P]),
     "reactive_amd_step_run": (C.c_int, [P, C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(StepReport)]),
