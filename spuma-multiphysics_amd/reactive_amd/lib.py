@@ -67,8 +67,29 @@ _SIGNATURES = {
     "reactive_amd_step_destroy": (None, [P]),
 }
 
-# exported stubs: every argument is passed by address or value; all take the handle first
-for _name, _n in (("reactive_transport_set_gas_thermo", 7), ("reactive_transport_stage_resident_gas", 9),
+# exported stubs with scalar arguments
+_SIGNATURES.update({
+    "reactive_transport_stage_resident_gas": (C.c_int, [P, P, P, C.c_double, C.c_int, C.c_uint64, C.c_uint64, P, P]),
+    "reactive_transport_replace_stage_state_v1": (C.c_int, [P, Token, C.c_uint64, P]),
+    "reactive_transport_capillary_resident_begin_v1": (C.c_int, [P, C.c_size_t, P, P, P, P, P, C.c_double, C.c_int,
+                                                                 P]),
+    "reactive_transport_capillary_resident_select_v1": (C.c_int, [P, C.c_int, P, P]),
+    "reactive_transport_capillary_resident_update_v1": (C.c_int, [P, C.c_size_t, P, P]),
+    "reactive_transport_capillary_resident_commit_v1": (C.c_int, [P, P, P, P, P]),
+    "reactive_transport_implicit_coefficients_v1": (C.c_int, [P, P, P, C.c_size_t, P]),
+    "reactive_transport_geometric_diagnostic_v1": (C.c_int, [P] * 10),
+    "reactive_transport_wale_scalar_fields_v1": (C.c_int, [P] * 5),
+    "reactive_transport_wale_pr_properties_v1": (C.c_int, [P, P, P, C.c_size_t, P]),
+    "reactive_transport_wale_pr_properties_v2": (C.c_int, [P, P, P, C.c_size_t, P, C.c_int]),
+    "reactive_closure_scalar_create_v1": (P, [C.c_size_t, C.c_char_p, C.c_size_t]),
+    "reactive_closure_scalar_destroy_v1": (None, [P]),
+    "reactive_closure_scalar_run_v1": (C.c_int, [P, P, C.c_size_t, P, P, C.c_char_p, C.c_size_t]),
+    "reactive_gpu_hem_initialize_tp_v1": (C.c_int, [P, P, P, P, C.c_size_t, P, P, C.POINTER(HemProfile), C.c_char_p,
+                                                    C.c_size_t]),
+})
+
+# exported stubs whose arguments are all addresses; all take the handle first
+for _name, _n in (("reactive_transport_set_gas_thermo", 7),
                   ("reactive_transport_rhs_gas", 6), ("reactive_transport_gas_properties_resident", 5),
                   ("reactive_transport_set_capillary_v1", 2), ("reactive_transport_capillary_geometry_v1", 6),
                   ("reactive_transport_capillary_profile_v1", 2),

@@ -60,3 +60,13 @@ def test_step_report_layout_matches_the_header(tmp_path):
     values = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     assert values[0] == C.sizeof(abi.StepReport)
     assert values[1:] == [getattr(abi.StepReport, f).offset for f in fields]
+
+
+def test_every_declared_entry_point_has_a_ctypes_signature():
+    """A ctypes call without argtypes passes a 64-bit handle as a C int and
+    truncates it; the binding table must cover every declared entry point."""
+    missing = sorted(declared() - set(rlib._SIGNATURES))
+    assert not missing, missing
+    lib = rlib.load()
+    for name in declared():
+        assert getattr(lib, name).argtypes is not None, name
