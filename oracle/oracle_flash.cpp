@@ -5,10 +5,13 @@
 // grouping follows the reference so that, compiled with -ffp-contract=off
 // against glibc, results equal the compiled reference bit for bit.
 //
-// Scope: pure-liquid condensables (condensedKind 0), no capillary interface
-// (color = pressureJump = 0), boundaryRecovery = 0; the finite-difference
-// Jacobian or, with analyticJacobian = 1, the analytic one for one active
-// liquid (reactiveDeviceFlashJacobian.h).  Other model images are rejected by
+// Scope: pure-liquid condensables (condensedKind 0); a capillary interface
+// (color, pressureJump: the gas at p - color J, the liquids at
+// p + (1 - color) J) and the bounded vapour-log ("adaptive") boundary search
+// of boundaryRecovery = 1 models and capillary equilibrium cells
+// (reactiveDeviceFlash.h:327-448,544-590); the finite-difference Jacobian or,
+// with analyticJacobian = 1, the analytic one for one active liquid
+// (reactiveDeviceFlashJacobian.h).  Other model images are rejected by
 // oracle_model_supported().
 #include "oracle.h"
 #include <cfloat>
@@ -32,12 +35,14 @@ inline double oexp(double x) { return ramd::elem::exp(x); }
 inline double othird(double x) { return ramd::elem::powThird(x); }
 inline double oacos(double x) { return ramd::elem::acos(x); }
 inline double ocos(double x) { return ramd::elem::cos(x); }
+inline double oexpm1(double x) { return ramd::elem::expm1(x); }
 #else
 inline double olog(double x) { return std::log(x); }
 inline double oexp(double x) { return std::exp(x); }
 inline double othird(double x) { return std::pow(x, 1.0 / 3.0); }
 inline double oacos(double x) { return std::acos(x); }
 inline double ocos(double x) { return std::cos(x); }
+inline double oexpm1(double x) { return std::expm1(x); }
 #endif
 
 constexpr double R = 8314.46261815324;            // reactiveDevicePR.h:31
@@ -631,6 +636,7 @@ struct Evaluation {  // :64-73
     ThermoState state;
     double volume, energy, entropyDensity, Vp, VT, Ep, ET, cpDensity;
     double muGas[2], muLiquid[2], gasCondensable[2];
+    bool explicitGas;  // the gas inventory was given (vapour coordinates), not q - liquid
     double liquidHbar[2], liquidVbar[2];
     unsigned liquidMask;
     double gasMolarVolume;
@@ -697,9 +703,14 @@ struct Flash {
     double cacheP = 0, cacheT = 0;  // one unavailable liquid-0 result (:142-145)
     int cacheStatus = -1;
     bool cacheVirtual = false;
+    bool adaptive = false;               // vapour-log coordinates of the boundary search (:145)
+    double color = 0, pressureJump = 0;  // capillary interface, J fixed during the solve (:146)
 
-    Flash(const Model& model, const double* mass, double e, const ThermoState& g, Counters& c)
-        : m(model), q(mass), energy(e), guess(g), count(c) {}
+    Flash(const Model& model, const double* mass, double e, const ThermoState& g, Counters& c, double geometricColor = 0,
+          double jump = 0)
+        : m(model), q(mass), energy(e), guess(g), count(c), color(geometricColor), pressureJump(jump) {}
+    double gasPressure(double pbar) const { return pbar - color * pressureJump; }           // :152
+    double liquidPressure(double pbar) const { return pbar + (1 - color) * pressureJump; }  // :153
 
     double density() const {
         double r = 0;
@@ -740,8 +751,10 @@ struct Flash {
         v.cpDensity += mass * p.cp;
     }
 
-    // Mixture of a gas phase and pure liquids at common (p, T) (:208-262).
-    bool evaluate(const double* mass, double p, double T, Evaluation& v, bool virtualLiquids = false) {
+    // Mixture of a gas phase and pure liquids at (p, T) (:208-262); vapor (the
+    // adaptive coordinates) gives each condensable's gas inventory directly.
+    bool evaluate(const double* mass, double p, double T, Evaluation& v, bool virtualLiquids = false,
+                  const double* vapor = nullptr) {
         v = Evaluation{};
         if (!bounds(p, T)) return false;
         if (m.nl > 0 && mass[0] > 0 && cacheStatus >= 0 && p == cacheP && T == cacheT && virtualLiquids == cacheVirtual)
@@ -757,14 +770,15 @@ struct Flash {
         for (int i = 0; i < m.nl; ++i) {
             const int c = m.condensable[i];
             if (!fin(mass[i]) || mass[i] < 0 || mass[i] > q[c]) return false;
-            gas[c] = gas[c] - mass[i];
+            gas[c] = vapor ? vapor[i] : gas[c] - mass[i];
             if (!fin(gas[c]) || gas[c] < 0 || gas[c] > q[c]) return false;
         }
         for (int i = 0; i < m.nl; ++i) v.gasCondensable[i] = gas[m.condensable[i]];
+        v.explicitGas = vapor != nullptr;
         double mg = 0;
         for (int k = 0; k < m.ns; ++k) mg += gas[k];
         s.gasMass = mg;
-        const double pg = p - 0.0 * 0.0, pl = p + (1 - 0.0) * 0.0;  // color = pressureJump = 0 (:151-152)
+        const double pg = gasPressure(p), pl = liquidPressure(p);
         if (mg > 0) {
             for (int k = 0; k < m.ns; ++k) gas[k] /= mg;
             PRState prop{};
@@ -828,7 +842,7 @@ struct Flash {
         ++count.phases;
         if (evaluateTrho(m.phase[0], T, rho, Y, prop, false) != OK) return false;
         const double gasP = prop.p;
-        p = gasP + 0.0 * 0.0;
+        p = gasP + color * pressureJump;
         if (!bounds(p, T)) return false;
         PRState gas{};
         if (!phase(-1, gasP, T, Y, gas, false) || std::fabs(gas.rho - rho) > 1e-8 * rho) return false;
@@ -930,15 +944,24 @@ struct Flash {
     // Active-set residual [V-1, (E-e)/scale, mu_l/RT - mu_g/RT] (:327-337).
     bool residual(const int* active, int n, const double* x, double scale, double* f, Evaluation* out = nullptr) {
         ++count.residuals;
-        double mass[2]{};
+        double mass[2]{}, vapor[2]{};
+        for (int i = 0; i < m.nl; ++i) vapor[i] = q[m.condensable[i]];
         for (int j = 0; j < n; ++j) {
             const int i = active[j];
             const double qq = q[m.condensable[i]];
-            if (!(x[j + 2] >= 0 && x[j + 2] < 1)) return false;
-            mass[i] = x[j + 2] * qq;
+            if (adaptive) {  // x = ln(vapour fraction) <= 0
+                if (!fin(x[j + 2]) || x[j + 2] > 0) return false;
+                vapor[i] = oexp(x[j + 2]) * qq;
+                if (!(vapor[i] > 0)) return false;
+                mass[i] = -oexpm1(x[j + 2]) * qq;
+            } else {
+                if (!(x[j + 2] >= 0 && x[j + 2] < 1)) return false;
+                mass[i] = x[j + 2] * qq;
+            }
         }
         Evaluation v;
-        if (!evaluate(mass, oexp(x[0]), oexp(x[1]), v, true) || !(v.state.gasMass > 0)) return false;
+        if (!evaluate(mass, oexp(x[0]), oexp(x[1]), v, true, adaptive ? vapor : nullptr) || !(v.state.gasMass > 0))
+            return false;
         f[0] = v.volume - 1;
         f[1] = (v.energy - energy) / scale;
         for (int j = 0; j < n; ++j) {
@@ -960,27 +983,43 @@ struct Flash {
                 for (int i = 0; i < size; ++i) xp[i] = xm[i] = x[i];
                 xp[j] += h;
                 xm[j] -= h;
+                if (adaptive) {  // a step lost to rounding moves by one ulp instead
+                    if (xp[j] == x[j]) xp[j] = std::nextafter(x[j], HUGE_VAL);
+                    if (xm[j] == x[j]) xm[j] = std::nextafter(x[j], -HUGE_VAL);
+                }
                 const bool plus = residual(active, n, xp, scale, fp), minus = residual(active, n, xm, scale, fm);
                 if (!plus && !minus) continue;
+                const double hp = xp[j] - x[j], hm = x[j] - xm[j];
+                double maximum = 0;
                 success = true;
                 for (int i = 0; i < size; ++i) {
-                    const double v = plus && minus ? (fp[i] - fm[i]) / (2 * h) : plus ? (fp[i] - f[i]) / h : (f[i] - fm[i]) / h;
+                    double v;
+                    if (adaptive)  // one-sided differences weighted by the actual steps
+                        v = plus && minus ? (hm / (hp + hm)) * ((fp[i] - f[i]) / hp) + (hp / (hp + hm)) * ((f[i] - fm[i]) / hm)
+                            : plus ? (fp[i] - f[i]) / hp : (f[i] - fm[i]) / hm;
+                    else
+                        v = plus && minus ? (fp[i] - fm[i]) / (2 * h) : plus ? (fp[i] - f[i]) / h : (f[i] - fm[i]) / h;
                     jac[i][j] = v;
+                    maximum = hi(maximum, std::fabs(v));
                     success = success && fin(v);
                 }
+                if (adaptive) success = success && maximum > 0;
             }
             if (!success) return false;
         }
         return true;
     }
     // Analytic Jacobian from the residual point's own properties
-    // (buildCached, reactiveDeviceFlashJacobian.h:535-640; one active liquid,
-    // flat interface, normal coordinates).
+    // (buildCached, reactiveDeviceFlashJacobian.h:535-640; one active liquid):
+    // the gas at its own pressure pg, the liquid's molar energy at pl.
     bool buildCached(const int* active, int n, const double* x, double scale, const Evaluation& v, double jac[4][4]) {
         const double p = v.state.p, T = v.state.T;
+        const double gp = gasPressure(p), lp = liquidPressure(p);
+        const double pg = gp == 0 ? p : gp, pl = lp == 0 ? p : lp;
+        if (!(pg > 0) || !(pl > 0) || !fin(pg) || !fin(pl)) return false;
         if (n != 1 || !(p > 0) || !(T > 0) || !(scale > 0) || !(v.state.gasMass > 0) || !(v.gasMolarVolume > 0) ||
             !fin(p) || !fin(T) || !fin(scale) || !fin(v.Vp) || !fin(v.VT) || !fin(v.Ep) || !fin(v.ET) ||
-            !fin(v.state.gasMass) || !fin(v.gasMolarVolume))
+            !fin(v.state.gasMass) || !fin(v.gasMolarVolume) || v.explicitGas != adaptive)
             return false;
         double Y[kMaxSpecies]{};
         for (int k = 0; k < m.ns; ++k) {
@@ -998,10 +1037,17 @@ struct Flash {
         if (i < 0 || i >= m.nl) return false;
         const int k = m.condensable[i];
         if (!(q[k] > 0)) return false;
-        if (!(x[2] >= 0 && x[2] < 1)) return false;
-        const double product = x[2] * q[k];  // the residual's own two roundings (:35-47)
-        const double expectedGas = q[k] - product;
-        const double dLiquid = q[k];
+        double expectedGas, dLiquid;
+        if (adaptive) {
+            if (!(x[2] <= 0) || !fin(x[2])) return false;
+            expectedGas = oexp(x[2]) * q[k];
+            dLiquid = -expectedGas;
+        } else {
+            if (!(x[2] >= 0 && x[2] < 1)) return false;
+            const double product = x[2] * q[k];  // the residual's own two roundings (:35-47)
+            expectedGas = q[k] - product;
+            dLiquid = q[k];
+        }
         if (!(expectedGas > 0) || v.gasCondensable[i] != expectedGas) return false;
         if (!(v.liquidMask & (1u << i)) || !fin(v.liquidHbar[i]) || !fin(v.liquidVbar[i]) || !(v.liquidVbar[i] > 0))
             return false;
@@ -1012,7 +1058,7 @@ struct Flash {
         if (reconstructed != v.state.gasMass) return false;
         for (int s = 0; s < m.ns; ++s) Y[s] /= v.state.gasMass;
         double hbar, ubar, vbar, dpsi;
-        if (!activeGasDerivatives(m.phase[0], T, p, v.gasMolarVolume, Y, v.state.gasMass, k, hbar, ubar, vbar, dpsi))
+        if (!activeGasDerivatives(m.phase[0], T, pg, v.gasMolarVolume, Y, v.state.gasMass, k, hbar, ubar, vbar, dpsi))
             return false;
         for (int r = 0; r < 4; ++r)
             for (int c = 0; c < 4; ++c) jac[r][c] = 0;
@@ -1021,7 +1067,7 @@ struct Flash {
         jac[1][0] = v.Ep * p / scale;
         jac[1][1] = v.ET * T / scale;
         const double Wk = m.phase[0].sp[k].W, Wl = m.phase[i + 1].sp[0].W;
-        const double liquidUbar = v.liquidHbar[i] - p * v.liquidVbar[i];
+        const double liquidUbar = v.liquidHbar[i] - pl * v.liquidVbar[i];
         jac[0][2] = dLiquid * (v.liquidVbar[i] / Wl - vbar / Wk);
         jac[1][2] = dLiquid * (liquidUbar / Wl - ubar / Wk) / scale;
         jac[2][0] = p * (v.liquidVbar[i] - vbar) / (R * T);
@@ -1037,21 +1083,32 @@ struct Flash {
     bool buildFresh(const int* active, int n, const double* x, double p, double T, double scale, const Evaluation& v,
                     double jac[4][4]) {
         if (n != 1 || !(p > 0) || !(T > 0) || !(scale > 0)) return false;
+        const double gp = gasPressure(p), lp = liquidPressure(p);
+        const double pg = gp == 0 ? p : gp, pl = lp == 0 ? p : lp;
+        if (!(pg > 0) || !(pl > 0) || !fin(pg) || !fin(pl)) return false;
         double gasMass[kMaxSpecies]{}, Y[kMaxSpecies]{}, total = 0;
         for (int s = 0; s < m.ns; ++s) gasMass[s] = q[s];
         const int i = active[0];
         if (i < 0 || i >= m.nl) return false;
         const int k = m.condensable[i];
         if (k < 0 || k >= m.ns || !(q[k] > 0)) return false;
-        if (!(x[2] >= 0 && x[2] < 1)) return false;
-        gasMass[k] -= x[2] * q[k];
-        const double dLiquid = q[k];
+        double dLiquid;
+        if (adaptive) {
+            if (!(x[2] <= 0) || !fin(x[2])) return false;
+            gasMass[k] = oexp(x[2]) * q[k];
+            if (!(gasMass[k] > 0)) return false;
+            dLiquid = -gasMass[k];
+        } else {
+            if (!(x[2] >= 0 && x[2] < 1)) return false;
+            gasMass[k] -= x[2] * q[k];
+            dLiquid = q[k];
+        }
         for (int s = 0; s < m.ns; ++s) total += gasMass[s];
         if (!(total > 0)) return false;
         for (int s = 0; s < m.ns; ++s) Y[s] = gasMass[s] / total;
         PRState gas{};
         ++count.phases;
-        if (evaluateTP(m.phase[0], T, p, Y, GasRoot, gas, false) != OK) return false;
+        if (evaluateTP(m.phase[0], T, pg, Y, GasRoot, gas, false) != OK) return false;
         double hbar[kMaxSpecies]{}, ubar[kMaxSpecies]{}, vbar[kMaxSpecies]{}, xm[kMaxSpecies]{}, W = 0;
         if (!phasePartials(m.phase[0], T, Y, gas, hbar, ubar, vbar, xm, W)) return false;
         const double totalMoles = total / W;
@@ -1059,7 +1116,7 @@ struct Flash {
         double pure[kMaxSpecies]{};
         pure[0] = 1;
         ++count.phases;
-        if (evaluateTP(m.phase[i + 1], T, p, pure, LiquidRoot, liquid, false) != OK) return false;
+        if (evaluateTP(m.phase[i + 1], T, pl, pure, LiquidRoot, liquid, false) != OK) return false;
         double dpsi;
         if (!gasCompositionPartial(m.phase[0], T, xm, totalMoles, gas, k, dpsi)) return false;
         for (int r = 0; r < 4; ++r)
@@ -1093,8 +1150,13 @@ struct Flash {
         }
         return jacobian(active, n, x, scale, f, jac);
     }
-    // Damped Newton on the active set, seeded by liquid fraction (:374-448).
-    bool activeFlash(const int* active, int n, double seed, Evaluation& value, const ThermoState* restart = nullptr) {
+    // Damped Newton on the active set, seeded by liquid fraction (:374-448);
+    // in adaptive coordinates by ln(vapour fraction), where seed -3 (and a
+    // previous state with almost no vapour) starts from a trace of vapour set
+    // by the noncondensable inventory.  secondSeed (-2: none) seeds the second
+    // active liquid.
+    bool activeFlash(const int* active, int n, double seed, Evaluation& value, double secondSeed = -2,
+                     const ThermoState* restart = nullptr) {
         ++count.candidates;
         const int size = n + 2;
         const double rho = density();
@@ -1103,8 +1165,23 @@ struct Flash {
         double x[4]{olog(clamp(g.p, m.pmin, m.pmax)), olog(clamp(g.T, m.Tmin, m.Tmax))};
         for (int j = 0; j < n; ++j) {
             const int i = active[j];
+            const double selected = j == 1 && secondSeed != -2 ? secondSeed : seed;
             const double previous = g.liquidMass[i] / q[m.condensable[i]];
-            x[j + 2] = seed < 0 ? clamp(previous, 1e-8, 1 - 1e-8) : seed;
+            x[j + 2] = selected < 0 ? clamp(previous, 1e-8, 1 - 1e-8) : selected;
+            if (adaptive) {
+                double vapor = selected < 0 ? 1 - previous : 1 - selected;
+                if (selected == -3 || (selected < 0 && vapor < 1e-6)) {
+                    double other = 0;
+                    for (int k = 0; k < m.ns; ++k) {
+                        bool selectedCondensable = false;
+                        for (int a = 0; a < n; ++a) selectedCondensable |= k == m.condensable[active[a]];
+                        if (!selectedCondensable) other += q[k];
+                    }
+                    vapor = other > 0 ? lo(1e-6, .01 * (other / q[m.condensable[i]])) : 1e-8;
+                }
+                if (!(vapor > 0 && fin(vapor))) return false;
+                x[j + 2] = olog(lo(1., vapor));
+            }
         }
         double f[4]{};
         if (!residual(active, n, x, scale, f, &value)) return false;
@@ -1129,7 +1206,7 @@ struct Flash {
             for (int j = 0; j < size; ++j) rhs[j] = -f[j];
             if (!solve(jac, rhs, size, step)) return false;
             double limiter = hi(1., hi(std::fabs(step[0]) / .7, std::fabs(step[1]) / .18));
-            for (int j = 2; j < size; ++j) limiter = hi(limiter, std::fabs(step[j]) / .3);
+            for (int j = 2; j < size; ++j) limiter = hi(limiter, std::fabs(step[j]) / (adaptive ? 2. : .3));
             for (int j = 0; j < size; ++j) step[j] /= limiter;
             bool accepted = false;
             const double previous = maxNorm(f, size);
@@ -1138,7 +1215,8 @@ struct Flash {
                 double tx[4]{}, tf[4]{};
                 for (int j = 0; j < size; ++j) tx[j] = x[j] + fraction * step[j];
                 bool rejected = false;
-                for (int j = 0; j < n; ++j) rejected |= !(tx[j + 2] >= 0 && tx[j + 2] < 1);
+                for (int j = 0; j < n; ++j)
+                    rejected |= adaptive ? (!fin(tx[j + 2]) || tx[j + 2] > 0) : !(tx[j + 2] >= 0 && tx[j + 2] < 1);
                 if (rejected) {
                     // rejected fractions form a prefix: bisect to its end (:419-438)
                     int lower = index, upper = 30;
@@ -1148,7 +1226,7 @@ struct Flash {
                         bool bad = false;
                         for (int j = 0; j < n; ++j) {
                             const double y = x[j + 2] + fm * step[j + 2];
-                            bad |= !(y >= 0 && y < 1);
+                            bad |= adaptive ? (!fin(y) || y > 0) : !(y >= 0 && y < 1);
                         }
                         if (bad) lower = mid; else upper = mid;
                     }
@@ -1178,7 +1256,7 @@ struct Flash {
         for (int k = 0; k < m.ns; ++k) { Y[k] = mole[k] * m.weights[k]; sum += Y[k]; }
         for (int k = 0; k < m.ns; ++k) Y[k] /= sum;
         PRState prop{};
-        const Status st = phaseStatus(-1, p, T, Y, prop, false);
+        const Status st = phaseStatus(-1, gasPressure(p), T, Y, prop, false);
         if (st != OK) return unavailable(st) ? HUGE_VAL : NAN;
         double reference = 0, W = 0;
         for (int i = 0; i < n; ++i) reference += mole[m.condensable[present[i]]] * mu[present[i]];
@@ -1191,7 +1269,8 @@ struct Flash {
         if (value.state.gasMass > 0) {
             double mass[2]{value.state.liquidMass[0], value.state.liquidMass[1]};
             Evaluation comparison;
-            if (!evaluate(mass, p, T, comparison, true)) return false;
+            if (!evaluate(mass, p, T, comparison, true, value.explicitGas ? value.gasCondensable : nullptr))
+                return false;
             for (int i = 0; i < m.nl; ++i) {
                 if (q[m.condensable[i]] == 0) continue;
                 const double affinity = comparison.muLiquid[i] - comparison.muGas[i];
@@ -1207,7 +1286,7 @@ struct Flash {
             if (value.state.liquidMass[i] > 0) {
                 present[n++] = i;
                 PRState prop{};
-                if (!phase(i, p, T, nullptr, prop)) return false;
+                if (!phase(i, liquidPressure(p), T, nullptr, prop)) return false;
                 mu[i] = prop.mu[0] / (8314.46261815324 * T);
             }
         if (!n) return false;
@@ -1263,7 +1342,7 @@ struct Flash {
                 const double frac = t == 0 ? 0.02 : t == 1 ? 0.25 : t == 2 ? 0.5 : 0.75;  // reactiveFlashSeeds.h:13-16
                 restart.T = lower + (upper - lower) * frac;
                 for (int j = 0; j < 5; ++j) {
-                    if (activeFlash(active, n, seeds[j], candidate, &restart)) consider(candidate, best, have);
+                    if (activeFlash(active, n, seeds[j], candidate, -2, &restart)) consider(candidate, best, have);
                     else ++count.failures;
                 }
             }
@@ -1275,7 +1354,8 @@ struct Flash {
         Evaluation candidate;
         double mass[2]{};
         bool gasStable = false;
-        const bool prunable = m.stableGasPrune && m.nl == 1 && m.condensedKind[0] == 0 && m.condensedKind[1] == 0;
+        const bool prunable =
+            m.stableGasPrune && m.nl == 1 && m.condensedKind[0] == 0 && m.condensedKind[1] == 0 && !adaptive;
         if (frozen(mass, candidate, prunable ? 4 : 0)) {
             const uint64_t before = count.stable;
             consider(candidate, best, have);
@@ -1309,18 +1389,54 @@ struct Flash {
                 if (activeFlash(active, n, seeds[j], candidate)) consider(candidate, best, have);
                 else ++count.failures;
             }
+            if (adaptive) {  // the trace-vapour start (:532)
+                if (activeFlash(active, n, -3, candidate)) consider(candidate, best, have);
+                else ++count.failures;
+            }
+            if (adaptive && n == 2)  // off-diagonal seed pairs in the order .1, .5, .95, .9999 (:533-536)
+                for (int first = 1; first < 5; ++first)
+                    for (int second = 1; second < 5; ++second)
+                        if (first != second) {
+                            const double off[4]{.1, .5, .95, .9999};
+                            if (activeFlash(active, n, off[first - 1], candidate, off[second - 1]))
+                                consider(candidate, best, have);
+                            else ++count.failures;
+                        }
         }
         if (!have) have = temperatureSearch(best);
         return have;
     }
-    bool run(Evaluation& result) {  // :544-560 with boundaryRecovery = 0
+    // :544-560.  With boundaryRecovery (or a capillary equilibrium cell), a
+    // reference result whose liquid holds nearly all of a condensable with gas
+    // present, or no result, runs the search again in vapour coordinates; the
+    // higher entropy density wins.
+    bool run(Evaluation& result, bool capillaryBoundary = false) {
         if (m.ns <= 0 || m.ns > kMaxSpecies || m.nl < 0 || m.nl > 2 || !fin(energy) || !(density() > 0)) return false;
         for (int k = 0; k < m.ns; ++k)
             if (!fin(q[k]) || q[k] < 0) return false;
         Evaluation reference;
         const bool have = search(reference);
-        if (have) result = reference;
-        return have;
+        if (!m.boundaryRecovery && !capillaryBoundary) {
+            if (have) result = reference;
+            return have;
+        }
+        if (have) {
+            bool boundary = false;
+            for (int i = 0; i < m.nl; ++i)
+                if (q[m.condensable[i]] > 0 && reference.state.gasMass > 0)
+                    boundary |= reference.state.liquidMass[i] / q[m.condensable[i]] > 1 - 1e-4;
+            if (!boundary) {
+                result = reference;
+                return true;
+            }
+        }
+        adaptive = true;
+        Evaluation recovered;
+        const bool found = search(recovered);
+        adaptive = false;
+        if (!found && !have) return false;
+        result = have && (!found || reference.entropyDensity >= recovered.entropyDensity) ? reference : recovered;
+        return true;
     }
 };
 
@@ -1369,7 +1485,6 @@ extern "C" {
 int oracle_model_supported(const void* image) {
     const Model& m = *static_cast<const Model*>(image);
     if (!validModel(m)) return 0;
-    if (m.boundaryRecovery) return 0;
     for (int i = 0; i < m.nl; ++i)
         if (m.condensedKind[i]) return 0;
     return 1;
@@ -1401,6 +1516,44 @@ int oracle_flash_frozen(const void* image, const double* q, double energy, const
     if (ok) *out = result.state;
     if (counters) std::memcpy(counters, &c, sizeof(c));
     return ok ? 1 : 0;
+}
+
+// One capillary-interface recovery (recoverCapillary, reactiveDeviceFlash.h:
+// 572-590): equilibrium 1 is the UV equilibrium with the boundary search,
+// 0 the frozen inventory at the guess's liquid masses; the interface's color
+// and pressure jump hold during the solve.  Returns 1 on success, 0 on a
+// failed recovery, -2 for an invalid color or jump (the reference's status 2).
+int oracle_flash_capillary(const void* image, const double* q, double energy, const ThermoState* guess, double color,
+                           double pressureJump, int equilibrium, ThermoState* out, uint64_t* counters) {
+    const Model& m = *static_cast<const Model*>(image);
+    if (!fin(color) || color < 0 || color > 1 || !fin(pressureJump) ||
+        (pressureJump != 0 && (m.nl != 1 || m.condensedKind[0] != 0)))
+        return -2;
+    Counters c{};
+    Flash flash(m, q, energy, *guess, c, color, pressureJump);
+    Evaluation result;
+    const double mass[2]{guess->liquidMass[0], guess->liquidMass[1]};
+    const bool ok = equilibrium ? flash.run(result, true) : flash.frozen(mass, result);
+    if (ok) *out = result.state;
+    if (counters) std::memcpy(counters, &c, sizeof(c));
+    return ok ? 1 : 0;
+}
+// Batch of capillary recoveries with per-cell color, jump and equilibrium.
+int oracle_flash_capillary_batch(const void* image, size_t count, const double* q, const double* energy,
+                                 ThermoState* states, const double* color, const double* jump, const int* equilibrium,
+                                 int* success, uint64_t* counters, int threads) {
+    const Model& m = *static_cast<const Model*>(image);
+    long ok = 0;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads > 0 ? threads : 1) reduction(+ : ok)
+    for (long c = 0; c < long(count); ++c) {
+        ThermoState result{};
+        const int s = oracle_flash_capillary(image, q + size_t(c) * m.ns, energy[c], &states[c], color[c], jump[c],
+                                             equilibrium[c], &result, counters ? counters + 9 * size_t(c) : nullptr);
+        success[c] = s > 0 ? 1 : 0;
+        if (s > 0) states[c] = result;
+        ok += s > 0;
+    }
+    return int(ok);
 }
 
 // Batch over cells, threads = OpenMP workers (1 = serial).  q is cell-major
@@ -1469,11 +1622,11 @@ int oracle_nasa(const void* image, int phase, int k, double T, double* out) {
 }
 
 // Elementary functions of this build, for tests: which = 0 log, 1 exp,
-// 2 x^(1/3) as pow(x, 1.0 / 3.0), 3 acos, 4 cos.
+// 2 x^(1/3) as pow(x, 1.0 / 3.0), 3 acos, 4 cos, 5 expm1.
 extern "C" void oracle_elementary(int which, size_t n, const double* x, double* y) {
     for (size_t i = 0; i < n; ++i) {
         const double v = x[i];
         y[i] = which == 0 ? oracle::olog(v) : which == 1 ? oracle::oexp(v) : which == 2 ? oracle::othird(v)
-             : which == 3 ? oracle::oacos(v) : oracle::ocos(v);
+             : which == 3 ? oracle::oacos(v) : which == 4 ? oracle::ocos(v) : oracle::oexpm1(v);
     }
 }

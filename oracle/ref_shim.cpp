@@ -86,6 +86,25 @@ int ref_flash_capillary_batch(const void* model, size_t count, const double* q, 
     return ok;
 }
 
+// Per-cell color, pressure jump and equilibrium flag (recoverCapillary for
+// each cell).  A cell whose interface inputs the reference rejects (status 2)
+// reports success 0.
+int ref_flash_capillary_cells(const void* model, size_t count, const double* q, const double* energy,
+                              const void* guess, const double* color, const double* jump, const int* equilibrium,
+                              void* out, int* success, uint64_t* counters) {
+    const auto& m = *static_cast<const ReactiveDeviceFlash::Model*>(model);
+    int ok = 0;
+    for (size_t c = 0; c < count; ++c) {
+        const char* g = static_cast<const char*>(guess) + c * sizeof(ReactiveThermoState);
+        char* o = static_cast<char*>(out) + c * sizeof(ReactiveThermoState);
+        std::memcpy(o, g, sizeof(ReactiveThermoState));
+        success[c] = ref_flash_capillary(model, q + c * m.ns, energy[c], g, color[c], jump[c], equilibrium[c], o,
+                                         counters ? counters + 9 * c : nullptr);
+        ok += success[c];
+    }
+    return ok;
+}
+
 // Batch form for fixture generation.
 int ref_flash_batch(const void* model, size_t count, const double* q, const double* energy,
                     const void* guess, int useCache, void* out, int* success, uint64_t* counters) {

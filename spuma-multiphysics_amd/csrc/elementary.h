@@ -350,6 +350,44 @@ RAMD_ELEM double exp(double x) {
     return scale(s + (se + (bl + (Th * tail + Tl * ((1.0 + u) + tail)))), m);
 }
 
+// e^x - 1 (the adaptive flash's liquid mass, -expm1(ln vapour fraction) q).
+// Below ln2/128 in magnitude the reduction is empty (r = x) and the result is
+// the series of e^r - 1 itself; otherwise e^x as exp forms it, a double-double
+// 2^m (S + E) good to about 2^-68, from which 1 is subtracted exactly.  The
+// error stays below 2^-60 of the result (at most 1/128 ulp before rounding).
+RAMD_ELEM double expm1(double x) {
+    if (x != x) return x;
+    if (x > 709.782712893384) return HUGE_VAL;
+    if (x < -37.5) return -1.0;  // e^x < 2^-54: -1 + e^x rounds to -1
+    const double kd = ::rint(x * kInvLn2x64);
+    const int k = int(kd), j = k & 63, m = (k - j) / 64;
+    const double r1 = fmaE(-kd, kL1, x);
+    double t, te, rh, rhe;
+    twoProd(kd, kL2, t, te);
+    twoSum(r1, -t, rh, rhe);
+    const double rl = rhe - (te + kd * kL3);
+    double sq, sqe;
+    twoProd(rh, rh, sq, sqe);
+    double Q = 1.9841269841269841e-04;
+    Q = fmaE(Q, rh, 1.3888888888888889e-03);
+    Q = fmaE(Q, rh, 8.3333333333333332e-03);
+    Q = fmaE(Q, rh, 4.1666666666666664e-02);
+    Q = fmaE(Q, rh, 1.6666666666666666e-01);
+    const double small = rl + (rh * rl + (0.5 * sqe + (rh * sq) * Q));
+    double u, ul;
+    fastTwoSum(rh, 0.5 * sq, u, ul);
+    if (k == 0) return u + (ul + small);  // r = x exactly
+    const double Th = kExp2Table[2 * j], Tl = kExp2Table[2 * j + 1];
+    double b, bl, s, se;
+    twoProd(Th, u, b, bl);
+    fastTwoSum(Th, b, s, se);
+    const double tail = ul + small;
+    const double E = se + (bl + (Th * tail + Tl * ((1.0 + u) + tail)));
+    double a, ae;
+    twoSum(scale(s, m), -1.0, a, ae);  // 2^m s and 2^m E are exact (m > -56)
+    return a + (ae + scale(E, m));
+}
+
 RAMD_ELEM double log(double x) {
     if (!(x > 0)) return x == 0 ? -HUGE_VAL : NAN;  // also NaN in, NaN out
     if (x == HUGE_VAL) return x;

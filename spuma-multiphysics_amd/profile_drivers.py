@@ -8,6 +8,12 @@
   cube N MODE STEPS
                  resident SSPRK2 steps (MODE 0 FP64, 1 mixed) on the seeded
                  pure-N2O N^3 box of the benchmark
+  bench N MODE STEPS
+                 the benchmark's own flow (bench.prepare + bench.run_steps) on its
+                 seeded pure-N2O N^3 box: an autotuning step on the stream, two
+                 graph-replayed warm-up steps, STEPS timed graph-replayed steps
+                 and one step on the stream (MODE 0 FP64, 1 mixed), so the
+                 kernels run with the launch shapes the benchmark times
   copy MIB       a plain device copy kernel of MIB MiB (torch elementwise copy),
                  to calibrate FETCH_SIZE and WRITE_SIZE against a known byte count
   uniform N T RHO MODE STEPS
@@ -150,6 +156,27 @@ def cube(n, mode, steps):
     lib.reactive_gpu_hem_destroy_v1(hem)
 
 
+def bench_flow(n, mode, steps):
+    sys.path.insert(0, os.path.dirname(HERE))
+    import bench
+    lib = rlib.load()
+    faces, volume, q, states, m, hem, info = bench.prepare(n, 2026)
+    cfg = abi.Config(cells=len(q), species=1, variables=5, faces=len(faces), fixed=0, viscosity=0, conductivity=0,
+                     diffusivity=0, waveFactor=1.1, maxBytes=200e9, mechanical=0)
+    err = C.create_string_buffer(1024)
+    tr = lib.reactive_transport_create(1, C.byref(cfg), rlib.ptr(volume), rlib.ptr(faces), None, None, None, None,
+                                       err, 1024)
+    step = lib.reactive_amd_step_create(tr, hem, err, 1024)
+    assert step, err.value
+    rep, wall, kernels, _, _ = bench.run_steps(lib, step, q, states, 0.4, steps, mode, 2, 1)
+    print(f"bench flow: {len(q)} cells, mode {mode}, {steps} timed steps, {wall / steps * 1e3:.2f} ms per step; "
+          f"shapes: flash block {kernels.flashBlock}, state bins {kernels.stateBins}, tile depth {kernels.tileDepth}",
+          flush=True)
+    lib.reactive_amd_step_destroy(step)
+    lib.reactive_transport_destroy(tr)
+    lib.reactive_gpu_hem_destroy_v1(hem)
+
+
 def copy(mib):
     import torch
     n = int(mib) << 20
@@ -169,6 +196,8 @@ if __name__ == "__main__":
         flash(int(sys.argv[2]))
     elif what == "copy":
         copy(int(sys.argv[2]))
+    elif what == "bench":
+        bench_flow(int(sys.argv[2]), int(sys.argv[3]), int(sys.argv[4]))
     elif what == "cube":
         cube(int(sys.argv[2]), int(sys.argv[3]), int(sys.argv[4]))
     elif what == "uniform":

@@ -63,10 +63,12 @@ def _fill_table(table, names, a_scale=1.0, b_scale=1.0):
 
 
 def build_model(gas_species=("N2O",), liquid="N2O", analytic_jacobian=0, stable_gas_prune=0,
-                scalar_recovery=1, limits=None, a_scale=1.0, b_scale=1.0):
+                scalar_recovery=1, limits=None, a_scale=1.0, b_scale=1.0, boundary_recovery=0):
     """Return the model image (numpy structured scalar) for a PR gas of
     `gas_species` with one pure liquid of `liquid` (None for no liquid).
-    a_scale, b_scale multiply every species' PR a (and a_ij) and b."""
+    a_scale, b_scale multiply every species' PR a (and a_ij) and b;
+    boundary_recovery = 1 selects the vapour-log boundary search
+    (reactiveDeviceFlash.h:544-560)."""
     lim = dict(mechanism.LIMITS)
     if limits:
         lim.update(limits)
@@ -76,6 +78,7 @@ def build_model(gas_species=("N2O",), liquid="N2O", analytic_jacobian=0, stable_
     m["scalarRecovery"] = scalar_recovery
     m["analyticJacobian"] = analytic_jacobian
     m["stableGasPrune"] = stable_gas_prune
+    m["boundaryRecovery"] = boundary_recovery
     for k, name in enumerate(gas_species):
         m["weights"][k] = mechanism.SPECIES[name]["W"]
     if liquid is not None:

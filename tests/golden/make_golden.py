@@ -35,6 +35,7 @@ def ref_lib():
     V = C.c_void_p
     lib.ref_flash_batch.argtypes = [V, C.c_size_t, V, V, V, C.c_int, V, V, V]
     lib.ref_flash_capillary_batch.argtypes = [V, C.c_size_t, V, V, V, C.c_int, V, V, V]
+    lib.ref_flash_capillary_cells.argtypes = [V, C.c_size_t, V, V, V, V, V, V, V, V, V]
     lib.reactive_transport_create.restype = V
     lib.reactive_transport_create.argtypes = [C.c_int, V, V, V, V, V, V, V, C.c_char_p, C.c_size_t]
     lib.reactive_transport_rhs.argtypes = [V] * 7
@@ -229,6 +230,61 @@ def base_fixtures(lib):
     print("binary success", int(success2.sum()), "/", len(e2))
 
 
+def capillary_cells(lib, mb, q, e, seeds, color, jump, equilibrium):
+    n = len(e)
+    out = seeds.copy()
+    success = np.zeros(n, np.int32)
+    counters = np.zeros((n, 9), np.uint64)
+    buf = C.create_string_buffer(mb, len(mb))
+    lib.ref_flash_capillary_cells(buf, n, p(np.ascontiguousarray(q)), p(np.ascontiguousarray(e)), p(seeds),
+                                  p(np.ascontiguousarray(color, dtype=float)), p(np.ascontiguousarray(jump, dtype=float)),
+                                  p(np.ascontiguousarray(equilibrium, dtype=np.int32)), p(out), p(success), p(counters))
+    return out, success, counters
+
+
+def capillary_fixture(lib):
+    """The boundary search in vapour coordinates (reactiveDeviceFlash.h:
+    327-448, 509-560) and capillary recoveries with a fixed interface
+    (recoverCapillary, :572-590): a seeded 8^3 N2O cube with per-cell colour,
+    pressure jump and equilibrium flag; binary N2O/N2 cubes with trace and
+    with ordinary N2 under a boundaryRecovery = 1 model (finite-difference
+    and analytic Jacobian), and capillary equilibrium cells on the trace box."""
+    sets = {}
+    mb = model.model_bytes(model.build_model())
+    _, _, q, seeds, _ = cases.pure_n2o_cube(8, seed=11)
+    e = cases.internal_energy(q, 1)
+    n = len(e)
+    rng = np.random.default_rng(11)
+    color, jump = rng.uniform(0, 1, n), rng.uniform(-2e4, 2e4, n)
+    equilibrium = (rng.uniform(size=n) < 0.7).astype(np.int32)
+    color[: n // 4], jump[: n // 4], equilibrium[: n // 4] = 0, 0, 1  # flat interface, equilibrium search
+    out, ok, cnt = capillary_cells(lib, mb, q[:, :1], e, seeds, color, jump, equilibrium)
+    sets["n2o"] = dict(model=np.frombuffer(mb, np.uint8), q=q[:, :1], energy=e, seeds=seeds, color=color, jump=jump,
+                       equilibrium=equilibrium, out=out, success=ok, counters=cnt)
+    for name, ymax, analytic in (("trace", 1e-4, 0), ("trace_analytic", 1e-4, 1), ("binary", 0.3, 0)):
+        mb2 = model.model_bytes(model.build_model(gas_species=("N2O", "N2"), analytic_jacobian=analytic,
+                                                  boundary_recovery=1))
+        _, _, q2, seeds2, _ = cases.binary_cube(8, seed=12, ymax=ymax)
+        e2 = cases.internal_energy(q2, 2)
+        out2, ok2, cnt2 = flash_set(lib, mb2, q2[:, :2], e2, seeds2)
+        sets[name] = dict(model=np.frombuffer(mb2, np.uint8), q=q2[:, :2], energy=e2, seeds=seeds2, out=out2,
+                          success=ok2, counters=cnt2)
+    mb3 = model.model_bytes(model.build_model(gas_species=("N2O", "N2")))
+    _, _, q3, seeds3, _ = cases.binary_cube(8, seed=13, ymax=1e-4)
+    e3 = cases.internal_energy(q3, 2)
+    n3 = len(e3)
+    color3, jump3 = rng.uniform(0, 1, n3), rng.uniform(-1e4, 1e4, n3)
+    eq3 = np.ones(n3, np.int32)
+    out3, ok3, cnt3 = capillary_cells(lib, mb3, q3[:, :2], e3, seeds3, color3, jump3, eq3)
+    sets["trace_capillary"] = dict(model=np.frombuffer(mb3, np.uint8), q=q3[:, :2], energy=e3, seeds=seeds3,
+                                   color=color3, jump=jump3, equilibrium=eq3, out=out3, success=ok3, counters=cnt3)
+    np.savez_compressed(os.path.join(HERE, "flash_capillary.npz"),
+                        **{f"{s_}__{k}": v for s_, d in sets.items() for k, v in d.items()})
+    for s_, d in sets.items():
+        print(s_, "success", int(d["success"].sum()), "/", len(d["energy"]), "candidates",
+              int(d["counters"][:, 2].sum()))
+
+
 # Viscous, conductive and WALE transport: (viscosity, conductivity, WALE Cw or None)
 VISCOUS_MODELS = {"mu": (0.05, 0.0, None), "k": (0.0, 40.0, None), "mu_k": (0.05, 40.0, None),
                   "wale": (0.0, 0.0, 0.325), "wale_mu_k": (2e-3, 15.0, 0.5)}
@@ -291,7 +347,7 @@ def viscous_fixture(lib):
 
 
 SETS = {"base": base_fixtures, "frozen": frozen_fixture, "prune": prune_fixture, "analytic": analytic_fixture,
-        "viscous": viscous_fixture}
+        "viscous": viscous_fixture, "capillary": capillary_fixture}
 
 
 def main(names):

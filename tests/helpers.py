@@ -46,6 +46,7 @@ def oracle(twin=False):
         lib.oracle_flash_batch.argtypes = [V, C.c_size_t, V, V, V, V, V, C.c_int]
         lib.oracle_flash_recover.argtypes = [V, V, C.c_double, V, V, V]
         lib.oracle_flash_frozen.argtypes = [V, V, C.c_double, V, V, V]
+        lib.oracle_flash_capillary_batch.argtypes = [V, C.c_size_t, V, V, V, V, V, V, V, V, C.c_int]
         lib.oracle_model_supported.argtypes = [V]
         lib.oracle_pr_tp.argtypes = [V, C.c_int, C.c_double, C.c_double, V, C.c_int, V]
         lib.oracle_pr_trho.argtypes = [V, C.c_int, C.c_double, C.c_double, V, V]
@@ -80,6 +81,24 @@ def oracle_flash(model_bytes, q_species, energy, seeds, threads=8, twin=False):
     mb = C.create_string_buffer(model_bytes, len(model_bytes))
     lib.oracle_flash_batch(mb, n, p(np.ascontiguousarray(q_species)), p(np.ascontiguousarray(energy)), p(states),
                            p(success), p(counters), threads)
+    return states, success, counters
+
+
+def oracle_capillary(model_bytes, q_species, energy, seeds, color, jump, equilibrium, threads=8, twin=False):
+    """Batch of capillary recoveries (recoverCapillary) on the oracle with
+    per-cell color, pressure jump and equilibrium flag: (states, success,
+    counters[9])."""
+    lib = oracle(twin)
+    n = len(energy)
+    states = seeds.copy()
+    success = np.zeros(n, np.int32)
+    counters = np.zeros((n, 9), np.uint64)
+    mb = C.create_string_buffer(model_bytes, len(model_bytes))
+    lib.oracle_flash_capillary_batch(mb, n, p(np.ascontiguousarray(q_species)), p(np.ascontiguousarray(energy)),
+                                     p(states), p(np.ascontiguousarray(color, dtype=float)),
+                                     p(np.ascontiguousarray(jump, dtype=float)),
+                                     p(np.ascontiguousarray(equilibrium, dtype=np.int32)), p(success), p(counters),
+                                     threads)
     return states, success, counters
 
 

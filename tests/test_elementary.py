@@ -60,6 +60,10 @@ CASES = {
         lambda x: mpmath.power(x, THIRD), False, 1.0),
     3: ("acos", args_acos, mpmath.acos, True, 0.5),
     4: ("cos", args_cos, mpmath.cos, True, 0.5),
+    # expm1 (the vapour-coordinate liquid mass): within 1/128 ulp before rounding
+    5: ("expm1", lambda r: np.concatenate([-np.exp(r.uniform(-30, 3.6, 1500)), r.uniform(-1e-3, 1e-3, 300),
+                                           r.uniform(-2, 5, 200), [-40.0, -37.6, -37.4, 0.0, -0.0054, 0.0054]]),
+        mpmath.expm1, False, 0.51),
 }
 
 
