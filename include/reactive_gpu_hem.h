@@ -71,10 +71,12 @@ typedef struct ReactiveGpuHemOutputV1 {
  * experimental", reactiveGpuHem.h:37): handles created in such a process run
  * the equilibrium flash of reactive_gpu_hem_run_v1 in the mixed arithmetic of
  * REACTIVE_AMD_FLASH_MIXED (include/reactive_amd.h).  The variable is read
- * once per process.  Frozen-inventory batches (run_v2, run_resident_v1) run
- * in FP64 either way. */
+ * once per process.  Capillary batches (run_v2, run_resident_v1) and
+ * boundaryRecovery models run in FP64 either way. */
 const char* reactive_gpu_hem_numerical_policy_v1(void);
-/* reactiveGpuHem.h:41 */
+/* reactiveGpuHem.h:41.  Model images with one pure liquid at most, no
+ * solids; boundaryRecovery = 1 selects the boundary search in every
+ * equilibrium flash (reactiveDeviceFlash.h:544-560). */
 void* reactive_gpu_hem_create_v1(const void* model, size_t modelBytes, size_t capacity, char* error, size_t errorSize);
 /* reactiveGpuHem.h:44 */
 void* reactive_gpu_hem_create_resident_v1(const void* model, size_t modelBytes, size_t capacity, char* error, size_t errorSize);
@@ -84,12 +86,14 @@ void reactive_gpu_hem_destroy_v1(void* hem);
 /* reactiveGpuHem.h:51-52: host arrays, count <= capacity */
 int reactive_gpu_hem_run_v1(void* hem, const double* q, const double* energy, size_t count,
     ReactiveThermoState* state, int* success, ReactiveGpuHemProfileV1* profile, char* error, size_t errorSize);
-/* reactiveGpuHem.h:63-65: capillary input per cell.  This build accepts
- * color = 0, pressureJump = 0 (no interface) with equilibrium = 0, the frozen
- * condensed-inventory recovery (reactiveDeviceFlash.h:572-590).  A batch
- * holding a cell with an interface or with equilibrium = 1 (the capillary
- * UV equilibrium search with boundary recovery) is rejected with rc 1 before
- * any flash runs; nothing is written. */
+/* reactiveGpuHem.h:63-65: capillary input per cell, recovered as
+ * recoverCapillary (reactiveDeviceFlash.h:572-590): the interface's color and
+ * pressure jump held during the solve; equilibrium 0 the frozen condensed
+ * inventory at the seed's liquid masses, 1 the UV equilibrium with the
+ * boundary search in vapour coordinates.  A batch with a color outside
+ * [0, 1], a non-finite jump or an equilibrium flag other than 0 / 1 is
+ * rejected with rc 1 before any flash runs (reactiveGpuHem.cuh:237-243); a
+ * cell with a jump but no liquid fails on its own (success 0). */
 int reactive_gpu_hem_run_v2(void* hem, const double* q, const double* energy,
     const ReactiveGpuHemCapillaryInputV2* capillary, size_t count,
     ReactiveThermoState* state, int* success, ReactiveGpuHemProfileV1* profile, char* error, size_t errorSize);
@@ -99,7 +103,8 @@ int reactive_gpu_hem_initialize_tp_v1(void* hem, double* q, double* energy,
     ReactiveThermoState* state, int* success, ReactiveGpuHemProfileV1* profile, char* error, size_t errorSize);
 /* reactiveGpuHem.h:81-84: every pointer is DEVICE memory; order[0..count)
  * names the cells, output[cell] receives each result.  Capillary input as
- * for run_v2 (frozen-inventory cells only). */
+ * for run_v2; as in the reference, the inputs are not checked as a batch: a
+ * cell whose color or jump recoverCapillary rejects fails on its own. */
 int reactive_gpu_hem_run_resident_v1(void* hem, const double* q, const double* energy,
     const ReactiveThermoState* seed, const ReactiveGpuHemCapillaryInputV2* capillary,
     const uint32_t* order, size_t count, ReactiveGpuHemOutputV1* output,

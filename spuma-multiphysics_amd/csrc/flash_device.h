@@ -61,7 +61,7 @@ struct HemModel {
     Table<1> liquid;
     PhaseHeuristic gasFixed;     // composition-independent heuristic when NS == 1
     PhaseHeuristic liquidFixed;  // pure liquid table
-    int nl, condensable, scalarRecovery, stableGasPrune, enforceBounds, analyticJacobian;
+    int nl, condensable, scalarRecovery, stableGasPrune, enforceBounds, analyticJacobian, boundaryRecovery;
     double weights[NS];
     double liquidTmin, liquidTc, Tmin, Tmax, pmin, pmax, vtol, etol, mutol;
 };
@@ -114,11 +114,30 @@ struct RefParts<true> {
     }
 };
 
+// The condensable's gas inventory of the residual point and whether it was
+// given (vapour coordinates, Evaluation::gasCondensable / explicitGas,
+// reactiveDeviceFlash.h:67): kept by the capillary / boundary-search flash.
+// A base (empty for the other flashes, so the record does not grow).
+template <bool C>
+struct CapParts {
+    RD void clearCap() {}
+};
+template <>
+struct CapParts<true> {
+    double capGasCondensable;
+    bool capExplicitGas;
+    RD void clearCap() {
+        capGasCondensable = 0;
+        capExplicitGas = false;
+    }
+};
+
 // Evaluation record (reactiveDeviceFlash.h:64-73) for at most one liquid;
 // A adds the per-phase parts of this build's analytic Jacobian (mixed
-// flash), RJ the inputs of the reference's analytic Jacobian.
-template <class R, bool A = false, bool RJ = false>
-struct EvalT {
+// flash), RJ the inputs of the reference's analytic Jacobian, C the
+// vapour-coordinate inventory of the capillary / boundary-search flash.
+template <class R, bool A = false, bool RJ = false, bool C = false>
+struct EvalT : CapParts<C> {
     R p, T, rho, e, entropy, cf, ceq, volRes, enRes, chemRes;
     R alphaGas, alphaLiq, rhoGas, rhoLiq, liqMass, gasMass, cp, cv;
     int active, iterations;
@@ -128,18 +147,19 @@ struct EvalT {
 };
 using Eval = EvalT<double>;
 
-template <class R, bool A, bool RJ>
-RD void zeroEval(EvalT<R, A, RJ>& v) {
+template <class R, bool A, bool RJ, bool C>
+RD void zeroEval(EvalT<R, A, RJ, C>& v) {
     v.p = v.T = v.rho = v.e = v.entropy = v.cf = v.ceq = v.volRes = v.enRes = v.chemRes = 0;
     v.alphaGas = v.alphaLiq = v.rhoGas = v.rhoLiq = v.liqMass = v.gasMass = v.cp = v.cv = 0;
     v.active = v.iterations = 0;
     v.volume = v.energy = v.entropyDensity = v.Vp = v.VT = v.Ep = v.ET = v.cpDensity = v.muGas = v.muLiquid = 0;
     v.parts.clear();
     v.ref.clear();
+    v.clearCap();
 }
 
-template <bool A, bool RJ>
-RD void writeOut(const EvalT<double, A, RJ>& v, ThermoOut& o) {
+template <bool A, bool RJ, bool C>
+RD void writeOut(const EvalT<double, A, RJ, C>& v, ThermoOut& o) {
     o.p = v.p; o.T = v.T; o.rho = v.rho; o.e = v.e; o.entropy = v.entropy;
     o.soundFrozen = v.cf; o.soundEquilibrium = v.ceq;
     o.volumeResidual = v.volRes; o.energyResidual = v.enRes; o.chemicalResidual = v.chemRes;
@@ -254,12 +274,28 @@ template <> struct FlashPrecision<float> {
 // flash.
 // RJ: the reference's analytic-Jacobian mode (Model::analyticJacobian = 1;
 // FP64 flash only).
-template <int NS, int NL, class R = double, bool A = FlashPrecision<R>::single, bool RJ = false>
-struct Flash {
+// CAP: the capillary / boundary-search flash (FP64 flash only): a fixed
+// interface (color, jump: the gas at p - color J, the liquid at
+// p + (1 - color) J) and the search in vapour coordinates of boundaryRecovery
+// models and capillary equilibrium cells (reactiveDeviceFlash.h:327-448,
+// 509-590).  Without it those members stay zero and no code reads them.
+// The capillary flash's interface and search mode, a base that is empty for
+// the other flashes (their per-lane record does not grow).
+template <bool CAP, class R>
+struct FlashCap {};
+template <class R>
+struct FlashCap<true, R> {
+    R color = 0, jump = 0;  // capillary interface, J fixed during the solve
+    bool adaptive = false;  // vapour-log coordinates of the boundary search
+};
+
+template <int NS, int NL, class R = double, bool A = FlashPrecision<R>::single, bool RJ = false, bool CAP = false>
+struct Flash : FlashCap<CAP, R> {
     static_assert(NL == 0 || NL == 1, "one pure liquid at most");
     static_assert(!RJ || (!A && !FlashPrecision<R>::single), "the reference's analytic mode is an FP64 flash");
+    static_assert(!CAP || (!A && !FlashPrecision<R>::single), "the capillary flash is an FP64 flash");
     using P = FlashPrecision<R>;
-    using E = EvalT<R, A, RJ>;
+    using E = EvalT<R, A, RJ, CAP>;
     using S = PRStateT<R>;
     using M = MixT<R>;
     const HemModel<NS>& m;
@@ -290,6 +326,15 @@ struct Flash {
         liquidTmin = R(m.liquidTmin); liquidTc = R(m.liquidTc);
         record = nullptr;
         stalled = false;
+    }
+    // Phase pressures of a curved interface (reactiveDeviceFlash.h:152-153).
+    RD R gasPressure(R p) const {
+        if constexpr (CAP) return p - this->color * this->jump;
+        else return p;
+    }
+    RD R liquidPressure(R p) const {
+        if constexpr (CAP) return p + (R(1) - this->color) * this->jump;
+        else return p;
     }
 
     RD R density() const {
@@ -486,15 +531,18 @@ struct Flash {
 
     // Gas + liquid mixture at common (p, T) (reactiveDeviceFlash.h:208-262).
     // Inlined into the solvers at inlining level 2 (see RAMD_SEARCH_INLINE).
-    RD bool evaluate(R mass0, R p, R T, E& v, bool virtualLiquids) {
+    // vapour >= 0 (CAP, vapour coordinates): the condensable's gas inventory
+    // itself rather than q - mass0.
+    RD bool evaluate(R mass0, R p, R T, E& v, bool virtualLiquids, R vapour = R(-1)) {
         if constexpr (P::single ? RAMD_SEARCH_INLINE >= 2 : RAMD_FLASH_INLINE >= 2)
-            return evaluateBody(mass0, p, T, v, virtualLiquids);
-        else return evaluateCall(mass0, p, T, v, virtualLiquids);
+            return evaluateBody(mass0, p, T, v, virtualLiquids, vapour);
+        else return evaluateCall(mass0, p, T, v, virtualLiquids, vapour);
     }
-    RAMD_NOINLINE bool evaluateCall(R mass0, R p, R T, E& v, bool virtualLiquids) {
-        return evaluateBody(mass0, p, T, v, virtualLiquids);
+    RAMD_NOINLINE bool evaluateCall(R mass0, R p, R T, E& v, bool virtualLiquids, R vapour) {
+        return evaluateBody(mass0, p, T, v, virtualLiquids, vapour);
     }
-    __device__ __attribute__((always_inline)) inline bool evaluateBody(R mass0, R p, R T, E& v, bool virtualLiquids) {
+    __device__ __attribute__((always_inline)) inline bool evaluateBody(R mass0, R p, R T, E& v, bool virtualLiquids,
+                                                                      R vapour) {
         zeroEval(v);
         if (!bounds(p, T)) return false;
         if (NL > 0 && mass0 > 0 && cacheStatus >= 0 && p == cacheP && T == cacheT && virtualLiquids == cacheVirtual)
@@ -509,20 +557,26 @@ struct Flash {
         const int c = NL ? m.condensable : 0;
         if (NL) {
             if (!fin(mass0) || mass0 < 0 || mass0 > q[c]) return false;
-            gas[c] = gas[c] - mass0;
+            if constexpr (CAP) gas[c] = vapour >= 0 ? vapour : gas[c] - mass0;
+            else gas[c] = gas[c] - mass0;
             if (!fin(gas[c]) || gas[c] < 0 || gas[c] > q[c]) return false;
             if constexpr (RJ) v.ref.gasCondensable = gas[c];
+            if constexpr (CAP) {
+                v.capGasCondensable = gas[c];
+                v.capExplicitGas = vapour >= 0;
+            }
         }
         R mg = 0;
         for (int k = 0; k < NS; ++k) mg += gas[k];
         v.gasMass = mg;
+        const R pg = gasPressure(p), pl = liquidPressure(p);
         if (mg > 0) {
             for (int k = 0; k < NS; ++k) gas[k] /= mg;
             S prop;
             R mu[NS];
-            if (phaseStatus(-1, p, T, gas, prop, virtualLiquids ? mu : nullptr) != kOk) return false;
+            if (phaseStatus(-1, pg, T, gas, prop, virtualLiquids ? mu : nullptr) != kOk) return false;
             if constexpr (RJ) v.ref.gasMolarVolume = prop.V;
-            accumulate(v, mg, prop, p);
+            accumulate(v, mg, prop, pg);
             v.alphaGas = mg / prop.rho;
             v.rhoGas = prop.rho;
             if constexpr (A) {
@@ -541,7 +595,7 @@ struct Flash {
                     ++n.phases;
                     st = cacheStatus;
                 } else {
-                    st = phaseStatus(0, p, T, nullptr, prop, virtualLiquids ? mu : nullptr);
+                    st = phaseStatus(0, pl, T, nullptr, prop, virtualLiquids ? mu : nullptr);
                     if (unavailable(st) && T >= liquidTmin && T < liquidTc * (R(1) - R(1e-10))) {
                         cacheP = p; cacheT = T; cacheVirtual = virtualLiquids; cacheStatus = st;
                     }
@@ -558,7 +612,7 @@ struct Flash {
                         v.ref.liquidMask |= 1;
                     }
                     if (mass0 > 0) {
-                        accumulate(v, mass0, prop, p);
+                        accumulate(v, mass0, prop, pl);
                         v.rhoLiq = prop.rho;
                         v.alphaLiq = mass0 / prop.rho;
                         v.active |= 1;
@@ -598,7 +652,8 @@ struct Flash {
             if (finishState<NS>(gasHeuristic(x, W, mx), T, prop, (const R*)nullptr) != kOk) return false;
         }
         const R gasP = prop.p;
-        p = gasP + R(0) * R(0);
+        if constexpr (CAP) p = gasP + this->color * this->jump;
+        else p = gasP + R(0) * R(0);
         if (!bounds(p, T)) return false;
         if (check) {
             S gas;
@@ -736,12 +791,26 @@ struct Flash {
     }
     // Residual [V-1, (E-e)/scale, mu_l/RT - mu_g/RT] in x = [ln p, ln T, beta]
     // (reactiveDeviceFlash.h:327-337).
+    // In vapour coordinates (CAP, this->adaptive) x[2] = ln(vapour fraction) <= 0.
     RAMD_NOINLINE bool residual(const R* x, R scale, R* f, E* out) {
         ++n.residuals;
-        if (!(x[2] >= 0 && x[2] < 1)) return false;
-        const R mass0 = x[2] * q[m.condensable];
+        R mass0, vapour = R(-1);
+        if constexpr (CAP) {
+            if (this->adaptive) {
+                if (!fin(x[2]) || x[2] > 0) return false;
+                vapour = m_exp(x[2]) * q[m.condensable];
+                if (!(vapour > 0)) return false;
+                mass0 = -m_expm1(x[2]) * q[m.condensable];
+            } else {
+                if (!(x[2] >= 0 && x[2] < 1)) return false;
+                mass0 = x[2] * q[m.condensable];
+            }
+        } else {
+            if (!(x[2] >= 0 && x[2] < 1)) return false;
+            mass0 = x[2] * q[m.condensable];
+        }
         E v;
-        if (!evaluate(mass0, m_exp(x[0]), m_exp(x[1]), v, true) || !(v.gasMass > 0)) return false;
+        if (!evaluate(mass0, m_exp(x[0]), m_exp(x[1]), v, true, vapour) || !(v.gasMass > 0)) return false;
         f[0] = v.volume - R(1);
         f[1] = (v.energy - energy) / scale;
         f[2] = v.muLiquid - v.muGas;
@@ -760,9 +829,31 @@ struct Flash {
                 for (int i = 0; i < 3; ++i) xp[i] = xm[i] = x[i];
                 xp[j] += h;
                 xm[j] -= h;
+                if constexpr (CAP) {
+                    if (this->adaptive) {  // a step lost to rounding moves by one ulp instead
+                        if (xp[j] == x[j]) xp[j] = ::nextafter(x[j], double(HUGE_VAL));
+                        if (xm[j] == x[j]) xm[j] = ::nextafter(x[j], -double(HUGE_VAL));
+                    }
+                }
                 const bool plus = residual(xp, scale, fp, nullptr), minus = residual(xm, scale, fm, nullptr);
                 if (!plus && !minus) continue;
                 success = true;
+                bool weighted = false;
+                if constexpr (CAP) weighted = this->adaptive;
+                if (weighted) {  // one-sided differences weighted by the actual steps (CAP: FP64)
+                    const R hp = xp[j] - x[j], hm = x[j] - xm[j];
+                    R maximum = 0;
+                    for (int i = 0; i < 3; ++i) {
+                        const R v = plus && minus ? (hm / (hp + hm)) * ((fp[i] - f[i]) / hp) +
+                                                        (hp / (hp + hm)) * ((f[i] - fm[i]) / hm)
+                                    : plus ? (fp[i] - f[i]) / hp : (f[i] - fm[i]) / hm;
+                        jac[i][j] = v;
+                        maximum = hi(maximum, ab(v));
+                        success = success && fin(v);
+                    }
+                    success = success && maximum > 0;
+                    continue;
+                }
                 for (int i = 0; i < 3; ++i) {
                     const R v = plus && minus ? (fp[i] - fm[i]) / (R(2) * h) : plus ? (fp[i] - f[i]) / h : (f[i] - fm[i]) / h;
                     jac[i][j] = v;
@@ -802,10 +893,28 @@ struct Flash {
             const double gc = v.ref.gasCondensable;
             if (!fin(gc) || gc < 0 || gc > q[c]) return false;
             Y[c] = gc;
-            if (!(q[c] > 0) || !(x[2] >= 0 && x[2] < 1)) return false;
-            const double product = x[2] * q[c];  // the residual's own two roundings (:35-47)
-            const double expectedGas = q[c] - product;
-            const double dLiquid = q[c];
+            // the phase pressures of a curved interface (zero: flat)
+            double pg = p, pl = p;
+            bool vapourCoordinates = false;
+            if constexpr (CAP) {
+                const double gp = gasPressure(p), lp = liquidPressure(p);
+                pg = gp == 0 ? p : gp;
+                pl = lp == 0 ? p : lp;
+                if (!(pg > 0) || !(pl > 0) || !fin(pg) || !fin(pl) || v.capExplicitGas != this->adaptive) return false;
+                vapourCoordinates = this->adaptive;
+            }
+            if (!(q[c] > 0)) return false;
+            double expectedGas, dLiquid;
+            if (vapourCoordinates) {
+                if (!(x[2] <= 0) || !fin(x[2])) return false;
+                expectedGas = m_exp(x[2]) * q[c];
+                dLiquid = -expectedGas;
+            } else {
+                if (!(x[2] >= 0 && x[2] < 1)) return false;
+                const double product = x[2] * q[c];  // the residual's own two roundings (:35-47)
+                expectedGas = q[c] - product;
+                dLiquid = q[c];
+            }
             if (!(expectedGas > 0) || gc != expectedGas) return false;
             const double Hl = v.ref.liquidHbar, Vl = v.ref.liquidVbar;
             if (!(v.ref.liquidMask & 1) || !fin(Hl) || !fin(Vl) || !(Vl > 0)) return false;
@@ -814,9 +923,9 @@ struct Flash {
             if (reconstructed != gm) return false;
             for (int k = 0; k < NS; ++k) Y[k] /= gm;
             GasPartials g;
-            if (!refActiveGasDerivatives(m.gas, T, p, V, Y, gm, c, g)) return false;
+            if (!refActiveGasDerivatives(m.gas, T, pg, V, Y, gm, c, g)) return false;
             const double Wk = m.gas.W[c], Wl = m.liquid.W[0];
-            const double liquidUbar = Hl - p * Vl;
+            const double liquidUbar = Hl - pl * Vl;
             jac[0][0] = v.Vp * p;
             jac[0][1] = v.VT * T;
             jac[1][0] = v.Ep * p / scale;
@@ -836,24 +945,42 @@ struct Flash {
     RAMD_NOINLINE bool jacobianFresh(const R* x, double p, double T, R scale, const E& v, R (&jac)[3][3]) {
         if constexpr (RJ) {
             if (!(p > 0) || !(T > 0) || !(scale > 0)) return false;
+            double pg = p, pl = p;
+            bool vapourCoordinates = false;
+            if constexpr (CAP) {
+                const double gp = gasPressure(p), lp = liquidPressure(p);
+                pg = gp == 0 ? p : gp;
+                pl = lp == 0 ? p : lp;
+                if (!(pg > 0) || !(pl > 0) || !fin(pg) || !fin(pl)) return false;
+                vapourCoordinates = this->adaptive;
+            }
             const int c = m.condensable;
             double gm[NS], Y[NS], total = 0;
             for (int k = 0; k < NS; ++k) gm[k] = q[k];
-            if (!(q[c] > 0) || !(x[2] >= 0 && x[2] < 1)) return false;
-            gm[c] -= x[2] * q[c];
-            const double dLiquid = q[c];
+            if (!(q[c] > 0)) return false;
+            double dLiquid;
+            if (vapourCoordinates) {
+                if (!(x[2] <= 0) || !fin(x[2])) return false;
+                gm[c] = m_exp(x[2]) * q[c];
+                if (!(gm[c] > 0)) return false;
+                dLiquid = -gm[c];
+            } else {
+                if (!(x[2] >= 0 && x[2] < 1)) return false;
+                gm[c] -= x[2] * q[c];
+                dLiquid = q[c];
+            }
             for (int k = 0; k < NS; ++k) total += gm[k];
             if (!(total > 0)) return false;
             for (int k = 0; k < NS; ++k) Y[k] = gm[k] / total;
             S gas;
             ++n.phases;
-            if (gasTP(T, p, Y, gas, nullptr) != kOk) return false;
+            if (gasTP(T, pg, Y, gas, nullptr) != kOk) return false;
             double hbar[NS], ubar[NS], vbar[NS], xm[NS], W = 0;
             if (!refPhasePartials(m.gas, T, Y, gas, hbar, ubar, vbar, xm, W)) return false;
             const double totalMoles = total / W;
             S liquid;
             ++n.phases;
-            if (liquidTP(T, p, liquid, nullptr) != kOk) return false;
+            if (liquidTP(T, pl, liquid, nullptr) != kOk) return false;
             double dpsi;
             if (!refGasCompositionPartial(m.gas, T, xm, totalMoles, gas, c, dpsi)) return false;
             const double Wk = m.gas.W[c], Wl = m.liquid.W[0];
@@ -876,7 +1003,8 @@ struct Flash {
     }
     // Start of an active-set solve: x = [ln p, ln T, seed fraction] and the
     // energy scale (reactiveDeviceFlash.h:374-390).
-    RD void activeStart(R seed, bool restart, R restartT, R* x, R& scale) const {
+    // False (CAP, vapour coordinates) where the start has no positive vapour.
+    RD bool activeStart(R seed, bool restart, R restartT, R* x, R& scale) const {
         const R rho = density();
         const R gT = restart ? restartT : guessT;
         scale = hi(R(1e5) * rho, hi(ab(energy), rho * R(2000) * hi(gT, Tmin)));
@@ -884,6 +1012,20 @@ struct Flash {
         x[1] = m_log(clamp(gT, Tmin, Tmax));
         const R previousFraction = guessLiquid / q[m.condensable];
         x[2] = seed < 0 ? clamp(previousFraction, R(P::fractionEdge), R(1) - R(P::fractionEdge)) : seed;
+        if constexpr (CAP) {
+            if (this->adaptive) {  // ln(vapour fraction); seed -3 (or almost no vapour) starts from a trace (:383-388)
+                double vapour = seed < 0 ? 1 - previousFraction : 1 - seed;
+                if (seed == R(-3) || (seed < 0 && vapour < 1e-6)) {
+                    double other = 0;
+                    for (int k = 0; k < NS; ++k)
+                        if (k != m.condensable) other += q[k];
+                    vapour = other > 0 ? lo(1e-6, .01 * (other / q[m.condensable])) : 1e-8;
+                }
+                if (!(vapour > 0 && fin(vapour))) return false;
+                x[2] = m_log(lo(1., vapour));
+            }
+        }
+        return true;
     }
     // Damped Newton on the one-liquid active set (reactiveDeviceFlash.h:391-448)
     // from x.  polish > 0 takes up to that many further Newton steps once the
@@ -947,7 +1089,9 @@ struct Flash {
             R rhs[3] = {-f[0], -f[1], -f[2]}, step[3];
             if (!solveN<3>(jac, rhs, step)) return polishing && activeFinish(x, scale, f, value);
             R limiter = hi(R(1), hi(ab(step[0]) / R(.7), ab(step[1]) / R(.18)));
-            limiter = hi(limiter, ab(step[2]) / R(.3));
+            bool vapourCoordinates = false;
+            if constexpr (CAP) vapourCoordinates = this->adaptive;
+            limiter = hi(limiter, ab(step[2]) / (vapourCoordinates ? R(2) : R(.3)));
             for (int j = 0; j < 3; ++j) step[j] /= limiter;
             bool accepted = false;
             const R previous = hi(hi(hi(R(0), ab(f[0])), ab(f[1])), ab(f[2]));
@@ -955,13 +1099,18 @@ struct Flash {
             for (int index = 0; index < 30; ++index, fraction *= R(.5)) {
                 R tx[3], tf[3];
                 for (int j = 0; j < 3; ++j) tx[j] = x[j] + fraction * step[j];
-                if (!(tx[2] >= 0 && tx[2] < 1)) {
+                // rejected partition coordinates (outside [0, 1), or positive or
+                // non-finite in vapour coordinates)
+                const auto outside = [vapourCoordinates](R y) {
+                    return vapourCoordinates ? (!fin(y) || y > 0) : !(y >= 0 && y < 1);
+                };
+                if (outside(tx[2])) {
                     // rejected fractions form a prefix; bisect to its end (:419-438)
                     int lower = index, upper = 30;
                     while (upper - lower > 1) {
                         const int mid = (lower + upper) / 2;
                         const R y = x[2] + R(::ldexp(1.0, -mid)) * step[2];
-                        if (!(y >= 0 && y < 1)) lower = mid; else upper = mid;
+                        if (outside(y)) lower = mid; else upper = mid;
                     }
                     n.residuals += uint32_t(upper - index);
                     if (upper == 30) break;
@@ -990,7 +1139,7 @@ struct Flash {
     RAMD_NOINLINE bool activeFlash(R seed, E& value, bool restart, R restartT) {
         ++n.candidates;
         R x[3], scale;
-        activeStart(seed, restart, restartT, x, scale);
+        if (!activeStart(seed, restart, restartT, x, scale)) return false;
         return activeNewton(x, scale, value, 0);
     }
     // Stability (reactiveDeviceFlash.h:449-485) for at most one liquid.
@@ -1001,8 +1150,11 @@ struct Flash {
         const R p = value.p, T = value.T;
         if (value.gasMass > 0) {
             E comparison;
+            R vapour = R(-1);  // the candidate's own gas inventory in vapour coordinates (:452)
+            if constexpr (CAP)
+                if (value.capExplicitGas) vapour = value.capGasCondensable;
             if (evaluated) comparison = value;
-            else if (!evaluate(value.liqMass, p, T, comparison, true)) return false;
+            else if (!evaluate(value.liqMass, p, T, comparison, true, vapour)) return false;
             if (!NL || q[m.condensable] == 0) return true;
             const R affinity = comparison.muLiquid - comparison.muGas;
             if (value.liqMass > 0) {
@@ -1013,7 +1165,7 @@ struct Flash {
         if (!NL || !(value.liqMass > 0)) return false;
         S prop;
         R mu[1];
-        if (phaseStatus(0, p, T, nullptr, prop, mu) != kOk) return false;
+        if (phaseStatus(0, liquidPressure(p), T, nullptr, prop, mu) != kOk) return false;
         const R muL = mu[0] / (R(8314.46261815324) * T);
         // Tangent-plane distance of the pure vapour of the condensable (:474-485, n = 1).
         R mole[NS], Y[NS], sum = 0;
@@ -1022,7 +1174,7 @@ struct Flash {
         for (int k = 0; k < NS; ++k) { Y[k] = mole[k] * R(m.weights[k]); sum += Y[k]; }
         for (int k = 0; k < NS; ++k) Y[k] /= sum;
         S gas;
-        const int st = phaseStatus(-1, p, T, Y, gas, nullptr);
+        const int st = phaseStatus(-1, gasPressure(p), T, Y, gas, nullptr);
         R tangent;
         if (st != kOk) tangent = unavailable(st) ? R(Real<R>::big) * R(2) : R(0) / R(0);
         else {
@@ -1049,12 +1201,16 @@ struct Flash {
     RD bool restartRange() const {
         return lo(Tmax, liquidTc * (R(1) - R(1e-10))) > hi(Tmin, liquidTmin);
     }
-    // Candidate search (reactiveDeviceFlash.h:490-543).
+    // Candidate search (reactiveDeviceFlash.h:490-543); in vapour
+    // coordinates (CAP) without the stable-gas pruning, and with the
+    // trace-vapour start after the five seeds.
     RAMD_NOINLINE bool search(E& best) {
         bool have = false;
         E candidate;
         bool gasStable = false;
-        const bool prunable = m.stableGasPrune != 0;
+        bool vapourCoordinates = false;
+        if constexpr (CAP) vapourCoordinates = this->adaptive;
+        const bool prunable = m.stableGasPrune != 0 && !vapourCoordinates;
         if (frozen(R(0), candidate, prunable ? 4 : 0)) {
             const uint32_t before = n.stable;
             consider(candidate, best, have);
@@ -1073,11 +1229,16 @@ struct Flash {
             else ++n.failures;
         }
         const R seeds[5] = {R(-1), R(.5), R(.95), R(.1), R(.9999)};
-        if (!prune && q[c] > 0)
+        if (!prune && q[c] > 0) {
             for (int j = 0; j < 5; ++j) {
                 if (activeFlash(seeds[j], candidate, false, R(0))) consider(candidate, best, have);
                 else ++n.failures;
             }
+            if (vapourCoordinates) {  // the trace-vapour start (:532)
+                if (activeFlash(R(-3), candidate, false, R(0))) consider(candidate, best, have);
+                else ++n.failures;
+            }
+        }
         if (!have && q[c] > 0 && restartRange())
             for (int t = 0; t < 4; ++t) {
                 const R restartT = restartTemperature(t);
@@ -1088,11 +1249,41 @@ struct Flash {
             }
         return have;
     }
-    RD bool run(E& result) {  // :544-560, boundaryRecovery = 0
+    // :544-560.  CAP: with boundaryRecovery, or for a capillary equilibrium
+    // cell (capillaryBoundary), a result whose liquid holds nearly all of the
+    // condensable with gas present, or no result, runs the search again in
+    // vapour coordinates and the higher entropy density wins.
+    RD bool run(E& result, bool capillaryBoundary = false) {
         if (!fin(energy) || !(density() > 0)) return false;
         for (int k = 0; k < NS; ++k)
             if (!fin(q[k]) || q[k] < 0) return false;
-        return search(result);
+        if constexpr (!CAP) {
+            return search(result);
+        } else {
+            E reference;
+            zeroEval(reference);
+            const bool have = search(reference);
+            if (!m.boundaryRecovery && !capillaryBoundary) {
+                if (have) result = reference;
+                return have;
+            }
+            if (have) {
+                const bool boundary =
+                    NL && q[m.condensable] > 0 && reference.gasMass > 0 && reference.liqMass / q[m.condensable] > 1 - 1e-4;
+                if (!boundary) {
+                    result = reference;
+                    return true;
+                }
+            }
+            this->adaptive = true;
+            E recovered;
+            zeroEval(recovered);
+            const bool found = search(recovered);
+            this->adaptive = false;
+            if (!found && !have) return false;
+            result = have && (!found || reference.entropyDensity >= recovered.entropyDensity) ? reference : recovered;
+            return true;
+        }
     }
 
     // ---- single-precision search (R = float): the candidate search with the

@@ -66,11 +66,14 @@ __device__ inline void storeHemCell(bool ok, const E& result, HemOutputDev& o) {
 }
 
 // Without capillary input: equilibrium UV flash (recover,
-// reactiveDeviceFlash.h:564-567).  With it (validated to color = jump = 0,
-// equilibrium = 0): frozen inventory at the seed's liquid masses
-// (recoverCapillary, reactiveDeviceFlash.h:572-590).  RJ: the model's
-// analytic-Jacobian mode (analyticJacobian = 1).
-template <int NS, int NL, bool RJ>
+// reactiveDeviceFlash.h:564-567).  With it (recoverCapillary, :572-590): the
+// cell's interface (color, pressure jump) held during the solve, then the
+// UV equilibrium with the boundary search (equilibrium = 1) or the frozen
+// inventory at the seed's liquid masses (0); a cell whose color or jump the
+// reference rejects (its status 2) fails.  RJ: the model's analytic-Jacobian
+// mode (analyticJacobian = 1); CAP: the capillary / boundary-search flash,
+// for capillary input and boundaryRecovery models.
+template <int NS, int NL, bool RJ, bool CAP>
 __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void hemFlashKernel(const HemModel<NS>* __restrict__ globalModel,
                                                      const double* __restrict__ q, const double* __restrict__ energy,
                                                      const ThermoOut* __restrict__ guess,
@@ -86,7 +89,7 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void hemFlashKernel(const
     uint32_t succeeded = 0;
     if (i < count) {
         const uint32_t c = order ? order[i] : i;
-        using F = Flash<NS, NL, double, false, RJ>;
+        using F = Flash<NS, NL, double, false, RJ, CAP>;
         F f(model, n);
         for (int k = 0; k < NS; ++k) f.q[k] = q[size_t(c) * NS + k];
         f.energy = energy[c];
@@ -97,8 +100,24 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void hemFlashKernel(const
         typename F::E result;
         zeroEval(result);
         bool ok;
-        if (capillary) ok = f.frozen(NL ? g.liquidMass[0] : 0.0, result, 0, g.liquidMass[1]);
-        else ok = f.run(result);
+        if (capillary) {
+            const HemCapillaryDev& x = capillary[c];
+            if constexpr (CAP) {
+                if (!fin(x.color) || x.color < 0 || x.color > 1 || !fin(x.pressureJump) ||
+                    (x.pressureJump != 0 && NL != 1)) {
+                    ok = false;
+                } else {
+                    f.color = x.color;
+                    f.jump = x.pressureJump;
+                    ok = x.equilibrium != 0 ? f.run(result, true)
+                                            : f.frozen(NL ? g.liquidMass[0] : 0.0, result, 0, g.liquidMass[1]);
+                }
+            } else {
+                ok = f.frozen(NL ? g.liquidMass[0] : 0.0, result, 0, g.liquidMass[1]);
+            }
+        } else {
+            ok = f.run(result);
+        }
         storeHemCell(ok, result, out[c]);
         succeeded = ok ? 1u : 0u;
         if (cellPhases) cellPhases[c] = n.phases;
@@ -183,28 +202,6 @@ __global__ void clearCountKernel(unsigned* __restrict__ count) {
     if (threadIdx.x == 0) *count = 0;
 }
 
-// Counts the ordered cells whose capillary input this build cannot run
-// (non-finite or out-of-range fields, an interface, or equilibrium = 1).
-// With partials (64-lane workgroups), each workgroup writes its count to its
-// own slot instead of adding it to *rejected.
-__global__ __launch_bounds__(256) void capillaryCheckKernel(const HemCapillaryDev* __restrict__ capillary,
-                                                            const uint32_t* __restrict__ order, uint32_t count,
-                                                            unsigned long long* __restrict__ rejected,
-                                                            unsigned long long* __restrict__ partials) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t bad = 0;
-    if (i < count) {
-        const HemCapillaryDev& x = capillary[order[i]];
-        bad = !(fin(x.color) && x.color == 0 && fin(x.pressureJump) && x.pressureJump == 0 && x.equilibrium == 0);
-    }
-    const uint32_t sum = waveSum(bad);
-    if (partials) {
-        if (threadIdx.x == 0) partials[blockIdx.x] = sum;
-    } else if ((threadIdx.x & 63) == 0 && sum) {
-        atomicAdd(rejected, (unsigned long long)sum);
-    }
-}
-
 // totals[k] = sum over b < blocks of partials[b * width + k], k < width: one
 // 64-lane workgroup, a fixed summation order (no atomics, nothing cleared).
 __global__ __launch_bounds__(64) void reducePartialsKernel(const unsigned long long* __restrict__ partials,
@@ -274,16 +271,21 @@ void HemHandle::launch(const double* q, const double* e, const ThermoOut* g, con
     if (!count) return;
     const unsigned blocks = unsigned((count + 63) / 64);
     if (!partials) hipLaunchKernelGGL(zeroTotalsKernel, dim3(1), dim3(64), 0, s, totals, 8);
-#define RAMD_HEM_LAUNCH(NS_, NL_, RJ_)                                                                             \
-    hipLaunchKernelGGL((hemFlashKernel<NS_, NL_, RJ_>), dim3(blocks), dim3(64), 0, s,                              \
+#define RAMD_HEM_LAUNCH(NS_, NL_, RJ_, CAP_)                                                                       \
+    hipLaunchKernelGGL((hemFlashKernel<NS_, NL_, RJ_, CAP_>), dim3(blocks), dim3(64), 0, s,                        \
                        static_cast<const HemModel<NS_>*>(model), q, e, g, order, uint32_t(count), capillary, out, totals, \
                        cellPhases, partials)
-    if (ns == 1 && nl == 0) RAMD_HEM_LAUNCH(1, 0, false);
-    else if (ns == 1 && !analytic) RAMD_HEM_LAUNCH(1, 1, false);
-    else if (ns == 1) RAMD_HEM_LAUNCH(1, 1, true);
-    else if (nl == 0) RAMD_HEM_LAUNCH(2, 0, false);
-    else if (!analytic) RAMD_HEM_LAUNCH(2, 1, false);
-    else RAMD_HEM_LAUNCH(2, 1, true);
+#define RAMD_HEM_SPECIES(CAP_)                                                                                    \
+    if (ns == 1 && nl == 0) RAMD_HEM_LAUNCH(1, 0, false, CAP_);                                                   \
+    else if (ns == 1 && !analytic) RAMD_HEM_LAUNCH(1, 1, false, CAP_);                                            \
+    else if (ns == 1) RAMD_HEM_LAUNCH(1, 1, true, CAP_);                                                          \
+    else if (nl == 0) RAMD_HEM_LAUNCH(2, 0, false, CAP_);                                                         \
+    else if (!analytic) RAMD_HEM_LAUNCH(2, 1, false, CAP_);                                                       \
+    else RAMD_HEM_LAUNCH(2, 1, true, CAP_);
+    // the capillary / boundary-search flash for interface input and boundaryRecovery models
+    if (capillary || boundary) { RAMD_HEM_SPECIES(true) }
+    else { RAMD_HEM_SPECIES(false) }
+#undef RAMD_HEM_SPECIES
 #undef RAMD_HEM_LAUNCH
     RAMD_HIP(hipGetLastError());
 }
@@ -335,7 +337,6 @@ HemHandle* createHem(const void* raw, size_t bytes, size_t capacity, bool hostPa
     require(m.ns == 1 || m.ns == 2, "This HEM build supports 1 or 2 gas species");
     require(m.nl <= 1, "This HEM build supports at most one pure liquid");
     require(!(m.nl && m.condensedKind[0]), "Solid condensed phases are not provided by this HEM build");
-    require(!m.boundaryRecovery, "Boundary-recovery (adaptive vapour) search is not provided by this HEM build");
     int devices = 0;
     RAMD_HIP(hipGetDeviceCount(&devices));
     require(devices > 0, "No HIP device");
@@ -343,6 +344,7 @@ HemHandle* createHem(const void* raw, size_t bytes, size_t capacity, bool hostPa
     h->ns = m.ns;
     h->nl = m.nl;
     h->analytic = m.analyticJacobian != 0;
+    h->boundary = m.boundaryRecovery != 0;
     h->arithmetic = defaultArithmetic();
     h->capacity = capacity;
     h->device = currentDevice();
@@ -377,11 +379,7 @@ int runHost(HemHandle* h, const double* q, const double* energy, const ReactiveG
             const auto& x = capillary[c];
             require(fin(x.color) && x.color >= 0 && x.color <= 1 && fin(x.pressureJump) &&
                         (x.equilibrium == 0 || x.equilibrium == 1),
-                    "Invalid HIP HEM capillary cell input");
-            require(x.color == 0 && x.pressureJump == 0,
-                    "Capillary interfaces (color, pressureJump != 0) are not provided by this HEM build");
-            require(x.equilibrium == 0,
-                    "Capillary equilibrium cells need the boundary-recovery search, not provided by this HEM build");
+                    "Invalid HIP HEM capillary cell input");  // reactiveGpuHem.cuh:237-243
         }
     }
     RAMD_HIP(hipSetDevice(h->device));
@@ -409,7 +407,8 @@ int runHost(HemHandle* h, const double* q, const double* energy, const ReactiveG
     if (capillary)
         RAMD_HIP(hipMemcpyAsync(h->dcap, capillary, count * sizeof(HemCapillaryDev), hipMemcpyHostToDevice, s));
     RAMD_HIP(hipEventRecord(h->events[1], s));
-    const bool mixed = !capillary && h->arithmetic == REACTIVE_AMD_FLASH_MIXED;
+    // the mixed arithmetic runs the equilibrium flash without the boundary search
+    const bool mixed = !capillary && !h->boundary && h->arithmetic == REACTIVE_AMD_FLASH_MIXED;
     if (mixed) h->launchMixed(h->dq, h->de, h->dguess, h->dorder, count, h->dout, s);
     else h->launch(h->dq, h->de, h->dguess, h->dorder, count, capillary ? h->dcap : nullptr, h->dout, nullptr, s);
     RAMD_HIP(hipEventRecord(h->events[2], s));
@@ -579,19 +578,10 @@ int reactive_gpu_hem_run_resident_v1(void* raw, const double* q, const double* e
         const auto begin = std::chrono::steady_clock::now();
         RAMD_HIP(hipSetDevice(h->device));
         hipStream_t s = h->stream;
-        if (!h->rejected) h->rejected = static_cast<unsigned long long*>(h->allocate(sizeof(unsigned long long)));
-        // every input is device memory indexed by global cell; reject the batch
-        // before any flash runs if a cell needs what this build does not provide
+        // every input is device memory indexed by global cell; as in the
+        // reference (reactiveGpuHem.cuh:360-380), a cell whose interface input
+        // recoverCapillary rejects fails on its own
         const auto* cap = reinterpret_cast<const HemCapillaryDev*>(capillary);
-        hipLaunchKernelGGL(zeroTotalsKernel, dim3(1), dim3(64), 0, s, h->rejected, 1);
-        hipLaunchKernelGGL(capillaryCheckKernel, dim3(unsigned((count + 255) / 256)), dim3(256), 0, s, cap, order,
-                           uint32_t(count), h->rejected, nullptr);
-        RAMD_HIP(hipGetLastError());
-        unsigned long long rejected = 0;
-        RAMD_HIP(hipMemcpyAsync(&rejected, h->rejected, sizeof(rejected), hipMemcpyDeviceToHost, s));
-        RAMD_HIP(hipStreamSynchronize(s));
-        require(rejected == 0, "Resident HIP HEM batch holds cells with an interface (color or pressureJump != 0), "
-                               "equilibrium = 1 or invalid capillary input; this build runs frozen-inventory cells only");
         RAMD_HIP(hipEventRecord(h->events[0], s));
         h->launch(q, energy, reinterpret_cast<const ThermoOut*>(seed), order, count, cap,
                   reinterpret_cast<HemOutputDev*>(output), nullptr, s);
@@ -616,7 +606,7 @@ int reactive_gpu_hem_run_resident_v1(void* raw, const double* q, const double* e
         p.stableCandidates = totals[3];
         p.finiteDifferenceJacobians = totals[5];
         p.analyticJacobians = totals[7];
-        p.transferBytes = sizeof(totals) + sizeof(rejected);
+        p.transferBytes = sizeof(totals);
         p.deviceBytes = h->deviceBytes;
         p.wallSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - begin).count();
         p.kernelSeconds = kernel / 1000.;
@@ -651,16 +641,6 @@ int reactive_gpu_hem_run_resident_noalloc_v1(void* raw, const double* q, const d
         hipStream_t s = h->stream;
         const unsigned blocks = unsigned((count + 63) / 64);
         const auto* cap = reinterpret_cast<const HemCapillaryDev*>(capillary);
-        // the capillary check: one count per 64-cell workgroup, then their sum
-        hipLaunchKernelGGL(capillaryCheckKernel, dim3(blocks), dim3(64), 0, s, cap, order, uint32_t(count), nullptr,
-                           partials);
-        hipLaunchKernelGGL(reducePartialsKernel, dim3(1), dim3(64), 0, s, partials, blocks, 1, totalsDev);
-        RAMD_HIP(hipGetLastError());
-        unsigned long long rejected = 0;
-        RAMD_HIP(hipMemcpyAsync(&rejected, totalsDev, sizeof(rejected), hipMemcpyDeviceToHost, s));
-        RAMD_HIP(hipStreamSynchronize(s));
-        require(rejected == 0, "Resident HIP HEM batch holds cells with an interface (color or pressureJump != 0), "
-                               "equilibrium = 1 or invalid capillary input; this build runs frozen-inventory cells only");
         RAMD_HIP(hipEventRecord(h->events[0], s));
         h->launch(q, energy, reinterpret_cast<const ThermoOut*>(seed), order, count, cap,
                   reinterpret_cast<HemOutputDev*>(output), nullptr, s, partials);
@@ -687,7 +667,7 @@ int reactive_gpu_hem_run_resident_noalloc_v1(void* raw, const double* q, const d
         p.stableCandidates = totals[3];
         p.finiteDifferenceJacobians = totals[5];
         p.analyticJacobians = totals[7];
-        p.transferBytes = sizeof(totals) + sizeof(rejected);
+        p.transferBytes = sizeof(totals);
         p.deviceBytes = h->deviceBytes;
         p.wallSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - begin).count();
         p.kernelSeconds = kernel / 1000.;

@@ -16,9 +16,8 @@ struct HemOutputDev {
 };
 
 // Per-cell capillary input, byte-compatible with ReactiveGpuHemCapillaryInputV2
-// (24 B).  This build runs cells with color = pressureJump = 0 only:
-// equilibrium 0 is the frozen-inventory recovery; equilibrium 1 would need the
-// boundary-recovery search and is rejected before any flash runs.
+// (24 B): the interface's color and pressure jump, and equilibrium 1 for the
+// UV equilibrium with the boundary search, 0 for the frozen inventory.
 struct HemCapillaryDev {
     double color, pressureJump;
     int equilibrium, pad;
@@ -27,6 +26,7 @@ struct HemCapillaryDev {
 struct HemHandle {
     int ns = 0, nl = 0, device = 0;
     bool analytic = false;  // the model's analytic-Jacobian mode (FP64 flash)
+    bool boundary = false;  // the model's boundary search (boundaryRecovery = 1)
     int arithmetic = 0;     // equilibrium flash of host batches: REACTIVE_AMD_FLASH_FP64 or _MIXED
     size_t capacity = 0, deviceBytes = 0, modelBytes = 0;
     void* model = nullptr;  // HemModel<ns> in device memory
@@ -47,7 +47,6 @@ struct HemHandle {
     HemOutputDev* dout = nullptr;
     uint32_t* dorder = nullptr;
     HemCapillaryDev* dcap = nullptr;      // host-path capillary staging
-    unsigned long long* rejected = nullptr;  // resident batches: unsupported capillary cells
     std::vector<uint32_t> hostOrder;
     std::vector<uint8_t> hostKey;
     std::vector<HemOutputDev> hostOut;

@@ -141,6 +141,7 @@ inline void buildDeviceModel(const Model& m, HemModel<NS>& d) {
     d.scalarRecovery = m.scalarRecovery;
     d.stableGasPrune = m.stableGasPrune;
     d.analyticJacobian = m.analyticJacobian;
+    d.boundaryRecovery = m.boundaryRecovery;
     d.enforceBounds = m.enforceSpeciesTemperatureBounds;
     for (int k = 0; k < NS; ++k) d.weights[k] = m.weights[k];
     d.liquidTmin = m.nl ? m.liquidTmin[0] : 0;

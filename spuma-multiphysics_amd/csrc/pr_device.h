@@ -47,6 +47,7 @@ RHD double m_third(double x) { return elem::powThird(x); }  // pow(x, 1.0 / 3.0)
 RHD double m_acos(double x) { return elem::acos(x); }
 RHD double m_cos(double x) { return elem::cos(x); }
 RHD double m_sqrt(double x) { return ::sqrt(x); }
+RHD double m_expm1(double x) { return elem::expm1(x); }
 RHD float m_log(float x) { return ::logf(x); }
 RHD float m_exp(float x) { return ::expf(x); }
 RHD float m_third(float x) { return ::cbrtf(x); }

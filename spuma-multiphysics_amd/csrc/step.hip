@@ -1101,6 +1101,8 @@ namespace ramd {
 // HEM handle's device with its species.
 static Step* createStep(const std::vector<Transport*>& list, HemHandle* hem) {
     require(!list.empty() && hem, "Null transport or HEM handle");
+    require(!hem->boundary, "The resident step's flash has no boundary search (boundaryRecovery = 1): run such "
+                            "models through reactive_gpu_hem_run_v1");
     std::unique_ptr<Step> s(new Step);
     s->t = list[0];
     s->h = hem;

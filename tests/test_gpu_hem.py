@@ -137,14 +137,24 @@ def test_resident_frozen_batch_matches_host_batch_and_oracle():
     good = out["success"][order] == 1
     assert out["state"][order][good].tobytes() == ref[True][0][good].tobytes()
     assert_reference_parity(out["state"][order], out["success"][order], ref[False][0], ref[False][1])
-    # a cell asking for the equilibrium (boundary-recovery) search: rejected, nothing written
+    # cells asking for the capillary equilibrium recovery and one with an
+    # interface the reference rejects (a jump without a liquid is fine here;
+    # color 2 is not): the first as the oracle recovers them, the last failed
+    from helpers import oracle_capillary
     cap2 = cap.copy()
-    cap2["equilibrium"][order[3]] = 1
+    cap2["equilibrium"][order[3]] = cap2["equilibrium"][order[5]] = 1
+    cap2["color"][order[5]], cap2["pressureJump"][order[5]] = 0.4, 3e3
+    cap2["color"][order[7]] = 2.0
     dc2 = dev.put(cap2)
     dout2 = dev.put(blank)
     rc = lib.reactive_gpu_hem_run_resident_v1(r, dq, de, ds, dc2, do, len(order), dout2, C.byref(rprof), err, 1024)
-    assert rc == 1 and b"frozen-inventory" in err.value
-    assert not dev.get(dout2, blank).view(np.uint8).any()
+    assert rc == 0, err.value
+    out2 = dev.get(dout2, blank)
+    ts, tok, _ = oracle_capillary(mb, q[order], e[order], seeds[order], cap2["color"][order],
+                                  cap2["pressureJump"][order], cap2["equilibrium"][order], twin=True)
+    assert np.array_equal(out2["success"][order], tok) and tok[7] == 0
+    good = tok == 1
+    assert out2["state"][order][good].tobytes() == ts[good].tobytes()
     dev.close()
     lib.reactive_gpu_hem_destroy_v1(r)
 
