@@ -289,7 +289,20 @@ struct FlashCap<true, R> {
     bool adaptive = false;  // vapour-log coordinates of the boundary search
 };
 
-template <int NS, int NL, class R = double, bool A = FlashPrecision<R>::single, bool RJ = false, bool CAP = false>
+// The workgroup's copy of the model image in LDS (filled by the kernels'
+// workgroupModel): a flash built with LDS reads its tables through this
+// object, whose address space the compiler knows in every solver function,
+// so the reads are LDS loads rather than generic (flat) ones.
+template <int NS>
+__device__ inline HemModel<NS>& sharedModel() {
+    __shared__ HemModel<NS> local;
+    return local;
+}
+
+// LDS: the model is the workgroup's LDS copy (sharedModel); otherwise the
+// reference given to the constructor.
+template <int NS, int NL, class R = double, bool A = FlashPrecision<R>::single, bool RJ = false, bool CAP = false,
+          bool LDS = false>
 struct Flash : FlashCap<CAP, R> {
     static_assert(NL == 0 || NL == 1, "one pure liquid at most");
     static_assert(!RJ || (!A && !FlashPrecision<R>::single), "the reference's analytic mode is an FP64 flash");
@@ -299,6 +312,10 @@ struct Flash : FlashCap<CAP, R> {
     using S = PRStateT<R>;
     using M = MixT<R>;
     const HemModel<NS>& m;
+    RD const HemModel<NS>& mod() const {
+        if constexpr (LDS) return sharedModel<NS>();
+        else return m;
+    }
     R q[NS];
     R energy;
     R guessP, guessT, guessLiquid;
@@ -319,11 +336,11 @@ struct Flash : FlashCap<CAP, R> {
         cacheStatus = -1;
         cacheVirtual = false;
         memoValid = false;
-        Tmin = R(m.Tmin); Tmax = R(m.Tmax); pmin = R(m.pmin); pmax = R(m.pmax);
-        vtol = hi(R(m.vtol), R(P::vtolFloor));
-        etol = hi(R(m.etol), R(P::etolFloor));
-        mutol = hi(R(m.mutol), R(P::mutolFloor));
-        liquidTmin = R(m.liquidTmin); liquidTc = R(m.liquidTc);
+        Tmin = R(mod().Tmin); Tmax = R(mod().Tmax); pmin = R(mod().pmin); pmax = R(mod().pmax);
+        vtol = hi(R(mod().vtol), R(P::vtolFloor));
+        etol = hi(R(mod().etol), R(P::etolFloor));
+        mutol = hi(R(mod().mutol), R(P::mutolFloor));
+        liquidTmin = R(mod().liquidTmin); liquidTc = R(mod().liquidTc);
         record = nullptr;
         stalled = false;
     }
@@ -348,12 +365,12 @@ struct Flash : FlashCap<CAP, R> {
     static RD bool unavailable(int s) { return s == kUnavailable || s == kNoRoot; }
 
     RD const PhaseHeuristic& gasHeuristic(const R* x, R W, const M& mx) {
-        if (NS == 1) return m.gasFixed;
+        if (NS == 1) return mod().gasFixed;
         bool hit = memoValid;
         for (int k = 0; k < NS; ++k)
             hit = hit && (P::single ? ab(memoKey[k] - x[k]) <= R(P::heuristicReuse) : memoKey[k] == x[k]);
         if (!hit) {
-            phaseHeuristic<NS, R>(m.gas, x, W, mx.a, mx.b, memo);
+            phaseHeuristic<NS, R>(mod().gas, x, W, mx.a, mx.b, memo);
             for (int k = 0; k < NS; ++k) memoKey[k] = x[k];
             memoValid = true;
         }
@@ -365,16 +382,16 @@ struct Flash : FlashCap<CAP, R> {
     __device__ __attribute__((always_inline)) inline int gasTPBody(R T, R p, const R* Y, S& s, R* mu) {
         if (!fin(T) || !fin(p) || T <= 0 || p <= 0) return kBadInput;
         R x[NS], W = 0;
-        int st = massToMole(m.gas, Y, x, W);
+        int st = massToMole(mod().gas, Y, x, W);
         if (st != kOk) return st;
         M mx;
-        st = mixing(m.gas, T, x, mx);
+        st = mixing(mod().gas, T, x, mx);
         if (st != kOk) return st;
         RootsT<R> r;
         st = roots(T, p, mx.a, mx.b, mx.aa, r);
         if (st != kOk) return st;
         if (!r.gas) return kUnavailable;
-        st = stateTrho(m.gas, T, W / r.vLast, x, W, mx, s, mu);
+        st = stateTrho(mod().gas, T, W / r.vLast, x, W, mx, s, mu);
         if (st != kOk) return st;
         st = finishState<NS>(gasHeuristic(x, W, mx), T, s, mu);
         if (st != kOk) return st;
@@ -389,18 +406,18 @@ struct Flash : FlashCap<CAP, R> {
         if (!fin(T) || !fin(p) || T <= 0 || p <= 0) return kBadInput;
         const R Y[1] = {R(1)};
         R x[1], W = 0;
-        int st = massToMole(m.liquid, Y, x, W);
+        int st = massToMole(mod().liquid, Y, x, W);
         if (st != kOk) return st;
         M mx;
-        st = mixing(m.liquid, T, x, mx);
+        st = mixing(mod().liquid, T, x, mx);
         if (st != kOk) return st;
         RootsT<R> r;
         st = roots(T, p, mx.a, mx.b, mx.aa, r);
         if (st != kOk) return st;
         if (!r.liquid) return kUnavailable;
-        st = stateTrho(m.liquid, T, W / r.v0, x, W, mx, s, mu);
+        st = stateTrho(mod().liquid, T, W / r.v0, x, W, mx, s, mu);
         if (st != kOk) return st;
-        st = finishState<1>(m.liquidFixed, T, s, mu);
+        st = finishState<1>(mod().liquidFixed, T, s, mu);
         if (st != kOk) return st;
         if (ab(s.p - p) > R(P::rootCheck) * hi(R(1), p)) return kNonFinite;
         s.p = p;
@@ -414,9 +431,9 @@ struct Flash : FlashCap<CAP, R> {
         if (!bounds(p, T)) return kBadInput;
         if (index >= 0 && T < liquidTmin) return kBadInput;
         if (index >= 0 && T >= liquidTc * (R(1) - R(1e-10))) return kUnavailable;
-        if (index < 0 && m.enforceBounds)
+        if (index < 0 && mod().enforceBounds)
             for (int k = 0; k < NS; ++k)
-                if (Y[k] > 0 && (T < R(m.gas.tlo[k]) || T > R(m.gas.thi[k]))) return kBadInput;
+                if (Y[k] > 0 && (T < R(mod().gas.tlo[k]) || T > R(mod().gas.thi[k]))) return kBadInput;
         if constexpr (P::single ? RAMD_SEARCH_INLINE != 0 : RAMD_FLASH_INLINE != 0) {
             if (index >= 0) return liquidTPBody(T, p, out, mu);
             return gasTPBody(T, p, Y, out, mu);
@@ -455,8 +472,8 @@ struct Flash : FlashCap<CAP, R> {
     // mu_c is the reference's gas chemical potential (reactiveDevicePR.h:900-916),
     // whose (T, p, n) derivatives these are.
     RD void gasPartials(R p, R T, R mg, const R* Y, const S& s, PhaseParts<R, true>& out) const {
-        const int c = m.condensable;
-        const auto& t = m.gas;
+        const int c = mod().condensable;
+        const auto& t = mod().gas;
         R x[NS], W = 0;
         if (massToMole(t, Y, x, W) != kOk) { out.muN = R(Real<R>::big) * R(2); return; }
         R qv[NS], dq[NS];
@@ -510,7 +527,7 @@ struct Flash : FlashCap<CAP, R> {
     // and, in the liquid amount, from the gas composition's change (muN; a
     // pure gas's mu does not depend on the phase amounts).
     RD void analyticJacobian(const E& v, R scale, R (&jac)[3][3]) const {
-        const R qc = q[m.condensable];
+        const R qc = q[mod().condensable];
         jac[0][0] = v.Vp * v.p;
         jac[0][1] = v.VT * v.T;
         jac[0][2] = qc * (v.parts.vL - v.parts.vG);
@@ -554,7 +571,7 @@ struct Flash : FlashCap<CAP, R> {
             if (!fin(q[k]) || q[k] < 0) return false;
             gas[k] = q[k];
         }
-        const int c = NL ? m.condensable : 0;
+        const int c = NL ? mod().condensable : 0;
         if (NL) {
             if (!fin(mass0) || mass0 < 0 || mass0 > q[c]) return false;
             if constexpr (CAP) gas[c] = vapour >= 0 ? vapour : gas[c] - mass0;
@@ -607,7 +624,7 @@ struct Flash : FlashCap<CAP, R> {
                     if (virtualLiquids) v.muLiquid = mu[0] / (R(8314.46261815324) * T);
                     if constexpr (A) recordPhase(prop, p, T, v.parts.vL, v.parts.eL, v.parts.hL, v.parts.zL);
                     if constexpr (RJ) {
-                        v.ref.liquidHbar = prop.h * m.liquid.W[0];
+                        v.ref.liquidHbar = prop.h * mod().liquid.W[0];
                         v.ref.liquidVbar = prop.V;
                         v.ref.liquidMask |= 1;
                     }
@@ -645,10 +662,10 @@ struct Flash : FlashCap<CAP, R> {
         {
             if (!fin(T) || !fin(rho) || T <= 0 || rho <= 0) return false;
             R x[NS], W = 0;
-            if (massToMole(m.gas, Y, x, W) != kOk) return false;
+            if (massToMole(mod().gas, Y, x, W) != kOk) return false;
             M mx;
-            if (mixing(m.gas, T, x, mx) != kOk) return false;
-            if (stateTrho(m.gas, T, rho, x, W, mx, prop, (R*)nullptr) != kOk) return false;
+            if (mixing(mod().gas, T, x, mx) != kOk) return false;
+            if (stateTrho(mod().gas, T, rho, x, W, mx, prop, (R*)nullptr) != kOk) return false;
             if (finishState<NS>(gasHeuristic(x, W, mx), T, prop, (const R*)nullptr) != kOk) return false;
         }
         const R gasP = prop.p;
@@ -734,7 +751,7 @@ struct Flash : FlashCap<CAP, R> {
     RAMD_NOINLINE bool frozen(R mass0, E& value, int stallLimit, R mass1 = R(0), int polish = 0) {
         stalled = false;
         if (!fin(energy)) return false;
-        if (m.scalarRecovery && mass0 == 0 && mass1 == 0 && scalarGas(value, polish)) return true;
+        if (mod().scalarRecovery && mass0 == 0 && mass1 == 0 && scalarGas(value, polish)) return true;
         R p = clamp(guessP, pmin, pmax), T = clamp(guessT, Tmin, Tmax);
         if (!evaluate(mass0, p, T, value, false)) return false;
         int stalls = 0;
@@ -798,16 +815,16 @@ struct Flash : FlashCap<CAP, R> {
         if constexpr (CAP) {
             if (this->adaptive) {
                 if (!fin(x[2]) || x[2] > 0) return false;
-                vapour = m_exp(x[2]) * q[m.condensable];
+                vapour = m_exp(x[2]) * q[mod().condensable];
                 if (!(vapour > 0)) return false;
-                mass0 = -m_expm1(x[2]) * q[m.condensable];
+                mass0 = -m_expm1(x[2]) * q[mod().condensable];
             } else {
                 if (!(x[2] >= 0 && x[2] < 1)) return false;
-                mass0 = x[2] * q[m.condensable];
+                mass0 = x[2] * q[mod().condensable];
             }
         } else {
             if (!(x[2] >= 0 && x[2] < 1)) return false;
-            mass0 = x[2] * q[m.condensable];
+            mass0 = x[2] * q[mod().condensable];
         }
         E v;
         if (!evaluate(mass0, m_exp(x[0]), m_exp(x[1]), v, true, vapour) || !(v.gasMass > 0)) return false;
@@ -884,7 +901,7 @@ struct Flash : FlashCap<CAP, R> {
             if (!(p > 0) || !(T > 0) || !(scale > 0) || !(gm > 0) || !(V > 0) || !fin(p) || !fin(T) || !fin(scale) ||
                 !fin(v.Vp) || !fin(v.VT) || !fin(v.Ep) || !fin(v.ET) || !fin(gm) || !fin(V))
                 return false;
-            const int c = m.condensable;
+            const int c = mod().condensable;
             double Y[NS];
             for (int k = 0; k < NS; ++k) {
                 if (!fin(q[k]) || q[k] < 0) return false;
@@ -923,8 +940,8 @@ struct Flash : FlashCap<CAP, R> {
             if (reconstructed != gm) return false;
             for (int k = 0; k < NS; ++k) Y[k] /= gm;
             GasPartials g;
-            if (!refActiveGasDerivatives(m.gas, T, pg, V, Y, gm, c, g)) return false;
-            const double Wk = m.gas.W[c], Wl = m.liquid.W[0];
+            if (!refActiveGasDerivatives(mod().gas, T, pg, V, Y, gm, c, g)) return false;
+            const double Wk = mod().gas.W[c], Wl = mod().liquid.W[0];
             const double liquidUbar = Hl - pl * Vl;
             jac[0][0] = v.Vp * p;
             jac[0][1] = v.VT * T;
@@ -954,7 +971,7 @@ struct Flash : FlashCap<CAP, R> {
                 if (!(pg > 0) || !(pl > 0) || !fin(pg) || !fin(pl)) return false;
                 vapourCoordinates = this->adaptive;
             }
-            const int c = m.condensable;
+            const int c = mod().condensable;
             double gm[NS], Y[NS], total = 0;
             for (int k = 0; k < NS; ++k) gm[k] = q[k];
             if (!(q[c] > 0)) return false;
@@ -976,14 +993,14 @@ struct Flash : FlashCap<CAP, R> {
             ++n.phases;
             if (gasTP(T, pg, Y, gas, nullptr) != kOk) return false;
             double hbar[NS], ubar[NS], vbar[NS], xm[NS], W = 0;
-            if (!refPhasePartials(m.gas, T, Y, gas, hbar, ubar, vbar, xm, W)) return false;
+            if (!refPhasePartials(mod().gas, T, Y, gas, hbar, ubar, vbar, xm, W)) return false;
             const double totalMoles = total / W;
             S liquid;
             ++n.phases;
             if (liquidTP(T, pl, liquid, nullptr) != kOk) return false;
             double dpsi;
-            if (!refGasCompositionPartial(m.gas, T, xm, totalMoles, gas, c, dpsi)) return false;
-            const double Wk = m.gas.W[c], Wl = m.liquid.W[0];
+            if (!refGasCompositionPartial(mod().gas, T, xm, totalMoles, gas, c, dpsi)) return false;
+            const double Wk = mod().gas.W[c], Wl = mod().liquid.W[0];
             jac[0][0] = v.Vp * p;
             jac[0][1] = v.VT * T;
             jac[1][0] = v.Ep * p / scale;
@@ -1010,7 +1027,7 @@ struct Flash : FlashCap<CAP, R> {
         scale = hi(R(1e5) * rho, hi(ab(energy), rho * R(2000) * hi(gT, Tmin)));
         x[0] = m_log(clamp(guessP, pmin, pmax));
         x[1] = m_log(clamp(gT, Tmin, Tmax));
-        const R previousFraction = guessLiquid / q[m.condensable];
+        const R previousFraction = guessLiquid / q[mod().condensable];
         x[2] = seed < 0 ? clamp(previousFraction, R(P::fractionEdge), R(1) - R(P::fractionEdge)) : seed;
         if constexpr (CAP) {
             if (this->adaptive) {  // ln(vapour fraction); seed -3 (or almost no vapour) starts from a trace (:383-388)
@@ -1018,8 +1035,8 @@ struct Flash : FlashCap<CAP, R> {
                 if (seed == R(-3) || (seed < 0 && vapour < 1e-6)) {
                     double other = 0;
                     for (int k = 0; k < NS; ++k)
-                        if (k != m.condensable) other += q[k];
-                    vapour = other > 0 ? lo(1e-6, .01 * (other / q[m.condensable])) : 1e-8;
+                        if (k != mod().condensable) other += q[k];
+                    vapour = other > 0 ? lo(1e-6, .01 * (other / q[mod().condensable])) : 1e-8;
                 }
                 if (!(vapour > 0 && fin(vapour))) return false;
                 x[2] = m_log(lo(1., vapour));
@@ -1155,7 +1172,7 @@ struct Flash : FlashCap<CAP, R> {
                 if (value.capExplicitGas) vapour = value.capGasCondensable;
             if (evaluated) comparison = value;
             else if (!evaluate(value.liqMass, p, T, comparison, true, vapour)) return false;
-            if (!NL || q[m.condensable] == 0) return true;
+            if (!NL || q[mod().condensable] == 0) return true;
             const R affinity = comparison.muLiquid - comparison.muGas;
             if (value.liqMass > 0) {
                 if (!fin(affinity) || ab(affinity) > mutol * R(2)) return false;
@@ -1170,17 +1187,17 @@ struct Flash : FlashCap<CAP, R> {
         // Tangent-plane distance of the pure vapour of the condensable (:474-485, n = 1).
         R mole[NS], Y[NS], sum = 0;
         for (int k = 0; k < NS; ++k) mole[k] = 0;
-        mole[m.condensable] = 1;
-        for (int k = 0; k < NS; ++k) { Y[k] = mole[k] * R(m.weights[k]); sum += Y[k]; }
+        mole[mod().condensable] = 1;
+        for (int k = 0; k < NS; ++k) { Y[k] = mole[k] * R(mod().weights[k]); sum += Y[k]; }
         for (int k = 0; k < NS; ++k) Y[k] /= sum;
         S gas;
         const int st = phaseStatus(-1, gasPressure(p), T, Y, gas, nullptr);
         R tangent;
         if (st != kOk) tangent = unavailable(st) ? R(Real<R>::big) * R(2) : R(0) / R(0);
         else {
-            const R reference = R(0) + mole[m.condensable] * muL;
+            const R reference = R(0) + mole[mod().condensable] * muL;
             R W = 0;
-            for (int k = 0; k < NS; ++k) W += mole[k] * R(m.weights[k]);
+            for (int k = 0; k < NS; ++k) W += mole[k] * R(mod().weights[k]);
             tangent = (gas.h - T * gas.s) * W / (R(8314.46261815324) * T) - reference;
         }
         return tangent >= -mutol;
@@ -1210,7 +1227,7 @@ struct Flash : FlashCap<CAP, R> {
         bool gasStable = false;
         bool vapourCoordinates = false;
         if constexpr (CAP) vapourCoordinates = this->adaptive;
-        const bool prunable = m.stableGasPrune != 0 && !vapourCoordinates;
+        const bool prunable = mod().stableGasPrune != 0 && !vapourCoordinates;
         if (frozen(R(0), candidate, prunable ? 4 : 0)) {
             const uint32_t before = n.stable;
             consider(candidate, best, have);
@@ -1220,7 +1237,7 @@ struct Flash : FlashCap<CAP, R> {
         }
         if (!NL) return have;
         const bool prune = prunable && gasStable;
-        const int c = m.condensable;
+        const int c = mod().condensable;
         R other = 0;
         for (int k = 0; k < NS; ++k)
             if (k != c) other += q[k];
@@ -1263,13 +1280,13 @@ struct Flash : FlashCap<CAP, R> {
             E reference;
             zeroEval(reference);
             const bool have = search(reference);
-            if (!m.boundaryRecovery && !capillaryBoundary) {
+            if (!mod().boundaryRecovery && !capillaryBoundary) {
                 if (have) result = reference;
                 return have;
             }
             if (have) {
                 const bool boundary =
-                    NL && q[m.condensable] > 0 && reference.gasMass > 0 && reference.liqMass / q[m.condensable] > 1 - 1e-4;
+                    NL && q[mod().condensable] > 0 && reference.gasMass > 0 && reference.liqMass / q[mod().condensable] > 1 - 1e-4;
                 if (!boundary) {
                     result = reference;
                     return true;
@@ -1319,7 +1336,7 @@ struct Flash : FlashCap<CAP, R> {
         E candidate, best;
         zeroEval(best);
         bool gasStable = false;
-        const bool prunable = m.stableGasPrune != 0;
+        const bool prunable = mod().stableGasPrune != 0;
         const bool gasOk = frozen(R(0), candidate, prunable ? 4 : 0);
         if (gasOk || stalled) {
             const R x[3] = {m_log(candidate.p), m_log(candidate.T), R(0)};
@@ -1333,7 +1350,7 @@ struct Flash : FlashCap<CAP, R> {
         if (!NL) return;
         const bool prune = prunable && gasStable;
         if (prune) set.flags |= kSeedsPruned;
-        const int c = m.condensable;
+        const int c = mod().condensable;
         R other = 0;
         for (int k = 0; k < NS; ++k)
             if (k != c) other += q[k];
@@ -1415,7 +1432,7 @@ struct Flash : FlashCap<CAP, R> {
     // failed also failed the frozen solve from the single-precision point,
     // which cost about 700 phase evaluations each.)
     RAMD_NOINLINE bool finishFrozen(R mass0, const Candidate& c, E& value, int stallLimit) {
-        if (mass0 == 0 && m.scalarRecovery) {
+        if (mass0 == 0 && mod().scalarRecovery) {
             if (gasPolish(m_exp(R(c.x[1])), value)) return true;
             return frozen(mass0, value, stallLimit);
         }
@@ -1446,7 +1463,7 @@ struct Flash : FlashCap<CAP, R> {
     // toward 1e-3 of the tolerances).  A polish that fails or lands on
     // another phase set keeps the searched result.
     RAMD_NOINLINE void polishBest(E& best) {
-        const int c = m.condensable;
+        const int c = mod().condensable;
         const R p0 = guessP, T0 = guessT;
         guessP = best.p;
         guessT = best.T;
@@ -1488,7 +1505,7 @@ struct Flash : FlashCap<CAP, R> {
         bool have = false;
         E candidate;
         bool gasStable = false;
-        const bool prunable = m.stableGasPrune != 0;
+        const bool prunable = mod().stableGasPrune != 0;
         uint32_t i = 0;
         const bool recorded = i < set.count && set.c[i].kind == kGasCandidate;
         // With seed pruning the reference decides on its own all-gas Newton,
@@ -1520,7 +1537,7 @@ struct Flash : FlashCap<CAP, R> {
         // FP64 prunes where the single-precision search did not: the
         // reference's search stops at the gas (and pure-liquid) candidate,
         // so the recorded seeds are skipped (have holds: the gas is stable)
-        const int c = m.condensable;
+        const int c = mod().condensable;
         if (i < set.count && set.c[i].kind == kLiquidCandidate) {
             if (finishFrozen(q[c], set.c[i], candidate, 0)) consider(candidate, best, have);
             else ++n.failures;

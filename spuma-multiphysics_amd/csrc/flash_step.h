@@ -23,6 +23,9 @@
 
 namespace ramd {
 
+// Flashes of the kernels read the workgroup's LDS model copy directly.
+constexpr bool kModelLds = RAMD_MODEL_LDS != 0;
+
 // The workgroup's copy of the model image: tables are read from LDS rather
 // than global memory.  Every thread of the workgroup must call this (it
 // synchronises the workgroup).
@@ -30,7 +33,7 @@ template <int NS>
 __device__ inline const HemModel<NS>& workgroupModel(const HemModel<NS>* __restrict__ global) {
 #if RAMD_MODEL_LDS
     static_assert(sizeof(HemModel<NS>) % 4 == 0, "model image in 4-byte words");
-    __shared__ HemModel<NS> local;
+    HemModel<NS>& local = sharedModel<NS>();
     const uint32_t* src = reinterpret_cast<const uint32_t*>(global);
     uint32_t* dst = reinterpret_cast<uint32_t*>(&local);
     for (uint32_t w = threadIdx.x; w < sizeof(HemModel<NS>) / 4; w += blockDim.x) dst[w] = src[w];

@@ -89,7 +89,7 @@ __global__ __launch_bounds__(64, RAMD_FLASH_MIN_WAVES) void hemFlashKernel(const
     uint32_t succeeded = 0;
     if (i < count) {
         const uint32_t c = order ? order[i] : i;
-        using F = Flash<NS, NL, double, false, RJ, CAP>;
+        using F = Flash<NS, NL, double, false, RJ, CAP, kModelLds>;
         F f(model, n);
         for (int k = 0; k < NS; ++k) f.q[k] = q[size_t(c) * NS + k];
         f.energy = energy[c];
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(64, RAMD_REFINE_MIN_WAVES) void hemRefineKernel(con
     uint32_t succeeded = 0;
     if (i < count) {
         const uint32_t c = order[i];
-        Flash<NS, NL, double, true> f(model, n);
+        Flash<NS, NL, double, true, false, false, kModelLds> f(model, n);
         loadHemCell<NS>(f, q, energy, guess[c], c);
         EvalT<double, true> result;
         zeroEval(result);
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(64, RAMD_REFINE_MIN_WAVES) void hemRefineQueueKerne
         uint32_t succeeded = 0;
         if (i < count) {
             const uint32_t c = queue[i];
-            Flash<NS, NL, double, true> f(model, n);
+            Flash<NS, NL, double, true, false, false, kModelLds> f(model, n);
             loadHemCell<NS>(f, q, energy, guess[c], c);
             EvalT<double, true> result;
             zeroEval(result);

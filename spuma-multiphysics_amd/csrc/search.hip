@@ -32,7 +32,7 @@ __global__ __launch_bounds__(256, RAMD_SEARCH_MIN_WAVES) void stepSearchKernel(c
     Counts n = {0, 0, 0, 0, 0, 0};
     if (i < count && *dtDev > 0 && !(rep && rep[order[i]] != order[i])) {
         const uint32_t c = order[i];
-        Flash<NS, NL, float> f(model, n);
+        Flash<NS, NL, float, true, false, false, kModelLds> f(model, n);
         loadCell(f, q, stride, seeds[c], c, NS);
         CandidateSet set;
         f.searchRecord(set);
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(256, RAMD_SEARCH_MIN_WAVES) void hemSearchKernel(co
     Counts n = {0, 0, 0, 0, 0, 0, 0};
     if (i < count) {
         const uint32_t c = order[i];
-        Flash<NS, NL, float> f(model, n);
+        Flash<NS, NL, float, true, false, false, kModelLds> f(model, n);
         loadHemCell<NS>(f, q, energy, guess[c], c);
         CandidateSet set;
         f.searchRecord(set);

@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256, RAMD_FLASH_MIN_WAVES) void stepFlashKernel(con
     if (i < count && *dtDev > 0 && !(rep && rep[order[i]] != order[i])) {
         const uint32_t c = order[i];
         const ThermoOut g = seeds[c];
-        using F = Flash<NS, NL, double, false, RJ>;
+        using F = Flash<NS, NL, double, false, RJ, false, kModelLds>;
         F f(model, n);
         loadCell(f, q, stride, g, c, NS);
         typename F::E result;
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256, RAMD_REFINE_MIN_WAVES) void stepRefineKernel(c
     if (i < count && *dtDev > 0 && !(rep && rep[order[i]] != order[i])) {
         const uint32_t c = order[i];
         const ThermoOut g = seeds[c];
-        Flash<NS, NL, double, true> f(model, n);
+        Flash<NS, NL, double, true, false, false, kModelLds> f(model, n);
         loadCell(f, q, stride, g, c, NS);
         EvalT<double, true> result;
         zeroEval(result);
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(64, RAMD_REFINE_MIN_WAVES) void stepRefineQueueKern
         if (i < count && *dtDev > 0) {
             const uint32_t c = queue[i];
             const ThermoOut g = seeds[c];
-            Flash<NS, NL, double, true> f(model, n);
+            Flash<NS, NL, double, true, false, false, kModelLds> f(model, n);
             loadCell(f, q, stride, g, c, NS);
             EvalT<double, true> result;
             zeroEval(result);
