@@ -22,6 +22,7 @@
 // FP64 Newton corrections and decides stability and maximum entropy in FP64.
 #ifndef REACTIVE_AMD_FLASH_DEVICE_H
 #define REACTIVE_AMD_FLASH_DEVICE_H
+#include <type_traits>
 #include "pr_device.h"
 #include "flash_jacobian.h"
 
@@ -132,15 +133,34 @@ struct CapParts<true> {
     }
 };
 
+// The fields of the evaluation record that only the committed state reads
+// (ReactiveThermoState's outputs): kept by the FP64 flashes, left out of the
+// FP32 search's record (a base, empty there), which never commits a state,
+// so its per-lane record is smaller.
+template <class R, bool Keep>
+struct Outputs {
+    R e, entropy, cf, ceq, volRes, enRes, chemRes;
+    R alphaGas, alphaLiq, rhoGas, rhoLiq, cp, cv;
+    int active, iterations;
+    RD void clearOutputs() {
+        e = entropy = cf = ceq = volRes = enRes = chemRes = 0;
+        alphaGas = alphaLiq = rhoGas = rhoLiq = cp = cv = 0;
+        active = iterations = 0;
+    }
+};
+template <class R>
+struct Outputs<R, false> {
+    RD void clearOutputs() {}
+};
+
 // Evaluation record (reactiveDeviceFlash.h:64-73) for at most one liquid;
 // A adds the per-phase parts of this build's analytic Jacobian (mixed
 // flash), RJ the inputs of the reference's analytic Jacobian, C the
 // vapour-coordinate inventory of the capillary / boundary-search flash.
 template <class R, bool A = false, bool RJ = false, bool C = false>
-struct EvalT : CapParts<C> {
-    R p, T, rho, e, entropy, cf, ceq, volRes, enRes, chemRes;
-    R alphaGas, alphaLiq, rhoGas, rhoLiq, liqMass, gasMass, cp, cv;
-    int active, iterations;
+struct EvalT : CapParts<C>, Outputs<R, !std::is_same<R, float>::value> {
+    static constexpr bool kOutputs = !std::is_same<R, float>::value;
+    R p, T, rho, liqMass, gasMass;
     R volume, energy, entropyDensity, Vp, VT, Ep, ET, cpDensity, muGas, muLiquid;
     PhaseParts<R, A> parts;
     RefParts<RJ> ref;
@@ -149,9 +169,8 @@ using Eval = EvalT<double>;
 
 template <class R, bool A, bool RJ, bool C>
 RD void zeroEval(EvalT<R, A, RJ, C>& v) {
-    v.p = v.T = v.rho = v.e = v.entropy = v.cf = v.ceq = v.volRes = v.enRes = v.chemRes = 0;
-    v.alphaGas = v.alphaLiq = v.rhoGas = v.rhoLiq = v.liqMass = v.gasMass = v.cp = v.cv = 0;
-    v.active = v.iterations = 0;
+    v.p = v.T = v.rho = v.liqMass = v.gasMass = 0;
+    v.clearOutputs();
     v.volume = v.energy = v.entropyDensity = v.Vp = v.VT = v.Ep = v.ET = v.cpDensity = v.muGas = v.muLiquid = 0;
     v.parts.clear();
     v.ref.clear();
@@ -594,8 +613,10 @@ struct Flash : FlashCap<CAP, R> {
             if (phaseStatus(-1, pg, T, gas, prop, virtualLiquids ? mu : nullptr) != kOk) return false;
             if constexpr (RJ) v.ref.gasMolarVolume = prop.V;
             accumulate(v, mg, prop, pg);
-            v.alphaGas = mg / prop.rho;
-            v.rhoGas = prop.rho;
+            if constexpr (E::kOutputs) {
+                v.alphaGas = mg / prop.rho;
+                v.rhoGas = prop.rho;
+            }
             if constexpr (A) {
                 if constexpr (NS == 1) recordPhase(prop, p, T, v.parts.vG, v.parts.eG, v.parts.hG, v.parts.zG);
                 else gasPartials(p, T, mg, gas, prop, v.parts);
@@ -630,22 +651,31 @@ struct Flash : FlashCap<CAP, R> {
                     }
                     if (mass0 > 0) {
                         accumulate(v, mass0, prop, pl);
-                        v.rhoLiq = prop.rho;
-                        v.alphaLiq = mass0 / prop.rho;
-                        v.active |= 1;
+                        if constexpr (E::kOutputs) {
+                            v.rhoLiq = prop.rho;
+                            v.alphaLiq = mass0 / prop.rho;
+                            v.active |= 1;
+                        }
                     }
                 }
             }
         }
-        v.e = v.energy / v.rho; v.entropy = v.entropyDensity / v.rho; v.cp = v.cpDensity / v.rho;
-        v.cv = (v.cpDensity + T * v.VT * v.VT / v.Vp) / v.rho;
-        if (!fin(v.cv) || v.cv <= 0) return false;
+        const R cp = v.cpDensity / v.rho;
+        const R cv = (v.cpDensity + T * v.VT * v.VT / v.Vp) / v.rho;
+        if constexpr (E::kOutputs) {
+            v.e = v.energy / v.rho; v.entropy = v.entropyDensity / v.rho; v.cp = cp;
+            v.cv = cv;
+        }
+        if (!fin(cv) || cv <= 0) return false;
         const R specific = v.volume / v.rho, vp = v.Vp / v.rho, vt = v.VT / v.rho;
-        const R adiabatic = vp + T * vt * vt / v.cp;
+        const R adiabatic = vp + T * vt * vt / cp;
         if (!(adiabatic < 0)) return false;
-        v.cf = m_sqrt(-specific * specific / adiabatic);
-        v.ceq = v.cf;
-        return fin(v.cf) && v.cf > 0;
+        const R cf = m_sqrt(-specific * specific / adiabatic);
+        if constexpr (E::kOutputs) {
+            v.cf = cf;
+            v.ceq = cf;
+        }
+        return fin(cf) && cf > 0;
     }
     RD R energyScale(const E& v) const { return hi(R(1e5) * v.rho, v.cpDensity * v.T); }
 
@@ -739,10 +769,13 @@ struct Flash : FlashCap<CAP, R> {
             T = trialT; F = trialF; cv = trialCv; p = trialP;
         }
         if (!evaluate(R(0), p, T, result, false)) return false;
-        result.volRes = ab(result.volume - R(1));
-        result.enRes = ab(result.energy - energy) / energyScale(result);
-        result.iterations = it;
-        return result.volRes <= vtol && result.enRes <= etol;
+        const R volRes = ab(result.volume - R(1)), enRes = ab(result.energy - energy) / energyScale(result);
+        if constexpr (E::kOutputs) {
+            result.volRes = volRes;
+            result.enRes = enRes;
+            result.iterations = it;
+        }
+        return volRes <= vtol && enRes <= etol;
     }
     // Fixed split, 2x2 Newton in (ln p, ln T) with line search (reactiveDeviceFlash.h:300-326).
     // polish > 0 (FP64 refinement only): up to that many further Newton steps
@@ -758,9 +791,11 @@ struct Flash : FlashCap<CAP, R> {
         bool polishing = false;
         for (int it = 0; it < 70; ++it) {
             const R scale = energyScale(value), rv = value.volume - R(1), re = (value.energy - energy) / scale;
-            value.volRes = ab(rv);
-            value.enRes = ab(re);
-            value.iterations = it;
+            if constexpr (E::kOutputs) {
+                value.volRes = ab(rv);
+                value.enRes = ab(re);
+                value.iterations = it;
+            }
             if (ab(rv) <= vtol && ab(re) <= etol) {
                 if (polish > 0 && !(ab(rv) <= R(1e-3) * vtol && ab(re) <= R(1e-3) * etol)) {
                     --polish;
@@ -1068,8 +1103,12 @@ struct Flash : FlashCap<CAP, R> {
         if (!solveN<3>(jac, rhs, response)) return false;
         const R squared = value.p * response[0];
         if (!fin(squared) || squared <= 0) return false;
-        value.ceq = m_sqrt(squared);
-        return value.ceq <= value.cf * (R(1) + R(2e-3));
+        if constexpr (E::kOutputs) {  // the FP64 flashes only
+            value.ceq = m_sqrt(squared);
+            return value.ceq <= value.cf * (R(1) + R(2e-3));
+        } else {
+            return false;
+        }
     }
     RAMD_NOINLINE bool activeNewton(R* x, R scale, E& value, int polish) {
         stalled = false;
@@ -1080,10 +1119,12 @@ struct Flash : FlashCap<CAP, R> {
             const R chemical = hi(R(0), ab(f[2]));
             R jac[3][3];
             if (ab(f[0]) <= vtol && ab(f[1]) <= etol && chemical <= mutol) {
-                value.volRes = ab(f[0]);
-                value.enRes = ab(f[1]);
-                value.chemRes = chemical;
-                value.iterations = it;
+                if constexpr (E::kOutputs) {
+                    value.volRes = ab(f[0]);
+                    value.enRes = ab(f[1]);
+                    value.chemRes = chemical;
+                    value.iterations = it;
+                }
                 if (P::single) return true;
                 const bool tight = ab(f[0]) <= R(1e-3) * vtol && ab(f[1]) <= R(1e-3) * etol &&
                                    chemical <= R(1e-3) * mutol;
