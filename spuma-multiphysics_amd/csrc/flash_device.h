@@ -258,6 +258,18 @@ RD bool solveN(const R (&m)[3][3], const R* rhs, R* x) {
     return true;
 }
 
+// Lane w's copy of a trivially copyable value, in every lane of the wave.
+template <class T>
+__device__ inline T waveBroadcast(const T& v, int w) {
+    static_assert(sizeof(T) % 4 == 0, "broadcast in 4-byte words");
+    uint32_t words[sizeof(T) / 4];
+    __builtin_memcpy(words, &v, sizeof(T));
+    for (size_t k = 0; k < sizeof(T) / 4; ++k) words[k] = uint32_t(__shfl(int(words[k]), w, 64));
+    T out;
+    __builtin_memcpy(&out, words, sizeof(T));
+    return out;
+}
+
 // Precision-dependent flash settings.  FP64: the reference's values.  FP32:
 // tolerances and finite-difference steps at single-precision rounding, and a
 // hand-over threshold: an FP32 Newton that stalls (rejected line search)
@@ -320,12 +332,24 @@ __device__ inline HemModel<NS>& sharedModel() {
 
 // LDS: the model is the workgroup's LDS copy (sharedModel); otherwise the
 // reference given to the constructor.
+// WAVE: every lane of a wave runs this flash on the same cell (so its
+// control flow is the wave's), and the frozen split's line search evaluates
+// its fractions on the wave's lanes at once, lane j the j-th fraction of the
+// loop; the first accepted fraction in loop order is taken, which is the
+// loop's choice, so the results are those of the one-lane flash bit for bit
+// (the rejected evaluations it adds change nothing: the flash's only memo,
+// an unavailable liquid at (p, T), repeats what recomputing gives).  For the
+// few long refinements: a frozen solve that crawls halves each of its seventy
+// Newton steps many times.  (The active-set line search stays one evaluation
+// at a time: spread over the lanes in the same way, its results depended on
+// what earlier work had left on the device, and that is not understood.)
 template <int NS, int NL, class R = double, bool A = FlashPrecision<R>::single, bool RJ = false, bool CAP = false,
-          bool LDS = false>
+          bool LDS = false, bool WAVE = false>
 struct Flash : FlashCap<CAP, R> {
     static_assert(NL == 0 || NL == 1, "one pure liquid at most");
     static_assert(!RJ || (!A && !FlashPrecision<R>::single), "the reference's analytic mode is an FP64 flash");
     static_assert(!CAP || (!A && !FlashPrecision<R>::single), "the capillary flash is an FP64 flash");
+    static_assert(!WAVE || !FlashPrecision<R>::single, "wave-shared line searches are an FP64 refinement");
     using P = FlashPrecision<R>;
     using E = EvalT<R, A, RJ, CAP>;
     using S = PRStateT<R>;
@@ -362,6 +386,20 @@ struct Flash : FlashCap<CAP, R> {
         liquidTmin = R(mod().liquidTmin); liquidTc = R(mod().liquidTc);
         record = nullptr;
         stalled = false;
+    }
+    // WAVE: after a line search shared by the wave, the counts the one-lane
+    // loop makes (the trials up to lane `last`, the winner or all), the same
+    // in every lane.
+    RD void waveCounts(const Counts& before, int last) {
+        const int lane = int(threadIdx.x & 63u);
+        uint32_t phases = lane <= last ? n.phases - before.phases : 0u;
+        uint32_t residuals = lane <= last ? n.residuals - before.residuals : 0u;
+        for (int o = 32; o > 0; o >>= 1) {
+            phases += uint32_t(__shfl_xor(int(phases), o, 64));
+            residuals += uint32_t(__shfl_xor(int(residuals), o, 64));
+        }
+        n.phases = before.phases + phases;
+        n.residuals = before.residuals + residuals;
     }
     // Phase pressures of a curved interface (reactiveDeviceFlash.h:152-153).
     RD R gasPressure(R p) const {
@@ -813,6 +851,34 @@ struct Flash : FlashCap<CAP, R> {
             step[0] /= limit; step[1] /= limit;
             const R previous = hi(ab(rv), ab(re));
             bool accepted = false;
+            if constexpr (WAVE) {
+                // lane j tries fraction 2^-j, the loop's j-th (fractions above 1e-9: j < 30)
+                const int lane = int(threadIdx.x & 63u);
+                const R fraction = R(::ldexp(1.0, -lane));
+                const Counts before = n;
+                E trial;
+                zeroEval(trial);
+                R pp = 0, tt = 0;
+                bool ok = false;
+                if (fraction > R(1e-9)) {
+                    pp = p * m_exp(fraction * step[0]);
+                    tt = T * m_exp(fraction * step[1]);
+                    if (evaluate(mass0, pp, tt, trial, false)) {
+                        const R next = hi(ab(trial.volume - R(1)), ab(trial.energy - energy) / scale);
+                        ok = next < previous * (R(1) - R(1e-4) * fraction) || next < lo(vtol, etol);
+                    }
+                }
+                const unsigned long long won = __ballot(ok);
+                waveCounts(before, won ? __ffsll(won) - 1 : 63);
+                if (won) {
+                    const int w = __ffsll(won) - 1;
+                    value = waveBroadcast(trial, w);
+                    p = waveBroadcast(pp, w);
+                    T = waveBroadcast(tt, w);
+                    accepted = true;
+                    stalls = R(::ldexp(1.0, -w)) < R(1) / R(32) ? stalls + 1 : 0;
+                }
+            } else
             for (R fraction = 1; fraction > R(1e-9); fraction *= R(.5)) {
                 E trial;
                 const R pp = p * m_exp(fraction * step[0]), tt = T * m_exp(fraction * step[1]);
@@ -1471,10 +1537,14 @@ struct Flash : FlashCap<CAP, R> {
     // from the cell's own guess, as the reference starts.  (A failed polish
     // goes straight to that solve: on a 160^3 box, every cell whose polish
     // failed also failed the frozen solve from the single-precision point,
-    // which cost about 700 phase evaluations each.)
-    RAMD_NOINLINE bool finishFrozen(R mass0, const Candidate& c, E& value, int stallLimit) {
+    // which cost about 700 phase evaluations each.)  deferred (when given):
+    // that solve from the guess, which can take a thousand phase evaluations,
+    // is left to the deferred refinement (*deferred = true, false returned),
+    // where a wave shares each line search.
+    RAMD_NOINLINE bool finishFrozen(R mass0, const Candidate& c, E& value, int stallLimit, bool* deferred = nullptr) {
         if (mass0 == 0 && mod().scalarRecovery) {
             if (gasPolish(m_exp(R(c.x[1])), value)) return true;
+            if (deferred) { *deferred = true; return false; }
             return frozen(mass0, value, stallLimit);
         }
         const R p0 = guessP, T0 = guessT;
@@ -1483,6 +1553,7 @@ struct Flash : FlashCap<CAP, R> {
         bool ok = frozen(mass0, value, stallLimit, R(0), 2);
         guessP = p0;
         guessT = T0;
+        if (!ok && deferred) { *deferred = true; return false; }
         if (!ok) ok = frozen(mass0, value, stallLimit);  // from the cell's own guess, as the reference starts
         return ok;
     }
@@ -1558,7 +1629,8 @@ struct Flash : FlashCap<CAP, R> {
         // change whether it converges.)
         bool gasOk = false;
         if (prunable) gasOk = frozen(R(0), candidate, 4, R(0), 2);
-        else if (recorded) gasOk = finishFrozen(R(0), set.c[i], candidate, 0);
+        else if (recorded) gasOk = finishFrozen(R(0), set.c[i], candidate, 0, deferred);
+        if (deferred && *deferred) return false;
         if (recorded) ++i;
         if (gasOk) {
             const uint32_t before = n.stable;
@@ -1580,7 +1652,8 @@ struct Flash : FlashCap<CAP, R> {
         // so the recorded seeds are skipped (have holds: the gas is stable)
         const int c = mod().condensable;
         if (i < set.count && set.c[i].kind == kLiquidCandidate) {
-            if (finishFrozen(q[c], set.c[i], candidate, 0)) consider(candidate, best, have);
+            if (finishFrozen(q[c], set.c[i], candidate, 0, deferred)) consider(candidate, best, have);
+            else if (deferred && *deferred) return false;
             else ++n.failures;
             ++i;
         }

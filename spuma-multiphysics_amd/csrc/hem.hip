@@ -170,31 +170,30 @@ __global__ __launch_bounds__(64, RAMD_REFINE_MIN_WAVES) void hemRefineKernel(con
     addTotals(n, succeeded, totals);
 }
 
-// The deferred cells, refined without deferral by a persistent grid that
-// sweeps the queue (the count is read on the device).
+// The deferred cells, refined without deferral by a persistent grid of
+// single-wave workgroups, one queued cell per wave at a time with its line
+// searches spread over the lanes (the flash's WAVE mode; the count is read
+// on the device).
 template <int NS, int NL>
 __global__ __launch_bounds__(64, RAMD_REFINE_MIN_WAVES) void hemRefineQueueKernel(
     const HemModel<NS>* __restrict__ globalModel, const double* __restrict__ q, const double* __restrict__ energy,
     const ThermoOut* __restrict__ guess, const uint32_t* __restrict__ queue, const unsigned* __restrict__ queueCount,
     const CandidateSet* __restrict__ sets, HemOutputDev* __restrict__ out, unsigned long long* __restrict__ totals) {
     const unsigned count = *queueCount;
-    if (blockIdx.x * 64u >= count) return;  // uniform per workgroup
+    if (blockIdx.x >= count) return;  // uniform per workgroup
     const HemModel<NS>& model = workgroupModel(globalModel);
-    for (unsigned base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u) {  // uniform per workgroup
-        const unsigned i = base + threadIdx.x;
+    const bool lead = threadIdx.x == 0;
+    for (unsigned i = blockIdx.x; i < count; i += gridDim.x) {  // uniform per workgroup
         Counts n = {0, 0, 0, 0, 0, 0, 0};
-        uint32_t succeeded = 0;
-        if (i < count) {
-            const uint32_t c = queue[i];
-            Flash<NS, NL, double, true, false, false, kModelLds> f(model, n);
-            loadHemCell<NS>(f, q, energy, guess[c], c);
-            EvalT<double, true> result;
-            zeroEval(result);
-            const bool ok = f.refine(sets[c], result);
-            storeHemCell(ok, result, out[c]);
-            succeeded = ok ? 1u : 0u;
-        }
-        addTotals(n, succeeded, totals);
+        const uint32_t c = queue[i];
+        Flash<NS, NL, double, true, false, false, kModelLds, true> f(model, n);
+        loadHemCell<NS>(f, q, energy, guess[c], c);
+        EvalT<double, true> result;
+        zeroEval(result);
+        const bool ok = f.refine(sets[c], result);
+        if (lead) storeHemCell(ok, result, out[c]);
+        const Counts none = {0, 0, 0, 0, 0, 0, 0};
+        addTotals(lead ? n : none, lead && ok ? 1u : 0u, totals);
     }
 }
 

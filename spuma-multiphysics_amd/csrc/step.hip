@@ -110,11 +110,19 @@ __global__ __launch_bounds__(256, RAMD_REFINE_MIN_WAVES) void stepRefineKernel(c
     countFlash(failed, n.phases, counters);
 }
 
-// The cells the refinement deferred (full FP64 search or FP64 restarts),
-// refined again without deferral: a persistent grid that sweeps the queue, so
-// the long searches of a few cells share waves instead of stalling the
-// waves of the bulk.  queueCount is read on the device (the launch does not
-// depend on it).
+// 1: the deferred refinements run one cell per wave with the frozen split's
+// line searches spread over its lanes; 0: one cell per lane.
+#ifndef RAMD_REFINE_WAVE
+#define RAMD_REFINE_WAVE 1
+#endif
+
+// The cells the refinement deferred (full FP64 search, FP64 restarts, or a
+// frozen candidate's solve from the cell's guess), refined again without
+// deferral: a persistent grid of single-wave workgroups, each wave on one
+// queued cell at a time with its line searches spread over its lanes (the
+// flash's WAVE mode), so that the few long refinements neither share a wave
+// nor halve their steps one evaluation at a time.  queueCount is read on the
+// device (the launch does not depend on it).
 template <int NS, int NL>
 __global__ __launch_bounds__(64, RAMD_REFINE_MIN_WAVES) void stepRefineQueueKernel(
     const HemModel<NS>* __restrict__ globalModel, const double* __restrict__ q, size_t stride,
@@ -123,26 +131,38 @@ __global__ __launch_bounds__(64, RAMD_REFINE_MIN_WAVES) void stepRefineQueueKern
     double* __restrict__ pOut, double* __restrict__ cOut, double* __restrict__ rOut, uint16_t* __restrict__ cost,
     unsigned long long* __restrict__ counters, uint8_t* __restrict__ failMark) {
     const unsigned count = *queueCount;
-    if (blockIdx.x * 64u >= count) return;  // uniform per workgroup: usually every workgroup leaves here
+#if RAMD_REFINE_WAVE
+    if (blockIdx.x >= count) return;  // uniform per workgroup: usually every workgroup leaves here
     const HemModel<NS>& model = workgroupModel(globalModel);
+    const bool lead = threadIdx.x == 0;
+    for (unsigned i = blockIdx.x; i < count; i += gridDim.x) {  // uniform per workgroup
+        const bool mine = true;
+#else
+    if (blockIdx.x * 64u >= count) return;
+    const HemModel<NS>& model = workgroupModel(globalModel);
+    const bool lead = true;
     for (unsigned base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u) {  // uniform per workgroup
         const unsigned i = base + threadIdx.x;
-        Counts n = {0, 0, 0, 0, 0, 0};
+        const bool mine = i < count;
+#endif
+        Counts n = {0, 0, 0, 0, 0, 0, 0};
         uint32_t failed = 0;
-        if (i < count && *dtDev > 0) {
+        if (mine && *dtDev > 0) {
             const uint32_t c = queue[i];
             const ThermoOut g = seeds[c];
-            Flash<NS, NL, double, true, false, false, kModelLds> f(model, n);
+            Flash<NS, NL, double, true, false, false, kModelLds, RAMD_REFINE_WAVE != 0> f(model, n);
             loadCell(f, q, stride, g, c, NS);
             EvalT<double, true> result;
             zeroEval(result);
             const bool ok = f.refine(sets[c], result);
-            storeCell(ok, result, g, c, out, pOut, cOut, rOut);
-            if (!ok && failMark) failMark[c] = 1;
-            failed = ok ? 0u : 1u;
-            if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
+            if (lead) {
+                storeCell(ok, result, g, c, out, pOut, cOut, rOut);
+                if (!ok && failMark) failMark[c] = 1;
+                if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
+                failed = ok ? 0u : 1u;
+            }
         }
-        countFlash(failed, n.phases, counters);
+        countFlash(failed, lead ? n.phases : 0u, counters);
     }
 }
 __global__ void clearQueueKernel(unsigned* __restrict__ count) {
