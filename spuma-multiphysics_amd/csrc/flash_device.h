@@ -333,16 +333,17 @@ __device__ inline HemModel<NS>& sharedModel() {
 // LDS: the model is the workgroup's LDS copy (sharedModel); otherwise the
 // reference given to the constructor.
 // WAVE: every lane of a wave runs this flash on the same cell (so its
-// control flow is the wave's), and the frozen split's line search evaluates
-// its fractions on the wave's lanes at once, lane j the j-th fraction of the
+// control flow is the wave's), and the Newton line searches evaluate their
+// fractions on the wave's lanes at once, lane j the j-th fraction of the
 // loop; the first accepted fraction in loop order is taken, which is the
 // loop's choice, so the results are those of the one-lane flash bit for bit
 // (the rejected evaluations it adds change nothing: the flash's only memo,
 // an unavailable liquid at (p, T), repeats what recomputing gives).  For the
 // few long refinements: a frozen solve that crawls halves each of its seventy
-// Newton steps many times.  (The active-set line search stays one evaluation
-// at a time: spread over the lanes in the same way, its results depended on
-// what earlier work had left on the device, and that is not understood.)
+// Newton steps many times.  Every lane makes the active set's residual call
+// (the lanes past the loop's end repeat its last fraction): with the call
+// made by only some lanes, the results depended on what earlier work had
+// left on the device.
 template <int NS, int NL, class R = double, bool A = FlashPrecision<R>::single, bool RJ = false, bool CAP = false,
           bool LDS = false, bool WAVE = false>
 struct Flash : FlashCap<CAP, R> {
@@ -1219,6 +1220,52 @@ struct Flash : FlashCap<CAP, R> {
             for (int j = 0; j < 3; ++j) step[j] /= limiter;
             bool accepted = false;
             const R previous = hi(hi(hi(R(0), ab(f[0])), ab(f[1])), ab(f[2]));
+            if constexpr (WAVE) {
+                const auto outsideAt = [vapourCoordinates](R y) {
+                    return vapourCoordinates ? (!fin(y) || y > 0) : !(y >= 0 && y < 1);
+                };
+                // the loop evaluates the indices from `first` on: 0, or the
+                // end of the rejected prefix of the partition coordinate
+                // (bisected as below); lane j tries index first + j, and
+                // every lane evaluates (lanes past the loop's end repeat its
+                // last index and are not counted)
+                int first = 0;
+                if (outsideAt(x[2] + step[2])) {
+                    int lower = 0, upper = 30;
+                    while (upper - lower > 1) {
+                        const int mid = (lower + upper) / 2;
+                        if (outsideAt(x[2] + R(::ldexp(1.0, -mid)) * step[2])) lower = mid;
+                        else upper = mid;
+                    }
+                    n.residuals += uint32_t(upper);
+                    first = upper;
+                }
+                if (first < 30) {
+                    const int lane = int(threadIdx.x & 63u);
+                    const int index = first + lane < 30 ? first + lane : 29;
+                    const R fraction = R(::ldexp(1.0, -index));
+                    const Counts before = n;
+                    R tx[3], tf[3];
+                    for (int j = 0; j < 3; ++j) tx[j] = x[j] + fraction * step[j];
+                    E trial;
+                    zeroEval(trial);
+                    const bool evaluated = residual(tx, scale, tf, &trial);
+                    const bool ok = first + lane < 30 && evaluated &&
+                                    hi(hi(hi(R(0), ab(tf[0])), ab(tf[1])), ab(tf[2])) <
+                                        previous * (R(1) - R(1e-4) * fraction);
+                    const unsigned long long won = __ballot(ok);
+                    waveCounts(before, won ? __ffsll(won) - 1 : 29 - first);
+                    if (won) {
+                        const int w = __ffsll(won) - 1;
+                        for (int j = 0; j < 3; ++j) {
+                            x[j] = waveBroadcast(tx[j], w);
+                            f[j] = waveBroadcast(tf[j], w);
+                        }
+                        value = waveBroadcast(trial, w);
+                        accepted = true;
+                    }
+                }
+            } else {
             R fraction = 1;
             for (int index = 0; index < 30; ++index, fraction *= R(.5)) {
                 R tx[3], tf[3];
@@ -1252,6 +1299,7 @@ struct Flash : FlashCap<CAP, R> {
                     accepted = true;
                     break;
                 }
+            }
             }
             if (!accepted) {
                 stalled = P::single && previous <= R(P::handOver);
