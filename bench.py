@@ -308,9 +308,13 @@ def stage_roofline(rep, cells, steps, n, nv=5):
 
 
 # HBM bytes per launch of the FP32 search at 160^3 pure N2O (the bench's
-# seeded box, mixed step), from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled as
-# below: profiles/stepSearch160_traffic.txt
-SEARCH_TRAFFIC_160 = 363.7e9
+# seeded box, mixed step, tuned launch shapes), from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE scaled as below: profiles/stepSearch160_traffic.txt
+SEARCH_TRAFFIC_160 = 135.97e9
+# The source commit the profiles behind SEARCH_TRAFFIC_160, STAGE_TRAFFIC_160
+# and FLOP_PER_PHASE were measured on (reported with the rooflines).
+PROFILED_COMMIT = "4d905b9"
+
 # HBM bytes per launch of the fused stage kernel at 160^3 pure N2O, stage 0 and
 # stage 1, from rocprofv3 FETCH_SIZE and WRITE_SIZE (each in its own run,
 # scaled by a plain copy kernel's counts): profiles/cartesianStage160_traffic.txt
@@ -322,7 +326,7 @@ GRAPH = 0x100  # REACTIVE_AMD_STEP_GRAPH
 # rocprofv3 on divergence-free uniform cells, mean of a two-phase cell and a
 # gas cell: profiles/flash_valu_uniform32.txt.  The flash roofline's achieved
 # rate is phase evaluations x this figure / event-timed kernel time.
-FLOP_PER_PHASE = {"fp64_flash": (1453 + 1860) / 2, "fp32_search": (414 + 502) / 2, "fp64_refine": (1845 + 1361) / 2}
+FLOP_PER_PHASE = {"fp64_flash": (1619 + 2115) / 2, "fp32_search": (370 + 523) / 2, "fp64_refine": (2253 + 1596) / 2}
 FP32_PEAK_TFLOPS = 157.3
 
 
@@ -364,7 +368,8 @@ def distance(s, states0, root, ok):
 def flash_roofline(rep, rep64, cells, steps, mixed):
     """VALU rate of the flash kernels against the vector peaks (see FLOP_PER_PHASE)."""
     flashes = 2 * steps
-    out = {"bound": "valu", "unit": "TFLOP/s", "flop_source": "profiles/flash_valu_uniform32.txt"}
+    out = {"bound": "valu", "unit": "TFLOP/s", "flop_source": "profiles/flash_valu_uniform32.txt",
+           "profiled_commit": PROFILED_COMMIT}
     t64 = (rep64.flashSeconds[0] + rep64.flashSeconds[1]) / flashes
     a64 = rep64.phaseEvaluations / flashes * FLOP_PER_PHASE["fp64_flash"] / t64 / 1e12
     out["fp64_flash"] = {"kernel": "stepFlashKernel", "avg_launch_ms": 1e3 * t64, "achieved": a64,
@@ -397,6 +402,7 @@ def search_roofline(flash, rep, steps, n, case="n2o"):
     return {"bound": "valu", "kernel": "stepSearchKernel (FP32 candidate search of the mixed flash)",
             "achieved": s["achieved"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": s["frac"],
             "traffic": SEARCH_TRAFFIC_160 if n == 160 and case == "n2o" else None,
+            "traffic_source": "profiles/stepSearch160_traffic.txt", "profiled_commit": PROFILED_COMMIT,
             "avg_launch_ms": s["avg_launch_ms"],
             "flop_per_phase_evaluation": FLOP_PER_PHASE["fp32_search"],
             "phase_evaluations_per_cell": s["phase_evaluations_per_cell"],

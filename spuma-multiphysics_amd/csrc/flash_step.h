@@ -20,6 +20,14 @@
 #ifndef RAMD_REFINE_MIN_WAVES
 #define RAMD_REFINE_MIN_WAVES 2
 #endif
+// Deferred refinements up to this many per flash run one cell per wave with
+// the Newton line searches spread over its lanes (the flash's WAVE mode);
+// more run one cell per lane (measured: one cell per wave took a binary
+// N2O/N2 96^3 step from 4.55 to 2.23 s with 16k deferred cells per flash, but
+// a 160^3 one from 6.74 to 7.18 s with 75k: too many rounds of waves).
+#ifndef RAMD_WAVE_QUEUE_CELLS
+#define RAMD_WAVE_QUEUE_CELLS 32768
+#endif
 
 namespace ramd {
 

@@ -171,29 +171,48 @@ __global__ __launch_bounds__(64, RAMD_REFINE_MIN_WAVES) void hemRefineKernel(con
 }
 
 // The deferred cells, refined without deferral by a persistent grid of
-// single-wave workgroups, one queued cell per wave at a time with its line
-// searches spread over the lanes (the flash's WAVE mode; the count is read
-// on the device).
+// single-wave workgroups: one queued cell per wave at a time with its line
+// searches spread over the lanes (the flash's WAVE mode) while the queue is
+// short, one cell per lane for a long queue (RAMD_WAVE_QUEUE_CELLS; the count
+// is read on the device).
+template <int NS, int NL, bool WAVE>
+__device__ inline void hemRefineQueued(const HemModel<NS>& model, const double* __restrict__ q,
+                                       const double* __restrict__ energy, const ThermoOut* __restrict__ guess,
+                                       uint32_t c, const CandidateSet& set, bool lead,
+                                       HemOutputDev* __restrict__ out, unsigned long long* __restrict__ totals) {
+    Counts n = {0, 0, 0, 0, 0, 0, 0};
+    Flash<NS, NL, double, true, false, false, kModelLds, WAVE> f(model, n);
+    loadHemCell<NS>(f, q, energy, guess[c], c);
+    EvalT<double, true> result;
+    zeroEval(result);
+    const bool ok = f.refine(set, result);
+    if (lead) storeHemCell(ok, result, out[c]);
+    const Counts none = {0, 0, 0, 0, 0, 0, 0};
+    addTotals(lead ? n : none, lead && ok ? 1u : 0u, totals);
+}
 template <int NS, int NL>
 __global__ __launch_bounds__(64, RAMD_REFINE_MIN_WAVES) void hemRefineQueueKernel(
     const HemModel<NS>* __restrict__ globalModel, const double* __restrict__ q, const double* __restrict__ energy,
     const ThermoOut* __restrict__ guess, const uint32_t* __restrict__ queue, const unsigned* __restrict__ queueCount,
     const CandidateSet* __restrict__ sets, HemOutputDev* __restrict__ out, unsigned long long* __restrict__ totals) {
     const unsigned count = *queueCount;
-    if (blockIdx.x >= count) return;  // uniform per workgroup
+    const bool perWave = count <= RAMD_WAVE_QUEUE_CELLS;  // uniform
+    if (blockIdx.x * (perWave ? 1u : 64u) >= count) return;  // uniform per workgroup
     const HemModel<NS>& model = workgroupModel(globalModel);
-    const bool lead = threadIdx.x == 0;
-    for (unsigned i = blockIdx.x; i < count; i += gridDim.x) {  // uniform per workgroup
-        Counts n = {0, 0, 0, 0, 0, 0, 0};
-        const uint32_t c = queue[i];
-        Flash<NS, NL, double, true, false, false, kModelLds, true> f(model, n);
-        loadHemCell<NS>(f, q, energy, guess[c], c);
-        EvalT<double, true> result;
-        zeroEval(result);
-        const bool ok = f.refine(sets[c], result);
-        if (lead) storeHemCell(ok, result, out[c]);
-        const Counts none = {0, 0, 0, 0, 0, 0, 0};
-        addTotals(lead ? n : none, lead && ok ? 1u : 0u, totals);
+    if (perWave) {
+        for (unsigned i = blockIdx.x; i < count; i += gridDim.x)  // uniform per workgroup
+            hemRefineQueued<NS, NL, true>(model, q, energy, guess, queue[i], sets[queue[i]], threadIdx.x == 0, out,
+                                          totals);
+    } else {
+        for (unsigned base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u) {  // uniform per workgroup
+            const unsigned i = base + threadIdx.x;
+            if (i < count) {
+                hemRefineQueued<NS, NL, false>(model, q, energy, guess, queue[i], sets[queue[i]], true, out, totals);
+            } else {
+                const Counts none = {0, 0, 0, 0, 0, 0, 0};
+                addTotals(none, 0u, totals);
+            }
+        }
     }
 }
 

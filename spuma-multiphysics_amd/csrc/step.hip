@@ -110,19 +110,35 @@ __global__ __launch_bounds__(256, RAMD_REFINE_MIN_WAVES) void stepRefineKernel(c
     countFlash(failed, n.phases, counters);
 }
 
-// 1: the deferred refinements run one cell per wave with the frozen split's
-// line searches spread over its lanes; 0: one cell per lane.
-#ifndef RAMD_REFINE_WAVE
-#define RAMD_REFINE_WAVE 1
-#endif
+// One deferred cell refined without deferral; lead: this lane stores it.
+template <int NS, int NL, bool WAVE>
+__device__ inline void refineQueued(const HemModel<NS>& model, const double* __restrict__ q, size_t stride,
+                                    const ThermoOut* __restrict__ seeds, uint32_t c, const CandidateSet& set,
+                                    bool lead, ThermoOut* __restrict__ out, double* __restrict__ pOut,
+                                    double* __restrict__ cOut, double* __restrict__ rOut, uint16_t* __restrict__ cost,
+                                    uint8_t* __restrict__ failMark, Counts& n, uint32_t& failed) {
+    const ThermoOut g = seeds[c];
+    Flash<NS, NL, double, true, false, false, kModelLds, WAVE> f(model, n);
+    loadCell(f, q, stride, g, c, NS);
+    EvalT<double, true> result;
+    zeroEval(result);
+    const bool ok = f.refine(set, result);
+    if (lead) {
+        storeCell(ok, result, g, c, out, pOut, cOut, rOut);
+        if (!ok && failMark) failMark[c] = 1;
+        if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
+        failed = ok ? 0u : 1u;
+    }
+}
 
 // The cells the refinement deferred (full FP64 search, FP64 restarts, or a
 // frozen candidate's solve from the cell's guess), refined again without
-// deferral: a persistent grid of single-wave workgroups, each wave on one
-// queued cell at a time with its line searches spread over its lanes (the
-// flash's WAVE mode), so that the few long refinements neither share a wave
-// nor halve their steps one evaluation at a time.  queueCount is read on the
-// device (the launch does not depend on it).
+// deferral by a persistent grid of single-wave workgroups: one queued cell
+// per wave at a time (every lane on the cell, its line searches spread over
+// the lanes) while the queue is short, so that the few long refinements
+// neither share a wave nor halve their steps one evaluation at a time; one
+// cell per lane for a long queue.  queueCount is read on the device (the
+// launch does not depend on it).
 template <int NS, int NL>
 __global__ __launch_bounds__(64, RAMD_REFINE_MIN_WAVES) void stepRefineQueueKernel(
     const HemModel<NS>* __restrict__ globalModel, const double* __restrict__ q, size_t stride,
@@ -131,38 +147,29 @@ __global__ __launch_bounds__(64, RAMD_REFINE_MIN_WAVES) void stepRefineQueueKern
     double* __restrict__ pOut, double* __restrict__ cOut, double* __restrict__ rOut, uint16_t* __restrict__ cost,
     unsigned long long* __restrict__ counters, uint8_t* __restrict__ failMark) {
     const unsigned count = *queueCount;
-#if RAMD_REFINE_WAVE
-    if (blockIdx.x >= count) return;  // uniform per workgroup: usually every workgroup leaves here
+    const bool perWave = count <= RAMD_WAVE_QUEUE_CELLS;  // uniform
+    if (blockIdx.x * (perWave ? 1u : 64u) >= count) return;  // uniform per workgroup: usually every workgroup leaves here
     const HemModel<NS>& model = workgroupModel(globalModel);
-    const bool lead = threadIdx.x == 0;
-    for (unsigned i = blockIdx.x; i < count; i += gridDim.x) {  // uniform per workgroup
-        const bool mine = true;
-#else
-    if (blockIdx.x * 64u >= count) return;
-    const HemModel<NS>& model = workgroupModel(globalModel);
-    const bool lead = true;
-    for (unsigned base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u) {  // uniform per workgroup
-        const unsigned i = base + threadIdx.x;
-        const bool mine = i < count;
-#endif
-        Counts n = {0, 0, 0, 0, 0, 0, 0};
-        uint32_t failed = 0;
-        if (mine && *dtDev > 0) {
-            const uint32_t c = queue[i];
-            const ThermoOut g = seeds[c];
-            Flash<NS, NL, double, true, false, false, kModelLds, RAMD_REFINE_WAVE != 0> f(model, n);
-            loadCell(f, q, stride, g, c, NS);
-            EvalT<double, true> result;
-            zeroEval(result);
-            const bool ok = f.refine(sets[c], result);
-            if (lead) {
-                storeCell(ok, result, g, c, out, pOut, cOut, rOut);
-                if (!ok && failMark) failMark[c] = 1;
-                if (cost) cost[c] = uint16_t(n.phases < 65535u ? n.phases : 65535u);
-                failed = ok ? 0u : 1u;
-            }
+    if (perWave) {
+        const bool lead = threadIdx.x == 0;
+        for (unsigned i = blockIdx.x; i < count; i += gridDim.x) {  // uniform per workgroup
+            Counts n = {0, 0, 0, 0, 0, 0, 0};
+            uint32_t failed = 0;
+            if (*dtDev > 0)
+                refineQueued<NS, NL, true>(model, q, stride, seeds, queue[i], sets[queue[i]], lead, out, pOut, cOut,
+                                           rOut, cost, failMark, n, failed);
+            countFlash(failed, lead ? n.phases : 0u, counters);
         }
-        countFlash(failed, lead ? n.phases : 0u, counters);
+    } else {
+        for (unsigned base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u) {  // uniform per workgroup
+            const unsigned i = base + threadIdx.x;
+            Counts n = {0, 0, 0, 0, 0, 0, 0};
+            uint32_t failed = 0;
+            if (i < count && *dtDev > 0)
+                refineQueued<NS, NL, false>(model, q, stride, seeds, queue[i], sets[queue[i]], true, out, pOut, cOut,
+                                            rOut, cost, failMark, n, failed);
+            countFlash(failed, n.phases, counters);
+        }
     }
 }
 __global__ void clearQueueKernel(unsigned* __restrict__ count) {

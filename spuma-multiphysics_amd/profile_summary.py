@@ -36,12 +36,18 @@ def pmc(db, pattern=None):
     for (name, did), counters in per.items():
         kernels[name].append((did, dur[(name, did)], counters))
     for name, items in kernels.items():
+        items.sort(key=lambda t: t[0])  # launch order
         names = sorted({c for _, _, cs in items for c in cs})
         print(f"{name[:100]}")
         print(f"  dispatches {len(items)}, mean duration {sum(d for _, d, _ in items) / len(items) / 1e3:.3f} us")
         for c in names:
             vals = [cs.get(c, 0.0) for _, _, cs in items]
-            print(f"  {c:28s} mean {sum(vals) / len(vals):16.1f}   per dispatch {[round(v, 1) for v in vals[:6]]}")
+            print(f"  {c:28s} mean {sum(vals) / len(vals):16.1f}   per dispatch {[round(v, 1) for v in vals[:6]]}"
+                  + (f" ... last {[round(v, 1) for v in vals[-4:]]}" if len(vals) > 6 else ""))
+        durations = [d for _, d, _ in items]
+        if len(durations) > 6:
+            print(f"  {'duration us':28s} first {[round(d / 1e3, 1) for d in durations[:6]]} ... last "
+                  f"{[round(d / 1e3, 1) for d in durations[-4:]]}")
 
 
 def dispatches(db, pattern):
