@@ -77,6 +77,17 @@ inline unsigned xcdGrid(long logicalBlocks) {
 #define RAMD_TILE_Z 4
 #endif
 constexpr int kTileX = 32, kTileY = 8, kChunkZ = RAMD_TILE_Z;
+
+// 1: interior cells whose faces sum in kOwnerNeighbourOrder take a path with
+// no per-value choice of face (the same sum, bit for bit).
+#ifndef RAMD_STAGE_INTERIOR
+#define RAMD_STAGE_INTERIOR 1
+#endif
+// Slot order (3 bits per CSR position, slots 0..5 = -x +x -y +y -z +z) of an
+// interior cell when internal faces are numbered by owner, then neighbour
+// (an upper-triangular face list): the faces it neighbours (-z, -y, -x, by
+// their owners' index), then the faces it owns (+x, +y, +z).
+constexpr uint32_t kOwnerNeighbourOrder = 4u | 2u << 3 | 0u << 6 | 1u << 9 | 3u << 12 | 5u << 15;
 constexpr int kChunkDepths[3] = {2, 4, 8};
 
 template <int NS, int NV, bool HLLC>
@@ -293,6 +304,18 @@ __device__ inline void tiledStageBlock(const CartesianArgs& a, long b, TileShare
             const uint32_t order = a.slotOrder[type];
             double rate[NV];
             for (int v = 0; v < NV; ++v) rate[v] = 0;
+            if (RAMD_STAGE_INTERIOR && type == 0 && order == kOwnerNeighbourOrder) {
+                // interior cell of an owner-then-neighbour face list: the
+                // generic sum below in its order, with no per-value choice of face
+                for (int v = 0; v < NV; ++v) {
+                    rate[v] += 1.0 * fzMinus[v] * a.invV;
+                    rate[v] += 1.0 * fyMinus[v] * a.invV;
+                    rate[v] += 1.0 * fxMinus[v] * a.invV;
+                    rate[v] += -1.0 * faceX[ty][tx][v] * a.invV;
+                    rate[v] += -1.0 * faceY[ty][tx][v] * a.invV;
+                    rate[v] += -1.0 * fzPlus[v] * a.invV;
+                }
+            } else
             for (int pos = 0; pos < 6; ++pos) {
                 const int slot = int((order >> (3 * pos)) & 7u);
                 const int ax = slot >> 1, plus = slot & 1;
