@@ -431,7 +431,8 @@ int reactive_amd_transport_stage_adjoint(void* transport, const double* q, const
     if (!lock.owns_lock()) return 2;
     try {
         require(q && states && rateBar && qBar && pBar && cBar, "Null stage adjoint argument");
-        require(!t.viscous(), "The stage adjoint covers the inviscid flux (viscosity, conductivity and WALE off)");
+        require(!t.viscous() && !t.capillary,
+                "The stage adjoint covers the inviscid flux (viscosity, conductivity, WALE and capillary model off)");
         require(t.cartesian.valid, "The stage adjoint needs a structured-box transport mesh");
         require(t.scheme == kHll, "The stage adjoint covers the HLL flux");
         require(t.ns == 1 && t.nv == 5, "The stage adjoint covers one species (nv = 5)");

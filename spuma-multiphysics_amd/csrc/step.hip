@@ -1138,7 +1138,8 @@ static Step* createStep(const std::vector<Transport*>& list, HemHandle* hem) {
     size_t total = 0;
     for (Transport* t : list) {
         require(t, "Null transport handle");
-        require(!t->viscous(), "The resident step carries the inviscid flux (viscosity, conductivity and WALE off)");
+        require(!t->viscous() && !t->capillary,
+                "The resident step carries the inviscid flux (viscosity, conductivity, WALE and capillary model off)");
         require(t->cartesian.valid, "The resident step needs a structured-box transport mesh");
         require(t->device == hem->device, "Transport and HEM handles are on different devices");
         require(size_t(hem->ns) == t->cfg.species && t->cfg.variables == t->cfg.species + 4,

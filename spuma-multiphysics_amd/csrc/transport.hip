@@ -199,8 +199,8 @@ __device__ inline double faceRhoNut(const FaceDev& f, double rhoL, double rhoR, 
 // the interpolated face velocity, and -k (T_R - T_L) / distance off slip walls.
 __device__ inline double4 viscousWork(const FaceDev& f, const double* ul, const double* ur,
                                       const ReactiveTransportState& sl, const ReactiveTransportState& sr,
-                                      const ViscousArgs& v) {
-    double t[3] = {0, 0, 0}, energy = 0;
+                                      const ViscousArgs& v, double energy0 = 0.0) {
+    double t[3] = {0, 0, 0}, energy = energy0;
     if (v.viscosity > 0 || v.nut) {
         double g[9];
         for (int j = 0; j < 9; ++j) {
@@ -229,6 +229,228 @@ __device__ inline double4 viscousWork(const FaceDev& f, const double* ul, const 
     return make_double4(t[0], t[1], t[2], energy);
 }
 
+// ---- resolved interface with surface tension (sigma > 0) -----------------
+// The material colour (cells + fixed) is supplied by the caller; the
+// geometry kernels below turn it into a per-cell interface normal, area
+// density |grad colour| and curvature, and the face kernel then takes the
+// contact-preserving capillary flux in place of HLL.
+struct CapillaryArgs {
+    const double* color;        // null: no capillary model
+    const double* normal;       // cells x 3, pointing from colour 1 to colour 0
+    const double* areaDensity;  // cells
+    const double* curvature;    // cells
+    unsigned* error;            // set by a rejected geometry or face
+    double sigma;
+    int species;                // the condensable; its liquid inventory is variable ns + 4
+};
+
+// Face colour (faceColor, reactiveUnstructuredInterface.h:33-42): the far
+// side is the neighbour, the fixed state's colour, or the owner itself.
+__device__ inline double interfaceFaceColor(const FaceDev& f, const double* __restrict__ color) {
+    return f.weight * color[f.owner] + (1 - f.weight) * color[f.right];
+}
+
+// Face normal of the interface (faceNormal, reactiveUnstructuredInterface.h:
+// 75-87): the interpolated cell normals (a boundary face takes the owner's on
+// both sides), scaled to unit length when not zero.
+__device__ inline void interfaceFaceNormal(const FaceDev& f, const double* __restrict__ normal, double* out) {
+    const size_t l = size_t(f.owner), r = f.kind == 0 ? size_t(f.right) : l;
+    double mag2 = 0;
+    for (int d = 0; d < 3; ++d) {
+        out[d] = f.weight * normal[3 * l + d] + (1 - f.weight) * normal[3 * r + d];
+        mag2 += out[d] * out[d];
+    }
+    if (mag2 > 0) {
+        const double inv = 1 / sqrt(mag2);
+        for (int d = 0; d < 3; ++d) out[d] *= inv;
+    }
+}
+
+// InterfaceGradient (reactiveTransportKernels.h:200-204 -> gradientCell,
+// reactiveUnstructuredInterface.h:44-73): Gauss gradient of the colour with
+// the cell's own colour taken from every face value (a uniform colour has
+// none), its length as the interface area density, and the unit normal
+// -g/|g| where |g| times the cell length cbrt(V) (length[c], taken on the
+// host) exceeds epsilon, else zero.  A colour outside [0, 1] or a non-finite
+// result sets *error.
+__global__ __launch_bounds__(256) void interfaceGradientKernel(
+    const FaceDev* __restrict__ faces, const uint32_t* __restrict__ row, const int32_t* __restrict__ incidence,
+    const double* __restrict__ invV, const double* __restrict__ length, size_t cells, double epsilon,
+    const double* __restrict__ color, double* __restrict__ gradient, double* __restrict__ normal,
+    double* __restrict__ areaDensity, unsigned* __restrict__ error) {
+    const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const double own = color[c];
+    if (!(isfinite(own) && own >= 0 && own <= 1)) {
+        atomicOr(error, 1u);
+        return;
+    }
+    double g[3] = {0, 0, 0};
+    for (uint32_t e = row[c]; e < row[c + 1]; ++e) {
+        const int32_t entry = incidence[e];
+        const FaceDev& f = faces[entry < 0 ? -entry - 1 : entry - 1];
+        const double orientation = entry < 0 ? 1.0 : -1.0;
+        const double cf = interfaceFaceColor(f, color);
+        if (!(isfinite(cf) && cf >= 0 && cf <= 1)) {
+            atomicOr(error, 1u);
+            return;
+        }
+        for (int d = 0; d < 3; ++d) g[d] += (cf - own) * orientation * f.area * f.n[d];
+    }
+    double mag2 = 0;
+    for (int d = 0; d < 3; ++d) {
+        g[d] *= invV[c];
+        mag2 += g[d] * g[d];
+        if (!isfinite(g[d])) {
+            atomicOr(error, 1u);
+            return;
+        }
+        gradient[3 * c + d] = g[d];
+    }
+    const double mag = sqrt(mag2);
+    if (!isfinite(mag) || mag < 0) {
+        atomicOr(error, 1u);
+        return;
+    }
+    areaDensity[c] = mag;
+    const bool active = mag * length[c] > epsilon;
+    for (int d = 0; d < 3; ++d) normal[3 * c + d] = active ? -g[d] / mag : 0.0;
+}
+
+// InterfaceCurvature (reactiveTransportKernels.h:205-210 -> curvatureCell,
+// reactiveUnstructuredInterface.h:89-100): divergence of the face normals
+// relative to the cell's own, and the surface energy sigma |grad colour|.
+__global__ __launch_bounds__(256) void interfaceCurvatureKernel(
+    const FaceDev* __restrict__ faces, const uint32_t* __restrict__ row, const int32_t* __restrict__ incidence,
+    const double* __restrict__ invV, size_t cells, double sigma, const double* __restrict__ normal,
+    const double* __restrict__ areaDensity, double* __restrict__ curvature, double* __restrict__ energy,
+    unsigned* __restrict__ error) {
+    const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    double sum = 0;
+    for (uint32_t e = row[c]; e < row[c + 1]; ++e) {
+        const int32_t entry = incidence[e];
+        const FaceDev& f = faces[entry < 0 ? -entry - 1 : entry - 1];
+        const double orientation = entry < 0 ? 1.0 : -1.0;
+        double nf[3];
+        interfaceFaceNormal(f, normal, nf);
+        for (int d = 0; d < 3; ++d) sum += (nf[d] - normal[3 * c + d]) * orientation * f.area * f.n[d];
+    }
+    const double candidate = sum * invV[c];
+    if (isfinite(candidate)) curvature[c] = candidate;
+    else atomicOr(error, 1u);
+    energy[c] = sigma * areaDensity[c];
+}
+
+// One side of a capillary face (State, reactiveBalancedCapillary.h:18-20).
+struct CapillarySide {
+    double rho, u[3], p, sound, energy, color;
+};
+
+__device__ inline bool capillarySideValid(const CapillarySide& s) {  // valid (:40-47)
+    if (!isfinite(s.rho) || s.rho <= 0 || !isfinite(s.p) || s.p <= 0 || !isfinite(s.sound) || s.sound <= 0 ||
+        !isfinite(s.energy) || !isfinite(s.color) || s.color < 0 || s.color > 1)
+        return false;
+    for (int d = 0; d < 3; ++d)
+        if (!isfinite(s.u[d])) return false;
+    return true;
+}
+
+// Capillary face flux: the face's interface geometry (faceGeometry,
+// reactiveUnstructuredInterface.h:102-128: interpolated curvature, colour
+// and area density, surface stress sigma |grad c| (I - n n)) and the
+// contact-preserving flux (faceFlux, reactiveBalancedCapillary.h:55-142):
+// the HLLC solve on pi = p - sigma kappa colour (HLL where the contact
+// leaves the acoustic bounds), plus the shared capillary traction
+// sigma kappa c_f n - C n and its work with the face velocity.  Writes the
+// four face coefficients to w, the capillary momentum flux to capM and its
+// energy flux to capE; false where the reference rejects the face.
+__device__ inline bool capillaryFlux(const FaceDev& f, const CapillarySide& L, const CapillarySide& R,
+                                     const CapillaryArgs& ca, double4& w, double* capM, double& capE) {
+    const size_t l = size_t(f.owner), r = f.kind == 0 ? size_t(f.right) : l;
+    const double curv = f.weight * ca.curvature[l] + (1 - f.weight) * ca.curvature[r];
+    const double colorFace = interfaceFaceColor(f, ca.color);
+    double ni[3];
+    interfaceFaceNormal(f, ca.normal, ni);
+    const double scale = ca.sigma * (f.weight * ca.areaDensity[l] + (1 - f.weight) * ca.areaDensity[r]);
+    double stress[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) stress[3 * i + j] = scale * ((i == j ? 1.0 : 0.0) - ni[i] * ni[j]);
+    if (!isfinite(curv) || !isfinite(colorFace) || colorFace < 0 || colorFace > 1 || !isfinite(scale)) return false;
+    if (!capillarySideValid(L) || !capillarySideValid(R)) return false;
+    double n2 = 0, unL = 0, unR = 0;
+    for (int d = 0; d < 3; ++d) {
+        if (!isfinite(f.n[d])) return false;
+        n2 += f.n[d] * f.n[d];
+        unL += L.u[d] * f.n[d];
+        unR += R.u[d] * f.n[d];
+    }
+    if (!isfinite(n2) || ::fabs(n2 - 1) > 1e-10) return false;
+    for (int j = 0; j < 9; ++j)
+        if (!isfinite(stress[j])) return false;
+    const double piL = L.p - ca.sigma * curv * L.color;
+    const double piR = R.p - ca.sigma * curv * R.color;
+    if (!isfinite(piL) || !isfinite(piR)) return false;
+    const double sl = dmin(0, dmin(unL - L.sound, unR - R.sound));
+    const double sr = dmax(0, dmax(unL + L.sound, unR + R.sound));
+    const double den = L.rho * (sl - unL) - R.rho * (sr - unR);
+    if (!isfinite(den) || den == 0 || !isfinite(sl) || !isfinite(sr) || sr <= sl) return false;
+    const double star = (piR - piL + L.rho * unL * (sl - unL) - R.rho * unR * (sr - unR)) / den;
+    double advectL = 0, advectR = 0, contact, mass, pMom, pEn;
+    if (!isfinite(star) || star <= sl || star >= sr) {  // the contact outran the bounds: HLL
+        const double inv = 1 / (sr - sl);
+        contact = (sr * unL - sl * unR) * inv;
+        advectL = sr * (unL - sl) * inv;
+        advectR = sl * (sr - unR) * inv;
+        mass = advectL * L.rho + advectR * R.rho;
+        pMom = (sr * piL - sl * piR) * inv;
+        pEn = (sr * piL * unL - sl * piR * unR) * inv;
+    } else {
+        const bool useLeft = star >= 0;
+        const CapillarySide& s = useLeft ? L : R;
+        const double un = useLeft ? unL : unR, wave = useLeft ? sl : sr, pi = useLeft ? piL : piR;
+        const bool supersonic = (useLeft && sl >= 0) || (!useLeft && sr <= 0);
+        contact = supersonic ? un : star;
+        if (supersonic) {
+            mass = s.rho * un;
+            pMom = pi;
+            pEn = pi * un;
+        } else {
+            const double rhoStar = s.rho * (wave - un) / (wave - star);
+            const double piStar = pi + s.rho * (wave - un) * (star - un);
+            const double energyStar = ((wave - un) * s.energy - pi * un + piStar * star) / (wave - star);
+            mass = rhoStar * star;
+            pMom = piStar + mass * (star - un);
+            pEn = star * (energyStar + piStar) - mass * s.energy / s.rho;
+        }
+        if (useLeft) advectL = mass / L.rho;
+        else advectR = mass / R.rho;
+    }
+    const double advective = advectL * L.energy + advectR * R.energy;
+    double fu[3], average = 0;
+    for (int d = 0; d < 3; ++d) {
+        fu[d] = 0.5 * (L.u[d] + R.u[d]);
+        average += fu[d] * f.n[d];
+    }
+    for (int d = 0; d < 3; ++d) fu[d] += (contact - average) * f.n[d];
+    capE = 0;
+    bool finiteMomentum = true;
+    for (int i = 0; i < 3; ++i) {
+        double traction = 0;
+        for (int j = 0; j < 3; ++j) traction += stress[3 * i + j] * f.n[j];
+        capM[i] = ca.sigma * curv * colorFace * f.n[i] - traction;
+        const double momentum = advectL * L.rho * L.u[i] + advectR * R.rho * R.u[i] + pMom * f.n[i] + capM[i];
+        finiteMomentum = finiteMomentum && isfinite(momentum) && isfinite(capM[i]);
+        capE += fu[i] * capM[i];
+    }
+    const double total = advective + pEn + capE;
+    if (!isfinite(mass) || !isfinite(pMom) || !isfinite(pEn) || !isfinite(capE) || !isfinite(total) ||
+        !finiteMomentum)
+        return false;
+    w = make_double4(advectL, advectR, pMom, pEn);
+    return true;
+}
+
 // Face coefficients (Faces, reactiveTransportKernels.h:343-356): HLL, or the
 // HLLC option (flux_device.h), which also reads the total energies from q.
 // A face the HLLC flux rejects keeps its HLL coefficients and is counted.
@@ -239,7 +461,7 @@ __global__ void hllKernel(const FaceDev* __restrict__ faces, size_t nf, const do
                           const ReactiveTransportState* __restrict__ state, double waveFactor,
                           double4* __restrict__ work, int scheme, const double* __restrict__ q, int ns,
                           unsigned long long* __restrict__ failures, ViscousArgs va, double4* __restrict__ visc,
-                          unsigned* __restrict__ waleError) {
+                          unsigned* __restrict__ waleError, CapillaryArgs ca) {
     const size_t fi = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (fi >= nf) return;
     const FaceDev f = faces[fi];
@@ -254,13 +476,41 @@ __global__ void hllKernel(const FaceDev* __restrict__ faces, size_t nf, const do
     w.y = left * (right - unR) * invWave;
     w.z = (right * pL - left * pR) * invWave;
     w.w = (right * pL * unL - left * pR * unR) * invWave;
-    if (scheme == kHllc &&
+    if (scheme == kHllc && !ca.color &&
         !hllcCoefficients(state[f.owner].rho, ul, pL, aL, q[size_t(ns + 3) * all + f.owner], state[f.right].rho, ur,
                           pR, aR, q[size_t(ns + 3) * all + f.right], f.n, w))
         atomicAdd(failures, 1ull);
+    // capillary model (reactiveTransportKernels.h:357-417): its flux replaces
+    // the coefficients, its energy flux opens the face's energy record and its
+    // momentum flux is taken from the traction after the viscous part (:493)
+    double capM[3] = {0, 0, 0}, energy0 = 0;
+    if (ca.color) {
+        const CapillarySide L{state[f.owner].rho, {ul[0], ul[1], ul[2]}, pL, aL,
+                              q[size_t(ns + 3) * all + f.owner], ca.color[f.owner]};
+        const CapillarySide R{state[f.right].rho, {ur[0], ur[1], ur[2]}, pR, aR,
+                              q[size_t(ns + 3) * all + f.right], ca.color[f.right]};
+        double capE = 0;
+        if (!capillaryFlux(f, L, R, ca, w, capM, capE)) {
+            atomicOr(ca.error, 1u);
+            return;
+        }
+        energy0 += capE;
+    }
     work[fi] = w;
     if (V) {
-        const double4 v = viscousWork(f, ul, ur, state[f.owner], state[f.right], va);
+        double4 v = viscousWork(f, ul, ur, state[f.owner], state[f.right], va, energy0);
+        if (ca.color) {
+            // the liquid inventory of the condensable on both sides (:478-486)
+            const size_t sk = size_t(ca.species), lk = size_t(ns + 4);
+            const double slq = q[sk * all + f.owner], srq = q[sk * all + f.right];
+            const double ml = q[lk * all + f.owner], mr = q[lk * all + f.right];
+            if (!isfinite(slq) || !isfinite(srq) || !isfinite(ml) || !isfinite(mr) || slq < 0 || srq < 0 || ml < 0 ||
+                mr < 0 || ml > slq || mr > srq)
+                atomicOr(ca.error, 1u);
+            v.x -= capM[0];
+            v.y -= capM[1];
+            v.z -= capM[2];
+        }
         if (va.nut && !(isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w))) atomicOr(waleError, 2u);
         visc[fi] = v;
     }
@@ -378,6 +628,15 @@ __global__ void faceSpeedKernel(const FaceDev* __restrict__ faces, size_t nf, co
     speed[fi] = s;
 }
 
+// The capillary wave step's inputs (areaDensity null: no capillary model);
+// length[c] = cbrt(V_c), taken on the host.
+struct CapillaryStepArgs {
+    const double* areaDensity;
+    const double* length;
+    const ReactiveTransportState* state;
+    double sigma, cfl, epsilon;
+};
+
 // Step (reactiveTransportKernels.h:529-550) + the absorbing validity map of
 // Minimum (reactiveTransport.cpp:222-232): invalid -> -1; block minimum.
 __global__ __launch_bounds__(256) void cellStepKernel(const FaceDev* __restrict__ faces,
@@ -385,7 +644,8 @@ __global__ __launch_bounds__(256) void cellStepKernel(const FaceDev* __restrict_
                                                       const uint32_t* __restrict__ row,
                                                       const int32_t* __restrict__ incidence,
                                                       const double* __restrict__ invV, size_t cells, double cfl,
-                                                      double maximum, double* __restrict__ blockMin) {
+                                                      double maximum, double* __restrict__ blockMin,
+                                                      CapillaryStepArgs cs) {
     const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     double v = __builtin_huge_val();
     if (c < cells) {
@@ -401,6 +661,14 @@ __global__ __launch_bounds__(256) void cellStepKernel(const FaceDev* __restrict_
         if (bad) step = -1;
         else {
             step = denominator > 0 ? dmin(maximum, cfl / (invV[c] * denominator)) : maximum;
+            if (cs.areaDensity && cs.sigma > 0) {  // capillary wave step (reactiveTransportKernels.h:540-547)
+                constexpr double pi = 3.141592653589793238462643383279502884;
+                const double volume = 1 / invV[c];
+                if (cs.areaDensity[c] * cs.length[c] > cs.epsilon) {
+                    const double capDt = cs.cfl * sqrt(cs.state[c].rho * volume / (pi * cs.sigma));
+                    step = dmin(step, capDt);
+                }
+            }
             if (!isfinite(denominator) || denominator < 0) step = -1;
         }
         v = isfinite(step) && step > 0 ? step : -1;
@@ -742,6 +1010,7 @@ void Transport::primitives() {
 // then the WALE viscosities or, for the flux of a viscous model, the velocity
 // gradients, then the face coefficients or the CFL face speeds.
 void Transport::prepareResident(const ReactiveTransportState* states, bool transport) {
+    require(!capillary || colorReady, "Capillary geometry must be uploaded for this stage");
     uploadState(states);
     primitives();
     if (wale) buildWale();
@@ -752,16 +1021,30 @@ void Transport::prepareResident(const ReactiveTransportState* states, bool trans
 
 void Transport::launchFaces() {
     const ViscousArgs va{gradient, wale ? nut : nullptr, cfg.viscosity, cfg.conductivity};
+    CapillaryArgs ca{nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, 0};
+    if (capillary) {
+        ca = CapillaryArgs{capColor, capNormal, capArea, capCurvature, capError, capSigma, capSpecies};
+        RAMD_HIP(hipMemsetAsync(capError, 0, sizeof(unsigned), stream));
+    }
     if (wale) RAMD_HIP(hipMemsetAsync(waleError, 0, sizeof(unsigned), stream));
-    if (viscous())
+    if (faceRecords())
         hipLaunchKernelGGL(hllKernel<true>, dim3(gridFor(cfg.faces, 256)), dim3(256), 0, stream, faces, cfg.faces,
-                           prim, all, state, cfg.waveFactor, work, scheme, q, ns, fluxFailures, va, visc, waleError);
+                           prim, all, state, cfg.waveFactor, work, scheme, q, ns, fluxFailures, va, visc, waleError,
+                           ca);
     else
         hipLaunchKernelGGL(hllKernel<false>, dim3(gridFor(cfg.faces, 256)), dim3(256), 0, stream, faces, cfg.faces,
-                           prim, all, state, cfg.waveFactor, work, scheme, q, ns, fluxFailures, va, visc, waleError);
+                           prim, all, state, cfg.waveFactor, work, scheme, q, ns, fluxFailures, va, visc, waleError,
+                           ca);
     RAMD_HIP(hipGetLastError());
     launched();
     ++deviceProfile.transportFaceLaunches;
+    if (capillary) {  // reactiveTransport.cpp:1405-1408
+        unsigned status = 0;
+        RAMD_HIP(hipMemcpyAsync(&status, capError, sizeof(status), hipMemcpyDeviceToHost, stream));
+        finish();
+        require(!status, "Invalid capillary face flux");
+        capillaryProfile.faceFluxBuilds += cfg.faces;
+    }
     if (wale) {
         unsigned status = 0;
         RAMD_HIP(hipMemcpyAsync(&status, waleError, sizeof(status), hipMemcpyDeviceToHost, stream));
@@ -829,6 +1112,108 @@ void Transport::installWale(const ReactiveWaleOptionsV1& options) {
     waleProfile.workspaceBytes = stats.allocatedBytes - before;
 }
 
+// setCapillary (reactiveTransport.cpp:859-885): once, before any stage.
+// The cell length cbrt(V) that gates the interface normal and the capillary
+// wave step is taken on the host from the uploaded inverse volumes, as the
+// reference's host arithmetic takes it.
+void Transport::installCapillary(const ReactiveCapillaryOptionsV1& o) {
+    require(o.abiVersion == 1 && o.structBytes == sizeof(o) && std::isfinite(o.sigma) && o.sigma > 0 &&
+                std::isfinite(o.capillaryCfl) && o.capillaryCfl > 0 && o.capillaryCfl <= 1 &&
+                std::isfinite(o.geometryEpsilon) && o.geometryEpsilon >= 0 && o.geometryEpsilon < 1,
+            "Invalid capillary options ABI/model");
+    require(!capillary && !attemptOpen && !stats.stages && !highestVersion,
+            "Capillary model must be installed once before stepping");
+    require(!cfg.mechanical && size_t(nv) == cfg.species + 5 && o.condensableSpecies >= 0 &&
+                size_t(o.condensableSpecies) < cfg.species,
+            "Capillary model needs one mapped conserved liquid inventory");
+    require(!stepAttached, "The capillary model must be installed before a resident step is created on this transport");
+    std::vector<double> inverse(cfg.cells), length(cfg.cells);
+    downloadBytes(inverse.data(), invV, cfg.cells * sizeof(double));
+    finish();
+    for (size_t c = 0; c < cfg.cells; ++c) length[c] = std::cbrt(1 / inverse[c]);
+    capColor = allocate<double>(all);
+    capGradient = allocate<double>(cfg.cells * 3);
+    capNormal = allocate<double>(cfg.cells * 3);
+    capArea = allocate<double>(cfg.cells);
+    capCurvature = allocate<double>(cfg.cells);
+    capEnergy = allocate<double>(cfg.cells);
+    capLength = allocate<double>(cfg.cells);
+    capError = allocate<unsigned>(1);
+    if (!visc) visc = allocate<double4>(cfg.faces);
+    for (double* a : {capGradient, capNormal}) RAMD_HIP(hipMemsetAsync(a, 0, cfg.cells * 3 * sizeof(double), stream));
+    for (double* a : {capArea, capCurvature, capEnergy})
+        RAMD_HIP(hipMemsetAsync(a, 0, cfg.cells * sizeof(double), stream));
+    RAMD_HIP(hipMemsetAsync(capColor, 0, all * sizeof(double), stream));
+    upload(capLength, length.data(), cfg.cells);
+    finish();
+    capSigma = o.sigma;
+    capCfl = o.capillaryCfl;
+    capEpsilon = o.geometryEpsilon;
+    capSpecies = int(o.condensableSpecies);
+    capillary = true;
+    cartesian.valid = false;
+    capillaryProfile.capillaryWorkspaceBytes = sizeof(double) * (all + 9 * cfg.cells) + sizeof(uint32_t);
+}
+
+// capillaryGeometry (reactiveTransport.cpp:886-962), diffuse geometry: the
+// colours are checked and uploaded, the gradient and curvature kernels run,
+// and the surface energy (and, when asked, curvature and normals) come back.
+// The geometry is a pure function of the colours, so a request with the same
+// colour bytes (and no normals asked) returns the kept outputs.
+void Transport::capillaryGeometry(const double* cellColor, const double* fixedColor, double* energy,
+                                  double* curvature, double* normal) {
+    require(capillary && cellColor && energy && (!cfg.fixed || fixedColor),
+            "Missing capillary model, color or surface-energy output");
+    for (size_t c = 0; c < cfg.cells; ++c)
+        require(std::isfinite(cellColor[c]) && cellColor[c] >= 0 && cellColor[c] <= 1, "Invalid material color");
+    for (size_t c = 0; c < cfg.fixed; ++c)
+        require(std::isfinite(fixedColor[c]) && fixedColor[c] >= 0 && fixedColor[c] <= 1,
+                "Invalid fixed material color");
+    auto same = [](const std::vector<double>& kept, const double* a, size_t n) {
+        return kept.size() == n && (n == 0 || std::memcmp(kept.data(), a, n * sizeof(double)) == 0);
+    };
+    if (!normal && colorReady && geometryShadowValid && (!curvature || !curvatureShadow.empty()) &&
+        same(colorShadow, cellColor, cfg.cells) && same(fixedColorShadow, fixedColor, cfg.fixed)) {
+        std::copy(energyShadow.begin(), energyShadow.end(), energy);
+        if (curvature) std::copy(curvatureShadow.begin(), curvatureShadow.end(), curvature);
+        return;
+    }
+    geometryShadowValid = false;
+    colorReady = false;
+    RAMD_HIP(hipMemsetAsync(capError, 0, sizeof(unsigned), stream));
+    upload(capColor, cellColor, cfg.cells);
+    if (cfg.fixed) upload(capColor + cfg.cells, fixedColor, cfg.fixed);
+    const unsigned blocks = gridFor(cfg.cells, 256);
+    hipLaunchKernelGGL(interfaceGradientKernel, dim3(blocks), dim3(256), 0, stream, faces, rowDev, incidenceDev, invV,
+                       capLength, cfg.cells, capEpsilon, capColor, capGradient, capNormal, capArea, capError);
+    RAMD_HIP(hipGetLastError());
+    launched();
+    hipLaunchKernelGGL(interfaceCurvatureKernel, dim3(blocks), dim3(256), 0, stream, faces, rowDev, incidenceDev,
+                       invV, cfg.cells, capSigma, capNormal, capArea, capCurvature, capEnergy, capError);
+    RAMD_HIP(hipGetLastError());
+    launched();
+    downloadBytes(energy, capEnergy, cfg.cells * sizeof(double));
+    if (curvature) downloadBytes(curvature, capCurvature, cfg.cells * sizeof(double));
+    if (normal) downloadBytes(normal, capNormal, cfg.cells * 3 * sizeof(double));
+    unsigned status = 0;
+    downloadBytes(&status, capError, sizeof(status));
+    finish();
+    require(!status, "Invalid capillary geometry");
+    colorReady = true;
+    ++capillaryProfile.geometryBuilds;
+    colorShadow.assign(cellColor, cellColor + cfg.cells);
+    fixedColorShadow.assign(fixedColor ? fixedColor : cellColor, fixedColor ? fixedColor + cfg.fixed : cellColor);
+    energyShadow.assign(energy, energy + cfg.cells);
+    if (curvature) curvatureShadow.assign(curvature, curvature + cfg.cells);
+    else curvatureShadow.clear();
+    geometryShadowValid = true;
+    capillaryProfile.geometryCells += cfg.cells;
+    capillaryProfile.geometryKernels += 2;
+    capillaryProfile.colorUploadBytes += (cfg.cells + cfg.fixed) * sizeof(double);
+    capillaryProfile.geometryDownloadBytes +=
+        cfg.cells * sizeof(double) * (1 + (curvature ? 1 : 0) + (normal ? 3 : 0));
+}
+
 void Transport::launchFaceSpeeds() {
     hipLaunchKernelGGL(faceSpeedKernel, dim3(gridFor(cfg.faces, 256)), dim3(256), 0, stream, faces, cfg.faces, prim,
                        all, state, cfg.waveFactor, cfg.viscosity, cfg.conductivity, wale ? nut : nullptr, faceSpeed);
@@ -867,8 +1252,9 @@ void Transport::stableStep(double cfl, double maximum, double* dt) {  // reactiv
     require(dt && std::isfinite(cfl) && cfl > 0 && cfl <= .5 && std::isfinite(maximum) && maximum > 0,
             "Invalid transport CFL controls");
     const unsigned blocks = gridFor(cfg.cells, 256);
+    const CapillaryStepArgs cs{capillary ? capArea : nullptr, capLength, state, capSigma, capCfl, capEpsilon};
     hipLaunchKernelGGL(cellStepKernel, dim3(blocks), dim3(256), 0, stream, faces, faceSpeed, rowDev, incidenceDev, invV,
-                       cfg.cells, cfl, maximum, blockMin);
+                       cfg.cells, cfl, maximum, blockMin, cs);
     RAMD_HIP(hipGetLastError());
     hipLaunchKernelGGL(minReduceKernel, dim3(1), dim3(256), 0, stream, blockMin, size_t(blocks), blockMin + blocks);
     RAMD_HIP(hipGetLastError());
@@ -1247,14 +1633,24 @@ int reactive_transport_gas_properties_resident(void* t, const ReactiveTransportS
                                                double*, double*) {
     return stub(t, "reactive_transport_gas_properties_resident", "diffusive gas transport");
 }
-int reactive_transport_set_capillary_v1(void* t, const ReactiveCapillaryOptionsV1*) {
-    return stub(t, "reactive_transport_set_capillary_v1", "capillary transport");
+int reactive_transport_set_capillary_v1(void* t, const ReactiveCapillaryOptionsV1* options) {
+    return protect(t, [&](Transport& x) {  // reactiveTransport.cpp:1566-1568
+        require(options, "Null capillary options");
+        x.installCapillary(*options);
+    });
 }
-int reactive_transport_capillary_geometry_v1(void* t, const double*, const double*, double*, double*, double*) {
-    return stub(t, "reactive_transport_capillary_geometry_v1", "capillary transport");
+int reactive_transport_capillary_geometry_v1(void* t, const double* color, const double* fixedColor,
+                                             double* surfaceEnergy, double* curvature, double* normalXYZ) {
+    return protect(t, [&](Transport& x) {  // reactiveTransport.cpp:1591-1594
+        x.capillaryGeometry(color, fixedColor, surfaceEnergy, curvature, normalXYZ);
+    });
 }
-int reactive_transport_capillary_profile_v1(void* t, ReactiveCapillaryProfileV1*) {
-    return stub(t, "reactive_transport_capillary_profile_v1", "capillary transport");
+int reactive_transport_capillary_profile_v1(void* t, ReactiveCapillaryProfileV1* profile) {
+    return protect(t, [&](Transport& x) {  // reactiveTransport.cpp:1620-1624
+        require(profile && profile->abiVersion == 1 && profile->structBytes == sizeof(*profile),
+                "Invalid capillary profile ABI");
+        *profile = x.capillaryProfile;
+    });
 }
 int reactive_transport_replace_stage_state_v1(void* t, ReactiveTransportToken, uint64_t, const double*) {
     return stub(t, "reactive_transport_replace_stage_state_v1", "capillary stage replacement");

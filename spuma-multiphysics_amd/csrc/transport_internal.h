@@ -97,6 +97,20 @@ public:
     bool wale = false;
     double waleCw = 0;
     bool stepAttached = false;      // a resident step was created on this transport
+    // resolved interface with surface tension (reactive_transport_set_capillary_v1)
+    bool capillary = false, colorReady = false, geometryShadowValid = false;
+    double capSigma = 0, capCfl = 0, capEpsilon = 0;
+    int capSpecies = 0;
+    double* capColor = nullptr;        // cells + fixed
+    double* capGradient = nullptr;     // cells x 3
+    double* capNormal = nullptr;       // cells x 3
+    double* capArea = nullptr;         // |grad colour| per cell
+    double* capCurvature = nullptr;    // per cell
+    double* capEnergy = nullptr;       // sigma |grad colour| per cell
+    double* capLength = nullptr;       // cbrt(V) per cell (host cbrt)
+    unsigned* capError = nullptr;
+    std::vector<double> colorShadow, fixedColorShadow, energyShadow, curvatureShadow;
+    ReactiveCapillaryProfileV1 capillaryProfile{1, sizeof(ReactiveCapillaryProfileV1), 0, 0, 0, 0, 0, 0, 0};
     ReactiveWaleProfileV1 waleProfile{1, sizeof(ReactiveWaleProfileV1), 0, 0, 0, 0};
     CartesianInfo cartesian;
     // versions, tokens, attempts (reactiveTransport.cpp:384-393)
@@ -134,7 +148,12 @@ public:
     void primitives();
     // true where the face flux carries viscous, conductive or WALE terms
     bool viscous() const { return cfg.viscosity > 0 || cfg.conductivity > 0 || wale; }
+    // true where the face records carry a traction / energy part (visc)
+    bool faceRecords() const { return viscous() || capillary; }
     void installWale(const ReactiveWaleOptionsV1& options);
+    void installCapillary(const ReactiveCapillaryOptionsV1& options);
+    void capillaryGeometry(const double* cellColor, const double* fixedColor, double* energy, double* curvature,
+                           double* normal);
     void launchGradients(bool withWale);
     void buildWale();
     void launchFaces();

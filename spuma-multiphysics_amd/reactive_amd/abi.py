@@ -127,6 +127,29 @@ class WaleOptions(C.Structure):
         return cls(abiVersion=1, structBytes=C.sizeof(cls), Cw=cw)
 
 
+class CapillaryOptions(C.Structure):
+    """ReactiveCapillaryOptionsV1 (reactiveTransport.h:38-42)."""
+    _fields_ = [("abiVersion", C.c_uint32), ("structBytes", C.c_uint32), ("sigma", C.c_double),
+                ("capillaryCfl", C.c_double), ("geometryEpsilon", C.c_double), ("condensableSpecies", C.c_int64)]
+
+    @classmethod
+    def make(cls, sigma, capillary_cfl=0.5, epsilon=1e-6, species=0):
+        return cls(abiVersion=1, structBytes=C.sizeof(cls), sigma=sigma, capillaryCfl=capillary_cfl,
+                   geometryEpsilon=epsilon, condensableSpecies=species)
+
+
+class CapillaryProfile(C.Structure):
+    """ReactiveCapillaryProfileV1 (reactiveTransport.h:43-47)."""
+    _fields_ = [("abiVersion", C.c_uint32), ("structBytes", C.c_uint32), ("geometryBuilds", C.c_uint64),
+                ("geometryCells", C.c_uint64), ("geometryKernels", C.c_uint64), ("faceFluxBuilds", C.c_uint64),
+                ("colorUploadBytes", C.c_uint64), ("geometryDownloadBytes", C.c_uint64),
+                ("capillaryWorkspaceBytes", C.c_uint64)]
+
+    @classmethod
+    def fresh(cls):
+        return cls(abiVersion=1, structBytes=C.sizeof(cls))
+
+
 class WaleProfile(C.Structure):
     """ReactiveWaleProfileV1 (reactiveTurbulence.h:16-19)."""
     _fields_ = [("abiVersion", C.c_uint32), ("structBytes", C.c_uint32), ("gradientBuilds", C.c_uint64),

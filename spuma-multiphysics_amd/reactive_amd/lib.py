@@ -88,7 +88,7 @@ _SIGNATURES.update({
                                                     C.c_size_t]),
 })
 
-# exported stubs whose arguments are all addresses; all take the handle first
+# entry points (and exported stubs) whose arguments are all addresses; all take the handle first
 for _name, _n in (("reactive_transport_set_gas_thermo", 7),
                   ("reactive_transport_rhs_gas", 6), ("reactive_transport_gas_properties_resident", 5),
                   ("reactive_transport_set_capillary_v1", 2), ("reactive_transport_capillary_geometry_v1", 6),

@@ -346,8 +346,98 @@ def viscous_fixture(lib):
     print("viscous sets", sorted(sets))
 
 
+# capillary transport models: (sigma, capillaryCfl, geometryEpsilon, viscosity, conductivity)
+CAPILLARY_MODELS = {"s002": (0.02, 0.5, 1e-6, 0.0, 0.0), "s1e6": (1e6, 0.5, 1e-6, 0.0, 0.0),
+                    "s002_mu_k": (0.02, 0.8, 1e-6, 0.05, 40.0)}
+
+
+def capillary_meshes(lib):
+    """Inputs of the capillary transport fixture: the oblique graph with every
+    face kind (helpers.random_graph) and a 6^3 N2O box, each with the liquid
+    inventory of species 0 as variable 5 and a material colour (random on the
+    graph, a smoothed sphere in the box)."""
+    rng = np.random.default_rng(11)
+    fg, vg, qg, tg, fqg, ftg = random_graph(193, 7)
+    qg = np.concatenate([qg, rng.uniform(0, 1, len(qg))[:, None] * qg[:, :1]], axis=1)
+    fqg = np.concatenate([fqg, rng.uniform(0, 1, len(fqg))[:, None] * fqg[:, :1]], axis=1)
+    colour_g = rng.uniform(0, 1, len(qg))
+    colour_g[::7] = 0.0
+    colour_g[3::11] = 1.0
+    fixed_colour_g = np.array([0.0, 1.0, 0.35])
+    faces6, volume6, q6, seeds6, _ = cases.pure_n2o_cube(6, seed=5)
+    mb = model.model_bytes(model.build_model())
+    st6, ok6, _ = flash_set(lib, mb, q6[:, :1], cases.internal_energy(q6, 1), seeds6)
+    st6[ok6 == 0] = st6[np.where(ok6 == 1)[0][0]]
+    liquid6 = np.clip(st6["liquidMass"][:, 0], 0, q6[:, 0])
+    q6 = np.concatenate([q6, liquid6[:, None]], axis=1)
+    n, edge = 6, 0.08
+    h = edge / n
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    centre = (np.stack([i, j, k], -1).reshape(-1, 3) + 0.5) * h
+    r = np.linalg.norm(centre - edge / 2, axis=1)
+    colour6 = np.clip(0.5 * (1 - np.tanh((r - 0.3 * edge) / h)), 0.0, 1.0)
+    return {"graph": (fg, vg, qg, tg, fqg, ftg, colour_g, fixed_colour_g),
+            "cube6": (faces6, volume6, q6, cases.transport_states(st6), None, None, colour6, None)}
+
+
+def capillary_transport_fixture(lib):
+    """The reference's serial operators with the resolved-interface capillary
+    model (reactiveTransport.cpp:859-962; reactiveTransportKernels.h:200-210,
+    357-417, 478-493, 540-547): the colour geometry (surface energy,
+    curvature, normals), the capillary profile, rhs, CFL step and both RK
+    stages, for each model of CAPILLARY_MODELS on both meshes."""
+    V = C.c_void_p
+    lib.reactive_transport_set_capillary_v1.argtypes = [V, V]
+    lib.reactive_transport_capillary_geometry_v1.argtypes = [V] * 6
+    lib.reactive_transport_capillary_profile_v1.argtypes = [V, V]
+    sets = {}
+    for mesh, (fc, vol, q, ts, fq, fst, colour, fixed_colour) in capillary_meshes(lib).items():
+        for name, (sigma, ccfl, eps, mu, k) in CAPILLARY_MODELS.items():
+            cfg = config(len(q), len(fc), nv=6, fixed=0 if fq is None else len(fq), viscosity=mu, conductivity=k)
+            err = C.create_string_buffer(1024)
+            h = lib.reactive_transport_create(0, C.byref(cfg), p(vol), p(fc), p(fq), p(fst), None, None, err, 1024)
+            assert h, err.value
+            opt = abi.CapillaryOptions.make(sigma, ccfl, eps, 0)
+            assert lib.reactive_transport_set_capillary_v1(h, C.byref(opt)) == 0, lib.reactive_transport_error(h)
+            energy, curvature, normal = np.zeros(len(q)), np.zeros(len(q)), np.zeros((len(q), 3))
+            assert lib.reactive_transport_capillary_geometry_v1(h, p(colour), p(fixed_colour), p(energy),
+                                                                 p(curvature), p(normal)) == 0, \
+                lib.reactive_transport_error(h)
+            rhs = np.zeros_like(q)
+            b = np.zeros(6)
+            assert lib.reactive_transport_rhs(h, p(q), p(ts), None, None, p(rhs), p(b)) == 0, \
+                lib.reactive_transport_error(h)
+            dt = C.c_double()
+            assert lib.reactive_transport_stable_step(h, p(q), p(ts), 0.4, 1.0, C.byref(dt)) == 0, \
+                lib.reactive_transport_error(h)
+            q1 = q.copy()
+            b0 = np.zeros(6)
+            assert lib.reactive_transport_stage(h, p(q1), p(ts), None, None, dt.value, 0, p(b0)) == 0, \
+                lib.reactive_transport_error(h)
+            q2 = q1.copy()
+            b1 = np.zeros(6)
+            assert lib.reactive_transport_stage(h, p(q2), p(ts), None, None, dt.value, 1, p(b1)) == 0, \
+                lib.reactive_transport_error(h)
+            prof = abi.CapillaryProfile.fresh()
+            assert lib.reactive_transport_capillary_profile_v1(h, C.byref(prof)) == 0
+            lib.reactive_transport_destroy(h)
+            sets[f"{mesh}_{name}"] = dict(
+                model=np.array([sigma, ccfl, eps, mu, k]), energy=energy, curvature=curvature, normal=normal,
+                rhs=rhs, boundary=b, dt=dt.value, q1=q1, b0=b0, q2=q2, b1=b1,
+                profile=np.array([getattr(prof, f) for f, _ in abi.CapillaryProfile._fields_[2:]], np.uint64))
+            print(mesh, name, "dt", dt.value, "profile", sets[f"{mesh}_{name}"]["profile"])
+        sets[mesh] = dict(faces=fc, volume=vol, q=q, states=ts, colour=colour,
+                          fixed_q=fq if fq is not None else np.zeros((0, 6)),
+                          fixed_states=fst if fst is not None else np.zeros(0, dtype=abi.TSTATE_DTYPE),
+                          fixed_colour=fixed_colour if fixed_colour is not None else np.zeros(0))
+    np.savez_compressed(os.path.join(HERE, "transport_capillary.npz"),
+                        **{f"{n}__{k}": v for n, d in sets.items() for k, v in d.items()})
+    print("capillary transport sets", sorted(sets))
+
+
 SETS = {"base": base_fixtures, "frozen": frozen_fixture, "prune": prune_fixture, "analytic": analytic_fixture,
-        "viscous": viscous_fixture, "capillary": capillary_fixture}
+        "viscous": viscous_fixture, "capillary": capillary_fixture,
+        "capillary_transport": capillary_transport_fixture}
 
 
 def main(names):
