@@ -1652,8 +1652,19 @@ int reactive_transport_capillary_profile_v1(void* t, ReactiveCapillaryProfileV1*
         *profile = x.capillaryProfile;
     });
 }
-int reactive_transport_replace_stage_state_v1(void* t, ReactiveTransportToken, uint64_t, const double*) {
-    return stub(t, "reactive_transport_replace_stage_state_v1", "capillary stage replacement");
+int reactive_transport_replace_stage_state_v1(void* t, ReactiveTransportToken before, uint64_t version,
+                                              const double* q) {
+    return protect(t, [&](Transport& x) {  // reactiveTransport.cpp:1636-1645
+        // the post-RK1 state of an open attempt is replaced (after the driver's
+        // coupled flash); the stage-0 snapshot that RK2 needs is kept
+        require(x.attemptOpen && x.tokenStage == 1 && before.attemptId == x.attemptId && before.stageId == 1 &&
+                    x.haveResident && before.contentVersion == x.residentVersion && x.haveInitial &&
+                    version > x.highestVersion && q,
+                "Invalid post-RK1 capillary/flash state replacement");
+        x.uploadQ(q);
+        x.finish();
+        x.residentVersion = x.highestVersion = version;
+    });
 }
 int reactive_transport_capillary_resident_begin_v1(void* t, size_t, const double*, const double*, const double*,
                                                    const void*, const double*, double, int, int*) {

@@ -180,6 +180,74 @@ def test_resident_tokens_and_rules():
     ora.close()
 
 
+def test_replace_stage_state():
+    """The post-RK1 state of an open attempt replaced between the stages
+    (reactiveTransport.cpp:1636-1645; ReactiveFoam.C:1000): RK2 then combines
+    the kept stage-0 input with the replacement, as the oracle does.  A
+    refused replacement (no stage-1 state, a stale token or version, no
+    array) fails like every failed call: the resident state is dropped
+    (reactiveTransport.cpp:1481), so the attempt must roll back."""
+    lib = rlib.load()
+    case = cube_case(5, seed=4)
+    q, st = case["q"], case["tstates"]
+    cfg = config(len(q), len(case["faces"]))
+    opts = abi.OptionsV2(abiVersion=1, structBytes=C.sizeof(abi.OptionsV2), bridgeCells=64, recomputeGas=0,
+                         physicalModelHash=b"a" * 64)
+    err = C.create_string_buffer(1024)
+    h = lib.reactive_transport_create_v2(1, C.byref(cfg), C.byref(opts), rlib.ptr(case["volume"]),
+                                         rlib.ptr(case["faces"]), None, None, None, None, err, 1024)
+    assert h, err.value
+    ora = OracleTransport(cfg, case["volume"], case["faces"])
+    dt = ora.stable_step(q, st, 0.4, 1.0)[1]
+    b = np.zeros(5)
+    T = abi.Token
+    try:
+        assert lib.reactive_transport_begin_attempt(h, b"a" * 64, 1) == 0
+        assert lib.reactive_transport_upload_conserved(h, rlib.ptr(q), 5) == 0
+        assert lib.reactive_transport_advance_resident_v2(h, T(1, 0, 5), T(1, 1, 6), rlib.ptr(st), None, None, None,
+                                                          dt, rlib.ptr(b)) == 0, lib.reactive_transport_error(h)
+        qo = q.copy()
+        ora.stage(qo, st, dt, 0)
+        replaced = qo.copy()
+        replaced[:, 4] *= 1.0 + 1e-3  # the driver's flash moved the energy
+        assert lib.reactive_transport_replace_stage_state_v1(h, T(1, 1, 6), 7, rlib.ptr(replaced)) == 0, \
+            lib.reactive_transport_error(h)
+        got = np.zeros_like(q)
+        assert lib.reactive_transport_download_conserved(h, T(1, 1, 7), rlib.ptr(got)) == 0
+        assert np.array_equal(got, replaced)
+        assert lib.reactive_transport_advance_resident_v2(h, T(1, 1, 7), T(1, 2, 8), rlib.ptr(st), None, None,
+                                                          None, dt, rlib.ptr(b)) == 0, lib.reactive_transport_error(h)
+        assert lib.reactive_transport_download_conserved(h, T(1, 2, 8), rlib.ptr(got)) == 0
+        ora.stage(replaced, st, dt, 1)
+        assert np.array_equal(got, replaced)
+        assert lib.reactive_transport_end_attempt(h, 1, 1) == 0
+        version = 10
+        refusals = [lambda v: (T(2, 0, v), v + 1, rlib.ptr(q)),  # before stage 1
+                    lambda v: (T(2, 1, v), v + 2, rlib.ptr(q)),  # stale content token
+                    lambda v: (T(2, 1, v + 1), v + 1, rlib.ptr(q)),  # version not increasing
+                    lambda v: (T(2, 1, v + 1), v + 2, None)]  # no array
+        for attempt, refusal in enumerate(refusals, start=2):
+            assert lib.reactive_transport_begin_attempt(h, b"a" * 64, attempt) == 0
+            assert lib.reactive_transport_upload_conserved(h, rlib.ptr(q), version) == 0
+            if attempt > 2:
+                assert lib.reactive_transport_advance_resident_v2(
+                    h, T(attempt, 0, version), T(attempt, 1, version + 1), rlib.ptr(st), None, None, None, dt,
+                    rlib.ptr(b)) == 0, lib.reactive_transport_error(h)
+            token, new_version, array = refusal(version)
+            token = T(attempt, token.stageId, token.contentVersion)
+            assert lib.reactive_transport_replace_stage_state_v1(h, token, new_version, array) == 1
+            assert b"replacement" in lib.reactive_transport_error(h)
+            # the resident state is gone: even the right replacement is refused now
+            assert lib.reactive_transport_replace_stage_state_v1(h, T(attempt, 1, version + 1), version + 5,
+                                                                 rlib.ptr(q)) == 1
+            assert lib.reactive_transport_end_attempt(h, attempt, 1) == 1
+            assert lib.reactive_transport_end_attempt(h, attempt, 0) == 0
+            version += 10
+    finally:
+        lib.reactive_transport_destroy(h)
+        ora.close()
+
+
 def test_error_contracts():
     lib = rlib.load()
     case = cube_case(4, seed=4)
