@@ -22,11 +22,12 @@
 #endif
 // Deferred refinements up to this many per flash run one cell per wave with
 // the Newton line searches spread over its lanes (the flash's WAVE mode);
-// more run one cell per lane (measured: one cell per wave took a binary
-// N2O/N2 96^3 step from 4.55 to 2.23 s with 16k deferred cells per flash, but
-// a 160^3 one from 6.74 to 7.18 s with 75k: too many rounds of waves).
+// more run one cell per lane.  Measured on the step's 4096-wave queue grid:
+// one cell per wave takes a binary N2O/N2 160^3 step (75k deferred cells per
+// flash) from 6.51 to 4.85 s; on a 1024-wave grid it had been slower there
+// (6.74 -> 7.18 s) and faster at 96^3 (16k cells, 4.55 -> 2.23 s).
 #ifndef RAMD_WAVE_QUEUE_CELLS
-#define RAMD_WAVE_QUEUE_CELLS 32768
+#define RAMD_WAVE_QUEUE_CELLS 131072
 #endif
 
 namespace ramd {
