@@ -20,6 +20,16 @@
 #ifndef RAMD_REFINE_MIN_WAVES
 #define RAMD_REFINE_MIN_WAVES 2
 #endif
+// Persistent grid of the queue kernels (the step's and the HEM ABI's mixed
+// flash): 4096 single-wave workgroups (sixteen
+// per CU, more than stay resident, so a wave that finishes its cell is
+// replaced at once) take up to 4096 deferred cells per round one cell per
+// wave, or 262,144 one cell per lane; workgroups beyond the queue leave at
+// once.  Against 1024: 160^3 pure N2O step 135.8 -> 132.5 ms (2,537 deferred
+// cells per flash), binary N2O/N2 96^3 2242 -> 1788 ms, bit for bit the same.
+#ifndef RAMD_QUEUE_BLOCKS
+#define RAMD_QUEUE_BLOCKS 4096
+#endif
 // Deferred refinements up to this many per flash run one cell per wave with
 // the Newton line searches spread over its lanes (the flash's WAVE mode);
 // more run one cell per lane.  Measured on the step's 4096-wave queue grid:

@@ -241,7 +241,7 @@ __global__ void queueCountKernel(const unsigned* __restrict__ count, unsigned lo
 }
 
 // Workgroups of the persistent deferred-refinement grid (64 lanes each).
-constexpr unsigned kQueueBlocks = 1024;
+constexpr unsigned kQueueBlocks = RAMD_QUEUE_BLOCKS;
 
 // Scheduling key (reactiveHemBucket.h:13-18).
 inline unsigned bucketKey(const double* q, int ns, int activeLiquids) {

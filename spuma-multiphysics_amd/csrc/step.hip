@@ -642,15 +642,6 @@ inline unsigned blocksFor(size_t n, unsigned b) { return unsigned((n + b - 1) / 
 #ifndef RAMD_REFINE_QUEUE
 #define RAMD_REFINE_QUEUE 1
 #endif
-// Persistent grid of the queue kernel: 4096 single-wave workgroups (sixteen
-// per CU, more than stay resident, so a wave that finishes its cell is
-// replaced at once) take up to 4096 deferred cells per round one cell per
-// wave, or 262,144 one cell per lane; workgroups beyond the queue leave at
-// once.  Against 1024: 160^3 pure N2O step 135.8 -> 132.5 ms (2,537 deferred
-// cells per flash), binary N2O/N2 96^3 2242 -> 1788 ms, bit for bit the same.
-#ifndef RAMD_QUEUE_BLOCKS
-#define RAMD_QUEUE_BLOCKS 4096
-#endif
 constexpr unsigned kQueueBlocks = RAMD_QUEUE_BLOCKS;
 constexpr int kMarks = 11;  // events per step
 
